@@ -1,0 +1,439 @@
+"""virginian_amd -- MI355X-native opcode-VM query executor.
+
+Python bindings over the C ABI of virginian_amd/lib/libvirginian_amd.so
+(declared in include/virginian.h and include/virg_gpu.h).  The library is
+built in-tree by ``python -m virginian_amd.build`` (or
+``__graft_entry__.build()``); importing this package without it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "lib" / "libvirginian_amd.so"
+
+VIRG_FAIL, VIRG_SUCCESS = 0, 1
+VIRG_MAX_COLUMNS = 16
+VIRG_MAX_COLUMN_NAME = 16
+VIRG_MAX_TABLES = 16
+VIRG_MAX_TABLE_NAME = 32
+VIRG_MEM_TABLETS = 64
+VIRG_OPS = 32
+VIRG_REGS = 16
+VIRG_GLOBAL_REGS = 16
+VIRG_ROW_BUFFER = 256
+VIRG_TABLET_SIZE = 8 * 1024 * 1024
+
+# opcodes (reference: src/virginian.h:166-193)
+(OP_Table, OP_ResultColumn, OP_Parallel, OP_Finish, OP_Column, OP_Rowid, OP_Result,
+ OP_Converge, OP_Invalid, OP_Cast, OP_Integer, OP_Float, OP_Le, OP_Lt, OP_Ge, OP_Gt,
+ OP_Eq, OP_Neq, OP_Add, OP_Sub, OP_Mul, OP_Div, OP_And, OP_Or, OP_Not, OP_Nop) = range(26)
+
+# value types (reference: src/virginian.h:197-205)
+VIRG_INT, VIRG_INT64, VIRG_FLOAT, VIRG_DOUBLE, VIRG_CHAR, VIRG_STRING, VIRG_NULL = range(7)
+NP_TYPE = {VIRG_INT: np.int32, VIRG_INT64: np.int64, VIRG_FLOAT: np.float32,
+           VIRG_DOUBLE: np.float64, VIRG_CHAR: np.int8}
+
+
+class TabletInfo(C.Structure):
+    _fields_ = [("id", C.c_uint), ("used", C.c_int), ("disk_slot", C.c_uint)]
+
+
+class TabletMeta(C.Structure):
+    """virg_tablet_meta (reference: src/virginian.h:251-302), 704 bytes."""
+    _fields_ = [
+        ("rows", C.c_uint), ("key_type", C.c_int), ("key_stride", C.c_size_t),
+        ("key_pointer_stride", C.c_size_t), ("id", C.c_uint), ("next", C.c_uint),
+        ("last_tablet", C.c_int), ("in_table", C.c_int), ("table_id", C.c_uint),
+        ("key_block", C.c_size_t), ("key_pointers_block", C.c_size_t),
+        ("fixed_block", C.c_size_t), ("variable_block", C.c_size_t), ("size", C.c_size_t),
+        ("possible_rows", C.c_uint), ("row_stride", C.c_size_t), ("fixed_columns", C.c_uint),
+        ("fixed_name", (C.c_char * VIRG_MAX_COLUMN_NAME) * VIRG_MAX_COLUMNS),
+        ("fixed_type", C.c_int * VIRG_MAX_COLUMNS),
+        ("fixed_stride", C.c_size_t * VIRG_MAX_COLUMNS),
+        ("fixed_offset", C.c_size_t * VIRG_MAX_COLUMNS),
+        ("info", C.POINTER(TabletInfo)),
+        ("_pad", C.c_char * 8),
+    ]
+
+
+class Var(C.Union):
+    _fields_ = [("i", C.c_int), ("f", C.c_float), ("li", C.c_longlong), ("d", C.c_double),
+                ("c", C.c_char), ("s", C.c_char_p)]
+
+
+class Op(C.Structure):
+    """virg_op (reference: src/virginian.h:326-332), 24 bytes."""
+    _fields_ = [("op", C.c_int), ("p1", C.c_int), ("p2", C.c_int), ("p3", C.c_int), ("p4", Var)]
+
+
+class ResultNode(C.Structure):
+    pass
+
+
+ResultNode._fields_ = [("id", C.c_uint), ("next", C.POINTER(ResultNode))]
+
+
+class VM(C.Structure):
+    """virg_vm (reference: src/virginian.h:356-377), 1016 bytes."""
+    _fields_ = [
+        ("pc", C.c_uint), ("stmt", Op * VIRG_OPS), ("num_ops", C.c_uint),
+        ("global_reg", Var * VIRG_GLOBAL_REGS), ("type", C.c_int * VIRG_GLOBAL_REGS),
+        ("table", C.c_uint * 1), ("num_tables", C.c_uint),
+        ("head_result", C.POINTER(ResultNode)), ("tail_result", C.POINTER(ResultNode)),
+        ("timing1", C.c_float), ("timing2", C.c_float), ("timing3", C.c_float),
+    ]
+
+
+class DB(C.Structure):
+    _fields_ = [
+        ("num_tablets", C.c_uint), ("alloced_tablets", C.c_uint), ("tablet_id_counter", C.c_uint),
+        ("block_size", C.c_size_t),
+        ("tables", (C.c_char * VIRG_MAX_TABLE_NAME) * VIRG_MAX_TABLES),
+        ("first_tablet", C.c_uint * VIRG_MAX_TABLES), ("last_tablet", C.c_uint * VIRG_MAX_TABLES),
+        ("write_cursor", C.c_uint * VIRG_MAX_TABLES), ("table_tablets", C.c_uint * VIRG_MAX_TABLES),
+        ("table_status", C.c_int * VIRG_MAX_TABLES), ("tablet_info", C.POINTER(TabletInfo)),
+    ]
+
+
+class State(C.Structure):
+    """virginian (reference: src/virginian.h:473-504), 1976 bytes."""
+    _fields_ = [
+        ("db", DB), ("tablet_slot_ids", C.c_uint * VIRG_MEM_TABLETS),
+        ("tablet_slot_status", C.c_int * VIRG_MEM_TABLETS),
+        ("tablet_slots_taken", C.c_uint), ("tablet_slot_counter", C.c_uint),
+        ("tablet_slots", C.POINTER(TabletMeta) * VIRG_MEM_TABLETS), ("gpu_slots", C.c_void_p),
+        ("slot_lock", C.c_byte * 40), ("dbfd", C.c_int), ("threads_per_block", C.c_uint),
+        ("multi_threads", C.c_uint), ("use_multi", C.c_int), ("use_gpu", C.c_int),
+        ("use_stream", C.c_int), ("use_mmap", C.c_int),
+    ]
+
+
+class Reader(C.Structure):
+    _fields_ = [("vm", C.POINTER(VM)), ("res", C.POINTER(TabletMeta)), ("row", C.c_uint),
+                ("buffer", C.c_char * VIRG_ROW_BUFFER)]
+
+
+class VirginianError(RuntimeError):
+    pass
+
+
+def _load() -> C.CDLL:
+    if not _LIB_PATH.exists():
+        raise VirginianError(
+            f"{_LIB_PATH} is missing: build it with `python -m virginian_amd.build` "
+            "(the executor has no CPU fallback)")
+    lib = C.CDLL(str(_LIB_PATH), mode=C.RTLD_GLOBAL)
+    P, V = C.c_void_p, C.POINTER(State)
+    sig = {
+        "virg_init": (C.c_int, [V]), "virg_close": (C.c_int, [V]),
+        "virg_gpu_getsizes": (C.POINTER(C.c_size_t), []),
+        "virg_table_create": (C.c_int, [V, C.c_char_p, C.c_int]),
+        "virg_table_addcolumn": (C.c_int, [V, C.c_uint, C.c_char_p, C.c_int]),
+        "virg_table_insert": (C.c_int, [V, C.c_uint, P, P, P]),
+        "virg_table_getid": (C.c_int, [V, C.c_char_p, C.POINTER(C.c_uint)]),
+        "virg_table_numrows": (C.c_int, [V, C.c_uint, C.POINTER(C.c_uint)]),
+        "virg_table_appendtablet": (C.c_int, [V, C.c_uint, C.c_uint, C.c_uint, P, C.POINTER(P)]),
+        "virg_table_tablets": (C.c_int, [V, C.c_uint, C.POINTER(C.POINTER(TabletMeta)), C.c_int]),
+        "virg_gen_table": (C.c_int, [V, C.c_char_p, C.c_uint64, C.c_uint, C.c_uint32, C.c_int]),
+        "virg_gen_rows_mt": (None, [C.c_uint32, C.c_uint64, C.c_uint64, P, P, P, P, P, P, P, C.c_int]),
+        "virg_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "virg_log": (C.c_double, [C.c_double]),
+        "virg_gen_normal": (C.c_double, [C.c_uint32, C.c_uint32, C.c_uint64]),
+        "virg_gen_uniform": (C.c_double, [C.c_uint32, C.c_uint32, C.c_uint64]),
+        "virg_vm_init": (C.POINTER(VM), []),
+        "virg_vm_cleanup": (None, [V, C.POINTER(VM)]),
+        "virg_vm_addop": (C.c_int, [C.POINTER(VM), C.c_int, C.c_int, C.c_int, C.c_int, Var]),
+        "virg_vm_execute": (C.c_int, [V, C.POINTER(VM)]),
+        "virg_vm_freeresults": (C.c_int, [V, C.POINTER(VM)]),
+        "virg_sql": (C.c_int, [V, C.c_char_p, C.POINTER(VM)]),
+        "virg_sql_compile": (C.c_int, [V, C.c_char_p, C.POINTER(VM), C.c_char_p, C.c_size_t]),
+        "virg_query": (C.c_int, [V, C.POINTER(C.POINTER(Reader)), C.c_char_p]),
+        "virg_release": (None, [V, C.POINTER(Reader)]),
+        "virg_reader_row": (C.c_int, [V, C.POINTER(Reader)]),
+        "virg_reader_getrows": (C.c_int, [V, C.POINTER(Reader), C.POINTER(C.c_uint)]),
+        "virg_result_columns": (C.c_int, [V, C.POINTER(VM), C.POINTER(P), C.POINTER(C.c_ulonglong)]),
+        "virg_opstring": (C.c_char_p, [C.c_int]),
+        "virg_tablet_check": (C.c_int, [C.POINTER(TabletMeta)]),
+        "virg_vm_gpu": (C.c_int, [V, C.POINTER(VM), C.POINTER(C.POINTER(TabletMeta)),
+                                  C.POINTER(C.POINTER(TabletMeta)), C.c_uint]),
+        "vg_query_create": (P, [V, C.POINTER(Op), C.c_uint, C.c_ulonglong, C.c_int, C.c_char_p, C.c_size_t]),
+        "vg_query_free": (None, [P]),
+        "vg_query_launch": (C.c_int, [P]),
+        "vg_query_sync": (C.c_int, [P, C.POINTER(C.c_ulonglong)]),
+        "vg_query_run": (C.c_int, [P, C.POINTER(C.c_ulonglong)]),
+        "vg_query_fetch": (C.c_int, [P, C.POINTER(P), C.c_ulonglong]),
+        "vg_query_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_ulonglong)]),
+        "vg_query_info": (C.c_int, [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                    C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
+        "vg_device_info": (C.c_int, [V, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
+        "vg_decode_program": (C.c_int, [C.POINTER(Op), C.c_uint, C.c_uint, C.POINTER(TabletMeta),
+                                        P, C.c_size_t, C.c_char_p, C.c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def struct_sizes() -> list[int]:
+    """virg_gpu_getsizes(): sizes of the 15 ABI structs as compiled."""
+    p = lib.virg_gpu_getsizes()
+    return [p[i] for i in range(15)]
+
+
+class Virginian:
+    """One database instance (virg_init .. virg_close)."""
+
+    def __init__(self) -> None:
+        self._st = State()
+        if lib.virg_init(C.byref(self._st)) != VIRG_SUCCESS:
+            raise VirginianError("virg_init failed")
+        self.ptr = C.pointer(self._st)
+        self.use_gpu = True
+
+    def close(self) -> None:
+        if self.ptr is not None:
+            lib.virg_close(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def use_gpu(self) -> bool:
+        return bool(self._st.use_gpu)
+
+    @use_gpu.setter
+    def use_gpu(self, on: bool) -> None:
+        self._st.use_gpu = 1 if on else 0
+
+    @property
+    def state(self) -> State:
+        return self._st
+
+    def device(self) -> tuple[str, int]:
+        name = C.create_string_buffer(64)
+        cus = C.c_int(0)
+        lib.vg_device_info(self.ptr, name, 64, C.byref(cus))
+        return name.value.decode(), cus.value
+
+    # ---- tables -----------------------------------------------------------
+    def table_id(self, name: str) -> int:
+        tid = C.c_uint(0)
+        if lib.virg_table_getid(self.ptr, name.encode(), C.byref(tid)) != VIRG_SUCCESS:
+            raise VirginianError(f"no table {name!r}")
+        return tid.value
+
+    def create_table(self, name: str, key_type: int, columns: list[tuple[str, int]]) -> int:
+        if lib.virg_table_create(self.ptr, name.encode(), key_type) != VIRG_SUCCESS:
+            raise VirginianError("virg_table_create failed")
+        tid = self.table_id(name)
+        for cname, ctype in columns:
+            if lib.virg_table_addcolumn(self.ptr, tid, cname.encode(), ctype) != VIRG_SUCCESS:
+                raise VirginianError(f"virg_table_addcolumn {cname} failed")
+        return tid
+
+    def append_tablet(self, table_id: int, key: np.ndarray, cols: list[np.ndarray],
+                      capacity: int | None = None) -> None:
+        n = len(key)
+        cap = capacity if capacity is not None else max(16, (n + 15) // 16 * 16)
+        key = np.ascontiguousarray(key)
+        cols = [np.ascontiguousarray(c) for c in cols]
+        arr = (C.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
+        if lib.virg_table_appendtablet(self.ptr, table_id, cap, n, key.ctypes.data, arr) != VIRG_SUCCESS:
+            raise VirginianError("virg_table_appendtablet failed")
+
+    def insert(self, table_id: int, key: np.ndarray, cols: list[np.ndarray]) -> None:
+        """Row-at-a-time insert (reference path: 8 MB tablets)."""
+        rowdt = np.dtype([(f"c{i}", c.dtype) for i, c in enumerate(cols)])
+        rows = np.empty(len(key), dtype=rowdt)
+        for i, c in enumerate(cols):
+            rows[f"c{i}"] = c
+        key = np.ascontiguousarray(key)
+        ks = key.dtype.itemsize
+        for r in range(len(key)):
+            if lib.virg_table_insert(self.ptr, table_id, key.ctypes.data + r * ks,
+                                     rows.ctypes.data + r * rowdt.itemsize, None) != VIRG_SUCCESS:
+                raise VirginianError("virg_table_insert failed")
+
+    def gen_table(self, name: str, rows: int, tablet_rows: int = 1 << 20, seed: int = 1,
+                  threads: int = 8) -> int:
+        """Standard 7-column synthetic table (see csrc/host/gen.c)."""
+        if lib.virg_gen_table(self.ptr, name.encode(), rows, tablet_rows, seed, threads) != VIRG_SUCCESS:
+            raise VirginianError("virg_gen_table failed")
+        return self.table_id(name)
+
+    def tablets(self, table_id: int) -> list:
+        n = lib.virg_table_tablets(self.ptr, table_id, None, 0)
+        arr = (C.POINTER(TabletMeta) * max(n, 1))()
+        lib.virg_table_tablets(self.ptr, table_id, arr, n)
+        return [arr[i] for i in range(n)]
+
+    def numrows(self, table_id: int) -> int:
+        n = C.c_uint(0)
+        lib.virg_table_numrows(self.ptr, table_id, C.byref(n))
+        return n.value
+
+    # ---- programs ---------------------------------------------------------
+    def compile(self, sql: str):
+        """SQL -> virg_vm (caller frees with free_vm)."""
+        vm = lib.virg_vm_init()
+        err = C.create_string_buffer(256)
+        if lib.virg_sql_compile(self.ptr, sql.encode(), vm, err, 256) != VIRG_SUCCESS:
+            lib.virg_vm_cleanup(self.ptr, vm)
+            raise VirginianError(err.value.decode())
+        return vm
+
+    def free_vm(self, vm) -> None:
+        lib.virg_vm_cleanup(self.ptr, vm)
+
+    def execute(self, vm) -> dict:
+        """virg_vm_execute on the GPU, then the logical result as arrays."""
+        if lib.virg_vm_execute(self.ptr, vm) != VIRG_SUCCESS:
+            raise VirginianError("virg_vm_execute failed")
+        return self.results(vm)
+
+    def results(self, vm) -> dict:
+        n = C.c_ulonglong(0)
+        lib.virg_result_columns(self.ptr, vm, None, C.byref(n))
+        names, types = result_schema(vm)
+        outs = [np.empty(n.value, dtype=NP_TYPE[t]) for t in types]
+        ptrs = (C.c_void_p * max(len(outs), 1))(*[o.ctypes.data for o in outs])
+        lib.virg_result_columns(self.ptr, vm, ptrs, C.byref(n))
+        return dict(zip(names, outs))
+
+    def query(self, sql: str) -> dict:
+        vm = self.compile(sql)
+        try:
+            return self.execute(vm)
+        finally:
+            self.free_vm(vm)
+
+    def prepare(self, sql_or_ops, capacity: int = 0, rows_per_lane: int = 0) -> "DeviceQuery":
+        return DeviceQuery(self, sql_or_ops, capacity, rows_per_lane)
+
+
+def program(vm) -> list[tuple]:
+    """The op-program of a virg_vm as (op, p1, p2, p3, p4) tuples (p4 as
+    int / float / str depending on the op)."""
+    out = []
+    for i in range(vm.contents.num_ops):
+        o = vm.contents.stmt[i]
+        if o.op == OP_ResultColumn:
+            p4 = o.p4.s.decode() if o.p4.s else None
+        elif o.op == OP_Float:
+            p4 = o.p4.f
+        else:
+            p4 = o.p4.i
+        out.append((o.op, o.p1, o.p2, o.p3, p4))
+    return out
+
+
+def result_schema(vm) -> tuple[list[str], list[int]]:
+    names, types = [], []
+    for i in range(vm.contents.num_ops):
+        o = vm.contents.stmt[i]
+        if o.op == OP_ResultColumn:
+            names.append(o.p4.s.decode() if o.p4.s else f"c{len(names)}")
+            types.append(o.p1)
+    return names, types
+
+
+def ops_array(vm) -> tuple:
+    n = vm.contents.num_ops
+    arr = (Op * n)()
+    for i in range(n):
+        arr[i] = vm.contents.stmt[i]
+    return arr, n
+
+
+def make_ops(ops: list[tuple]) -> tuple:
+    """(op, p1, p2, p3, p4) tuples -> Op array; p4 may be int, float or str."""
+    arr = (Op * len(ops))()
+    keep = []
+    for i, (op, p1, p2, p3, p4) in enumerate(ops):
+        arr[i].op, arr[i].p1, arr[i].p2, arr[i].p3 = op, p1, p2, p3
+        arr[i].p4.li = 0
+        if isinstance(p4, float):
+            arr[i].p4.f = p4
+        elif isinstance(p4, str):
+            b = C.create_string_buffer(p4.encode(), VIRG_MAX_COLUMN_NAME)
+            keep.append(b)
+            arr[i].p4.s = C.cast(b, C.c_char_p)
+        elif p4 is not None:
+            arr[i].p4.i = int(p4)
+    return arr, len(ops), keep
+
+
+class DeviceQuery:
+    """A query prepared for repeated execution with its table resident in
+    HBM (vg_query_* in include/virg_gpu.h)."""
+
+    def __init__(self, db: Virginian, sql_or_ops, capacity: int = 0, rows_per_lane: int = 0):
+        self.db = db
+        if isinstance(sql_or_ops, str):
+            vm = db.compile(sql_or_ops)
+            try:
+                arr, n = ops_array(vm)
+                self.names, self.types = result_schema(vm)
+                self._keep = []
+            finally:
+                db.free_vm(vm)
+        else:
+            arr, n, self._keep = make_ops(sql_or_ops)
+            self.names = [o[4] if isinstance(o[4], str) else f"c{k}"
+                          for k, o in enumerate([o for o in sql_or_ops if o[0] == OP_ResultColumn])]
+            self.types = [o[1] for o in sql_or_ops if o[0] == OP_ResultColumn]
+        self._ops = arr
+        err = C.create_string_buffer(256)
+        self.h = lib.vg_query_create(db.ptr, arr, n, capacity, rows_per_lane, err, 256)
+        if not self.h:
+            raise VirginianError(err.value.decode() or "vg_query_create failed")
+        rows, tiles, rb, wb, nout = (C.c_ulonglong(), C.c_uint(), C.c_uint(), C.c_uint(), C.c_uint())
+        lib.vg_query_info(self.h, C.byref(rows), C.byref(tiles), C.byref(rb), C.byref(wb), C.byref(nout))
+        self.rows, self.tiles = rows.value, tiles.value
+        self.read_bytes_per_row, self.write_bytes_per_row = rb.value, wb.value
+        self.last_rows = 0
+
+    def run(self) -> int:
+        n = C.c_ulonglong(0)
+        if lib.vg_query_run(self.h, C.byref(n)) != VIRG_SUCCESS:
+            raise VirginianError("vg_query_run failed")
+        self.last_rows = n.value
+        return n.value
+
+    def fetch(self) -> dict:
+        outs = [np.empty(self.last_rows, dtype=NP_TYPE[t]) for t in self.types]
+        ptrs = (C.c_void_p * max(len(outs), 1))(*[o.ctypes.data for o in outs])
+        if lib.vg_query_fetch(self.h, ptrs, self.last_rows) != VIRG_SUCCESS:
+            raise VirginianError("vg_query_fetch failed")
+        return dict(zip(self.names, outs))
+
+    def time(self, warmup: int, iters: int) -> tuple[float, int]:
+        ms = C.c_double(0)
+        n = C.c_ulonglong(0)
+        if lib.vg_query_time(self.h, warmup, iters, C.byref(ms), C.byref(n)) != VIRG_SUCCESS:
+            raise VirginianError("vg_query_time failed")
+        self.last_rows = n.value
+        return ms.value, n.value
+
+    def close(self) -> None:
+        if self.h:
+            lib.vg_query_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,690 @@
+/*
+ * Device layer: HIP context, pinned host slots, the device-resident tablet
+ * cache, the virg_vm_gpu boundary and the vg_query C ABI.
+ *
+ * Differences from the reference host driver (src/vm/vm_gpu.cu:107-578):
+ *  - one kernel launch covers every tablet of the query instead of one
+ *    launch per tablet with a full 8 MB result copy after each;
+ *  - tablets are copied to HBM once and stay resident (keyed by tablet id,
+ *    re-copied when their row count or layout changes), so repeated
+ *    queries read them from HBM; the reference's serial / stream / mapped
+ *    transfer modes are all served by this cache;
+ *  - the program is decoded on the host and passed as a kernel argument
+ *    instead of copying virg_vm to __constant__ memory;
+ *  - results are produced in row order and copied back only as large as
+ *    they are, filling each host result tablet to capacity.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+#include <stdarg.h>
+#include <unordered_map>
+#include <vector>
+#include <string>
+
+extern "C" {
+#include "../host/internal.h"
+#include "vg_prog.h"
+}
+
+extern "C" int vg_kernel_launch(const vg_prog *P, const vg_launch *L, int rows_per_lane, hipStream_t stream);
+extern "C" uint32_t vg_kernel_tile_rows(int rows_per_lane);
+extern "C" int vg_kernel_pick_rpl(const vg_prog *P, int requested);
+extern "C" int vg_decode(const virg_op *stmt, unsigned num_ops, unsigned pc,
+	const virg_tablet_meta *tab, const virg_tablet_meta *res, vg_prog *P, char *err, size_t errlen);
+
+#define VG_EPOCH_MAX 0xFFFFFFu
+
+struct resident {
+	void *dptr = nullptr;
+	size_t bytes = 0;
+	unsigned rows = 0;
+	size_t size = 0, fixed_block = 0;
+	unsigned possible_rows = 0;
+};
+
+/* launch resources owned by one query stream: control block, tile status
+ * words and the epoch that tags them */
+struct launch_res {
+	vg_ctrl *ctrl = nullptr;
+	uint64_t *status = nullptr;
+	size_t status_cap = 0;
+	uint32_t epoch = 0;
+};
+
+struct vgd_device {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	std::unordered_map<unsigned, resident> tablets;
+	launch_res lr;
+	void *outbuf = nullptr;
+	size_t outbuf_bytes = 0;
+	int rows_per_lane = 4;
+	hipDeviceProp_t prop;
+};
+
+static int herr(hipError_t e, const char *what)
+{
+	if (e != hipSuccess) {
+		fprintf(stderr, "::: Virginian HIP error in %s: %s\n", what, hipGetErrorString(e));
+		return 1;
+	}
+	return 0;
+}
+#define HCHK(call, what) do { if (herr((call), (what))) return VIRG_FAIL; } while (0)
+
+static vgd_device *dev_of(virginian *v)
+{
+	virg_state *s = virg_state_of(v);
+	return s ? s->dev : nullptr;
+}
+
+extern "C" int vgd_open(virginian *v)
+{
+	virg_state *s = virg_state_of(v);
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+		(void)hipGetLastError();
+		s->dev = nullptr;
+		return 0;
+	}
+	vgd_device *d = new vgd_device();
+	d->device = VIRG_CUDADEVICE;
+	if (hipSetDevice(d->device) != hipSuccess ||
+		hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess ||
+		hipGetDeviceProperties(&d->prop, d->device) != hipSuccess) {
+		(void)hipGetLastError();
+		delete d;
+		s->dev = nullptr;
+		return 0;
+	}
+	const char *rpl = getenv("VIRG_ROWS_PER_LANE");
+	if (rpl) {
+		int x = atoi(rpl);
+		if (x == 1 || x == 2 || x == 4)
+			d->rows_per_lane = x;
+	}
+	s->dev = d;
+	return 1;
+}
+
+static void free_lr(launch_res &lr)
+{
+	if (lr.ctrl) (void)hipFree(lr.ctrl);
+	if (lr.status) (void)hipFree(lr.status);
+	lr = launch_res();
+}
+
+extern "C" void vgd_close(virginian *v)
+{
+	virg_state *s = virg_state_of(v);
+	if (!s || !s->dev)
+		return;
+	vgd_device *d = s->dev;
+	(void)hipStreamSynchronize(d->stream);
+	for (auto &kv : d->tablets)
+		(void)hipFree(kv.second.dptr);
+	free_lr(d->lr);
+	if (d->outbuf)
+		(void)hipFree(d->outbuf);
+	(void)hipStreamDestroy(d->stream);
+	delete d;
+	s->dev = nullptr;
+}
+
+extern "C" void vgd_forget_tablet(virginian *v, unsigned id)
+{
+	vgd_device *d = dev_of(v);
+	if (!d)
+		return;
+	auto it = d->tablets.find(id);
+	if (it == d->tablets.end())
+		return;
+	(void)hipStreamSynchronize(d->stream);
+	(void)hipFree(it->second.dptr);
+	d->tablets.erase(it);
+}
+
+extern "C" void *vgd_host_alloc(size_t bytes)
+{
+	void *p = nullptr;
+	if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+		(void)hipGetLastError();
+		return nullptr;
+	}
+	return p;
+}
+
+extern "C" void vgd_host_free(void *p)
+{
+	(void)hipHostFree(p);
+}
+
+/* Make a tablet resident in HBM; returns its device copy. */
+static void *resident_tablet(vgd_device *d, const virg_tablet_meta *t, hipStream_t stream)
+{
+	auto it = d->tablets.find(t->id);
+	if (it != d->tablets.end()) {
+		resident &r = it->second;
+		if (r.rows == t->rows && r.size == t->size && r.fixed_block == t->fixed_block &&
+			r.possible_rows == t->possible_rows)
+			return r.dptr;
+		(void)hipStreamSynchronize(stream);
+		(void)hipFree(r.dptr);
+		d->tablets.erase(it);
+	}
+	resident r;
+	r.bytes = t->size;
+	if (hipMalloc(&r.dptr, r.bytes) != hipSuccess) {
+		(void)hipGetLastError();
+		fprintf(stderr, "::: Virginian: cannot make tablet %u (%zu bytes) resident in HBM\n", t->id, r.bytes);
+		return nullptr;
+	}
+	if (hipMemcpyAsync(r.dptr, t, r.bytes, hipMemcpyHostToDevice, stream) != hipSuccess) {
+		(void)hipGetLastError();
+		(void)hipFree(r.dptr);
+		return nullptr;
+	}
+	r.rows = t->rows;
+	r.size = t->size;
+	r.fixed_block = t->fixed_block;
+	r.possible_rows = t->possible_rows;
+	d->tablets[t->id] = r;
+	return r.dptr;
+}
+
+static int ensure_lr(launch_res &lr, uint32_t num_tiles, hipStream_t stream)
+{
+	if (!lr.ctrl) {
+		HCHK(hipMalloc((void **)&lr.ctrl, sizeof(vg_ctrl)), "control block");
+		HCHK(hipMemsetAsync(lr.ctrl, 0, sizeof(vg_ctrl), stream), "control block");
+	}
+	if (lr.status_cap < num_tiles) {
+		if (lr.status) {
+			HCHK(hipStreamSynchronize(stream), "status");
+			HCHK(hipFree(lr.status), "status");
+		}
+		size_t cap = num_tiles < 1024 ? 1024 : num_tiles;
+		HCHK(hipMalloc((void **)&lr.status, cap * sizeof(uint64_t)), "tile status");
+		HCHK(hipMemsetAsync(lr.status, 0, cap * sizeof(uint64_t), stream), "tile status");
+		lr.status_cap = cap;
+		lr.epoch = 0;
+	}
+	if (++lr.epoch > VG_EPOCH_MAX) {
+		HCHK(hipMemsetAsync(lr.status, 0, lr.status_cap * sizeof(uint64_t), stream), "tile status");
+		lr.epoch = 1;
+	}
+	return VIRG_SUCCESS;
+}
+
+/* device description of a list of resident tablets for one tile size */
+struct plan {
+	std::vector<vg_tabdesc> desc;
+	std::vector<uint32_t> tile_tab;
+	vg_tabdesc *d_desc = nullptr;
+	uint32_t *d_tile = nullptr;
+	uint32_t num_tiles = 0;
+	uint64_t rows = 0;
+};
+
+static void free_plan(plan &p)
+{
+	if (p.d_desc) (void)hipFree(p.d_desc);
+	if (p.d_tile) (void)hipFree(p.d_tile);
+	p.d_desc = nullptr;
+	p.d_tile = nullptr;
+}
+
+static int build_plan(vgd_device *d, const std::vector<const virg_tablet_meta *> &tabs, int rpl,
+	hipStream_t stream, plan &p)
+{
+	const uint32_t T = vg_kernel_tile_rows(rpl);
+	p.desc.clear();
+	p.tile_tab.clear();
+	p.rows = 0;
+	for (size_t i = 0; i < tabs.size(); i++) {
+		const virg_tablet_meta *t = tabs[i];
+		const virg_tablet_meta *t0 = tabs[0];
+		VIRG_CHECK(t->fixed_columns != t0->fixed_columns || t->key_type != t0->key_type ||
+			memcmp(t->fixed_type, t0->fixed_type, sizeof t->fixed_type) != 0,
+			"tablets of one table differ in schema");
+		VIRG_CHECK(t->fixed_columns > VG_MAX_COLS - 1, "too many columns");
+		char *base = (char *)resident_tablet(d, t, stream);
+		if (!base)
+			return VIRG_FAIL;
+		vg_tabdesc td;
+		memset(&td, 0, sizeof td);
+		bool aligned = true;
+		for (unsigned c = 0; c < t->fixed_columns && c < VG_MAX_COLS - 1; c++) {
+			uint64_t a = (uint64_t)(base + t->fixed_block + t->fixed_offset[c]);
+			td.col[c] = a;
+			aligned = aligned && (a % 16 == 0);
+		}
+		uint64_t ka = (uint64_t)(base + t->key_block);
+		td.col[VG_KEY_SRC] = ka;
+		aligned = aligned && (ka % 16 == 0);
+		td.rows = t->rows;
+		td.first_tile = (uint32_t)p.tile_tab.size();
+		td.aligned = aligned ? 1u : 0u;
+		uint32_t nt = (t->rows + T - 1) / T;
+		for (uint32_t k = 0; k < nt; k++)
+			p.tile_tab.push_back((uint32_t)i);
+		p.desc.push_back(td);
+		p.rows += t->rows;
+	}
+	p.num_tiles = (uint32_t)p.tile_tab.size();
+	free_plan(p);
+	if (!p.desc.empty()) {
+		HCHK(hipMalloc((void **)&p.d_desc, p.desc.size() * sizeof(vg_tabdesc)), "descriptors");
+		HCHK(hipMemcpyAsync(p.d_desc, p.desc.data(), p.desc.size() * sizeof(vg_tabdesc),
+			hipMemcpyHostToDevice, stream), "descriptors");
+	}
+	if (p.num_tiles) {
+		HCHK(hipMalloc((void **)&p.d_tile, p.tile_tab.size() * sizeof(uint32_t)), "tile table");
+		HCHK(hipMemcpyAsync(p.d_tile, p.tile_tab.data(), p.tile_tab.size() * sizeof(uint32_t),
+			hipMemcpyHostToDevice, stream), "tile table");
+	}
+	return VIRG_SUCCESS;
+}
+
+static int launch(const vg_prog &P, const plan &p, launch_res &lr, void *const *out,
+	uint64_t capacity, int rpl, hipStream_t stream)
+{
+	if (p.num_tiles == 0)
+		return VIRG_SUCCESS;
+	if (ensure_lr(lr, p.num_tiles, stream) == VIRG_FAIL)
+		return VIRG_FAIL;
+	vg_launch L;
+	memset(&L, 0, sizeof L);
+	L.tabs = p.d_desc;
+	L.tile_tab = p.d_tile;
+	L.num_tiles = p.num_tiles;
+	L.epoch = lr.epoch;
+	L.capacity = capacity;
+	L.ctrl = lr.ctrl;
+	L.status = lr.status;
+	for (unsigned j = 0; j < P.nout; j++)
+		L.out[j] = out[j];
+	int rc = vg_kernel_launch(&P, &L, rpl, stream);
+	if (rc != 0) {
+		fprintf(stderr, "::: Virginian: kernel launch failed (%d)\n", rc);
+		return VIRG_FAIL;
+	}
+	return VIRG_SUCCESS;
+}
+
+static int read_ctrl(const launch_res &lr, hipStream_t stream, uint64_t *rows)
+{
+	vg_ctrl c;
+	HCHK(hipMemcpyAsync(&c, lr.ctrl, sizeof c, hipMemcpyDeviceToHost, stream), "control block");
+	HCHK(hipStreamSynchronize(stream), "kernel");
+	if (c.error != 0 || c.overflow != 0) {
+		/* leave a clean control block for the next launch */
+		HCHK(hipMemsetAsync(lr.ctrl, 0, sizeof(vg_ctrl), stream), "control block");
+		HCHK(hipStreamSynchronize(stream), "control block");
+	}
+	VIRG_CHECK(c.error != 0, "a tile waited too long for its predecessors (look-back gave up)");
+	VIRG_CHECK(c.overflow != 0, "result exceeded the output capacity");
+	rows[0] = c.total;
+	return VIRG_SUCCESS;
+}
+
+/* ------------------------------------------------------------ boundary */
+
+/* reference: src/vm/vm_gpu.cu:27 */
+extern "C" const size_t *virg_gpu_getsizes(void)
+{
+	return &virg_testsizes[0];
+}
+
+/* reference: src/vm/vm_gpu.cu:107 */
+extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
+	virg_tablet_meta **res, unsigned num_tablets)
+{
+	vgd_device *d = dev_of(v);
+	VIRG_CHECK(d == nullptr, "no usable GPU: the data-parallel VM needs a HIP device");
+	VIRG_CHECK(v->threads_per_block != VIRG_THREADSPERBLOCK, "Cannot change compile-time threads per block");
+	HCHK(hipSetDevice(d->device), "set device");
+
+	char err[256];
+	vg_prog P;
+	if (vg_decode(vm->stmt, vm->num_ops, vm->pc, tab[0], res[0], &P, err, sizeof err) == VIRG_FAIL) {
+		fprintf(stderr, "::: Virginian: op-program not supported on the GPU: %s\n", err);
+		return VIRG_FAIL;
+	}
+
+	/* collect the tablets: tab[0] arrives locked; like the reference the
+	 * walk unlocks each tablet as it moves on and leaves the last locked */
+	std::vector<const virg_tablet_meta *> tabs;
+	unsigned proced = 0;
+	while (1) {
+		tabs.push_back(tab[0]);
+		proced++;
+		if (tab[0]->last_tablet || (num_tablets != 0 && proced >= num_tablets))
+			break;
+		VIRG_CHECK(virg_db_loadnext(v, tab) == VIRG_FAIL, "could not load next tablet");
+		if (tab[0]->rows == 0)
+			break;
+	}
+
+	hipEvent_t ev[4];
+	for (int i = 0; i < 4; i++)
+		HCHK(hipEventCreate(&ev[i]), "event");
+	HCHK(hipEventRecord(ev[0], d->stream), "event");
+	plan p;
+	int rpl = vg_kernel_pick_rpl(&P, d->rows_per_lane);
+	VIRG_CHECK(rpl == 0, "op-program needs more register slots than the LDS budget allows");
+	if (build_plan(d, tabs, rpl, d->stream, p) == VIRG_FAIL) {
+		free_plan(p);
+		return VIRG_FAIL;
+	}
+	HCHK(hipEventRecord(ev[1], d->stream), "event");
+
+	/* device output columns, capacity = every row selected */
+	size_t widths[VG_MAX_OUT], total_w = 0;
+	for (unsigned j = 0; j < P.nout; j++) {
+		widths[j] = P.out[j].width;
+		total_w += widths[j];
+	}
+	size_t need = (size_t)p.rows * total_w + 256 * (P.nout + 1);
+	if (need > d->outbuf_bytes) {
+		if (d->outbuf) {
+			HCHK(hipStreamSynchronize(d->stream), "sync");
+			HCHK(hipFree(d->outbuf), "free");
+		}
+		d->outbuf = nullptr;
+		d->outbuf_bytes = 0;
+		HCHK(hipMalloc(&d->outbuf, need), "result buffer");
+		d->outbuf_bytes = need;
+	}
+	void *out[VG_MAX_OUT];
+	size_t off = 0;
+	for (unsigned j = 0; j < P.nout; j++) {
+		out[j] = (char *)d->outbuf + off;
+		off += ((size_t)p.rows * widths[j] + 255) / 256 * 256;
+	}
+
+	int ok = launch(P, p, d->lr, out, p.rows, rpl, d->stream);
+	HCHK(hipEventRecord(ev[2], d->stream), "event");
+	uint64_t nres = 0;
+	if (ok == VIRG_SUCCESS && p.num_tiles)
+		ok = read_ctrl(d->lr, d->stream, &nres);
+	free_plan(p);
+	if (ok == VIRG_FAIL)
+		return VIRG_FAIL;
+
+	/* fill host result tablets in order */
+	uint64_t done = 0;
+	while (1) {
+		virg_tablet_meta *r = res[0];
+		uint64_t room = r->possible_rows > r->rows ? r->possible_rows - r->rows : 0;
+		uint64_t n = nres - done < room ? nres - done : room;
+		for (unsigned j = 0; j < P.nout && n > 0; j++) {
+			char *dst = (char *)r + r->fixed_block + r->fixed_offset[j] + (size_t)r->rows * widths[j];
+			HCHK(hipMemcpyAsync(dst, (char *)out[j] + done * widths[j], n * widths[j],
+				hipMemcpyDeviceToHost, d->stream), "result copy");
+		}
+		r->rows += (unsigned)n;
+		done += n;
+		if (done >= nres)
+			break;
+		VIRG_CHECK(room == 0 && n == 0 && r->possible_rows == 0, "result tablet has no capacity");
+		virg_tablet_meta *old = r;
+		VIRG_CHECK(virg_vm_allocresult(v, vm, res, old) == VIRG_FAIL, "result allocation failed");
+		virg_tablet_unlock(v, old->id);
+	}
+	HCHK(hipEventRecord(ev[3], d->stream), "event");
+	HCHK(hipStreamSynchronize(d->stream), "result copy");
+	float ms[3];
+	for (int i = 0; i < 3; i++)
+		HCHK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]), "event time");
+	vm->timing1 = ms[0] / 1000.0f;
+	vm->timing2 = ms[1] / 1000.0f;
+	vm->timing3 = ms[2] / 1000.0f;
+	for (int i = 0; i < 4; i++)
+		(void)hipEventDestroy(ev[i]);
+	return VIRG_SUCCESS;
+}
+
+/* -------------------------------------------------------- vg_query ABI */
+
+struct vg_query {
+	virginian *v = nullptr;
+	vgd_device *d = nullptr;
+	vg_prog P;
+	std::vector<const virg_tablet_meta *> tabs;
+	plan p;
+	launch_res lr;
+	void *out[VG_MAX_OUT];
+	size_t width[VG_MAX_OUT];
+	uint64_t capacity = 0;
+	int rpl = 4;
+	hipStream_t stream = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	uint64_t last_rows = 0;
+	uint64_t read_bytes_per_row = 0;
+};
+
+static int qerr(char *err, size_t n, const char *fmt, ...)
+{
+	if (err && n) {
+		va_list ap;
+		va_start(ap, fmt);
+		vsnprintf(err, n, fmt, ap);
+		va_end(ap);
+	}
+	return VIRG_FAIL;
+}
+
+extern "C" void vg_query_free(vg_query *q)
+{
+	if (!q)
+		return;
+	if (q->stream)
+		(void)hipStreamSynchronize(q->stream);
+	free_plan(q->p);
+	free_lr(q->lr);
+	for (unsigned j = 0; j < VG_MAX_OUT; j++)
+		if (q->out[j])
+			(void)hipFree(q->out[j]);
+	if (q->e0) (void)hipEventDestroy(q->e0);
+	if (q->e1) (void)hipEventDestroy(q->e1);
+	if (q->stream) (void)hipStreamDestroy(q->stream);
+	delete q;
+}
+
+/* Prepare the data-parallel section of a generated program (ops from the
+ * SQL front-end or virg_vm_addop: Table, ResultColumn..., Parallel, ...,
+ * Result, Converge, Finish) for repeated execution over its table with the
+ * table resident in HBM.  Output capacity is `capacity` rows (0 = every
+ * row of the table). */
+extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned num_ops,
+	unsigned long long capacity, int rows_per_lane, char *err, size_t errlen)
+{
+	vgd_device *d = dev_of(v);
+	if (!d) {
+		qerr(err, errlen, "no usable GPU: the data-parallel VM needs a HIP device");
+		return nullptr;
+	}
+	if (hipSetDevice(d->device) != hipSuccess) {
+		qerr(err, errlen, "cannot select device");
+		return nullptr;
+	}
+	int table = -1, par = -1;
+	virg_tablet_meta res;
+	memset(&res, 0, sizeof res);
+	for (unsigned i = 0; i < num_ops; i++) {
+		if (ops[i].op == OP_Table && table < 0)
+			table = ops[i].p1;
+		if (ops[i].op == OP_ResultColumn && res.fixed_columns < VIRG_MAX_COLUMNS) {
+			res.fixed_type[res.fixed_columns] = (virg_t)ops[i].p1;
+			res.fixed_stride[res.fixed_columns] = virg_sizeof((virg_t)ops[i].p1);
+			res.fixed_columns++;
+		}
+		if (ops[i].op == OP_Parallel && par < 0)
+			par = (int)i;
+	}
+	if (table < 0 || table >= VIRG_MAX_TABLES || !v->db.table_status[table] || par < 0) {
+		qerr(err, errlen, "program has no Table/Parallel op or names an unknown table");
+		return nullptr;
+	}
+	vg_query *q = new vg_query();
+	memset(q->out, 0, sizeof q->out);
+	q->v = v;
+	q->d = d;
+	q->rpl = rows_per_lane ? rows_per_lane : d->rows_per_lane;
+	virg_tablet_meta *t;
+	if (virg_db_load(v, v->db.first_tablet[table], &t) == VIRG_FAIL) {
+		qerr(err, errlen, "table has no tablets");
+		delete q;
+		return nullptr;
+	}
+	if (vg_decode(ops, num_ops, (unsigned)par + 1, t, &res, &q->P, err, errlen) == VIRG_FAIL) {
+		virg_tablet_unlock(v, t->id);
+		delete q;
+		return nullptr;
+	}
+	while (1) {
+		q->tabs.push_back(t);
+		if (t->last_tablet)
+			break;
+		virg_db_loadnext(v, &t);
+	}
+	virg_tablet_unlock(v, t->id);
+	if (hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess ||
+		hipEventCreate(&q->e0) != hipSuccess || hipEventCreate(&q->e1) != hipSuccess) {
+		qerr(err, errlen, "cannot create stream/events");
+		vg_query_free(q);
+		return nullptr;
+	}
+	q->rpl = vg_kernel_pick_rpl(&q->P, q->rpl);
+	if (q->rpl == 0) {
+		qerr(err, errlen, "op-program needs more register slots than the LDS budget allows");
+		vg_query_free(q);
+		return nullptr;
+	}
+	if (build_plan(d, q->tabs, q->rpl, q->stream, q->p) == VIRG_FAIL) {
+		qerr(err, errlen, "cannot make the table resident");
+		vg_query_free(q);
+		return nullptr;
+	}
+	q->capacity = capacity ? capacity : q->p.rows;
+	for (unsigned j = 0; j < q->P.nout; j++) {
+		q->width[j] = q->P.out[j].width;
+		if (hipMalloc(&q->out[j], q->capacity * q->width[j] + 256) != hipSuccess) {
+			qerr(err, errlen, "cannot allocate %llu-row output", (unsigned long long)q->capacity);
+			vg_query_free(q);
+			return nullptr;
+		}
+	}
+	uint64_t rb = 0;
+	for (unsigned c = 0; c < q->P.ncol; c++)
+		rb += q->P.col[c].width;
+	q->read_bytes_per_row = rb;
+	if (ensure_lr(q->lr, q->p.num_tiles, q->stream) == VIRG_FAIL ||
+		hipStreamSynchronize(q->stream) != hipSuccess) {
+		qerr(err, errlen, "cannot set up the launch");
+		vg_query_free(q);
+		return nullptr;
+	}
+	return q;
+}
+
+/* Launch once, asynchronously, on the query's stream. */
+extern "C" int vg_query_launch(vg_query *q)
+{
+	return launch(q->P, q->p, q->lr, q->out, q->capacity, q->rpl, q->stream);
+}
+
+/* Wait for the last launch; returns the selected row count. */
+extern "C" int vg_query_sync(vg_query *q, unsigned long long *rows)
+{
+	uint64_t n = 0;
+	if (q->p.num_tiles && read_ctrl(q->lr, q->stream, &n) == VIRG_FAIL)
+		return VIRG_FAIL;
+	q->last_rows = n;
+	rows[0] = n;
+	return VIRG_SUCCESS;
+}
+
+extern "C" int vg_query_run(vg_query *q, unsigned long long *rows)
+{
+	if (vg_query_launch(q) == VIRG_FAIL)
+		return VIRG_FAIL;
+	return vg_query_sync(q, rows);
+}
+
+/* Copy the first min(rows, max_rows) result rows to host arrays. */
+extern "C" int vg_query_fetch(vg_query *q, void **cols, unsigned long long max_rows)
+{
+	uint64_t n = q->last_rows < max_rows ? q->last_rows : max_rows;
+	for (unsigned j = 0; j < q->P.nout; j++)
+		if (cols[j] && n)
+			HCHK(hipMemcpyAsync(cols[j], q->out[j], n * q->width[j], hipMemcpyDeviceToHost, q->stream), "fetch");
+	HCHK(hipStreamSynchronize(q->stream), "fetch");
+	return VIRG_SUCCESS;
+}
+
+/* Time `iters` back-to-back launches after `warmup` untimed ones with HIP
+ * events on the query's stream (the stream the kernel runs on).  Reports
+ * the mean time per launch. */
+extern "C" int vg_query_time(vg_query *q, int warmup, int iters, double *ms_per_launch,
+	unsigned long long *rows)
+{
+	for (int i = 0; i < warmup; i++)
+		if (vg_query_launch(q) == VIRG_FAIL)
+			return VIRG_FAIL;
+	if (vg_query_sync(q, rows) == VIRG_FAIL)
+		return VIRG_FAIL;
+	HCHK(hipEventRecord(q->e0, q->stream), "event");
+	for (int i = 0; i < iters; i++)
+		if (vg_query_launch(q) == VIRG_FAIL)
+			return VIRG_FAIL;
+	HCHK(hipEventRecord(q->e1, q->stream), "event");
+	if (vg_query_sync(q, rows) == VIRG_FAIL)
+		return VIRG_FAIL;
+	float ms = 0;
+	HCHK(hipEventElapsedTime(&ms, q->e0, q->e1), "event time");
+	ms_per_launch[0] = iters > 0 ? ms / iters : 0.0;
+	return VIRG_SUCCESS;
+}
+
+/* Shape of a prepared query: rows scanned, tiles, referenced-column bytes
+ * per row, result bytes per row, result columns. */
+extern "C" int vg_query_info(vg_query *q, unsigned long long *rows, unsigned *tiles,
+	unsigned *read_bytes_per_row, unsigned *write_bytes_per_row, unsigned *nout)
+{
+	rows[0] = q->p.rows;
+	tiles[0] = q->p.num_tiles;
+	read_bytes_per_row[0] = (unsigned)q->read_bytes_per_row;
+	unsigned w = 0;
+	for (unsigned j = 0; j < q->P.nout; j++)
+		w += (unsigned)q->width[j];
+	write_bytes_per_row[0] = w;
+	nout[0] = q->P.nout;
+	return VIRG_SUCCESS;
+}
+
+/* Name and compute-unit count of the device, 0 if there is none. */
+extern "C" int vg_device_info(virginian *v, char *name, size_t n, int *cus)
+{
+	vgd_device *d = dev_of(v);
+	if (!d) {
+		if (n) name[0] = '\0';
+		cus[0] = 0;
+		return VIRG_FAIL;
+	}
+	snprintf(name, n, "%s", d->prop.gcnArchName);
+	cus[0] = d->prop.multiProcessorCount;
+	return VIRG_SUCCESS;
+}
+
+/* Decode only (no device needed): used to inspect the micro-program. */
+extern "C" int vg_decode_program(const virg_op *ops, unsigned num_ops, unsigned pc,
+	const virg_tablet_meta *tab, void *out, size_t out_bytes, char *err, size_t errlen)
+{
+	if (out_bytes < sizeof(vg_prog))
+		return qerr(err, errlen, "output buffer smaller than vg_prog (%zu bytes)", sizeof(vg_prog));
+	return vg_decode(ops, num_ops, pc, tab, nullptr, (vg_prog *)out, err, errlen);
+}

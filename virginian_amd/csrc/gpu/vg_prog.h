@@ -1,0 +1,140 @@
+/*
+ * Decoded op-program: the form in which the data-parallel section of a
+ * virg_vm program reaches the gfx950 kernels.
+ *
+ * The host decoder (decode.cpp) resolves everything that is uniform across
+ * rows before launch: register types (the reference keeps one type per
+ * register for a whole row block, src/vm/virginia.c:449 and :144-145),
+ * constants (Integer/Float registers become immediates), column loads
+ * (Column/Rowid copy a whole block in the reference, virginia.c:445-448, so
+ * a column register is simply the loaded column), jump targets (remapped to
+ * micro-op indices) and physical register slots.  The kernel then walks the
+ * micro-ops with wave-uniform dispatch: every field below is read with
+ * scalar loads.
+ *
+ * Physical register file: up to VG_NSLOT 32-bit slots per row, held in
+ * LDS as [slot][row of the tile].  Loaded columns take the first slots
+ * (8-byte types two: low and high word), computed values follow; the
+ * decoder reuses a computed value's slot once its last reader has run.
+ */
+#ifndef VG_PROG_H
+#define VG_PROG_H
+
+#include <stdint.h>
+
+#define VG_MAX_UOPS      48
+#define VG_MAX_COLS      17      /* distinct column sources: 16 fixed + key */
+#define VG_MAX_OUT       16
+#define VG_NSLOT         48
+#define VG_KEY_SRC       16      /* column source index of the key column */
+
+/* micro-op kinds */
+enum {
+	VG_K_LT = 0, VG_K_LE, VG_K_GT, VG_K_GE, VG_K_EQ, VG_K_NE, VG_K_AND, VG_K_OR,
+	VG_K_NOT,        /* branch if a == 0 */
+	VG_K_INVALID,    /* active rows become invalid */
+	VG_K_ADD, VG_K_SUB, VG_K_MUL, VG_K_DIV,
+	VG_K_MOV,        /* dst = a (typed copy) */
+	VG_K_CAST,       /* dst = (type b) a, in place on the reference's register */
+	VG_K_COUNT
+};
+
+/* flags */
+#define VG_F_AIMM    0x01   /* operand a is the immediate ia */
+#define VG_F_BIMM    0x02   /* operand b is the immediate ib */
+#define VG_F_MASKED  0x04   /* write only rows active at this micro-op */
+#define VG_F_KEEP    0x08   /* branch keeps validity (p4 != 0) */
+
+/* value types: the virg_t numbering */
+enum { VG_T_INT = 0, VG_T_INT64 = 1, VG_T_FLOAT = 2, VG_T_DOUBLE = 3, VG_T_CHAR = 4 };
+
+typedef struct {
+	uint8_t  kind;
+	uint8_t  type;     /* operand type; for CAST the source type */
+	uint8_t  flags;
+	uint8_t  target;   /* branch: micro-op index where taken rows resume */
+	uint8_t  dst, dst_hi;
+	uint8_t  a, a_hi;
+	uint8_t  b, b_hi;  /* for CAST: b = destination type */
+	uint16_t src_op;   /* op index in the original program (diagnostics) */
+	uint32_t pad;
+	uint64_t ia, ib;   /* immediates (bit patterns, 32-bit types in the low word) */
+} vg_uop;              /* 32 bytes */
+
+/* column loads */
+enum { VG_PHASE_FILTER = 0, VG_PHASE_PROJECT = 1, VG_PHASE_EMIT = 2 };
+
+typedef struct {
+	uint8_t src;       /* 0..15 fixed column, VG_KEY_SRC = key */
+	uint8_t width;     /* 1, 4 or 8 bytes */
+	uint8_t phase;
+	uint8_t type;
+	uint8_t slot, slot_hi;
+	uint8_t pad[2];
+} vg_col;              /* 8 bytes */
+
+/* result columns */
+enum { VG_OUT_SLOT = 0, VG_OUT_COLUMN = 1, VG_OUT_CONST = 2 };
+
+typedef struct {
+	uint8_t  kind;
+	uint8_t  width;    /* bytes written per row */
+	uint8_t  slot, slot_hi;
+	uint8_t  col;      /* VG_OUT_COLUMN: index into vg_prog.col */
+	uint8_t  type;
+	uint8_t  pad[2];
+	uint64_t imm;      /* VG_OUT_CONST value (bit pattern) */
+} vg_out;              /* 16 bytes */
+
+typedef struct {
+	uint32_t nuop;      /* micro-ops in total */
+	uint32_t nfilter;   /* micro-ops [0, nfilter) can change validity */
+	uint32_t ncol;      /* entries in col[] (loaded ones first) */
+	uint32_t nload;     /* entries of col[] that occupy slots */
+	uint32_t nout;
+	uint32_t nslot;     /* highest slot used + 1 */
+	uint32_t has_project_loads;
+	uint32_t pad;
+	vg_col   col[VG_MAX_COLS + 1];
+	vg_out   out[VG_MAX_OUT];
+	vg_uop   uop[VG_MAX_UOPS];
+} vg_prog;
+
+/* per-tablet device descriptor: base address of every column of one
+ * device-resident tablet, its row count and its first tile */
+typedef struct {
+	uint64_t col[VG_MAX_COLS];   /* device address of column (or key) row 0 */
+	uint32_t rows;
+	uint32_t first_tile;
+	uint32_t aligned;            /* every column base is 16-byte aligned */
+	uint32_t pad;
+} vg_tabdesc;                    /* 152 bytes */
+
+/* per-launch control block in device memory */
+typedef struct {
+	uint32_t tile_counter;   /* next tile to hand out */
+	uint32_t done;           /* workgroups finished */
+	uint64_t total;          /* rows selected (written by the last tile) */
+	uint32_t overflow;       /* output capacity was exceeded */
+	uint32_t error;          /* a bounded wait gave up */
+	uint32_t pad[2];
+} vg_ctrl;
+
+/* kernel arguments besides the program */
+typedef struct {
+	const vg_tabdesc *tabs;
+	const uint32_t *tile_tab;
+	uint32_t num_tiles;
+	uint32_t epoch;
+	uint64_t capacity;
+	vg_ctrl *ctrl;
+	uint64_t *status;
+	void *out[VG_MAX_OUT];
+} vg_launch;
+
+/* per-tile status word: [63:40] epoch, [39:38] flag, [37:0] value */
+#define VG_ST_AGG     1ull
+#define VG_ST_PREFIX  2ull
+#define VG_ST_VALMASK ((1ull << 38) - 1)
+
+#endif
