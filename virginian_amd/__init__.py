@@ -8,11 +8,14 @@ built in-tree by ``python -m virginian_amd.build`` (or
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-_LIB_PATH = Path(__file__).resolve().parent / "lib" / "libvirginian_amd.so"
+# VIRGINIAN_AMD_LIB selects another build of the same library (A/B timing)
+_LIB_PATH = Path(os.environ.get("VIRGINIAN_AMD_LIB") or
+                 Path(__file__).resolve().parent / "lib" / "libvirginian_amd.so")
 
 VIRG_FAIL, VIRG_SUCCESS = 0, 1
 VIRG_MAX_COLUMNS = 16

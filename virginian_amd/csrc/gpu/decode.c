@@ -548,5 +548,65 @@ int vg_decode(const virg_op *stmt, unsigned num_ops, unsigned pc,
 	}
 	P->nout = nres;
 	P->nslot = (uint32_t)maxslot;
+
+	/* Write pass: validity is already known from the count pass, so it
+	 * only needs the micro-ops that compute result registers (and what
+	 * they read).  Masked writes depend on per-row resume points, which
+	 * only a full rerun reproduces. */
+	int full = 0;
+	for (int k = 0; k < nu; k++)
+		if (P->uop[k].flags & VG_F_MASKED)
+			full = 1;
+	uint64_t need = 0;
+	uint8_t slot_live[VG_NSLOT];
+	memset(slot_live, 0, sizeof slot_live);
+	for (unsigned j = 0; j < nres; j++)
+		if (P->out[j].kind == VG_OUT_SLOT) {
+			slot_live[P->out[j].slot] = 1;
+			slot_live[P->out[j].slot_hi] = 1;
+		}
+	if (full) {
+		need = nu >= 64 ? ~0ull : ((1ull << nu) - 1);
+	} else {
+		/* backward over micro-ops: a value op is needed if it writes a
+		 * live slot; its operands then become live */
+		for (int k = nu - 1; k >= 0; k--) {
+			const vg_uop *o = &P->uop[k];
+			if (o->kind <= VG_K_INVALID)
+				continue;
+			if (!slot_live[o->dst] && !slot_live[o->dst_hi])
+				continue;
+			need |= 1ull << k;
+			slot_live[o->dst] = slot_live[o->dst_hi] = 0;
+			if (!(o->flags & VG_F_AIMM)) {
+				slot_live[o->a] = 1;
+				slot_live[o->a_hi] = 1;
+			}
+			if (o->kind != VG_K_MOV && o->kind != VG_K_CAST && !(o->flags & VG_F_BIMM)) {
+				slot_live[o->b] = 1;
+				slot_live[o->b_hi] = 1;
+			}
+		}
+	}
+	P->wfull = (uint32_t)full;
+	P->wneed = need;
+	for (unsigned c = 0; c < P->nload; c++) {
+		vg_col *col = &P->col[c];
+		int used = full;
+		for (int k = 0; k < nu && !used; k++) {
+			if (!((need >> k) & 1))
+				continue;
+			const vg_uop *o = &P->uop[k];
+			if (!(o->flags & VG_F_AIMM) && (o->a == col->slot || o->a_hi == col->slot))
+				used = 1;
+			if (o->kind != VG_K_MOV && o->kind != VG_K_CAST && !(o->flags & VG_F_BIMM) &&
+				(o->b == col->slot || o->b_hi == col->slot))
+				used = 1;
+		}
+		for (unsigned j = 0; j < nres && !used; j++)
+			if (P->out[j].kind == VG_OUT_SLOT && P->out[j].slot == col->slot)
+				used = 1;
+		col->wload = (uint8_t)used;
+	}
 	return VIRG_SUCCESS;
 }

@@ -28,13 +28,15 @@ extern "C" {
 #include "vg_prog.h"
 }
 
-extern "C" int vg_kernel_launch(const vg_prog *P, const vg_launch *L, int rows_per_lane, hipStream_t stream);
+extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, int rows_per_lane, hipStream_t stream);
 extern "C" uint32_t vg_kernel_tile_rows(int rows_per_lane);
+extern "C" uint32_t vg_kernel_grid(const vg_prog *P, int rows_per_lane, uint32_t num_tiles);
 extern "C" int vg_kernel_pick_rpl(const vg_prog *P, int requested);
 extern "C" int vg_decode(const virg_op *stmt, unsigned num_ops, unsigned pc,
 	const virg_tablet_meta *tab, const virg_tablet_meta *res, vg_prog *P, char *err, size_t errlen);
 
-#define VG_EPOCH_MAX 0xFFFFFFu
+/* validity bytes per tile: one per lane of the 256-thread workgroup */
+#define VG_VBYTES_PER_TILE 256u
 
 struct resident {
 	void *dptr = nullptr;
@@ -44,13 +46,14 @@ struct resident {
 	unsigned possible_rows = 0;
 };
 
-/* launch resources owned by one query stream: control block, tile status
- * words and the epoch that tags them */
+/* launch resources owned by one query stream: control block, validity
+ * bytes of every tile and the per-workgroup counts */
 struct launch_res {
 	vg_ctrl *ctrl = nullptr;
-	uint64_t *status = nullptr;
-	size_t status_cap = 0;
-	uint32_t epoch = 0;
+	uint8_t *vbits = nullptr;
+	size_t vbits_cap = 0;      /* tiles */
+	uint32_t *wgcount = nullptr;
+	size_t wgcount_cap = 0;    /* workgroups */
 };
 
 struct vgd_device {
@@ -112,7 +115,8 @@ extern "C" int vgd_open(virginian *v)
 static void free_lr(launch_res &lr)
 {
 	if (lr.ctrl) (void)hipFree(lr.ctrl);
-	if (lr.status) (void)hipFree(lr.status);
+	if (lr.vbits) (void)hipFree(lr.vbits);
+	if (lr.wgcount) (void)hipFree(lr.wgcount);
 	lr = launch_res();
 }
 
@@ -194,26 +198,29 @@ static void *resident_tablet(vgd_device *d, const virg_tablet_meta *t, hipStream
 	return r.dptr;
 }
 
-static int ensure_lr(launch_res &lr, uint32_t num_tiles, hipStream_t stream)
+static int ensure_lr(launch_res &lr, uint32_t num_tiles, uint32_t grid, hipStream_t stream)
 {
 	if (!lr.ctrl) {
 		HCHK(hipMalloc((void **)&lr.ctrl, sizeof(vg_ctrl)), "control block");
 		HCHK(hipMemsetAsync(lr.ctrl, 0, sizeof(vg_ctrl), stream), "control block");
 	}
-	if (lr.status_cap < num_tiles) {
-		if (lr.status) {
-			HCHK(hipStreamSynchronize(stream), "status");
-			HCHK(hipFree(lr.status), "status");
+	if (lr.vbits_cap < num_tiles) {
+		if (lr.vbits) {
+			HCHK(hipStreamSynchronize(stream), "validity bytes");
+			HCHK(hipFree(lr.vbits), "validity bytes");
 		}
-		size_t cap = num_tiles < 1024 ? 1024 : num_tiles;
-		HCHK(hipMalloc((void **)&lr.status, cap * sizeof(uint64_t)), "tile status");
-		HCHK(hipMemsetAsync(lr.status, 0, cap * sizeof(uint64_t), stream), "tile status");
-		lr.status_cap = cap;
-		lr.epoch = 0;
+		size_t cap = num_tiles < 64 ? 64 : num_tiles;
+		HCHK(hipMalloc((void **)&lr.vbits, cap * VG_VBYTES_PER_TILE), "validity bytes");
+		lr.vbits_cap = cap;
 	}
-	if (++lr.epoch > VG_EPOCH_MAX) {
-		HCHK(hipMemsetAsync(lr.status, 0, lr.status_cap * sizeof(uint64_t), stream), "tile status");
-		lr.epoch = 1;
+	if (lr.wgcount_cap < grid) {
+		if (lr.wgcount) {
+			HCHK(hipStreamSynchronize(stream), "counts");
+			HCHK(hipFree(lr.wgcount), "counts");
+		}
+		size_t cap = grid < 4096 ? 4096 : grid;
+		HCHK(hipMalloc((void **)&lr.wgcount, cap * sizeof(uint32_t)), "counts");
+		lr.wgcount_cap = cap;
 	}
 	return VIRG_SUCCESS;
 }
@@ -293,17 +300,17 @@ static int launch(const vg_prog &P, const plan &p, launch_res &lr, void *const *
 {
 	if (p.num_tiles == 0)
 		return VIRG_SUCCESS;
-	if (ensure_lr(lr, p.num_tiles, stream) == VIRG_FAIL)
+	if (ensure_lr(lr, p.num_tiles, vg_kernel_grid(&P, rpl, p.num_tiles), stream) == VIRG_FAIL)
 		return VIRG_FAIL;
 	vg_launch L;
 	memset(&L, 0, sizeof L);
 	L.tabs = p.d_desc;
 	L.tile_tab = p.d_tile;
 	L.num_tiles = p.num_tiles;
-	L.epoch = lr.epoch;
 	L.capacity = capacity;
 	L.ctrl = lr.ctrl;
-	L.status = lr.status;
+	L.vbits = lr.vbits;
+	L.wgcount = lr.wgcount;
 	for (unsigned j = 0; j < P.nout; j++)
 		L.out[j] = out[j];
 	int rc = vg_kernel_launch(&P, &L, rpl, stream);
@@ -319,12 +326,11 @@ static int read_ctrl(const launch_res &lr, hipStream_t stream, uint64_t *rows)
 	vg_ctrl c;
 	HCHK(hipMemcpyAsync(&c, lr.ctrl, sizeof c, hipMemcpyDeviceToHost, stream), "control block");
 	HCHK(hipStreamSynchronize(stream), "kernel");
-	if (c.error != 0 || c.overflow != 0) {
+	if (c.overflow != 0) {
 		/* leave a clean control block for the next launch */
 		HCHK(hipMemsetAsync(lr.ctrl, 0, sizeof(vg_ctrl), stream), "control block");
 		HCHK(hipStreamSynchronize(stream), "control block");
 	}
-	VIRG_CHECK(c.error != 0, "a tile waited too long for its predecessors (look-back gave up)");
 	VIRG_CHECK(c.overflow != 0, "result exceeded the output capacity");
 	rows[0] = c.total;
 	return VIRG_SUCCESS;
@@ -582,7 +588,7 @@ extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned 
 	for (unsigned c = 0; c < q->P.ncol; c++)
 		rb += q->P.col[c].width;
 	q->read_bytes_per_row = rb;
-	if (ensure_lr(q->lr, q->p.num_tiles, q->stream) == VIRG_FAIL ||
+	if (ensure_lr(q->lr, q->p.num_tiles, vg_kernel_grid(&q->P, q->rpl, q->p.num_tiles), q->stream) == VIRG_FAIL ||
 		hipStreamSynchronize(q->stream) != hipSuccess) {
 		qerr(err, errlen, "cannot set up the launch");
 		vg_query_free(q);

@@ -70,7 +70,8 @@ typedef struct {
 	uint8_t phase;
 	uint8_t type;
 	uint8_t slot, slot_hi;
-	uint8_t pad[2];
+	uint8_t wload;     /* loaded again by the write pass */
+	uint8_t pad;
 } vg_col;              /* 8 bytes */
 
 /* result columns */
@@ -94,7 +95,8 @@ typedef struct {
 	uint32_t nout;
 	uint32_t nslot;     /* highest slot used + 1 */
 	uint32_t has_project_loads;
-	uint32_t pad;
+	uint32_t wfull;     /* the write pass reruns every micro-op (masked writes) */
+	uint64_t wneed;     /* micro-ops the write pass runs (bit k = micro-op k) */
 	vg_col   col[VG_MAX_COLS + 1];
 	vg_out   out[VG_MAX_OUT];
 	vg_uop   uop[VG_MAX_UOPS];
@@ -112,29 +114,24 @@ typedef struct {
 
 /* per-launch control block in device memory */
 typedef struct {
-	uint32_t tile_counter;   /* next tile to hand out */
-	uint32_t done;           /* workgroups finished */
-	uint64_t total;          /* rows selected (written by the last tile) */
+	uint64_t total;          /* rows selected (written by the last workgroup) */
 	uint32_t overflow;       /* output capacity was exceeded */
-	uint32_t error;          /* a bounded wait gave up */
-	uint32_t pad[2];
+	uint32_t pad[5];
 } vg_ctrl;
 
-/* kernel arguments besides the program */
+/* kernel arguments besides the program.  The count pass and the write
+ * pass run with the same grid: workgroup g owns tiles
+ * [g * tiles_per_wg, (g + 1) * tiles_per_wg). */
 typedef struct {
 	const vg_tabdesc *tabs;
 	const uint32_t *tile_tab;
 	uint32_t num_tiles;
-	uint32_t epoch;
+	uint32_t tiles_per_wg;
 	uint64_t capacity;
 	vg_ctrl *ctrl;
-	uint64_t *status;
+	uint8_t *vbits;          /* one validity byte per lane per tile */
+	uint32_t *wgcount;       /* rows selected per workgroup */
 	void *out[VG_MAX_OUT];
 } vg_launch;
-
-/* per-tile status word: [63:40] epoch, [39:38] flag, [37:0] value */
-#define VG_ST_AGG     1ull
-#define VG_ST_PREFIX  2ull
-#define VG_ST_VALMASK ((1ull << 38) - 1)
 
 #endif

@@ -3,32 +3,30 @@
  * reference src/vm/vm_gpu.cu:1152-1225 on the GPU, src/vm/virginia.c:228-891
  * on the CPU).
  *
- * One workgroup processes one tile of T = NT*R consecutive rows of one
- * tablet; lane l of wave w owns rows (w*64+l)*R .. +R-1, so every column
- * load is one contiguous R*width-byte vector per lane and a wave reads
- * 64*R*width contiguous bytes of HBM.  Tiles are handed out in order by an
- * atomic counter, which is what makes the ordered-output look-back below
- * safe: a tile only waits for tiles taken before it by running workgroups.
+ * A query is two launches over the same persistent grid.  Rows are cut
+ * into tiles of T = NT*R consecutive rows of one tablet; workgroup g owns
+ * a contiguous run of tiles.  Inside a tile, lane l of wave w owns rows
+ * (w*64+l)*R .. +R-1, so a column load is one contiguous R*width-byte
+ * vector per lane and a wave reads 64*R*width contiguous bytes of HBM.
+ *
+ *   count pass: load the columns the filter micro-ops read, run them,
+ *     store one validity byte per lane per tile and one row count per
+ *     workgroup;
+ *   write pass: a workgroup's output offset is the sum of the counts of
+ *     the workgroups before it.  Per tile, each wave gets its own offset
+ *     from the tile's validity bytes, loads the columns the result needs
+ *     only in lanes holding a selected row, runs the micro-ops computing
+ *     result registers, ranks its rows with ballot + mbcnt and writes its
+ *     contiguous slice of every result column in row order.
+ * Nothing waits on another workgroup, so neither pass has atomics, spins
+ * or cross-workgroup ordering assumptions; results come out in the
+ * reference's single-core row order.
  *
  * The VM registers live in LDS as [slot][row of the tile]; a lane's R rows
  * of a slot are contiguous, so reading an operand for all of them is one
  * ds_read_b128 (R = 4) at a wave-uniform slot base.  Micro-op fields are
  * read with scalar loads from the kernel arguments, so dispatch is a
  * scalar branch.
- *
- * Per tile:
- *   1. filter loads: every column read by a validity-changing micro-op is
- *      loaded for all rows (the reference copies whole blocks the same way);
- *   2. filter micro-ops [0, nfilter) with a per-row resume index and
- *      validity flag;
- *   3. projection: only in waves that still hold a valid row, load the
- *      remaining columns for lanes holding one and run [nfilter, nuop);
- *   4. ordered compaction: ballot + mbcnt rank each valid row inside its
- *      wave, an LDS scan over waves inside the tile, and a decoupled
- *      look-back over per-tile status words gives the tile's global offset;
- *      each valid row writes its row number to srcrow[rank];
- *   5. for every result column, consecutive lanes gather srcrow[i] and
- *      store output element base+i: coalesced, in row order.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -36,10 +34,6 @@
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-
-/* bounded waits: a tile gives up on its predecessors after this many polls
- * (about a second) and reports an error instead of hanging the device */
-#define VG_SPIN_LIMIT (1u << 23)
 
 __device__ __forceinline__ uint32_t uni(uint32_t x)
 {
@@ -306,10 +300,12 @@ __device__ __forceinline__ void cast_from(const vg_uop &u, uint32_t k, uint32_t 
 }
 
 template <int R, uint32_t TR>
-__device__ __forceinline__ void run_uops(const vg_prog &P, uint32_t kbeg, uint32_t kend,
+__device__ __forceinline__ void run_uops(const vg_prog &P, uint32_t kbeg, uint32_t kend, uint64_t mask,
 	uint32_t *regs, uint32_t lrow, uint32_t (&resume)[R], bool (&valid)[R])
 {
 	for (uint32_t k = kbeg; k < kend; ++k) {
+		if (!((mask >> k) & 1ull))
+			continue;
 		const vg_uop &u = P.uop[k];
 		if (u.kind == VG_K_INVALID) {
 #pragma unroll
@@ -402,246 +398,234 @@ __device__ __forceinline__ void load_col(const vg_col &c, const char *base, uint
 }
 
 template <int R, uint32_t TR>
-__device__ __forceinline__ void load_phase(const vg_prog &P, const vg_tabdesc *D, uint32_t row0,
-	uint32_t lrow, uint32_t nrows, uint32_t phase, uint32_t *regs)
+__device__ __forceinline__ void load_cols(const vg_prog &P, const vg_tabdesc *D, uint32_t row0,
+	uint32_t lrow, uint32_t nrows, bool write_pass, uint32_t *regs)
 {
 	const bool aligned = D->aligned != 0;
 	for (uint32_t k = 0; k < P.nload; ++k) {
 		const vg_col &c = P.col[k];
-		if (c.phase != phase)
+		if (write_pass ? !c.wload : c.phase != VG_PHASE_FILTER)
 			continue;
 		const char *base = (const char *)D->col[c.src] + (size_t)row0 * c.width;
 		load_col<R, TR>(c, base, lrow, nrows, aligned, regs);
 	}
 }
 
-/* ----------------------------------------------------------- look-back */
-__device__ __forceinline__ uint64_t st_make(uint32_t epoch, uint64_t flag, uint64_t v)
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
 {
-	return ((uint64_t)epoch << 40) | (flag << 38) | (v & VG_ST_VALMASK);
-}
-
-/* Exclusive prefix of `tile`: publish its aggregate, then walk back over
- * predecessors 64 at a time (one per lane) until one with a published
- * inclusive prefix is found.  Called by one full wave. */
-__device__ uint64_t lookback(const vg_launch &L, uint32_t tile, uint32_t agg, uint32_t lane)
-{
-	uint64_t *st = L.status;
-	if (tile == 0) {
-		if (lane == 0)
-			__hip_atomic_store(&st[0], st_make(L.epoch, VG_ST_PREFIX, agg),
-				__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		return 0;
-	}
-	if (lane == 0)
-		__hip_atomic_store(&st[tile], st_make(L.epoch, VG_ST_AGG, agg),
-			__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	uint64_t excl = 0;
-	int64_t j = (int64_t)tile - 1;
-	uint32_t spins = 0;
-	while (true) {
-		const int64_t idx = j - (int64_t)lane;
-		uint64_t flag = VG_ST_PREFIX, val = 0;
-		if (idx >= 0) {
-			const uint64_t w = __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			flag = (uint32_t)(w >> 40) == L.epoch ? (w >> 38) & 3ull : 0ull;
-			val = w & VG_ST_VALMASK;
-		}
-		const uint64_t pmask = __ballot(flag == VG_ST_PREFIX);
-		const uint64_t xmask = __ballot(flag == 0);
-		const uint32_t first_p = pmask ? (uint32_t)__builtin_ctzll(pmask) : 64u;
-		const uint64_t upto = first_p >= 63 ? ~0ull : ((2ull << first_p) - 1);
-		if (xmask & upto) {
-			if (++spins > VG_SPIN_LIMIT) {
-				if (lane == 0)
-					atomicOr(&L.ctrl->error, 1u);
-				break;
-			}
-			__builtin_amdgcn_s_sleep(2);
-			continue;
-		}
-		uint64_t part = lane <= first_p ? val : 0;
 #pragma unroll
-		for (int off = 32; off > 0; off >>= 1)
-			part += __shfl_xor(part, off, 64);
-		excl += part;
-		if (first_p < 64)
-			break;
-		j -= 64;
-	}
-	if (lane == 0)
-		__hip_atomic_store(&st[tile], st_make(L.epoch, VG_ST_PREFIX, excl + agg),
-			__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	return excl;
+	for (int off = 32; off > 0; off >>= 1)
+		x += __shfl_xor(x, off, 64);
+	return x;
 }
 
-/* --------------------------------------------------------------- kernel */
-/* dynamic LDS: nslot * T words of registers, then T 16-bit row numbers */
+/* ----------------------------------------------------------- count pass */
+/* Workgroup g evaluates the filter micro-ops over its tiles, stores one
+ * validity byte per lane per tile (bit r = row lrow+r) and the number of
+ * selected rows of all its tiles.  LDS: nslot * T words of registers. */
 template <int R, int NT>
-__global__ __launch_bounds__(NT) void vg_select_kernel(const vg_prog P, const vg_launch L)
+__global__ __launch_bounds__(NT) void vg_count_kernel(const vg_prog P, const vg_launch L)
 {
 	constexpr uint32_t T = NT * R;
 	constexpr int NW = NT / 64;
 	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
-	__shared__ uint32_t s_tile;
 	__shared__ uint32_t s_wcount[NW];
-	__shared__ uint64_t s_base;
 	uint32_t *regs = lds;
-	uint16_t *srcrow = (uint16_t *)(lds + P.nslot * T);
-
 	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	if (tid == 0)
-		s_tile = atomicAdd(&L.ctrl->tile_counter, 1u);
-	__syncthreads();
-	const uint32_t tile = uni(s_tile);
-	if (tile >= L.num_tiles) {
-		/* the counter was not reset by an earlier launch: report, touch nothing */
-		if (tid == 0)
-			atomicOr(&L.ctrl->error, 2u);
-		return;
-	}
-	const uint32_t tb = uni(L.tile_tab[tile]);
-	const vg_tabdesc *D = &L.tabs[tb];
-	const uint32_t row0 = (tile - D->first_tile) * T;
-	const uint32_t left = D->rows - row0;
-	const uint32_t nrows = left < T ? left : T;
 	const uint32_t lrow = (wave * 64 + lane) * R;
-
-	uint32_t resume[R];
-	bool valid[R];
+	const uint32_t g = blockIdx.x;
+	const uint32_t t0 = g * L.tiles_per_wg;
+	const uint32_t t1 = min(t0 + L.tiles_per_wg, L.num_tiles);
+	uint32_t count = 0;
+	for (uint32_t tile = t0; tile < t1; ++tile) {
+		const uint32_t tb = uni(L.tile_tab[tile]);
+		const vg_tabdesc *D = &L.tabs[tb];
+		const uint32_t row0 = (tile - D->first_tile) * T;
+		const uint32_t left = D->rows - row0;
+		const uint32_t nrows = left < T ? left : T;
+		uint32_t resume[R];
+		bool valid[R];
 #pragma unroll
-	for (int r = 0; r < R; ++r) {
-		resume[r] = 0;
-		valid[r] = lrow + r < nrows;
-	}
-
-	/* every LDS row a lane touches is its own, so no barrier is needed
-	 * between the register-file phases */
-	load_phase<R, T>(P, D, row0, lrow, nrows, VG_PHASE_FILTER, regs);
-	run_uops<R, T>(P, 0, P.nfilter, regs, lrow, resume, valid);
-
-	if (P.nuop > P.nfilter || P.has_project_loads) {
-		bool lane_any = false;
+		for (int r = 0; r < R; ++r) {
+			resume[r] = 0;
+			valid[r] = lrow + r < nrows;
+		}
+		/* every LDS row a lane touches is its own: no barriers needed */
+		load_cols<R, T>(P, D, row0, lrow, nrows, false, regs);
+		run_uops<R, T>(P, 0, P.nfilter, ~0ull, regs, lrow, resume, valid);
+		uint32_t bits = 0;
 #pragma unroll
 		for (int r = 0; r < R; ++r)
-			lane_any = lane_any || valid[r];
-		if (__ballot(lane_any) != 0) {
-			if (lane_any)
-				load_phase<R, T>(P, D, row0, lrow, nrows, VG_PHASE_PROJECT, regs);
-			run_uops<R, T>(P, P.nfilter, P.nuop, regs, lrow, resume, valid);
-		}
+			bits |= valid[r] ? (1u << r) : 0u;
+		L.vbits[(size_t)tile * NT + tid] = (uint8_t)bits;
+		count += (uint32_t)__builtin_popcount(bits);
 	}
-
-	/* ranks inside the wave and the tile */
-	uint32_t lane_pref = 0, wave_tot = 0;
-#pragma unroll
-	for (int r = 0; r < R; ++r) {
-		const uint64_t b = __ballot(valid[r]);
-		lane_pref += rank_below(b);
-		wave_tot += (uint32_t)__builtin_popcountll(b);
-	}
+	count = wave_sum(count);
 	if (lane == 0)
-		s_wcount[wave] = wave_tot;
-	__syncthreads();
-	uint32_t wave_off = 0, tile_tot = 0;
-#pragma unroll
-	for (int w = 0; w < NW; ++w) {
-		const uint32_t c = s_wcount[w];
-		wave_off += (uint32_t)w < wave ? c : 0u;
-		tile_tot += c;
-	}
-	{
-		uint32_t pos = wave_off + lane_pref;
-#pragma unroll
-		for (int r = 0; r < R; ++r)
-			if (valid[r])
-				srcrow[pos++] = (uint16_t)(lrow + r);
-	}
-	if (wave == 0) {
-		const uint64_t base = lookback(L, tile, tile_tot, lane);
-		if (lane == 0) {
-			s_base = base;
-			if (tile == L.num_tiles - 1)
-				L.ctrl->total = base + tile_tot;
-			if (base + tile_tot > L.capacity)
-				atomicOr(&L.ctrl->overflow, 1u);
-		}
-	}
-	__syncthreads();
-	const uint64_t base = s_base;
-	const uint64_t room = L.capacity > base ? L.capacity - base : 0;
-	const uint32_t nwrite = (uint64_t)tile_tot < room ? tile_tot : (uint32_t)room;
-
-	/* gather every result column in row order */
-	for (uint32_t j = 0; j < P.nout; ++j) {
-		const vg_out &o = P.out[j];
-		char *dst = (char *)L.out[j];
-		if (o.kind == VG_OUT_SLOT) {
-			const uint32_t *lo = regs + o.slot * T, *hi = regs + o.slot_hi * T;
-			if (o.width == 8) {
-				for (uint32_t i = tid; i < nwrite; i += NT) {
-					const uint32_t s = srcrow[i];
-					((uint64_t *)dst)[base + i] = ((uint64_t)hi[s] << 32) | lo[s];
-				}
-			} else if (o.width == 4) {
-				for (uint32_t i = tid; i < nwrite; i += NT)
-					((uint32_t *)dst)[base + i] = lo[srcrow[i]];
-			} else {
-				for (uint32_t i = tid; i < nwrite; i += NT)
-					((uint8_t *)dst)[base + i] = (uint8_t)lo[srcrow[i]];
-			}
-		} else if (o.kind == VG_OUT_COLUMN) {
-			const vg_col &c = P.col[o.col];
-			const char *src = (const char *)D->col[c.src] + (size_t)row0 * c.width;
-			if (o.width == 8) {
-				for (uint32_t i = tid; i < nwrite; i += NT)
-					((uint64_t *)dst)[base + i] = ((const uint64_t *)src)[srcrow[i]];
-			} else if (o.width == 4) {
-				for (uint32_t i = tid; i < nwrite; i += NT)
-					((uint32_t *)dst)[base + i] = ((const uint32_t *)src)[srcrow[i]];
-			} else {
-				for (uint32_t i = tid; i < nwrite; i += NT)
-					((uint8_t *)dst)[base + i] = ((const uint8_t *)src)[srcrow[i]];
-			}
-		} else {
-			if (o.width == 8) {
-				for (uint32_t i = tid; i < nwrite; i += NT)
-					((uint64_t *)dst)[base + i] = o.imm;
-			} else if (o.width == 4) {
-				for (uint32_t i = tid; i < nwrite; i += NT)
-					((uint32_t *)dst)[base + i] = (uint32_t)o.imm;
-			} else {
-				for (uint32_t i = tid; i < nwrite; i += NT)
-					((uint8_t *)dst)[base + i] = (uint8_t)o.imm;
-			}
-		}
-	}
-
+		s_wcount[wave] = count;
 	__syncthreads();
 	if (tid == 0) {
-		__threadfence();
-		const uint32_t d = atomicAdd(&L.ctrl->done, 1u);
-		if (d == L.num_tiles - 1) {
-			L.ctrl->tile_counter = 0;
-			L.ctrl->done = 0;
-		}
+		uint32_t s = 0;
+#pragma unroll
+		for (int w = 0; w < NW; ++w)
+			s += s_wcount[w];
+		L.wgcount[g] = s;
 	}
+}
+
+/* ----------------------------------------------------------- write pass */
+/* Workgroup g starts at the sum of the counts of workgroups < g.  For each
+ * of its tiles the four waves compute their offsets from the tile's
+ * validity bytes; a wave with selected rows loads what the result needs
+ * (only in lanes holding a selected row), runs the micro-ops that compute
+ * result registers, ranks its rows with ballot + mbcnt, and writes its
+ * contiguous slice of every result column.  Waves never wait for each
+ * other.  LDS: nslot * T words of registers, then R*64 row numbers per
+ * wave. */
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void vg_write_kernel(const vg_prog P, const vg_launch L)
+{
+	constexpr uint32_t T = NT * R;
+	constexpr int NW = NT / 64;
+	static_assert(NT / 4 == 64, "one 4-byte word of validity bytes per lane covers the tile");
+	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+	uint32_t *regs = lds;
+	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	uint16_t *srcrow = (uint16_t *)(lds + P.nslot * T) + wave * (64 * R);
+	const uint32_t lrow = (wave * 64 + lane) * R;
+	const uint32_t g = blockIdx.x;
+	const uint32_t t0 = g * L.tiles_per_wg;
+	const uint32_t t1 = min(t0 + L.tiles_per_wg, L.num_tiles);
+
+	/* offset of this workgroup: sum of the counts before it */
+	uint64_t base = 0;
+	for (uint32_t i = lane; i < g; i += 64)
+		base += L.wgcount[i];
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1)
+		base += __shfl_xor(base, off, 64);
+
+	for (uint32_t tile = t0; tile < t1; ++tile) {
+		/* validity bytes of the whole tile: lane l holds threads 4l..4l+3 */
+		const uint32_t word = ((const uint32_t *)(L.vbits + (size_t)tile * NT))[lane];
+		const uint32_t c = (uint32_t)__builtin_popcount(word);
+		const uint32_t tile_tot = wave_sum(c);
+		if (tile_tot == 0)
+			continue;
+		const uint32_t wave_off = wave_sum(lane < 16 * wave ? c : 0u);
+		const uint32_t wave_tot = wave_sum(lane >= 16 * wave && lane < 16 * wave + 16 ? c : 0u);
+		if (wave_tot != 0) {
+			const uint32_t mybits = (__shfl(word, (int)(16 * wave + (lane >> 2)), 64) >> ((lane & 3) * 8)) & 0xFFu;
+			const uint32_t tb = uni(L.tile_tab[tile]);
+			const vg_tabdesc *D = &L.tabs[tb];
+			const uint32_t row0 = (tile - D->first_tile) * T;
+			const uint32_t left = D->rows - row0;
+			const uint32_t nrows = left < T ? left : T;
+			uint32_t resume[R];
+			bool valid[R];
+#pragma unroll
+			for (int r = 0; r < R; ++r) {
+				resume[r] = 0;
+				valid[r] = (mybits >> r) & 1u;
+			}
+			if (mybits)
+				load_cols<R, T>(P, D, row0, lrow, nrows, true, regs);
+			if (P.wfull) {
+				/* rerun everything: resume points and validity come back
+				 * exactly as in the count pass */
+				run_uops<R, T>(P, 0, P.nuop, ~0ull, regs, lrow, resume, valid);
+			} else if (P.wneed) {
+				run_uops<R, T>(P, 0, P.nuop, P.wneed, regs, lrow, resume, valid);
+			}
+			/* ranks inside the wave, row numbers in rank order */
+			uint32_t pos = 0;
+#pragma unroll
+			for (int r = 0; r < R; ++r) {
+				const uint64_t b = __ballot((mybits >> r) & 1u);
+				pos += rank_below(b);
+			}
+#pragma unroll
+			for (int r = 0; r < R; ++r)
+				if ((mybits >> r) & 1u)
+					srcrow[pos++] = (uint16_t)(lrow + r);
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+			const uint64_t at = base + wave_off;
+			const uint64_t room = L.capacity > at ? L.capacity - at : 0;
+			const uint32_t n = (uint64_t)wave_tot < room ? wave_tot : (uint32_t)room;
+			if (n < wave_tot && lane == 0)
+				atomicOr(&L.ctrl->overflow, 1u);
+			for (uint32_t j = 0; j < P.nout; ++j) {
+				const vg_out &o = P.out[j];
+				char *dst = (char *)L.out[j];
+				if (o.kind == VG_OUT_SLOT) {
+					const uint32_t *lo = regs + o.slot * T, *hi = regs + o.slot_hi * T;
+					if (o.width == 8) {
+						for (uint32_t i = lane; i < n; i += 64) {
+							const uint32_t s = srcrow[i];
+							((uint64_t *)dst)[at + i] = ((uint64_t)hi[s] << 32) | lo[s];
+						}
+					} else if (o.width == 4) {
+						for (uint32_t i = lane; i < n; i += 64)
+							((uint32_t *)dst)[at + i] = lo[srcrow[i]];
+					} else {
+						for (uint32_t i = lane; i < n; i += 64)
+							((uint8_t *)dst)[at + i] = (uint8_t)lo[srcrow[i]];
+					}
+				} else if (o.kind == VG_OUT_COLUMN) {
+					const vg_col &cc = P.col[o.col];
+					const char *src = (const char *)D->col[cc.src] + (size_t)row0 * cc.width;
+					if (o.width == 8) {
+						for (uint32_t i = lane; i < n; i += 64)
+							((uint64_t *)dst)[at + i] = ((const uint64_t *)src)[srcrow[i]];
+					} else if (o.width == 4) {
+						for (uint32_t i = lane; i < n; i += 64)
+							((uint32_t *)dst)[at + i] = ((const uint32_t *)src)[srcrow[i]];
+					} else {
+						for (uint32_t i = lane; i < n; i += 64)
+							((uint8_t *)dst)[at + i] = ((const uint8_t *)src)[srcrow[i]];
+					}
+				} else {
+					if (o.width == 8) {
+						for (uint32_t i = lane; i < n; i += 64)
+							((uint64_t *)dst)[at + i] = o.imm;
+					} else if (o.width == 4) {
+						for (uint32_t i = lane; i < n; i += 64)
+							((uint32_t *)dst)[at + i] = (uint32_t)o.imm;
+					} else {
+						for (uint32_t i = lane; i < n; i += 64)
+							((uint8_t *)dst)[at + i] = (uint8_t)o.imm;
+					}
+				}
+			}
+			/* srcrow and the register rows are rewritten by the next tile */
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		}
+		base += tile_tot;
+	}
+	if (g == gridDim.x - 1 && tid == 0)
+		L.ctrl->total = base;
 }
 
 /* --------------------------------------------------------------- launch */
 template <int R>
-static int launch_r(const vg_prog *P, const vg_launch *L, hipStream_t stream)
+static size_t lds_bytes(const vg_prog *P)
 {
-	constexpr uint32_t T = 256u * R;
-	const size_t lds = (size_t)P->nslot * T * 4 + T * 2;
-	hipLaunchKernelGGL((vg_select_kernel<R, 256>), dim3(L->num_tiles), dim3(256), lds, stream, *P, *L);
-	return hipGetLastError() == hipSuccess ? 0 : -2;
+	constexpr size_t T = 256u * R;
+	return (size_t)P->nslot * T * 4 + 4 * (64 * R) * 2;
 }
 
 extern "C" size_t vg_kernel_lds_bytes(const vg_prog *P, int rows_per_lane)
 {
-	const size_t T = 256u * (size_t)rows_per_lane;
-	return (size_t)P->nslot * T * 4 + T * 2;
+	switch (rows_per_lane) {
+	case 1: return lds_bytes<1>(P);
+	case 2: return lds_bytes<2>(P);
+	default: return lds_bytes<4>(P);
+	}
 }
 
 #define VG_LDS_LIMIT (64u * 1024u)
@@ -656,9 +640,63 @@ extern "C" int vg_kernel_pick_rpl(const vg_prog *P, int requested)
 	return vg_kernel_lds_bytes(P, r) > VG_LDS_LIMIT ? 0 : r;
 }
 
-extern "C" int vg_kernel_launch(const vg_prog *P, const vg_launch *L, int rows_per_lane,
+/* Workgroups per pass: enough to fill every CU at the occupancy the LDS
+ * footprint allows, never more than there are tiles. */
+template <int R>
+static uint32_t grid_for(const vg_prog *P, uint32_t num_tiles)
+{
+	static int cus = 0;
+	if (cus == 0) {
+		int dev = 0;
+		hipDeviceProp_t prop;
+		if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+			cus = 256;
+		else
+			cus = prop.multiProcessorCount;
+	}
+	int per_cu = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, vg_write_kernel<R, 256>, 256,
+			lds_bytes<R>(P)) != hipSuccess || per_cu < 1)
+		per_cu = 1;
+	uint32_t g = (uint32_t)cus * (uint32_t)per_cu;
+	return g < num_tiles ? g : num_tiles;
+}
+
+template <int R>
+static int launch_r(const vg_prog *P, vg_launch *L, hipStream_t stream)
+{
+	const uint32_t g = grid_for<R>(P, L->num_tiles);
+	L->tiles_per_wg = (L->num_tiles + g - 1) / g;
+	const uint32_t grid = (L->num_tiles + L->tiles_per_wg - 1) / L->tiles_per_wg;
+	const size_t lds = lds_bytes<R>(P);
+	hipLaunchKernelGGL((vg_count_kernel<R, 256>), dim3(grid), dim3(256), lds, stream, *P, *L);
+	if (hipGetLastError() != hipSuccess)
+		return -2;
+	hipLaunchKernelGGL((vg_write_kernel<R, 256>), dim3(grid), dim3(256), lds, stream, *P, *L);
+	return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+/* Number of workgroups a launch of this program over num_tiles tiles uses
+ * (the size of the per-workgroup count array). */
+extern "C" uint32_t vg_kernel_grid(const vg_prog *P, int rows_per_lane, uint32_t num_tiles)
+{
+	if (num_tiles == 0)
+		return 0;
+	uint32_t g;
+	switch (rows_per_lane) {
+	case 1: g = grid_for<1>(P, num_tiles); break;
+	case 2: g = grid_for<2>(P, num_tiles); break;
+	default: g = grid_for<4>(P, num_tiles); break;
+	}
+	const uint32_t per = (num_tiles + g - 1) / g;
+	return (num_tiles + per - 1) / per;
+}
+
+extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, int rows_per_lane,
 	hipStream_t stream)
 {
+	if (L->num_tiles == 0)
+		return 0;
 	if (vg_kernel_lds_bytes(P, rows_per_lane) > VG_LDS_LIMIT)
 		return -3;
 	switch (rows_per_lane) {
