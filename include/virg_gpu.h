@@ -1,0 +1,76 @@
+/*
+ * Entry points of the gfx950 executor that sit beside the drop-in boundary.
+ *
+ * The boundary itself is declared in virginian.h exactly as the reference
+ * declares it:
+ *   int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
+ *                   virg_tablet_meta **res, unsigned num_tablets);
+ *       replaces src/vm/vm_gpu.cu:107 (declared at src/virginian.h:620-621)
+ *   const size_t *virg_gpu_getsizes(void);
+ *       replaces src/vm/vm_gpu.cu:27-30 (src/virginian.h:622)
+ *
+ * The functions below have no counterpart in the reference.  They keep a
+ * query's table resident in HBM and run its data-parallel section any
+ * number of times, which is how the reference's "data already on the GPU"
+ * timings (src/vm/vm_gpu.cu:152-199 with the tablet cached) are reproduced
+ * without a host round trip per tablet.  All are plain C: pointers, sizes
+ * and the reference's own structs; VIRG_SUCCESS (1) / VIRG_FAIL (0).
+ * A vg_query is used by one host thread at a time.
+ */
+#ifndef VIRG_GPU_H
+#define VIRG_GPU_H
+
+#include <stddef.h>
+#include "virginian.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vg_query vg_query;
+
+/* Prepare a generated program (Table, ResultColumn..., Parallel, ...,
+ * Result, Converge, Finish: what virg_sql or virg_vm_addop produce) for
+ * repeated execution over the table its Table op names.  The table's
+ * tablets are copied to HBM once.  Output capacity is `capacity` rows
+ * (0 = the table's row count).  `reserved` must be 0.  Returns NULL with a
+ * message in err when the program cannot run (no device, unsupported op,
+ * operand types that differ, too many registers). */
+vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned num_ops,
+	unsigned long long capacity, int reserved, char *err, size_t errlen);
+void vg_query_free(vg_query *q);
+
+/* Launch the section once on the query's stream (asynchronous). */
+int vg_query_launch(vg_query *q);
+/* Wait for the last launch; *rows = selected rows.  Fails when the result
+ * exceeded the capacity. */
+int vg_query_sync(vg_query *q, unsigned long long *rows);
+/* launch + sync */
+int vg_query_run(vg_query *q, unsigned long long *rows);
+/* Copy the first min(last rows, max_rows) result rows into host arrays,
+ * one per result column, in the reference's single-core row order. */
+int vg_query_fetch(vg_query *q, void **cols, unsigned long long max_rows);
+/* Mean milliseconds per launch over `iters` back-to-back launches after
+ * `warmup` untimed ones, measured with HIP events on the query's stream. */
+int vg_query_time(vg_query *q, int warmup, int iters, double *ms_per_launch,
+	unsigned long long *rows);
+/* Rows scanned, tiles, referenced-column bytes per row, result bytes per
+ * selected row, result columns. */
+int vg_query_info(vg_query *q, unsigned long long *rows, unsigned *tiles,
+	unsigned *read_bytes_per_row, unsigned *write_bytes_per_row, unsigned *nout);
+
+/* Architecture name and compute-unit count of the device in use
+ * (VIRG_FAIL and *cus = 0 without one). */
+int vg_device_info(virginian *v, char *name, size_t n, int *cus);
+
+/* Decode the data-parallel section starting at op `pc` against tablet
+ * `tab`'s schema into the kernel's micro-program (no device needed).
+ * `out` receives a vg_prog (see virginian_amd/csrc/gpu/vg_prog.h). */
+int vg_decode_program(const virg_op *ops, unsigned num_ops, unsigned pc,
+	const virg_tablet_meta *tab, void *out, size_t out_bytes, char *err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -385,7 +385,9 @@ int vg_decode(const virg_op *stmt, unsigned num_ops, unsigned pc,
 			col->type = (uint8_t)ty;
 			col->width = (uint8_t)tsize(ty);
 			col->phase = (uint8_t)phase;
-			if (phase != VG_PHASE_EMIT) {
+			if (phase == VG_PHASE_FILTER) {
+				/* filter columns take the lowest slots so the count pass,
+				 * which only touches filter slots, needs the least LDS */
 				int need = col->width == 8 ? 2 : 1;
 				if (next_slot + need > VG_NSLOT)
 					return derr(err, errlen, "computation reads too many columns");
@@ -393,8 +395,10 @@ int vg_decode(const virg_op *stmt, unsigned num_ops, unsigned pc,
 				col->slot_hi = (uint8_t)(next_slot + need - 1);
 				next_slot += need;
 				P->nload = (uint32_t)(nc + 1);
-				if (phase == VG_PHASE_PROJECT)
-					P->has_project_loads = 1;
+			} else if (phase == VG_PHASE_PROJECT) {
+				col->slot = col->slot_hi = 0xFF;   /* placed after the temporaries */
+				P->nload = (uint32_t)(nc + 1);
+				P->has_project_loads = 1;
 			}
 			col_index[c] = nc++;
 		}
@@ -446,8 +450,34 @@ int vg_decode(const virg_op *stmt, unsigned num_ops, unsigned pc,
 			slot_free_at[got[j]] = R[r].last_use_uop;
 	}
 
+	/* slots the count pass touches: filter columns and the values the
+	 * filter micro-ops write */
+	int nslot_count = next_slot;
+	for (int k = 0; k < nfilter; k++) {
+		int r = U[k].dst_reg;
+		if (r >= 0 && R[r].slot_hi + 1 > nslot_count)
+			nslot_count = R[r].slot_hi + 1;
+	}
+	/* projection columns go above every temporary */
+	int top = next_slot;
+	for (int r = 0; r < VIRG_REGS; r++)
+		if (R[r].cls == RC_SLOT && R[r].slot_hi + 1 > top)
+			top = R[r].slot_hi + 1;
+	for (int c = 0; c < nc; c++) {
+		vg_col *col = &P->col[c];
+		if (col->phase != VG_PHASE_PROJECT)
+			continue;
+		int need = col->width == 8 ? 2 : 1;
+		if (top + need > VG_NSLOT)
+			return derr(err, errlen, "program needs more than %d register slots", VG_NSLOT);
+		col->slot = (uint8_t)top;
+		col->slot_hi = (uint8_t)(top + need - 1);
+		top += need;
+	}
+	P->nslot_count = (uint32_t)nslot_count;
+
 	/* final micro-ops */
-	int maxslot = next_slot;
+	int maxslot = top;
 	for (int k = 0; k < nu; k++) {
 		vg_uop *o = &P->uop[k];
 		puop *u = &U[k];

@@ -3,8 +3,9 @@
  * cache, the virg_vm_gpu boundary and the vg_query C ABI.
  *
  * Differences from the reference host driver (src/vm/vm_gpu.cu:107-578):
- *  - one kernel launch covers every tablet of the query instead of one
- *    launch per tablet with a full 8 MB result copy after each;
+ *  - one pair of launches (count pass, write pass) covers every tablet of
+ *    the query instead of one launch per tablet with a full 8 MB result
+ *    copy after each;
  *  - tablets are copied to HBM once and stay resident (keyed by tablet id,
  *    re-copied when their row count or layout changes), so repeated
  *    queries read them from HBM; the reference's serial / stream / mapped
@@ -21,21 +22,20 @@
 #include <stdarg.h>
 #include <unordered_map>
 #include <vector>
-#include <string>
 
 extern "C" {
 #include "../host/internal.h"
 #include "vg_prog.h"
 }
 
-extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, int rows_per_lane, hipStream_t stream);
-extern "C" uint32_t vg_kernel_tile_rows(int rows_per_lane);
-extern "C" uint32_t vg_kernel_grid(const vg_prog *P, int rows_per_lane, uint32_t num_tiles);
-extern "C" int vg_kernel_pick_rpl(const vg_prog *P, int requested);
+extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stream);
+extern "C" uint32_t vg_kernel_tile_rows(void);
+extern "C" uint32_t vg_kernel_grid(const vg_prog *P, uint32_t num_tiles);
+extern "C" int vg_kernel_check(const vg_prog *P);
 extern "C" int vg_decode(const virg_op *stmt, unsigned num_ops, unsigned pc,
 	const virg_tablet_meta *tab, const virg_tablet_meta *res, vg_prog *P, char *err, size_t errlen);
 
-/* validity bytes per tile: one per lane of the 256-thread workgroup */
+/* validity bytes per tile: one per lane of the 256-thread write workgroup */
 #define VG_VBYTES_PER_TILE 256u
 
 struct resident {
@@ -47,13 +47,14 @@ struct resident {
 };
 
 /* launch resources owned by one query stream: control block, validity
- * bytes of every tile and the per-workgroup counts */
+ * bytes and per-wave counts of every tile, per-workgroup counts */
 struct launch_res {
 	vg_ctrl *ctrl = nullptr;
 	uint8_t *vbits = nullptr;
-	size_t vbits_cap = 0;      /* tiles */
+	uint32_t *tcount = nullptr;
+	size_t tiles_cap = 0;
 	uint32_t *wgcount = nullptr;
-	size_t wgcount_cap = 0;    /* workgroups */
+	size_t wgcount_cap = 0;
 };
 
 struct vgd_device {
@@ -63,7 +64,6 @@ struct vgd_device {
 	launch_res lr;
 	void *outbuf = nullptr;
 	size_t outbuf_bytes = 0;
-	int rows_per_lane = 4;
 	hipDeviceProp_t prop;
 };
 
@@ -102,12 +102,6 @@ extern "C" int vgd_open(virginian *v)
 		s->dev = nullptr;
 		return 0;
 	}
-	const char *rpl = getenv("VIRG_ROWS_PER_LANE");
-	if (rpl) {
-		int x = atoi(rpl);
-		if (x == 1 || x == 2 || x == 4)
-			d->rows_per_lane = x;
-	}
 	s->dev = d;
 	return 1;
 }
@@ -116,6 +110,7 @@ static void free_lr(launch_res &lr)
 {
 	if (lr.ctrl) (void)hipFree(lr.ctrl);
 	if (lr.vbits) (void)hipFree(lr.vbits);
+	if (lr.tcount) (void)hipFree(lr.tcount);
 	if (lr.wgcount) (void)hipFree(lr.wgcount);
 	lr = launch_res();
 }
@@ -145,7 +140,7 @@ extern "C" void vgd_forget_tablet(virginian *v, unsigned id)
 	auto it = d->tablets.find(id);
 	if (it == d->tablets.end())
 		return;
-	(void)hipStreamSynchronize(d->stream);
+	(void)hipDeviceSynchronize();
 	(void)hipFree(it->second.dptr);
 	d->tablets.erase(it);
 }
@@ -174,7 +169,7 @@ static void *resident_tablet(vgd_device *d, const virg_tablet_meta *t, hipStream
 		if (r.rows == t->rows && r.size == t->size && r.fixed_block == t->fixed_block &&
 			r.possible_rows == t->possible_rows)
 			return r.dptr;
-		(void)hipStreamSynchronize(stream);
+		(void)hipDeviceSynchronize();
 		(void)hipFree(r.dptr);
 		d->tablets.erase(it);
 	}
@@ -204,14 +199,18 @@ static int ensure_lr(launch_res &lr, uint32_t num_tiles, uint32_t grid, hipStrea
 		HCHK(hipMalloc((void **)&lr.ctrl, sizeof(vg_ctrl)), "control block");
 		HCHK(hipMemsetAsync(lr.ctrl, 0, sizeof(vg_ctrl), stream), "control block");
 	}
-	if (lr.vbits_cap < num_tiles) {
+	if (lr.tiles_cap < num_tiles) {
 		if (lr.vbits) {
 			HCHK(hipStreamSynchronize(stream), "validity bytes");
 			HCHK(hipFree(lr.vbits), "validity bytes");
+			HCHK(hipFree(lr.tcount), "tile counts");
+			lr.vbits = nullptr;
+			lr.tcount = nullptr;
 		}
 		size_t cap = num_tiles < 64 ? 64 : num_tiles;
 		HCHK(hipMalloc((void **)&lr.vbits, cap * VG_VBYTES_PER_TILE), "validity bytes");
-		lr.vbits_cap = cap;
+		HCHK(hipMalloc((void **)&lr.tcount, cap * 4 * sizeof(uint32_t)), "tile counts");
+		lr.tiles_cap = cap;
 	}
 	if (lr.wgcount_cap < grid) {
 		if (lr.wgcount) {
@@ -225,7 +224,7 @@ static int ensure_lr(launch_res &lr, uint32_t num_tiles, uint32_t grid, hipStrea
 	return VIRG_SUCCESS;
 }
 
-/* device description of a list of resident tablets for one tile size */
+/* device description of a list of resident tablets */
 struct plan {
 	std::vector<vg_tabdesc> desc;
 	std::vector<uint32_t> tile_tab;
@@ -243,10 +242,10 @@ static void free_plan(plan &p)
 	p.d_tile = nullptr;
 }
 
-static int build_plan(vgd_device *d, const std::vector<const virg_tablet_meta *> &tabs, int rpl,
+static int build_plan(vgd_device *d, const std::vector<const virg_tablet_meta *> &tabs,
 	hipStream_t stream, plan &p)
 {
-	const uint32_t T = vg_kernel_tile_rows(rpl);
+	const uint32_t T = vg_kernel_tile_rows();
 	p.desc.clear();
 	p.tile_tab.clear();
 	p.rows = 0;
@@ -263,7 +262,7 @@ static int build_plan(vgd_device *d, const std::vector<const virg_tablet_meta *>
 		vg_tabdesc td;
 		memset(&td, 0, sizeof td);
 		bool aligned = true;
-		for (unsigned c = 0; c < t->fixed_columns && c < VG_MAX_COLS - 1; c++) {
+		for (unsigned c = 0; c < t->fixed_columns; c++) {
 			uint64_t a = (uint64_t)(base + t->fixed_block + t->fixed_offset[c]);
 			td.col[c] = a;
 			aligned = aligned && (a % 16 == 0);
@@ -296,11 +295,11 @@ static int build_plan(vgd_device *d, const std::vector<const virg_tablet_meta *>
 }
 
 static int launch(const vg_prog &P, const plan &p, launch_res &lr, void *const *out,
-	uint64_t capacity, int rpl, hipStream_t stream)
+	uint64_t capacity, hipStream_t stream)
 {
 	if (p.num_tiles == 0)
 		return VIRG_SUCCESS;
-	if (ensure_lr(lr, p.num_tiles, vg_kernel_grid(&P, rpl, p.num_tiles), stream) == VIRG_FAIL)
+	if (ensure_lr(lr, p.num_tiles, vg_kernel_grid(&P, p.num_tiles), stream) == VIRG_FAIL)
 		return VIRG_FAIL;
 	vg_launch L;
 	memset(&L, 0, sizeof L);
@@ -310,10 +309,11 @@ static int launch(const vg_prog &P, const plan &p, launch_res &lr, void *const *
 	L.capacity = capacity;
 	L.ctrl = lr.ctrl;
 	L.vbits = lr.vbits;
+	L.tcount = lr.tcount;
 	L.wgcount = lr.wgcount;
 	for (unsigned j = 0; j < P.nout; j++)
 		L.out[j] = out[j];
-	int rc = vg_kernel_launch(&P, &L, rpl, stream);
+	int rc = vg_kernel_launch(&P, &L, stream);
 	if (rc != 0) {
 		fprintf(stderr, "::: Virginian: kernel launch failed (%d)\n", rc);
 		return VIRG_FAIL;
@@ -359,6 +359,7 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 		fprintf(stderr, "::: Virginian: op-program not supported on the GPU: %s\n", err);
 		return VIRG_FAIL;
 	}
+	VIRG_CHECK(vg_kernel_check(&P) != 0, "op-program needs more register slots than the LDS holds");
 
 	/* collect the tablets: tab[0] arrives locked; like the reference the
 	 * walk unlocks each tablet as it moves on and leaves the last locked */
@@ -379,9 +380,7 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 		HCHK(hipEventCreate(&ev[i]), "event");
 	HCHK(hipEventRecord(ev[0], d->stream), "event");
 	plan p;
-	int rpl = vg_kernel_pick_rpl(&P, d->rows_per_lane);
-	VIRG_CHECK(rpl == 0, "op-program needs more register slots than the LDS budget allows");
-	if (build_plan(d, tabs, rpl, d->stream, p) == VIRG_FAIL) {
+	if (build_plan(d, tabs, d->stream, p) == VIRG_FAIL) {
 		free_plan(p);
 		return VIRG_FAIL;
 	}
@@ -411,7 +410,7 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 		off += ((size_t)p.rows * widths[j] + 255) / 256 * 256;
 	}
 
-	int ok = launch(P, p, d->lr, out, p.rows, rpl, d->stream);
+	int ok = launch(P, p, d->lr, out, p.rows, d->stream);
 	HCHK(hipEventRecord(ev[2], d->stream), "event");
 	uint64_t nres = 0;
 	if (ok == VIRG_SUCCESS && p.num_tiles)
@@ -435,7 +434,7 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 		done += n;
 		if (done >= nres)
 			break;
-		VIRG_CHECK(room == 0 && n == 0 && r->possible_rows == 0, "result tablet has no capacity");
+		VIRG_CHECK(r->possible_rows == 0, "result tablet has no capacity");
 		virg_tablet_meta *old = r;
 		VIRG_CHECK(virg_vm_allocresult(v, vm, res, old) == VIRG_FAIL, "result allocation failed");
 		virg_tablet_unlock(v, old->id);
@@ -465,7 +464,6 @@ struct vg_query {
 	void *out[VG_MAX_OUT];
 	size_t width[VG_MAX_OUT];
 	uint64_t capacity = 0;
-	int rpl = 4;
 	hipStream_t stream = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	uint64_t last_rows = 0;
@@ -504,10 +502,11 @@ extern "C" void vg_query_free(vg_query *q)
  * SQL front-end or virg_vm_addop: Table, ResultColumn..., Parallel, ...,
  * Result, Converge, Finish) for repeated execution over its table with the
  * table resident in HBM.  Output capacity is `capacity` rows (0 = every
- * row of the table). */
+ * row of the table).  `reserved` must be 0. */
 extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned num_ops,
-	unsigned long long capacity, int rows_per_lane, char *err, size_t errlen)
+	unsigned long long capacity, int reserved, char *err, size_t errlen)
 {
+	(void)reserved;
 	vgd_device *d = dev_of(v);
 	if (!d) {
 		qerr(err, errlen, "no usable GPU: the data-parallel VM needs a HIP device");
@@ -539,7 +538,6 @@ extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned 
 	memset(q->out, 0, sizeof q->out);
 	q->v = v;
 	q->d = d;
-	q->rpl = rows_per_lane ? rows_per_lane : d->rows_per_lane;
 	virg_tablet_meta *t;
 	if (virg_db_load(v, v->db.first_tablet[table], &t) == VIRG_FAIL) {
 		qerr(err, errlen, "table has no tablets");
@@ -558,19 +556,18 @@ extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned 
 		virg_db_loadnext(v, &t);
 	}
 	virg_tablet_unlock(v, t->id);
+	if (vg_kernel_check(&q->P) != 0) {
+		qerr(err, errlen, "op-program needs more register slots than the LDS holds");
+		delete q;
+		return nullptr;
+	}
 	if (hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess ||
 		hipEventCreate(&q->e0) != hipSuccess || hipEventCreate(&q->e1) != hipSuccess) {
 		qerr(err, errlen, "cannot create stream/events");
 		vg_query_free(q);
 		return nullptr;
 	}
-	q->rpl = vg_kernel_pick_rpl(&q->P, q->rpl);
-	if (q->rpl == 0) {
-		qerr(err, errlen, "op-program needs more register slots than the LDS budget allows");
-		vg_query_free(q);
-		return nullptr;
-	}
-	if (build_plan(d, q->tabs, q->rpl, q->stream, q->p) == VIRG_FAIL) {
+	if (build_plan(d, q->tabs, q->stream, q->p) == VIRG_FAIL) {
 		qerr(err, errlen, "cannot make the table resident");
 		vg_query_free(q);
 		return nullptr;
@@ -588,7 +585,7 @@ extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned 
 	for (unsigned c = 0; c < q->P.ncol; c++)
 		rb += q->P.col[c].width;
 	q->read_bytes_per_row = rb;
-	if (ensure_lr(q->lr, q->p.num_tiles, vg_kernel_grid(&q->P, q->rpl, q->p.num_tiles), q->stream) == VIRG_FAIL ||
+	if (ensure_lr(q->lr, q->p.num_tiles, vg_kernel_grid(&q->P, q->p.num_tiles), q->stream) == VIRG_FAIL ||
 		hipStreamSynchronize(q->stream) != hipSuccess) {
 		qerr(err, errlen, "cannot set up the launch");
 		vg_query_free(q);
@@ -600,7 +597,7 @@ extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned 
 /* Launch once, asynchronously, on the query's stream. */
 extern "C" int vg_query_launch(vg_query *q)
 {
-	return launch(q->P, q->p, q->lr, q->out, q->capacity, q->rpl, q->stream);
+	return launch(q->P, q->p, q->lr, q->out, q->capacity, q->stream);
 }
 
 /* Wait for the last launch; returns the selected row count. */
@@ -633,8 +630,8 @@ extern "C" int vg_query_fetch(vg_query *q, void **cols, unsigned long long max_r
 }
 
 /* Time `iters` back-to-back launches after `warmup` untimed ones with HIP
- * events on the query's stream (the stream the kernel runs on).  Reports
- * the mean time per launch. */
+ * events on the query's stream (the stream the kernels run on).  Reports
+ * the mean time per launch pair. */
 extern "C" int vg_query_time(vg_query *q, int warmup, int iters, double *ms_per_launch,
 	unsigned long long *rows)
 {

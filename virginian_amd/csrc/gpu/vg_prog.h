@@ -97,6 +97,8 @@ typedef struct {
 	uint32_t has_project_loads;
 	uint32_t wfull;     /* the write pass reruns every micro-op (masked writes) */
 	uint64_t wneed;     /* micro-ops the write pass runs (bit k = micro-op k) */
+	uint32_t nslot_count; /* slots [0, nslot_count) hold everything the count pass uses */
+	uint32_t pad2;
 	vg_col   col[VG_MAX_COLS + 1];
 	vg_out   out[VG_MAX_OUT];
 	vg_uop   uop[VG_MAX_UOPS];
@@ -129,7 +131,8 @@ typedef struct {
 	uint32_t tiles_per_wg;
 	uint64_t capacity;
 	vg_ctrl *ctrl;
-	uint8_t *vbits;          /* one validity byte per lane per tile */
+	uint8_t *vbits;          /* one validity byte per write-pass lane per tile */
+	uint32_t *tcount;        /* rows selected per tile and wave: 4 per tile */
 	uint32_t *wgcount;       /* rows selected per workgroup */
 	void *out[VG_MAX_OUT];
 } vg_launch;

@@ -4,29 +4,30 @@
  * on the CPU).
  *
  * A query is two launches over the same persistent grid.  Rows are cut
- * into tiles of T = NT*R consecutive rows of one tablet; workgroup g owns
- * a contiguous run of tiles.  Inside a tile, lane l of wave w owns rows
- * (w*64+l)*R .. +R-1, so a column load is one contiguous R*width-byte
- * vector per lane and a wave reads 64*R*width contiguous bytes of HBM.
+ * into tiles of 1024 consecutive rows of one tablet; workgroup g owns a
+ * contiguous run of tiles.  Rows are handled in strips of 4 consecutive
+ * rows per lane, so a wave's strip is 256 rows: one 1 KB contiguous
+ * access in HBM and in LDS.
  *
- *   count pass: load the columns the filter micro-ops read, run them,
- *     store one validity byte per lane per tile and one row count per
- *     workgroup;
+ *   count pass: each wave takes a whole tile (4 strips per lane), copies
+ *     the columns the filter micro-ops read from HBM straight into its LDS
+ *     register file, runs those micro-ops, and stores one validity byte per
+ *     write-pass lane, the selected-row count of each write-pass wave, and
+ *     one count per workgroup;
  *   write pass: a workgroup's output offset is the sum of the counts of
- *     the workgroups before it.  Per tile, each wave gets its own offset
- *     from the tile's validity bytes, loads the columns the result needs
- *     only in lanes holding a selected row, runs the micro-ops computing
- *     result registers, ranks its rows with ballot + mbcnt and writes its
- *     contiguous slice of every result column in row order.
+ *     the workgroups before it, and each tile's wave offsets follow from
+ *     the per-wave counts.  A wave with selected rows loads what the result
+ *     needs (only in lanes holding a selected row), runs the micro-ops
+ *     computing result registers, ranks its rows with ballot + mbcnt and
+ *     writes its contiguous slice of every result column in row order.
  * Nothing waits on another workgroup, so neither pass has atomics, spins
  * or cross-workgroup ordering assumptions; results come out in the
  * reference's single-core row order.
  *
- * The VM registers live in LDS as [slot][row of the tile]; a lane's R rows
- * of a slot are contiguous, so reading an operand for all of them is one
- * ds_read_b128 (R = 4) at a wave-uniform slot base.  Micro-op fields are
- * read with scalar loads from the kernel arguments, so dispatch is a
- * scalar branch.
+ * The VM registers live in LDS as [slot][row of the tile]; reading an
+ * operand for a strip is one ds_read_b128 at a wave-uniform slot base.
+ * Micro-op fields are read with scalar loads from the kernel arguments,
+ * so dispatch is a scalar branch.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -38,6 +39,43 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t uni(uint32_t x)
 {
 	return __builtin_amdgcn_readfirstlane(x);
+}
+
+/* Read memory that no kernel of the launch writes (tile table, tablet
+ * descriptors) through the constant address space: with a wave-uniform
+ * address this is a scalar load, which neither occupies VGPRs nor waits
+ * behind the vector loads already in flight. */
+template <typename T>
+__device__ __forceinline__ T ldc(const T *p)
+{
+	return *(const __attribute__((address_space(4))) T *)p;
+}
+
+/* The same for a small struct of byte fields (micro-ops, column and
+ * output descriptors): fetched as whole dwords, so the fields are
+ * extracted from SGPRs instead of being read by vector byte loads. */
+template <typename T>
+__device__ __forceinline__ T ldc_struct(const T *p)
+{
+	static_assert(sizeof(T) % 4 == 0, "dword-sized descriptors only");
+	union {
+		T t;
+		uint32_t w[sizeof(T) / 4];
+	} u;
+	const __attribute__((address_space(4))) uint32_t *s =
+		(const __attribute__((address_space(4))) uint32_t *)p;
+#pragma unroll
+	for (unsigned i = 0; i < sizeof(T) / 4; ++i)
+		u.w[i] = s[i];
+	return u.t;
+}
+
+/* global (not flat) view of column data, so vector loads count only
+ * against vmcnt */
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gptr(const void *p)
+{
+	return (const __attribute__((address_space(1))) T *)p;
 }
 
 __device__ __forceinline__ uint32_t rank_below(uint64_t m)
@@ -299,304 +337,352 @@ __device__ __forceinline__ void cast_from(const vg_uop &u, uint32_t k, uint32_t 
 	}
 }
 
-template <int R, uint32_t TR>
+/* ------------------------------------------------------ strips of rows */
+/* A lane owns NS strips of 4 consecutive rows.  Strip q of a lane starts
+ * at row wbase + q*stride + 4*lane of the tile: wbase and stride are
+ * wave-uniform, so a wave's strip q is 256 consecutive rows, which is one
+ * 16-byte-per-lane vector access both in HBM and in the LDS register
+ * file ([slot][row of the tile]). */
+template <int NS>
+struct strips {
+	uint32_t wbase, stride, lane;
+	__device__ __forceinline__ uint32_t row(int q) const { return wbase + (uint32_t)q * stride + 4 * lane; }
+	__device__ __forceinline__ uint32_t ubase(int q) const { return wbase + (uint32_t)q * stride; }
+};
+
+template <int NS, uint32_t TR>
 __device__ __forceinline__ void run_uops(const vg_prog &P, uint32_t kbeg, uint32_t kend, uint64_t mask,
-	uint32_t *regs, uint32_t lrow, uint32_t (&resume)[R], bool (&valid)[R])
+	uint32_t *regs, const strips<NS> &S, uint32_t (&resume)[NS * 4], bool (&valid)[NS * 4])
 {
 	for (uint32_t k = kbeg; k < kend; ++k) {
 		if (!((mask >> k) & 1ull))
 			continue;
-		const vg_uop &u = P.uop[k];
-		if (u.kind == VG_K_INVALID) {
+		const vg_uop u = ldc_struct(&P.uop[k]);
 #pragma unroll
-			for (int r = 0; r < R; ++r)
-				if (resume[r] <= k)
-					valid[r] = false;
-			continue;
-		}
-		if (u.kind == VG_K_CAST) {
-			switch (u.type) {
-			case VG_T_INT: cast_from<int32_t, R, TR>(u, k, regs, lrow, resume); break;
-			case VG_T_INT64: cast_from<int64_t, R, TR>(u, k, regs, lrow, resume); break;
-			case VG_T_FLOAT: cast_from<float, R, TR>(u, k, regs, lrow, resume); break;
-			case VG_T_DOUBLE: cast_from<double, R, TR>(u, k, regs, lrow, resume); break;
-			case VG_T_CHAR: cast_from<chr, R, TR>(u, k, regs, lrow, resume); break;
+		for (int q = 0; q < NS; ++q) {
+			uint32_t (&res4)[4] = *(uint32_t (*)[4])(resume + 4 * q);
+			bool (&val4)[4] = *(bool (*)[4])(valid + 4 * q);
+			const uint32_t r0 = S.row(q);
+			if (u.kind == VG_K_INVALID) {
+#pragma unroll
+				for (int r = 0; r < 4; ++r)
+					if (res4[r] <= k)
+						val4[r] = false;
+				continue;
 			}
-			continue;
-		}
-		switch (u.type) {
-		case VG_T_INT: exec_typed<int32_t, R, TR>(u, k, regs, lrow, resume, valid); break;
-		case VG_T_INT64: exec_typed<int64_t, R, TR>(u, k, regs, lrow, resume, valid); break;
-		case VG_T_FLOAT: exec_typed<float, R, TR>(u, k, regs, lrow, resume, valid); break;
-		case VG_T_DOUBLE: exec_typed<double, R, TR>(u, k, regs, lrow, resume, valid); break;
-		case VG_T_CHAR: exec_typed<chr, R, TR>(u, k, regs, lrow, resume, valid); break;
+			if (u.kind == VG_K_CAST) {
+				switch (u.type) {
+				case VG_T_INT: cast_from<int32_t, 4, TR>(u, k, regs, r0, res4); break;
+				case VG_T_INT64: cast_from<int64_t, 4, TR>(u, k, regs, r0, res4); break;
+				case VG_T_FLOAT: cast_from<float, 4, TR>(u, k, regs, r0, res4); break;
+				case VG_T_DOUBLE: cast_from<double, 4, TR>(u, k, regs, r0, res4); break;
+				case VG_T_CHAR: cast_from<chr, 4, TR>(u, k, regs, r0, res4); break;
+				}
+				continue;
+			}
+			switch (u.type) {
+			case VG_T_INT: exec_typed<int32_t, 4, TR>(u, k, regs, r0, res4, val4); break;
+			case VG_T_INT64: exec_typed<int64_t, 4, TR>(u, k, regs, r0, res4, val4); break;
+			case VG_T_FLOAT: exec_typed<float, 4, TR>(u, k, regs, r0, res4, val4); break;
+			case VG_T_DOUBLE: exec_typed<double, 4, TR>(u, k, regs, r0, res4, val4); break;
+			case VG_T_CHAR: exec_typed<chr, 4, TR>(u, k, regs, r0, res4, val4); break;
+			}
 		}
 	}
 }
 
 /* ---------------------------------------------------------- column loads */
-/* Rows [lrow, lrow+R) of column c into its LDS slot(s).  The vector path
- * needs the lane's whole group in range and a 16-byte aligned column;
- * otherwise rows are loaded one by one (tails, unaligned tablets). */
-template <int R, uint32_t TR>
-__device__ __forceinline__ void load_col(const vg_col &c, const char *base, uint32_t lrow,
+typedef __attribute__((address_space(3))) void lds_void;
+
+/* Strips of column c into its LDS slot(s).  A full strip of a 16-byte
+ * aligned 4-byte column is copied from HBM straight into the register
+ * file (global_load_lds: no VGPRs, and every column of a tile is in
+ * flight at once; the compiler waits for them before the first LDS read).
+ * 8-byte and 1-byte columns, tails and unaligned tablets go through VGPRs
+ * (8-byte values are split into a low-word and a high-word slot). */
+template <int NS, uint32_t TR>
+__device__ __forceinline__ void load_col(const vg_col &c, const char *base, const strips<NS> &S,
 	uint32_t nrows, bool aligned, uint32_t *regs)
 {
-	const bool full = aligned && lrow + R <= nrows;
-	rv<R> l, h;
 #pragma unroll
-	for (int r = 0; r < R; ++r)
-		l.v[r] = h.v[r] = 0;
-	if (c.width == 4) {
-		const uint32_t *p = (const uint32_t *)base + lrow;
-		if (full) {
-			if constexpr (R % 4 == 0) {
+	for (int q = 0; q < NS; ++q) {
+		const uint32_t r0 = S.row(q);
+		if (c.width == 4 && aligned && r0 + 4 <= nrows) {
+			__builtin_amdgcn_global_load_lds((const void *)((const uint32_t *)base + r0),
+				(lds_void *)(regs + c.slot * TR + S.ubase(q)), 16, 0, 0);
+			continue;
+		}
+		rv<4> l, h;
 #pragma unroll
-				for (int q = 0; q < R / 4; ++q) {
-					const u32x4 v = __builtin_nontemporal_load((const u32x4 *)p + q);
-					l.v[4 * q] = v.x; l.v[4 * q + 1] = v.y; l.v[4 * q + 2] = v.z; l.v[4 * q + 3] = v.w;
+		for (int r = 0; r < 4; ++r)
+			l.v[r] = h.v[r] = 0;
+		if (c.width == 4) {
+#pragma unroll
+			for (int r = 0; r < 4; ++r)
+				if (r0 + r < nrows)
+					l.v[r] = gptr<uint32_t>(base)[r0 + r];
+		} else if (c.width == 8) {
+			const __attribute__((address_space(1))) u32x2 *p = gptr<u32x2>(base) + r0;
+			if (aligned && r0 + 4 <= nrows) {
+#pragma unroll
+				for (int j = 0; j < 2; ++j) {
+					const u32x4 v = __builtin_nontemporal_load((const __attribute__((address_space(1))) u32x4 *)p + j);
+					l.v[2 * j] = v.x; h.v[2 * j] = v.y; l.v[2 * j + 1] = v.z; h.v[2 * j + 1] = v.w;
 				}
-			} else if constexpr (R == 2) {
-				const u32x2 v = __builtin_nontemporal_load((const u32x2 *)p);
-				l.v[0] = v.x; l.v[1] = v.y;
 			} else {
-				l.v[0] = __builtin_nontemporal_load(p);
+#pragma unroll
+				for (int r = 0; r < 4; ++r)
+					if (r0 + r < nrows) {
+						const u32x2 v = p[r];
+						l.v[r] = v.x;
+						h.v[r] = v.y;
+					}
 			}
 		} else {
+			const __attribute__((address_space(1))) int8_t *p = gptr<int8_t>(base) + r0;
 #pragma unroll
-			for (int r = 0; r < R; ++r)
-				if (lrow + r < nrows)
-					l.v[r] = p[r];
+			for (int r = 0; r < 4; ++r)
+				if (r0 + r < nrows)
+					l.v[r] = (uint32_t)(int32_t)p[r];
 		}
-	} else if (c.width == 8) {
-		const u32x2 *p = (const u32x2 *)base + lrow;
-		if (full && R % 2 == 0) {
-#pragma unroll
-			for (int q = 0; q < R / 2; ++q) {
-				const u32x4 v = __builtin_nontemporal_load((const u32x4 *)p + q);
-				l.v[2 * q] = v.x; h.v[2 * q] = v.y; l.v[2 * q + 1] = v.z; h.v[2 * q + 1] = v.w;
-			}
-		} else {
-#pragma unroll
-			for (int r = 0; r < R; ++r)
-				if (lrow + r < nrows) {
-					const u32x2 v = p[r];
-					l.v[r] = v.x;
-					h.v[r] = v.y;
-				}
-		}
-	} else {
-		const int8_t *p = (const int8_t *)base + lrow;
-#pragma unroll
-		for (int r = 0; r < R; ++r)
-			if (lrow + r < nrows)
-				l.v[r] = (uint32_t)(int32_t)p[r];
+		lds_put<4>(regs + c.slot * TR + r0, l);
+		if (c.width == 8)
+			lds_put<4>(regs + c.slot_hi * TR + r0, h);
 	}
-	lds_put<R>(regs + c.slot * TR + lrow, l);
-	if (c.width == 8)
-		lds_put<R>(regs + c.slot_hi * TR + lrow, h);
 }
 
-template <int R, uint32_t TR>
+template <int NS, uint32_t TR>
 __device__ __forceinline__ void load_cols(const vg_prog &P, const vg_tabdesc *D, uint32_t row0,
-	uint32_t lrow, uint32_t nrows, bool write_pass, uint32_t *regs)
+	const strips<NS> &S, uint32_t nrows, bool write_pass, uint32_t *regs)
 {
-	const bool aligned = D->aligned != 0;
+	const bool aligned = ldc(&D->aligned) != 0;
 	for (uint32_t k = 0; k < P.nload; ++k) {
-		const vg_col &c = P.col[k];
+		const vg_col c = ldc_struct(&P.col[k]);
 		if (write_pass ? !c.wload : c.phase != VG_PHASE_FILTER)
 			continue;
-		const char *base = (const char *)D->col[c.src] + (size_t)row0 * c.width;
-		load_col<R, TR>(c, base, lrow, nrows, aligned, regs);
+		const char *base = (const char *)ldc(&D->col[c.src]) + (size_t)row0 * c.width;
+		load_col<NS, TR>(c, base, S, nrows, aligned, regs);
 	}
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
-{
-#pragma unroll
-	for (int off = 32; off > 0; off >>= 1)
-		x += __shfl_xor(x, off, 64);
-	return x;
+	/* The compiler does not count global_load_lds as a write of the LDS
+	 * it later reads, so it inserts no wait before those reads: wait for
+	 * every vector load of this wave here (and keep memory accesses from
+	 * moving across the wait). */
+	asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 /* ----------------------------------------------------------- count pass */
-/* Workgroup g evaluates the filter micro-ops over its tiles, stores one
- * validity byte per lane per tile (bit r = row lrow+r) and the number of
- * selected rows of all its tiles.  LDS: nslot * T words of registers. */
-template <int R, int NT>
-__global__ __launch_bounds__(NT) void vg_count_kernel(const vg_prog P, const vg_launch L)
+/* Each wave evaluates one whole tile of T = 1024 rows: lane l owns strips
+ * q*256 + 4l .. +3 (q < 4), so every load of the wave is 1 KB contiguous
+ * and four of them per column are in flight.  The row set of count lane l,
+ * strip q, is exactly that of write lane l of wave q, so the validity byte
+ * the write pass reads for thread 64q + l is bits [4q, 4q+4) here, and the
+ * write pass's wave q count is the strip-q count.  Workgroup g covers
+ * tiles [g*tiles_per_wg, (g+1)*tiles_per_wg), its waves taking every
+ * fourth tile, and stores the workgroup's total.
+ * LDS: one register file of nslot_count * T words per wave. */
+#define VG_T 1024u
+__global__ __launch_bounds__(256) void vg_count_kernel(const vg_prog P, const vg_launch L)
 {
-	constexpr uint32_t T = NT * R;
-	constexpr int NW = NT / 64;
+	constexpr uint32_t T = VG_T;
 	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
-	__shared__ uint32_t s_wcount[NW];
-	uint32_t *regs = lds;
-	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	const uint32_t lrow = (wave * 64 + lane) * R;
+	__shared__ uint32_t s_wcount[4];
+	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
+	uint32_t *regs = lds + wave * P.nslot_count * T;
+	const strips<4> S{0u, 256u, lane};
 	const uint32_t g = blockIdx.x;
 	const uint32_t t0 = g * L.tiles_per_wg;
 	const uint32_t t1 = min(t0 + L.tiles_per_wg, L.num_tiles);
 	uint32_t count = 0;
-	for (uint32_t tile = t0; tile < t1; ++tile) {
-		const uint32_t tb = uni(L.tile_tab[tile]);
+	for (uint32_t tile = t0 + wave; tile < t1; tile += 4) {
+		const uint32_t tb = ldc(&L.tile_tab[tile]);
 		const vg_tabdesc *D = &L.tabs[tb];
-		const uint32_t row0 = (tile - D->first_tile) * T;
-		const uint32_t left = D->rows - row0;
+		const uint32_t row0 = (tile - ldc(&D->first_tile)) * T;
+		const uint32_t left = ldc(&D->rows) - row0;
 		const uint32_t nrows = left < T ? left : T;
-		uint32_t resume[R];
-		bool valid[R];
+		uint32_t resume[16];
+		bool valid[16];
 #pragma unroll
-		for (int r = 0; r < R; ++r) {
-			resume[r] = 0;
-			valid[r] = lrow + r < nrows;
+		for (int q = 0; q < 4; ++q)
+#pragma unroll
+			for (int j = 0; j < 4; ++j) {
+				resume[4 * q + j] = 0;
+				valid[4 * q + j] = S.row(q) + j < nrows;
+			}
+		/* a lane only ever touches its own rows of the register file */
+		load_cols<4, T>(P, D, row0, S, nrows, false, regs);
+		run_uops<4, T>(P, 0, P.nfilter, ~0ull, regs, S, resume, valid);
+		uint32_t cnt[4];
+#pragma unroll
+		for (int q = 0; q < 4; ++q) {
+			uint32_t byte = 0;
+			cnt[q] = 0;
+#pragma unroll
+			for (int j = 0; j < 4; ++j) {
+				byte |= valid[4 * q + j] ? 1u << j : 0u;
+				cnt[q] += (uint32_t)__builtin_popcountll(__ballot(valid[4 * q + j]));
+			}
+			L.vbits[(size_t)tile * 256 + 64 * q + lane] = (uint8_t)byte;
 		}
-		/* every LDS row a lane touches is its own: no barriers needed */
-		load_cols<R, T>(P, D, row0, lrow, nrows, false, regs);
-		run_uops<R, T>(P, 0, P.nfilter, ~0ull, regs, lrow, resume, valid);
-		uint32_t bits = 0;
-#pragma unroll
-		for (int r = 0; r < R; ++r)
-			bits |= valid[r] ? (1u << r) : 0u;
-		L.vbits[(size_t)tile * NT + tid] = (uint8_t)bits;
-		count += (uint32_t)__builtin_popcount(bits);
+		if (lane < 4)
+			L.tcount[(size_t)tile * 4 + lane] = lane == 0 ? cnt[0] : lane == 1 ? cnt[1] : lane == 2 ? cnt[2] : cnt[3];
+		count += cnt[0] + cnt[1] + cnt[2] + cnt[3];
 	}
-	count = wave_sum(count);
 	if (lane == 0)
 		s_wcount[wave] = count;
 	__syncthreads();
-	if (tid == 0) {
-		uint32_t s = 0;
-#pragma unroll
-		for (int w = 0; w < NW; ++w)
-			s += s_wcount[w];
-		L.wgcount[g] = s;
-	}
+	if (tid == 0)
+		L.wgcount[g] = s_wcount[0] + s_wcount[1] + s_wcount[2] + s_wcount[3];
 }
 
 /* ----------------------------------------------------------- write pass */
-/* Workgroup g starts at the sum of the counts of workgroups < g.  For each
- * of its tiles the four waves compute their offsets from the tile's
- * validity bytes; a wave with selected rows loads what the result needs
- * (only in lanes holding a selected row), runs the micro-ops that compute
- * result registers, ranks its rows with ballot + mbcnt, and writes its
- * contiguous slice of every result column.  Waves never wait for each
- * other.  LDS: nslot * T words of registers, then R*64 row numbers per
+/* Workgroup g starts at the sum of the counts of workgroups < g.  Its
+ * tiles' per-wave counts are staged in LDS 64 tiles at a time with their
+ * running offsets, so empty tiles cost nothing.  For a tile with selected
+ * rows, each wave with selected rows loads what the result needs (only in
+ * lanes holding a selected row), runs the micro-ops computing result
+ * registers, ranks its rows with ballot + mbcnt and writes its contiguous
+ * slice of every result column; waves never wait for each other inside a
+ * tile.  LDS: nslot * T words of registers, then 256 row numbers per
  * wave. */
-template <int R, int NT>
-__global__ __launch_bounds__(NT) void vg_write_kernel(const vg_prog P, const vg_launch L)
+#define VG_CHUNK 64
+__global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg_launch L)
 {
-	constexpr uint32_t T = NT * R;
-	constexpr int NW = NT / 64;
-	static_assert(NT / 4 == 64, "one 4-byte word of validity bytes per lane covers the tile");
+	constexpr uint32_t T = VG_T;
 	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+	__shared__ uint32_t s_cnt[VG_CHUNK][4];
+	__shared__ uint64_t s_off[VG_CHUNK];
+	__shared__ uint64_t s_base;
 	uint32_t *regs = lds;
-	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	uint16_t *srcrow = (uint16_t *)(lds + P.nslot * T) + wave * (64 * R);
-	const uint32_t lrow = (wave * 64 + lane) * R;
+	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
+	uint16_t *srcrow = (uint16_t *)(lds + P.nslot * T) + wave * 256;
+	const strips<1> S{wave * 256u, 0u, lane};
+	const uint32_t lrow = S.row(0);
 	const uint32_t g = blockIdx.x;
 	const uint32_t t0 = g * L.tiles_per_wg;
 	const uint32_t t1 = min(t0 + L.tiles_per_wg, L.num_tiles);
 
-	/* offset of this workgroup: sum of the counts before it */
-	uint64_t base = 0;
-	for (uint32_t i = lane; i < g; i += 64)
-		base += L.wgcount[i];
+	if (wave == 0) {
+		/* offset of this workgroup: sum of the counts before it */
+		uint64_t base = 0;
+		for (uint32_t i = lane; i < g; i += 64)
+			base += L.wgcount[i];
 #pragma unroll
-	for (int off = 32; off > 0; off >>= 1)
-		base += __shfl_xor(base, off, 64);
+		for (int off = 32; off > 0; off >>= 1)
+			base += __shfl_xor(base, off, 64);
+		if (lane == 0)
+			s_base = base;
+	}
+	__syncthreads();
+	uint64_t chunk_base = s_base;
 
-	for (uint32_t tile = t0; tile < t1; ++tile) {
-		/* validity bytes of the whole tile: lane l holds threads 4l..4l+3 */
-		const uint32_t word = ((const uint32_t *)(L.vbits + (size_t)tile * NT))[lane];
-		const uint32_t c = (uint32_t)__builtin_popcount(word);
-		const uint32_t tile_tot = wave_sum(c);
-		if (tile_tot == 0)
-			continue;
-		const uint32_t wave_off = wave_sum(lane < 16 * wave ? c : 0u);
-		const uint32_t wave_tot = wave_sum(lane >= 16 * wave && lane < 16 * wave + 16 ? c : 0u);
-		if (wave_tot != 0) {
-			const uint32_t mybits = (__shfl(word, (int)(16 * wave + (lane >> 2)), 64) >> ((lane & 3) * 8)) & 0xFFu;
-			const uint32_t tb = uni(L.tile_tab[tile]);
-			const vg_tabdesc *D = &L.tabs[tb];
-			const uint32_t row0 = (tile - D->first_tile) * T;
-			const uint32_t left = D->rows - row0;
-			const uint32_t nrows = left < T ? left : T;
-			uint32_t resume[R];
-			bool valid[R];
+	for (uint32_t c0 = t0; c0 < t1; c0 += VG_CHUNK) {
+		const uint32_t nt = min((uint32_t)VG_CHUNK, t1 - c0);
+		if (wave == 0) {
+			/* lane i: counts of tile c0+i and its offset inside the chunk */
+			uint32_t c[4] = {0, 0, 0, 0};
+			if (lane < nt) {
+				const u32x4 v = *(const u32x4 *)(L.tcount + (size_t)(c0 + lane) * 4);
+				c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w;
+			}
+			const uint32_t tot = c[0] + c[1] + c[2] + c[3];
+			uint32_t incl = tot;
 #pragma unroll
-			for (int r = 0; r < R; ++r) {
+			for (int off = 1; off < 64; off <<= 1) {
+				const uint32_t y = __shfl_up(incl, off, 64);
+				if (lane >= (uint32_t)off)
+					incl += y;
+			}
+			s_cnt[lane][0] = c[0]; s_cnt[lane][1] = c[1]; s_cnt[lane][2] = c[2]; s_cnt[lane][3] = c[3];
+			s_off[lane] = chunk_base + (incl - tot);
+		}
+		__syncthreads();
+		for (uint32_t i = 0; i < nt; ++i) {
+			const uint32_t c0w = s_cnt[i][0], c1w = s_cnt[i][1], c2w = s_cnt[i][2], c3w = s_cnt[i][3];
+			const uint32_t wave_tot = wave == 0 ? c0w : wave == 1 ? c1w : wave == 2 ? c2w : c3w;
+			if (wave_tot == 0)
+				continue;
+			const uint32_t wave_off = (wave > 0 ? c0w : 0u) + (wave > 1 ? c1w : 0u) + (wave > 2 ? c2w : 0u);
+			const uint32_t tile = c0 + i;
+			const uint32_t mybits = L.vbits[(size_t)tile * 256 + tid];
+			const uint32_t tb = ldc(&L.tile_tab[tile]);
+			const vg_tabdesc *D = &L.tabs[tb];
+			const uint32_t row0 = (tile - ldc(&D->first_tile)) * T;
+			const uint32_t left = ldc(&D->rows) - row0;
+			const uint32_t nrows = left < T ? left : T;
+			uint32_t resume[4];
+			bool valid[4];
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
 				resume[r] = 0;
 				valid[r] = (mybits >> r) & 1u;
 			}
 			if (mybits)
-				load_cols<R, T>(P, D, row0, lrow, nrows, true, regs);
+				load_cols<1, T>(P, D, row0, S, nrows, true, regs);
 			if (P.wfull) {
 				/* rerun everything: resume points and validity come back
 				 * exactly as in the count pass */
-				run_uops<R, T>(P, 0, P.nuop, ~0ull, regs, lrow, resume, valid);
+				run_uops<1, T>(P, 0, P.nuop, ~0ull, regs, S, resume, valid);
 			} else if (P.wneed) {
-				run_uops<R, T>(P, 0, P.nuop, P.wneed, regs, lrow, resume, valid);
+				run_uops<1, T>(P, 0, P.nuop, P.wneed, regs, S, resume, valid);
 			}
 			/* ranks inside the wave, row numbers in rank order */
 			uint32_t pos = 0;
 #pragma unroll
-			for (int r = 0; r < R; ++r) {
-				const uint64_t b = __ballot((mybits >> r) & 1u);
-				pos += rank_below(b);
-			}
+			for (int r = 0; r < 4; ++r)
+				pos += rank_below(__ballot((mybits >> r) & 1u));
 #pragma unroll
-			for (int r = 0; r < R; ++r)
+			for (int r = 0; r < 4; ++r)
 				if ((mybits >> r) & 1u)
 					srcrow[pos++] = (uint16_t)(lrow + r);
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 			__builtin_amdgcn_wave_barrier();
 			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-			const uint64_t at = base + wave_off;
+			const uint64_t at = s_off[i] + wave_off;
 			const uint64_t room = L.capacity > at ? L.capacity - at : 0;
 			const uint32_t n = (uint64_t)wave_tot < room ? wave_tot : (uint32_t)room;
 			if (n < wave_tot && lane == 0)
 				atomicOr(&L.ctrl->overflow, 1u);
 			for (uint32_t j = 0; j < P.nout; ++j) {
-				const vg_out &o = P.out[j];
+				const vg_out o = ldc_struct(&P.out[j]);
 				char *dst = (char *)L.out[j];
 				if (o.kind == VG_OUT_SLOT) {
 					const uint32_t *lo = regs + o.slot * T, *hi = regs + o.slot_hi * T;
 					if (o.width == 8) {
-						for (uint32_t i = lane; i < n; i += 64) {
-							const uint32_t s = srcrow[i];
-							((uint64_t *)dst)[at + i] = ((uint64_t)hi[s] << 32) | lo[s];
+						for (uint32_t k = lane; k < n; k += 64) {
+							const uint32_t s = srcrow[k];
+							((uint64_t *)dst)[at + k] = ((uint64_t)hi[s] << 32) | lo[s];
 						}
 					} else if (o.width == 4) {
-						for (uint32_t i = lane; i < n; i += 64)
-							((uint32_t *)dst)[at + i] = lo[srcrow[i]];
+						for (uint32_t k = lane; k < n; k += 64)
+							((uint32_t *)dst)[at + k] = lo[srcrow[k]];
 					} else {
-						for (uint32_t i = lane; i < n; i += 64)
-							((uint8_t *)dst)[at + i] = (uint8_t)lo[srcrow[i]];
+						for (uint32_t k = lane; k < n; k += 64)
+							((uint8_t *)dst)[at + k] = (uint8_t)lo[srcrow[k]];
 					}
 				} else if (o.kind == VG_OUT_COLUMN) {
-					const vg_col &cc = P.col[o.col];
-					const char *src = (const char *)D->col[cc.src] + (size_t)row0 * cc.width;
+					const vg_col cc = ldc_struct(&P.col[o.col]);
+					const char *src = (const char *)ldc(&D->col[cc.src]) + (size_t)row0 * cc.width;
 					if (o.width == 8) {
-						for (uint32_t i = lane; i < n; i += 64)
-							((uint64_t *)dst)[at + i] = ((const uint64_t *)src)[srcrow[i]];
+						for (uint32_t k = lane; k < n; k += 64)
+							((uint64_t *)dst)[at + k] = gptr<uint64_t>(src)[srcrow[k]];
 					} else if (o.width == 4) {
-						for (uint32_t i = lane; i < n; i += 64)
-							((uint32_t *)dst)[at + i] = ((const uint32_t *)src)[srcrow[i]];
+						for (uint32_t k = lane; k < n; k += 64)
+							((uint32_t *)dst)[at + k] = gptr<uint32_t>(src)[srcrow[k]];
 					} else {
-						for (uint32_t i = lane; i < n; i += 64)
-							((uint8_t *)dst)[at + i] = ((const uint8_t *)src)[srcrow[i]];
+						for (uint32_t k = lane; k < n; k += 64)
+							((uint8_t *)dst)[at + k] = gptr<uint8_t>(src)[srcrow[k]];
 					}
 				} else {
 					if (o.width == 8) {
-						for (uint32_t i = lane; i < n; i += 64)
-							((uint64_t *)dst)[at + i] = o.imm;
+						for (uint32_t k = lane; k < n; k += 64)
+							((uint64_t *)dst)[at + k] = o.imm;
 					} else if (o.width == 4) {
-						for (uint32_t i = lane; i < n; i += 64)
-							((uint32_t *)dst)[at + i] = (uint32_t)o.imm;
+						for (uint32_t k = lane; k < n; k += 64)
+							((uint32_t *)dst)[at + k] = (uint32_t)o.imm;
 					} else {
-						for (uint32_t i = lane; i < n; i += 64)
-							((uint8_t *)dst)[at + i] = (uint8_t)o.imm;
+						for (uint32_t k = lane; k < n; k += 64)
+							((uint8_t *)dst)[at + k] = (uint8_t)o.imm;
 					}
 				}
 			}
@@ -605,45 +691,37 @@ __global__ __launch_bounds__(NT) void vg_write_kernel(const vg_prog P, const vg_
 			__builtin_amdgcn_wave_barrier();
 			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 		}
-		base += tile_tot;
+		{
+			const uint32_t lastc = s_cnt[nt - 1][0] + s_cnt[nt - 1][1] + s_cnt[nt - 1][2] + s_cnt[nt - 1][3];
+			chunk_base = s_off[nt - 1] + lastc;
+		}
+		__syncthreads();
 	}
 	if (g == gridDim.x - 1 && tid == 0)
-		L.ctrl->total = base;
+		L.ctrl->total = chunk_base;
 }
 
 /* --------------------------------------------------------------- launch */
-template <int R>
-static size_t lds_bytes(const vg_prog *P)
+static size_t count_lds_bytes(const vg_prog *P)
 {
-	constexpr size_t T = 256u * R;
-	return (size_t)P->nslot * T * 4 + 4 * (64 * R) * 2;
+	return (size_t)4 * P->nslot_count * VG_T * 4;
 }
 
-extern "C" size_t vg_kernel_lds_bytes(const vg_prog *P, int rows_per_lane)
+static size_t write_lds_bytes(const vg_prog *P)
 {
-	switch (rows_per_lane) {
-	case 1: return lds_bytes<1>(P);
-	case 2: return lds_bytes<2>(P);
-	default: return lds_bytes<4>(P);
-	}
+	return (size_t)P->nslot * VG_T * 4 + 4 * 256 * 2;
 }
 
-#define VG_LDS_LIMIT (64u * 1024u)
+/* the gfx950 LDS is 160 KB per CU; keep room for the static arrays */
+#define VG_LDS_LIMIT (150u * 1024u)
 
-/* Rows per lane to use for a program: the requested value, halved until
- * its register file fits the workgroup LDS budget. */
-extern "C" int vg_kernel_pick_rpl(const vg_prog *P, int requested)
+extern "C" size_t vg_kernel_lds_bytes(const vg_prog *P)
 {
-	int r = requested == 1 || requested == 2 || requested == 4 ? requested : 4;
-	while (r > 1 && vg_kernel_lds_bytes(P, r) > VG_LDS_LIMIT)
-		r /= 2;
-	return vg_kernel_lds_bytes(P, r) > VG_LDS_LIMIT ? 0 : r;
+	const size_t a = count_lds_bytes(P), b = write_lds_bytes(P);
+	return a > b ? a : b;
 }
 
-/* Workgroups per pass: enough to fill every CU at the occupancy the LDS
- * footprint allows, never more than there are tiles. */
-template <int R>
-static uint32_t grid_for(const vg_prog *P, uint32_t num_tiles)
+static int num_cus()
 {
 	static int cus = 0;
 	if (cus == 0) {
@@ -654,60 +732,66 @@ static uint32_t grid_for(const vg_prog *P, uint32_t num_tiles)
 		else
 			cus = prop.multiProcessorCount;
 	}
+	return cus;
+}
+
+static void allow_lds()
+{
+	static bool done = false;
+	if (!done) {
+		(void)hipFuncSetAttribute((const void *)vg_count_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, VG_LDS_LIMIT);
+		(void)hipFuncSetAttribute((const void *)vg_write_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, VG_LDS_LIMIT);
+		done = true;
+	}
+}
+
+/* Workgroups per pass: enough to fill every CU at the occupancy the write
+ * pass's LDS footprint allows, never more than there are tiles. */
+static uint32_t grid_for(const vg_prog *P, uint32_t num_tiles)
+{
+	allow_lds();
 	int per_cu = 0;
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, vg_write_kernel<R, 256>, 256,
-			lds_bytes<R>(P)) != hipSuccess || per_cu < 1)
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, vg_write_kernel, 256,
+			write_lds_bytes(P)) != hipSuccess || per_cu < 1)
 		per_cu = 1;
-	uint32_t g = (uint32_t)cus * (uint32_t)per_cu;
+	uint32_t g = (uint32_t)num_cus() * (uint32_t)per_cu;
 	return g < num_tiles ? g : num_tiles;
 }
 
-template <int R>
-static int launch_r(const vg_prog *P, vg_launch *L, hipStream_t stream)
-{
-	const uint32_t g = grid_for<R>(P, L->num_tiles);
-	L->tiles_per_wg = (L->num_tiles + g - 1) / g;
-	const uint32_t grid = (L->num_tiles + L->tiles_per_wg - 1) / L->tiles_per_wg;
-	const size_t lds = lds_bytes<R>(P);
-	hipLaunchKernelGGL((vg_count_kernel<R, 256>), dim3(grid), dim3(256), lds, stream, *P, *L);
-	if (hipGetLastError() != hipSuccess)
-		return -2;
-	hipLaunchKernelGGL((vg_write_kernel<R, 256>), dim3(grid), dim3(256), lds, stream, *P, *L);
-	return hipGetLastError() == hipSuccess ? 0 : -2;
-}
-
-/* Number of workgroups a launch of this program over num_tiles tiles uses
- * (the size of the per-workgroup count array). */
-extern "C" uint32_t vg_kernel_grid(const vg_prog *P, int rows_per_lane, uint32_t num_tiles)
+/* Number of workgroups a launch uses (the size of the per-workgroup count
+ * array). */
+extern "C" uint32_t vg_kernel_grid(const vg_prog *P, uint32_t num_tiles)
 {
 	if (num_tiles == 0)
 		return 0;
-	uint32_t g;
-	switch (rows_per_lane) {
-	case 1: g = grid_for<1>(P, num_tiles); break;
-	case 2: g = grid_for<2>(P, num_tiles); break;
-	default: g = grid_for<4>(P, num_tiles); break;
-	}
+	const uint32_t g = grid_for(P, num_tiles);
 	const uint32_t per = (num_tiles + g - 1) / g;
 	return (num_tiles + per - 1) / per;
 }
 
-extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, int rows_per_lane,
-	hipStream_t stream)
+/* 0 if the program's register file fits the LDS of both passes */
+extern "C" int vg_kernel_check(const vg_prog *P)
+{
+	return vg_kernel_lds_bytes(P) > VG_LDS_LIMIT ? -3 : 0;
+}
+
+extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stream)
 {
 	if (L->num_tiles == 0)
 		return 0;
-	if (vg_kernel_lds_bytes(P, rows_per_lane) > VG_LDS_LIMIT)
+	if (vg_kernel_check(P) != 0)
 		return -3;
-	switch (rows_per_lane) {
-	case 1: return launch_r<1>(P, L, stream);
-	case 2: return launch_r<2>(P, L, stream);
-	case 4: return launch_r<4>(P, L, stream);
-	}
-	return -1;
+	const uint32_t g = grid_for(P, L->num_tiles);
+	L->tiles_per_wg = (L->num_tiles + g - 1) / g;
+	const uint32_t grid = (L->num_tiles + L->tiles_per_wg - 1) / L->tiles_per_wg;
+	hipLaunchKernelGGL(vg_count_kernel, dim3(grid), dim3(256), count_lds_bytes(P), stream, *P, *L);
+	if (hipGetLastError() != hipSuccess)
+		return -2;
+	hipLaunchKernelGGL(vg_write_kernel, dim3(grid), dim3(256), write_lds_bytes(P), stream, *P, *L);
+	return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-extern "C" uint32_t vg_kernel_tile_rows(int rows_per_lane)
+extern "C" uint32_t vg_kernel_tile_rows(void)
 {
-	return 256u * (uint32_t)rows_per_lane;
+	return VG_T;
 }
