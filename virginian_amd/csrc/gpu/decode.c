@@ -542,6 +542,40 @@ int vg_decode(const virg_op *stmt, unsigned num_ops, unsigned pc,
 	P->nuop = (uint32_t)nu;
 	P->nfilter = (uint32_t)nfilter;
 
+	/* Wait slots.  A row that takes a branch keeping its validity skips
+	 * the micro-ops up to the target (the reference's row_pc jumps
+	 * forward, virginia.c:59-62) and resumes there; rows that lose
+	 * validity never reach the result again, so only keep-branches need a
+	 * slot.  Branches with the same target share one; a slot is free
+	 * again once its target is reached. */
+	{
+		int slot_target[VG_NWAIT];
+		for (int j = 0; j < VG_NWAIT; j++)
+			slot_target[j] = -1;
+		for (int k = 0; k < nu; k++) {
+			vg_uop *o = &P->uop[k];
+			for (int j = 0; j < VG_NWAIT; j++)
+				if (slot_target[j] == k) {
+					o->join |= (uint8_t)(1u << j);
+					slot_target[j] = -1;
+				}
+			if (o->kind > VG_K_NOT || !(o->flags & VG_F_KEEP))
+				continue;
+			int s = -1;
+			for (int j = 0; j < VG_NWAIT && s < 0; j++)
+				if (slot_target[j] == (int)o->target)
+					s = j;
+			for (int j = 0; j < VG_NWAIT && s < 0; j++)
+				if (slot_target[j] < 0)
+					s = j;
+			if (s < 0)
+				return derr(err, errlen, "op %d: more than %d forward-jump targets pending at once",
+					o->src_op, VG_NWAIT);
+			o->wslot = (uint8_t)s;
+			slot_target[s] = o->target;
+		}
+	}
+
 	/* outputs */
 	for (unsigned j = 0; j < nres; j++) {
 		int r = res_p1 + (int)j;

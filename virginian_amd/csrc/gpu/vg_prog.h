@@ -57,9 +57,16 @@ typedef struct {
 	uint8_t  a, a_hi;
 	uint8_t  b, b_hi;  /* for CAST: b = destination type */
 	uint16_t src_op;   /* op index in the original program (diagnostics) */
-	uint32_t pad;
+	uint8_t  wslot;    /* branch with VG_F_KEEP: wait slot of the rows it takes */
+	uint8_t  join;     /* bit j: rows waiting in slot j become active here */
+	uint16_t pad;
 	uint64_t ia, ib;   /* immediates (bit patterns, 32-bit types in the low word) */
 } vg_uop;              /* 32 bytes */
+
+/* Rows that take a forward branch which keeps their validity wait until
+ * the branch target; at most VG_NWAIT different targets can be pending at
+ * once (the decoder assigns the slots). */
+#define VG_NWAIT 4
 
 /* column loads */
 enum { VG_PHASE_FILTER = 0, VG_PHASE_PROJECT = 1, VG_PHASE_EMIT = 2 };

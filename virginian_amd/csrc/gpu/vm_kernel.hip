@@ -78,6 +78,15 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T *gptr(const
 	return (const __attribute__((address_space(1))) T *)p;
 }
 
+/* sum of x over the wave, in every lane */
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1)
+		x += __shfl_xor(x, off, 64);
+	return x;
+}
+
 __device__ __forceinline__ uint32_t rank_below(uint64_t m)
 {
 	return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
@@ -215,88 +224,150 @@ __device__ __forceinline__ void operand(const uint32_t *regs, bool imm, uint64_t
 	}
 }
 
-/* write R typed values to a slot; masked writes keep inactive rows */
-template <typename T, int R, uint32_t TR>
-__device__ __forceinline__ void result_put(uint32_t *regs, uint32_t lo, uint32_t hi, uint32_t lrow,
-	const T (&x)[R], bool masked, uint32_t k, const uint32_t (&resume)[R])
+/* ------------------------------------------------------ strips of rows */
+/* A lane owns NS strips of 4 consecutive rows.  Strip q of a lane starts
+ * at row wbase + q*stride + 4*lane of the tile: wbase and stride are
+ * wave-uniform, so a wave's strip q is 256 consecutive rows, which is one
+ * 16-byte-per-lane vector access both in HBM and in the LDS register
+ * file ([slot][row of the tile]). */
+template <int NS>
+struct strips {
+	uint32_t wbase, stride, lane;
+	__device__ __forceinline__ uint32_t row(int q) const { return wbase + (uint32_t)q * stride + 4 * lane; }
+	__device__ __forceinline__ uint32_t ubase(int q) const { return wbase + (uint32_t)q * stride; }
+};
+
+/* ------------------------------------------------------------ row state */
+/* The rows of a lane as bits: bit 4q+j is row j of strip q.  `active`
+ * rows execute the micro-op at hand (the reference's row_pc == pc).  A row
+ * that takes a forward branch keeping its validity waits in wait[slot]
+ * until the branch target, where it is active again.  A row that loses
+ * validity can never reach the result (validity only ever drops,
+ * virginia.c:59-62), so it leaves `active` for good and what it would
+ * compute afterwards is irrelevant.  With rows as bit masks every
+ * micro-op is straight-line vector code: no per-row branches, hence no
+ * exec-mask bookkeeping on the scalar unit. */
+struct rowstate {
+	uint32_t valid, active;
+	uint32_t wait[VG_NWAIT];
+};
+
+__device__ __forceinline__ void rows_init(rowstate &RS, uint32_t valid)
 {
-	rv<R> l, h;
-	if (masked) {
-		l = lds_get<R>(regs + lo * TR + lrow);
-		if constexpr (vt<T>::wide)
-			h = lds_get<R>(regs + hi * TR + lrow);
-	}
+	RS.valid = RS.active = valid;
 #pragma unroll
-	for (int r = 0; r < R; ++r) {
-		if (!masked || resume[r] <= k) {
+	for (int j = 0; j < VG_NWAIT; ++j)
+		RS.wait[j] = 0;
+}
+
+/* ------------------------------------------------------------- micro-ops */
+/* condition bits of a compare / logic micro-op over the lane's rows */
+template <typename T, int NS, uint32_t TR, bool USE_B, typename F>
+__device__ __forceinline__ uint32_t cond_strips(const vg_uop &u, const uint32_t *regs,
+	const strips<NS> &S, F f)
+{
+	uint32_t c = 0;
+#pragma unroll
+	for (int q = 0; q < NS; ++q) {
+		T a[4], b[4];
+		operand<T, 4, TR>(regs, u.flags & VG_F_AIMM, u.ia, u.a, u.a_hi, S.row(q), a);
+		if (USE_B)
+			operand<T, 4, TR>(regs, u.flags & VG_F_BIMM, u.ib, u.b, u.b_hi, S.row(q), b);
+#pragma unroll
+		for (int r = 0; r < 4; ++r)
+			c |= f(a[r], USE_B ? b[r] : a[r]) ? 1u << (4 * q + r) : 0u;
+	}
+	return c;
+}
+
+template <typename T, int NS, uint32_t TR>
+__device__ __forceinline__ uint32_t cond_typed(const vg_uop &u, const uint32_t *regs, const strips<NS> &S)
+{
+	switch (u.kind) {
+	case VG_K_LT: return cond_strips<T, NS, TR, true>(u, regs, S, [](T x, T y) { return x < y; });
+	case VG_K_LE: return cond_strips<T, NS, TR, true>(u, regs, S, [](T x, T y) { return x <= y; });
+	case VG_K_GT: return cond_strips<T, NS, TR, true>(u, regs, S, [](T x, T y) { return x > y; });
+	case VG_K_GE: return cond_strips<T, NS, TR, true>(u, regs, S, [](T x, T y) { return x >= y; });
+	case VG_K_EQ: return cond_strips<T, NS, TR, true>(u, regs, S, [](T x, T y) { return x == y; });
+	case VG_K_NE: return cond_strips<T, NS, TR, true>(u, regs, S, [](T x, T y) { return x != y; });
+	case VG_K_AND: return cond_strips<T, NS, TR, true>(u, regs, S, [](T x, T y) { return nz(x) && nz(y); });
+	case VG_K_OR: return cond_strips<T, NS, TR, true>(u, regs, S, [](T x, T y) { return nz(x) || nz(y); });
+	default: return cond_strips<T, NS, TR, false>(u, regs, S, [](T x, T) { return !nz(x); });
+	}
+}
+
+/* write 4 typed values of strip row r0 to a slot; a masked write keeps
+ * the old value of every row that is not active (bits `act`) */
+template <typename T, uint32_t TR>
+__device__ __forceinline__ void result_put(uint32_t *regs, uint32_t lo, uint32_t hi, uint32_t r0,
+	const T (&x)[4], bool masked, uint32_t act)
+{
+	rv<4> l, h;
+	if (masked) {
+		l = lds_get<4>(regs + lo * TR + r0);
+		if constexpr (vt<T>::wide)
+			h = lds_get<4>(regs + hi * TR + r0);
+#pragma unroll
+		for (int r = 0; r < 4; ++r) {
+			const bool on = (act >> r) & 1u;
+			l.v[r] = on ? vt<T>::lo(x[r]) : l.v[r];
+			if constexpr (vt<T>::wide)
+				h.v[r] = on ? vt<T>::hi(x[r]) : h.v[r];
+		}
+	} else {
+#pragma unroll
+		for (int r = 0; r < 4; ++r) {
 			l.v[r] = vt<T>::lo(x[r]);
 			if constexpr (vt<T>::wide)
 				h.v[r] = vt<T>::hi(x[r]);
 		}
 	}
-	lds_put<R>(regs + lo * TR + lrow, l);
+	lds_put<4>(regs + lo * TR + r0, l);
 	if constexpr (vt<T>::wide)
-		lds_put<R>(regs + hi * TR + lrow, h);
+		lds_put<4>(regs + hi * TR + r0, h);
 }
 
-/* ------------------------------------------------------------- micro-ops */
-template <typename T, int R, uint32_t TR>
-__device__ __forceinline__ void exec_typed(const vg_uop &u, uint32_t k, uint32_t *regs,
-	uint32_t lrow, uint32_t (&resume)[R], bool (&valid)[R])
+template <typename T, int NS, uint32_t TR, typename F>
+__device__ __forceinline__ void math_strips(const vg_uop &u, uint32_t *regs, const strips<NS> &S,
+	uint32_t active, F f)
 {
-	T a[R], b[R], x[R];
-	operand<T, R, TR>(regs, u.flags & VG_F_AIMM, u.ia, u.a, u.a_hi, lrow, a);
-	const uint32_t kind = u.kind;
-	if (kind <= VG_K_NOT) {
-		if (kind != VG_K_NOT)
-			operand<T, R, TR>(regs, u.flags & VG_F_BIMM, u.ib, u.b, u.b_hi, lrow, b);
-		const bool keep = u.flags & VG_F_KEEP;
+	const bool masked = u.flags & VG_F_MASKED;
 #pragma unroll
-		for (int r = 0; r < R; ++r) {
-			bool c;
-			switch (kind) {
-			case VG_K_LT: c = a[r] < b[r]; break;
-			case VG_K_LE: c = a[r] <= b[r]; break;
-			case VG_K_GT: c = a[r] > b[r]; break;
-			case VG_K_GE: c = a[r] >= b[r]; break;
-			case VG_K_EQ: c = a[r] == b[r]; break;
-			case VG_K_NE: c = a[r] != b[r]; break;
-			case VG_K_AND: c = nz(a[r]) && nz(b[r]); break;
-			case VG_K_OR: c = nz(a[r]) || nz(b[r]); break;
-			default: c = !nz(a[r]); break;
-			}
-			if (resume[r] <= k && c) {
-				valid[r] = valid[r] && keep;
-				resume[r] = u.target;
-			}
-		}
-		return;
+	for (int q = 0; q < NS; ++q) {
+		T a[4], b[4], x[4];
+		operand<T, 4, TR>(regs, u.flags & VG_F_AIMM, u.ia, u.a, u.a_hi, S.row(q), a);
+		operand<T, 4, TR>(regs, u.flags & VG_F_BIMM, u.ib, u.b, u.b_hi, S.row(q), b);
+#pragma unroll
+		for (int r = 0; r < 4; ++r)
+			x[r] = f(a[r], b[r]);
+		result_put<T, TR>(regs, u.dst, u.dst_hi, S.row(q), x, masked, active >> (4 * q));
 	}
-	if (kind != VG_K_MOV)
-		operand<T, R, TR>(regs, u.flags & VG_F_BIMM, u.ib, u.b, u.b_hi, lrow, b);
-	switch (kind) {
-	case VG_K_ADD:
+}
+
+template <typename T, int NS, uint32_t TR>
+__device__ __forceinline__ void mov_strips(const vg_uop &u, uint32_t *regs, const strips<NS> &S,
+	uint32_t active)
+{
+	const bool masked = u.flags & VG_F_MASKED;
 #pragma unroll
-		for (int r = 0; r < R; ++r) x[r] = vt<T>::add(a[r], b[r]);
-		break;
-	case VG_K_SUB:
-#pragma unroll
-		for (int r = 0; r < R; ++r) x[r] = vt<T>::sub(a[r], b[r]);
-		break;
-	case VG_K_MUL:
-#pragma unroll
-		for (int r = 0; r < R; ++r) x[r] = vt<T>::mul(a[r], b[r]);
-		break;
-	case VG_K_DIV:
-#pragma unroll
-		for (int r = 0; r < R; ++r) x[r] = vt<T>::div(a[r], b[r]);
-		break;
-	default:
-#pragma unroll
-		for (int r = 0; r < R; ++r) x[r] = a[r];
-		break;
+	for (int q = 0; q < NS; ++q) {
+		T a[4];
+		operand<T, 4, TR>(regs, u.flags & VG_F_AIMM, u.ia, u.a, u.a_hi, S.row(q), a);
+		result_put<T, TR>(regs, u.dst, u.dst_hi, S.row(q), a, masked, active >> (4 * q));
 	}
-	result_put<T, R, TR>(regs, u.dst, u.dst_hi, lrow, x, u.flags & VG_F_MASKED, k, resume);
+}
+
+template <typename T, int NS, uint32_t TR>
+__device__ __forceinline__ void math_typed(const vg_uop &u, uint32_t *regs, const strips<NS> &S,
+	uint32_t active)
+{
+	switch (u.kind) {
+	case VG_K_ADD: math_strips<T, NS, TR>(u, regs, S, active, [](T x, T y) { return vt<T>::add(x, y); }); break;
+	case VG_K_SUB: math_strips<T, NS, TR>(u, regs, S, active, [](T x, T y) { return vt<T>::sub(x, y); }); break;
+	case VG_K_MUL: math_strips<T, NS, TR>(u, regs, S, active, [](T x, T y) { return vt<T>::mul(x, y); }); break;
+	case VG_K_DIV: math_strips<T, NS, TR>(u, regs, S, active, [](T x, T y) { return vt<T>::div(x, y); }); break;
+	default: mov_strips<T, NS, TR>(u, regs, S, active); break;
+	}
 }
 
 template <typename S, typename D>
@@ -311,82 +382,99 @@ template <> __device__ __forceinline__ chr convert<int64_t, chr>(int64_t x) { re
 template <> __device__ __forceinline__ chr convert<float, chr>(float x) { return chr{(int32_t)(int8_t)(int32_t)x}; }
 template <> __device__ __forceinline__ chr convert<double, chr>(double x) { return chr{(int32_t)(int8_t)(int32_t)x}; }
 
-template <typename S, typename D, int R, uint32_t TR>
-__device__ __forceinline__ void cast_rows(const vg_uop &u, uint32_t k, uint32_t *regs, uint32_t lrow,
-	const uint32_t (&resume)[R])
+/* Cast converts the register in place for the rows at this op
+ * (virginia.c:682-862 tests row_pc <= pc, which only active rows meet) */
+template <typename Sx, typename Dx, int NS, uint32_t TR>
+__device__ __forceinline__ void cast_strips(const vg_uop &u, uint32_t *regs, const strips<NS> &S,
+	uint32_t active)
 {
-	S a[R];
-	D x[R];
-	operand<S, R, TR>(regs, false, 0, u.a, u.a_hi, lrow, a);
 #pragma unroll
-	for (int r = 0; r < R; ++r)
-		x[r] = convert<S, D>(a[r]);
-	result_put<D, R, TR>(regs, u.dst, u.dst_hi, lrow, x, true, k, resume);
-}
-
-template <typename S, int R, uint32_t TR>
-__device__ __forceinline__ void cast_from(const vg_uop &u, uint32_t k, uint32_t *regs, uint32_t lrow,
-	const uint32_t (&resume)[R])
-{
-	switch (u.b) {
-	case VG_T_INT: cast_rows<S, int32_t, R, TR>(u, k, regs, lrow, resume); break;
-	case VG_T_INT64: cast_rows<S, int64_t, R, TR>(u, k, regs, lrow, resume); break;
-	case VG_T_FLOAT: cast_rows<S, float, R, TR>(u, k, regs, lrow, resume); break;
-	case VG_T_DOUBLE: cast_rows<S, double, R, TR>(u, k, regs, lrow, resume); break;
-	case VG_T_CHAR: cast_rows<S, chr, R, TR>(u, k, regs, lrow, resume); break;
+	for (int q = 0; q < NS; ++q) {
+		Sx a[4];
+		Dx x[4];
+		operand<Sx, 4, TR>(regs, false, 0, u.a, u.a_hi, S.row(q), a);
+#pragma unroll
+		for (int r = 0; r < 4; ++r)
+			x[r] = convert<Sx, Dx>(a[r]);
+		result_put<Dx, TR>(regs, u.dst, u.dst_hi, S.row(q), x, true, active >> (4 * q));
 	}
 }
 
-/* ------------------------------------------------------ strips of rows */
-/* A lane owns NS strips of 4 consecutive rows.  Strip q of a lane starts
- * at row wbase + q*stride + 4*lane of the tile: wbase and stride are
- * wave-uniform, so a wave's strip q is 256 consecutive rows, which is one
- * 16-byte-per-lane vector access both in HBM and in the LDS register
- * file ([slot][row of the tile]). */
-template <int NS>
-struct strips {
-	uint32_t wbase, stride, lane;
-	__device__ __forceinline__ uint32_t row(int q) const { return wbase + (uint32_t)q * stride + 4 * lane; }
-	__device__ __forceinline__ uint32_t ubase(int q) const { return wbase + (uint32_t)q * stride; }
-};
+template <typename Sx, int NS, uint32_t TR>
+__device__ __forceinline__ void cast_from(const vg_uop &u, uint32_t *regs, const strips<NS> &S,
+	uint32_t active)
+{
+	switch (u.b) {
+	case VG_T_INT: cast_strips<Sx, int32_t, NS, TR>(u, regs, S, active); break;
+	case VG_T_INT64: cast_strips<Sx, int64_t, NS, TR>(u, regs, S, active); break;
+	case VG_T_FLOAT: cast_strips<Sx, float, NS, TR>(u, regs, S, active); break;
+	case VG_T_DOUBLE: cast_strips<Sx, double, NS, TR>(u, regs, S, active); break;
+	case VG_T_CHAR: cast_strips<Sx, chr, NS, TR>(u, regs, S, active); break;
+	}
+}
 
+/* Micro-ops [kbeg, kend) whose bit is set in `mask`, over the lane's
+ * rows.  Dispatch is scalar: every field of a micro-op is wave-uniform. */
 template <int NS, uint32_t TR>
 __device__ __forceinline__ void run_uops(const vg_prog &P, uint32_t kbeg, uint32_t kend, uint64_t mask,
-	uint32_t *regs, const strips<NS> &S, uint32_t (&resume)[NS * 4], bool (&valid)[NS * 4])
+	uint32_t *regs, const strips<NS> &S, rowstate &RS)
 {
 	for (uint32_t k = kbeg; k < kend; ++k) {
 		if (!((mask >> k) & 1ull))
 			continue;
 		const vg_uop u = ldc_struct(&P.uop[k]);
+		if (u.join) {
 #pragma unroll
-		for (int q = 0; q < NS; ++q) {
-			uint32_t (&res4)[4] = *(uint32_t (*)[4])(resume + 4 * q);
-			bool (&val4)[4] = *(bool (*)[4])(valid + 4 * q);
-			const uint32_t r0 = S.row(q);
-			if (u.kind == VG_K_INVALID) {
-#pragma unroll
-				for (int r = 0; r < 4; ++r)
-					if (res4[r] <= k)
-						val4[r] = false;
-				continue;
-			}
-			if (u.kind == VG_K_CAST) {
-				switch (u.type) {
-				case VG_T_INT: cast_from<int32_t, 4, TR>(u, k, regs, r0, res4); break;
-				case VG_T_INT64: cast_from<int64_t, 4, TR>(u, k, regs, r0, res4); break;
-				case VG_T_FLOAT: cast_from<float, 4, TR>(u, k, regs, r0, res4); break;
-				case VG_T_DOUBLE: cast_from<double, 4, TR>(u, k, regs, r0, res4); break;
-				case VG_T_CHAR: cast_from<chr, 4, TR>(u, k, regs, r0, res4); break;
+			for (int j = 0; j < VG_NWAIT; ++j)
+				if ((u.join >> j) & 1u) {
+					RS.active |= RS.wait[j];
+					RS.wait[j] = 0;
 				}
-				continue;
-			}
+		}
+		if (u.kind <= VG_K_NOT) {
+			uint32_t c = 0;
 			switch (u.type) {
-			case VG_T_INT: exec_typed<int32_t, 4, TR>(u, k, regs, r0, res4, val4); break;
-			case VG_T_INT64: exec_typed<int64_t, 4, TR>(u, k, regs, r0, res4, val4); break;
-			case VG_T_FLOAT: exec_typed<float, 4, TR>(u, k, regs, r0, res4, val4); break;
-			case VG_T_DOUBLE: exec_typed<double, 4, TR>(u, k, regs, r0, res4, val4); break;
-			case VG_T_CHAR: exec_typed<chr, 4, TR>(u, k, regs, r0, res4, val4); break;
+			case VG_T_INT: c = cond_typed<int32_t, NS, TR>(u, regs, S); break;
+			case VG_T_INT64: c = cond_typed<int64_t, NS, TR>(u, regs, S); break;
+			case VG_T_FLOAT: c = cond_typed<float, NS, TR>(u, regs, S); break;
+			case VG_T_DOUBLE: c = cond_typed<double, NS, TR>(u, regs, S); break;
+			case VG_T_CHAR: c = cond_typed<chr, NS, TR>(u, regs, S); break;
 			}
+			/* taken rows jump to the target: kept rows wait for it,
+			 * the others lose validity (virginia.c:59-62) */
+			const uint32_t taken = c & RS.active;
+			RS.active &= ~taken;
+			if (u.flags & VG_F_KEEP) {
+#pragma unroll
+				for (int j = 0; j < VG_NWAIT; ++j)
+					if (u.wslot == j)
+						RS.wait[j] |= taken;
+			} else {
+				RS.valid &= ~taken;
+			}
+			continue;
+		}
+		if (u.kind == VG_K_INVALID) {
+			RS.valid &= ~RS.active;
+			RS.active = 0;
+			continue;
+		}
+		if (u.kind == VG_K_CAST) {
+			switch (u.type) {
+			case VG_T_INT: cast_from<int32_t, NS, TR>(u, regs, S, RS.active); break;
+			case VG_T_INT64: cast_from<int64_t, NS, TR>(u, regs, S, RS.active); break;
+			case VG_T_FLOAT: cast_from<float, NS, TR>(u, regs, S, RS.active); break;
+			case VG_T_DOUBLE: cast_from<double, NS, TR>(u, regs, S, RS.active); break;
+			case VG_T_CHAR: cast_from<chr, NS, TR>(u, regs, S, RS.active); break;
+			}
+			continue;
+		}
+		switch (u.type) {
+		case VG_T_INT: math_typed<int32_t, NS, TR>(u, regs, S, RS.active); break;
+		case VG_T_INT64: math_typed<int64_t, NS, TR>(u, regs, S, RS.active); break;
+		case VG_T_FLOAT: math_typed<float, NS, TR>(u, regs, S, RS.active); break;
+		case VG_T_DOUBLE: math_typed<double, NS, TR>(u, regs, S, RS.active); break;
+		case VG_T_CHAR: math_typed<chr, NS, TR>(u, regs, S, RS.active); break;
 		}
 	}
 }
@@ -451,23 +539,97 @@ __device__ __forceinline__ void load_col(const vg_col &c, const char *base, cons
 	}
 }
 
+/* rows per tile */
+#define VG_T 1024u
+
+/* Wait for every vector memory access of this wave (loads into VGPRs and
+ * into LDS alike).  Through the intrinsic the compiler's own wait
+ * bookkeeping sees that nothing is outstanding afterwards; the empty asm
+ * keeps memory accesses from being moved across it. */
+__device__ __forceinline__ void wait_vmem()
+{
+	__builtin_amdgcn_s_waitcnt(0x0F70);   /* vmcnt(0); expcnt, lgkmcnt unconstrained */
+	asm volatile("" ::: "memory");
+}
+
+/* ------------------------------------------------------- tablet cursor */
+/* Where a wave's current tile lives, kept across tiles so that the chain
+ * tile table -> tablet descriptor -> column address is walked once per
+ * tablet instead of once per tile.  Lane k of cd_lo/cd_hi holds column
+ * descriptor P.col[k] (8 bytes) and lane k of cb_lo/cb_hi the address of
+ * that column's row 0 in the current tablet; readlane with a wave-uniform
+ * k brings either into SGPRs without a memory access. */
+struct cursor {
+	uint32_t first, end, rows, aligned;   /* tiles [first, end) of the tablet */
+	uint32_t cd_lo, cd_hi, cb_lo, cb_hi;  /* per-lane: column k */
+};
+
+__device__ __forceinline__ void cursor_init(cursor &C, const vg_prog &P, uint32_t lane)
+{
+	C.first = C.end = C.rows = C.aligned = 0;
+	C.cd_lo = C.cd_hi = C.cb_lo = C.cb_hi = 0;
+	if (lane < P.ncol) {
+		const uint32_t *w = (const uint32_t *)&P.col[lane];
+		C.cd_lo = w[0];
+		C.cd_hi = w[1];
+	}
+}
+
+/* make `tile` the cursor's tile (tiles only move forward) */
+__device__ __forceinline__ void cursor_seek(cursor &C, const vg_prog &P, const vg_launch &L,
+	uint32_t tile, uint32_t lane)
+{
+	if (tile < C.end)
+		return;
+	const uint32_t tb = ldc(&L.tile_tab[tile]);
+	const vg_tabdesc *D = &L.tabs[tb];
+	C.first = ldc(&D->first_tile);
+	C.rows = ldc(&D->rows);
+	C.end = C.first + (C.rows + VG_T - 1) / VG_T;
+	C.aligned = ldc(&D->aligned);
+	uint64_t b = 0;
+	if (lane < P.ncol)
+		b = gptr<uint64_t>(&D->col[0])[C.cd_lo & 0xFFu];   /* vg_col.src is byte 0 */
+	C.cb_lo = (uint32_t)b;
+	C.cb_hi = (uint32_t)(b >> 32);
+	wait_vmem();
+}
+
+__device__ __forceinline__ vg_col cursor_col(const cursor &C, uint32_t k)
+{
+	union {
+		vg_col c;
+		uint32_t w[2];
+	} u;
+	u.w[0] = __builtin_amdgcn_readlane(C.cd_lo, k);
+	u.w[1] = __builtin_amdgcn_readlane(C.cd_hi, k);
+	return u.c;
+}
+
+__device__ __forceinline__ const char *cursor_base(const cursor &C, uint32_t k)
+{
+	/* readlane yields an int: widen both halves unsigned, or a low word
+	 * with bit 31 set would be sign-extended over the high word */
+	const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(C.cb_lo, k);
+	const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(C.cb_hi, k);
+	return (const char *)(((uint64_t)hi << 32) | (uint64_t)lo);
+}
+
 template <int NS, uint32_t TR>
-__device__ __forceinline__ void load_cols(const vg_prog &P, const vg_tabdesc *D, uint32_t row0,
+__device__ __forceinline__ void load_cols(const vg_prog &P, const cursor &C, uint32_t row0,
 	const strips<NS> &S, uint32_t nrows, bool write_pass, uint32_t *regs)
 {
-	const bool aligned = ldc(&D->aligned) != 0;
+	const bool aligned = C.aligned != 0;
 	for (uint32_t k = 0; k < P.nload; ++k) {
-		const vg_col c = ldc_struct(&P.col[k]);
+		const vg_col c = cursor_col(C, k);
 		if (write_pass ? !c.wload : c.phase != VG_PHASE_FILTER)
 			continue;
-		const char *base = (const char *)ldc(&D->col[c.src]) + (size_t)row0 * c.width;
+		const char *base = cursor_base(C, k) + (size_t)row0 * c.width;
 		load_col<NS, TR>(c, base, S, nrows, aligned, regs);
 	}
 	/* The compiler does not count global_load_lds as a write of the LDS
-	 * it later reads, so it inserts no wait before those reads: wait for
-	 * every vector load of this wave here (and keep memory accesses from
-	 * moving across the wait). */
-	asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+	 * it later reads, so it inserts no wait before those reads. */
+	wait_vmem();
 }
 
 /* ----------------------------------------------------------- count pass */
@@ -480,7 +642,6 @@ __device__ __forceinline__ void load_cols(const vg_prog &P, const vg_tabdesc *D,
  * tiles [g*tiles_per_wg, (g+1)*tiles_per_wg), its waves taking every
  * fourth tile, and stores the workgroup's total.
  * LDS: one register file of nslot_count * T words per wave. */
-#define VG_T 1024u
 __global__ __launch_bounds__(256) void vg_count_kernel(const vg_prog P, const vg_launch L)
 {
 	constexpr uint32_t T = VG_T;
@@ -492,40 +653,41 @@ __global__ __launch_bounds__(256) void vg_count_kernel(const vg_prog P, const vg
 	const uint32_t g = blockIdx.x;
 	const uint32_t t0 = g * L.tiles_per_wg;
 	const uint32_t t1 = min(t0 + L.tiles_per_wg, L.num_tiles);
+	uint16_t *vmask = (uint16_t *)L.vbits;
 	uint32_t count = 0;
+	cursor C;
+	cursor_init(C, P, lane);
 	for (uint32_t tile = t0 + wave; tile < t1; tile += 4) {
-		const uint32_t tb = ldc(&L.tile_tab[tile]);
-		const vg_tabdesc *D = &L.tabs[tb];
-		const uint32_t row0 = (tile - ldc(&D->first_tile)) * T;
-		const uint32_t left = ldc(&D->rows) - row0;
+		cursor_seek(C, P, L, tile, lane);
+		const uint32_t row0 = (tile - C.first) * T;
+		const uint32_t left = C.rows - row0;
 		const uint32_t nrows = left < T ? left : T;
-		uint32_t resume[16];
-		bool valid[16];
+		uint32_t live = 0xFFFFu;
+		if (nrows < T) {
+			live = 0;
 #pragma unroll
-		for (int q = 0; q < 4; ++q)
+			for (int q = 0; q < 4; ++q)
 #pragma unroll
-			for (int j = 0; j < 4; ++j) {
-				resume[4 * q + j] = 0;
-				valid[4 * q + j] = S.row(q) + j < nrows;
-			}
-		/* a lane only ever touches its own rows of the register file */
-		load_cols<4, T>(P, D, row0, S, nrows, false, regs);
-		run_uops<4, T>(P, 0, P.nfilter, ~0ull, regs, S, resume, valid);
-		uint32_t cnt[4];
-#pragma unroll
-		for (int q = 0; q < 4; ++q) {
-			uint32_t byte = 0;
-			cnt[q] = 0;
-#pragma unroll
-			for (int j = 0; j < 4; ++j) {
-				byte |= valid[4 * q + j] ? 1u << j : 0u;
-				cnt[q] += (uint32_t)__builtin_popcountll(__ballot(valid[4 * q + j]));
-			}
-			L.vbits[(size_t)tile * 256 + 64 * q + lane] = (uint8_t)byte;
+				for (int j = 0; j < 4; ++j)
+					live |= S.row(q) + j < nrows ? 1u << (4 * q + j) : 0u;
 		}
+		rowstate RS;
+		rows_init(RS, live);
+		/* a lane only ever touches its own rows of the register file */
+		load_cols<4, T>(P, C, row0, S, nrows, false, regs);
+		run_uops<4, T>(P, 0, P.nfilter, ~0ull, regs, S, RS);
+		/* validity of write-pass lane (wave q, lane l) = bits [4q, 4q+4) of
+		 * this lane's mask; per-strip counts summed over the wave, two
+		 * 16-bit counts per word */
+		const uint32_t v = RS.valid;
+		vmask[(size_t)tile * 64 + lane] = (uint16_t)v;
+		const uint32_t n01 = wave_sum((uint32_t)__builtin_popcount(v & 0xFu) |
+			((uint32_t)__builtin_popcount(v & 0xF0u) << 16));
+		const uint32_t n23 = wave_sum((uint32_t)__builtin_popcount(v & 0xF00u) |
+			((uint32_t)__builtin_popcount(v & 0xF000u) << 16));
 		if (lane < 4)
-			L.tcount[(size_t)tile * 4 + lane] = lane == 0 ? cnt[0] : lane == 1 ? cnt[1] : lane == 2 ? cnt[2] : cnt[3];
-		count += cnt[0] + cnt[1] + cnt[2] + cnt[3];
+			L.tcount[(size_t)tile * 4 + lane] = ((lane < 2 ? n01 : n23) >> (16 * (lane & 1))) & 0xFFFFu;
+		count += (n01 & 0xFFFFu) + (n01 >> 16) + (n23 & 0xFFFFu) + (n23 >> 16);
 	}
 	if (lane == 0)
 		s_wcount[wave] = count;
@@ -574,6 +736,9 @@ __global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg
 	}
 	__syncthreads();
 	uint64_t chunk_base = s_base;
+	const uint16_t *vmask = (const uint16_t *)L.vbits;
+	cursor C;
+	cursor_init(C, P, lane);
 
 	for (uint32_t c0 = t0; c0 < t1; c0 += VG_CHUNK) {
 		const uint32_t nt = min((uint32_t)VG_CHUNK, t1 - c0);
@@ -603,27 +768,22 @@ __global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg
 				continue;
 			const uint32_t wave_off = (wave > 0 ? c0w : 0u) + (wave > 1 ? c1w : 0u) + (wave > 2 ? c2w : 0u);
 			const uint32_t tile = c0 + i;
-			const uint32_t mybits = L.vbits[(size_t)tile * 256 + tid];
-			const uint32_t tb = ldc(&L.tile_tab[tile]);
-			const vg_tabdesc *D = &L.tabs[tb];
-			const uint32_t row0 = (tile - ldc(&D->first_tile)) * T;
-			const uint32_t left = ldc(&D->rows) - row0;
+			const uint32_t mybits = (vmask[(size_t)tile * 64 + lane] >> (4 * wave)) & 0xFu;
+			cursor_seek(C, P, L, tile, lane);
+			const uint32_t row0 = (tile - C.first) * T;
+			const uint32_t left = C.rows - row0;
 			const uint32_t nrows = left < T ? left : T;
-			uint32_t resume[4];
-			bool valid[4];
-#pragma unroll
-			for (int r = 0; r < 4; ++r) {
-				resume[r] = 0;
-				valid[r] = (mybits >> r) & 1u;
-			}
+			rowstate RS;
+			rows_init(RS, mybits);
 			if (mybits)
-				load_cols<1, T>(P, D, row0, S, nrows, true, regs);
+				load_cols<1, T>(P, C, row0, S, nrows, true, regs);
 			if (P.wfull) {
-				/* rerun everything: resume points and validity come back
-				 * exactly as in the count pass */
-				run_uops<1, T>(P, 0, P.nuop, ~0ull, regs, S, resume, valid);
+				/* rerun everything: the selected rows take the same
+				 * branches as in the count pass, so masked writes see the
+				 * same active rows */
+				run_uops<1, T>(P, 0, P.nuop, ~0ull, regs, S, RS);
 			} else if (P.wneed) {
-				run_uops<1, T>(P, 0, P.nuop, P.wneed, regs, S, resume, valid);
+				run_uops<1, T>(P, 0, P.nuop, P.wneed, regs, S, RS);
 			}
 			/* ranks inside the wave, row numbers in rank order */
 			uint32_t pos = 0;
@@ -661,8 +821,8 @@ __global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg
 							((uint8_t *)dst)[at + k] = (uint8_t)lo[srcrow[k]];
 					}
 				} else if (o.kind == VG_OUT_COLUMN) {
-					const vg_col cc = ldc_struct(&P.col[o.col]);
-					const char *src = (const char *)ldc(&D->col[cc.src]) + (size_t)row0 * cc.width;
+					const vg_col cc = cursor_col(C, o.col);
+					const char *src = cursor_base(C, o.col) + (size_t)row0 * cc.width;
 					if (o.width == 8) {
 						for (uint32_t k = lane; k < n; k += 64)
 							((uint64_t *)dst)[at + k] = gptr<uint64_t>(src)[srcrow[k]];
