@@ -1,24 +1,32 @@
 """Benchmark: filter + math SELECT over a GPU-resident 64M-row table.
 
-Workload (the benchmark configuration the headline metric is quoted on):
-67,108,864 rows of the 7-column schema in 1M-row tablets, resident in HBM;
+Workload: 67,108,864 rows of the 7-column benchmark schema in 1M-row
+tablets, resident in HBM;
     select (uniformf * 2.0) + normalf5, uniformi - normali20,
            uniformf / (normalf20 + 1000.0) from test where uniformf < T
 for T in 0.1, 10, 50, 90 (selectivity 0.1 %, 10 %, 50 %, 90 %).  One step
-runs the four queries (each a single kernel launch over all 64 tablets,
-each on its own stream).  value = scanned bytes (referenced columns, full
-length) of all queries / wall time of the timed steps.
+runs the four queries, each one count-pass + write-pass launch pair over
+all 64 tablets on its own stream.  value = scanned bytes (referenced
+columns at full length) of all queries / wall time of the K timed steps.
 
-Also reported: per-query kernel time from HIP events on the query's own
-stream, the roofline of the kernel (algorithmic bytes read + written per
-launch / mean launch time, against 8 TB/s), and the CPU baseline: the
-oracle's multi-threaded VM on this host over a bounded sample.
+Also reported:
+  roofline      the query's kernel pair: algorithmic bytes per launch
+                (referenced columns x rows + selected rows x result bytes)
+                / mean launch-pair time from HIP events on the query's
+                stream, against 8 TB/s; per pass in config.per_query;
+  traffic       HBM bytes per launch pair measured with rocprofv3
+                FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a
+                device copy of known size (profiles/traffic_*.json), or null;
+  cpu_baseline  the oracle's multi-threaded CPU VM on this host's cores
+                over a bounded sample of the same workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --copy-probe 256     # known-size device copies (counter calibration)
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -30,6 +38,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "GB/s scanned (% HBM peak) for filter SELECT; p50/p99 latency, batched vs plain"
+THRESHOLDS = [0.1, 10.0, 50.0, 90.0]
 
 
 def sql_for(t: float) -> str:
@@ -37,21 +46,53 @@ def sql_for(t: float) -> str:
             f"uniformf / (normalf20 + 1000.0) as c from test where uniformf < {t!r}")
 
 
-def cpu_baseline(db, sql: str, seconds: float = 12.0) -> dict:
-    """Oracle multi-threaded CPU VM on this host's cores, repeated over the
-    table until about `seconds` of wall time have run."""
+def copy_probe(mib: int, reps: int = 5) -> None:
+    """Device-to-device copies of `mib` MiB through the HIP runtime: a
+    kernel of known traffic (read and write mib MiB) for calibrating the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), 3) != 0:     # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_probe_mib": mib, "reps": reps, "ms_per_copy": round(dt * 1e3, 3),
+                      "read_bytes": n, "write_bytes": n}))
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def cpu_baseline(db, sql: str, seconds: float) -> dict:
+    """The oracle's multi-threaded CPU VM (the reference's multi-core
+    scheduling: threads take whole tablets) over the table, repeated until
+    about `seconds` of wall time have run."""
     from oracle import oracle
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    rows_total = 0
+    threads = max(1, min(threads, 16))
     reps = 0
     t0 = time.perf_counter()
     while True:
-        cols, _ = oracle.select(db, sql, threads=threads, ordered=False, count_only=True)
+        oracle.select(db, sql, threads=threads, ordered=False, count_only=True)
         reps += 1
         if time.perf_counter() - t0 >= seconds:
             break
-    dt = time.perf_counter() - t0
-    return {"reps": reps, "seconds": dt, "threads": threads}
+    return {"reps": reps, "seconds": time.perf_counter() - t0, "threads": threads}
+
+
+def measured_traffic(workload: str) -> dict | None:
+    """HBM traffic per launch pair recorded from counter runs, if present."""
+    p = ROOT / "profiles" / f"traffic_{workload}.json"
+    if not p.exists():
+        return None
+    return json.loads(p.read_text())
 
 
 def main() -> int:
@@ -62,14 +103,18 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=64 << 20)
     ap.add_argument("--tablet-rows", type=int, default=1 << 20)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--rows-per-lane", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this executor runs on one GPU; multi-GPU sharding is described in "
-              "DESIGN.md and not launched here", file=sys.stderr)
+        print("bench.py: this build runs the executor on one GPU; figures for several GPUs are "
+              "not produced (tablets shard across GPUs by design, but no multi-GPU path is "
+              "launched)", file=sys.stderr)
         return 2
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return 0
 
     import virginian_amd as va
 
@@ -80,72 +125,76 @@ def main() -> int:
         return 1
     t0 = time.perf_counter()
     db.gen_table("test", a.rows, a.tablet_rows, seed=a.seed, threads=16)
-    gen_s = time.perf_counter() - t0
-    print(f"# generated {a.rows} rows in {gen_s:.1f}s on {name} ({cus} CUs)", file=sys.stderr, flush=True)
+    print(f"# generated {a.rows} rows in {time.perf_counter() - t0:.1f}s on {name} ({cus} CUs)",
+          file=sys.stderr, flush=True)
 
-    thresholds = [0.1, 10.0, 50.0, 90.0]
-    queries = [db.prepare(sql_for(t), rows_per_lane=a.rows_per_lane) for t in thresholds]
+    queries = [db.prepare(sql_for(t)) for t in THRESHOLDS]
     for q in queries:
         q.run()
 
-    # per-query kernel time (events on the query's stream, nothing else running)
+    # per query: launch-pair time back to back, and each pass on its own
     per = []
-    for t, q in zip(thresholds, queries):
+    for t, q in zip(THRESHOLDS, queries):
         ms, n = q.time(a.warmup, a.steps)
+        ms_c, ms_w, _ = q.time_passes(a.warmup, a.steps)
         read_b = q.rows * q.read_bytes_per_row
         write_b = n * q.write_bytes_per_row
         per.append({"threshold": t, "selected": n, "selectivity": n / q.rows, "kernel_ms": ms,
+                    "count_pass_ms": ms_c, "write_pass_ms": ms_w,
                     "read_bytes": read_b, "write_bytes": write_b,
                     "gbs_scanned": read_b / ms / 1e6, "gbs_total": (read_b + write_b) / ms / 1e6})
-        print(f"# T={t}: {n} rows ({n / q.rows:.4%}) {ms:.3f} ms/launch "
-              f"{read_b / ms / 1e6:.0f} GB/s scanned, {(read_b + write_b) / ms / 1e6:.0f} GB/s total",
-              file=sys.stderr, flush=True)
+        print(f"# T={t}: {n} rows ({n / q.rows:.4%}) {ms:.3f} ms/launch pair "
+              f"(count {ms_c:.3f} + write {ms_w:.3f}), {read_b / ms / 1e6:.0f} GB/s scanned, "
+              f"{(read_b + write_b) / ms / 1e6:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
 
-    # whole-job throughput: K steps of the 4-query sweep, all streams
-    import ctypes as C
+    # whole job: K steps of the 4-query sweep, one stream per query
+    def sync_all():
+        for q in queries:
+            n = C.c_ulonglong()
+            if va.lib.vg_query_sync(q.h, C.byref(n)) != va.VIRG_SUCCESS:
+                raise RuntimeError("query failed")
+
     for _ in range(a.warmup):
         for q in queries:
             va.lib.vg_query_launch(q.h)
-        for q in queries:
-            n = C.c_ulonglong()
-            va.lib.vg_query_sync(q.h, C.byref(n))
+        sync_all()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         for q in queries:
             if va.lib.vg_query_launch(q.h) != va.VIRG_SUCCESS:
                 raise RuntimeError("launch failed")
-    for q in queries:
-        n = C.c_ulonglong()
-        if va.lib.vg_query_sync(q.h, C.byref(n)) != va.VIRG_SUCCESS:
-            raise RuntimeError("query failed")
+    sync_all()
     wall = time.perf_counter() - t0
     scanned = sum(p["read_bytes"] for p in per) * a.steps
     value = scanned / wall / 1e9
-    ms_per_step = wall / a.steps * 1e3
 
-    sum_bytes = sum(p["read_bytes"] + p["write_bytes"] for p in per)
-    sum_ms = sum(p["kernel_ms"] for p in per)
-    achieved = sum_bytes / len(per) / (sum_ms / len(per)) / 1e6
+    mean_bytes = sum(p["read_bytes"] + p["write_bytes"] for p in per) / len(per)
+    mean_ms = sum(p["kernel_ms"] for p in per) / len(per)
+    achieved = mean_bytes / mean_ms / 1e6
+    tr = measured_traffic("config2_64M")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": tr["bytes_per_launch_pair_mean"] if tr else None,
+                "kernel": "vg_count_kernel + vg_write_kernel (one query)",
+                "algorithmic_bytes_per_launch": round(mean_bytes)}
 
     cpu = None
     if not a.no_cpu:
-        sample_sql = sql_for(10.0)
-        r = cpu_baseline(db, sample_sql, a.cpu_seconds)
-        rows = queries[0].rows
-        rb = queries[1].rows * queries[1].read_bytes_per_row
+        q10 = queries[1]
+        r = cpu_baseline(db, sql_for(10.0), a.cpu_seconds)
+        rb = q10.rows * q10.read_bytes_per_row
         cpu = {"value": round(rb * r["reps"] / r["seconds"] / 1e9, 3), "unit": "GB/s",
                "cores": r["threads"], "kind": "port",
-               "sample": f"T=10 query over the same {rows}-row table, {r['reps']} full passes in "
-                         f"{r['seconds']:.1f}s ({rows * r['reps'] / r['seconds']:.3e} rows/s)"}
+               "sample": f"the T=10 query over the same {q10.rows}-row resident table, "
+                         f"{r['reps']} full passes in {r['seconds']:.1f}s "
+                         f"({q10.rows * r['reps'] / r['seconds']:.3e} rows/s)"}
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "GB/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3),
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+i32",
-        "data": "synthetic (seeded Philox generator)",
-        "config": {"workload": "filter+math SELECT, 64M-row 7-column table GPU-resident in 1M-row "
+        "data": "synthetic (seeded Philox generator, 7-column benchmark schema)",
+        "config": {"workload": "filter+math SELECT, 64M-row 7-column table resident in HBM in 1M-row "
                                "tablets, selectivity sweep 0.1/10/50/90 %",
                    "rows": a.rows, "tablet_rows": a.tablet_rows, "queries_per_step": len(queries),
                    "pct_hbm_peak": round(value / HBM_PEAK_GBS * 100, 2),

@@ -54,6 +54,11 @@ int vg_query_fetch(vg_query *q, void **cols, unsigned long long max_rows);
  * `warmup` untimed ones, measured with HIP events on the query's stream. */
 int vg_query_time(vg_query *q, int warmup, int iters, double *ms_per_launch,
 	unsigned long long *rows);
+/* Mean milliseconds of the count pass and of the write pass over `iters`
+ * launches after `warmup` untimed ones, each launch bracketed by HIP
+ * events on the query's stream and waited for before the next. */
+int vg_query_time_passes(vg_query *q, int warmup, int iters, double *ms_count,
+	double *ms_write, unsigned long long *rows);
 /* Rows scanned, tiles, referenced-column bytes per row, result bytes per
  * selected row, result columns. */
 int vg_query_info(vg_query *q, unsigned long long *rows, unsigned *tiles,

@@ -118,12 +118,15 @@ API long long ref_run(const virg_op *ops, unsigned num_ops, void **out, unsigned
 	virg_tablet_meta *tab = NULL, *res;
 	unsigned i;
 	long long total = 0;
-	char names[VIRG_OPS][VIRG_MAX_COLUMN_NAME];
 	for (i = 0; i < num_ops; i++) {
 		virg_var p4 = ops[i].p4;
+		/* virg_vm_cleanup frees ResultColumn names (vm/cleanup.c:17-22) */
 		if (ops[i].op == OP_ResultColumn) {
-			snprintf(names[i], sizeof names[i], "%s", ops[i].p4.s ? ops[i].p4.s : "");
-			p4.s = names[i];
+			const char *nm = ops[i].p4.s ? ops[i].p4.s : "";
+			p4.s = malloc(strlen(nm) + 1);
+			if (!p4.s)
+				return -1;
+			memcpy(p4.s, nm, strlen(nm) + 1);
 		}
 		if (virg_vm_addop(vm, ops[i].op, ops[i].p1, ops[i].p2, ops[i].p3, p4) == VIRG_FAIL)
 			return -1;

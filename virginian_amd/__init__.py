@@ -169,6 +169,8 @@ def _load() -> C.CDLL:
         "vg_query_run": (C.c_int, [P, C.POINTER(C.c_ulonglong)]),
         "vg_query_fetch": (C.c_int, [P, C.POINTER(P), C.c_ulonglong]),
         "vg_query_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_ulonglong)]),
+        "vg_query_time_passes": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_ulonglong)]),
         "vg_query_info": (C.c_int, [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint), C.POINTER(C.c_uint),
                                     C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
         "vg_device_info": (C.c_int, [V, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
@@ -182,7 +184,19 @@ def _load() -> C.CDLL:
     return lib
 
 
-lib = _load()
+class _Lib:
+    """The native library, loaded on first use (so that the package can be
+    imported to build it); any use without it raises VirginianError."""
+
+    _cdll = None
+
+    def __getattr__(self, name: str):
+        if _Lib._cdll is None:
+            _Lib._cdll = _load()
+        return getattr(_Lib._cdll, name)
+
+
+lib = _Lib()
 
 
 def struct_sizes() -> list[int]:
@@ -252,6 +266,10 @@ class Virginian:
         cap = capacity if capacity is not None else max(16, (n + 15) // 16 * 16)
         key = np.ascontiguousarray(key)
         cols = [np.ascontiguousarray(c) for c in cols]
+        meta = self.tablets(table_id)[0].contents
+        if len(cols) != meta.fixed_columns or key.dtype.itemsize != meta.key_stride or any(
+                len(c) != n or c.dtype.itemsize != meta.fixed_stride[j] for j, c in enumerate(cols)):
+            raise VirginianError("append_tablet: arrays do not match the table's key and columns")
         arr = (C.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
         if lib.virg_table_appendtablet(self.ptr, table_id, cap, n, key.ctypes.data, arr) != VIRG_SUCCESS:
             raise VirginianError("virg_table_appendtablet failed")
@@ -429,6 +447,15 @@ class DeviceQuery:
             raise VirginianError("vg_query_time failed")
         self.last_rows = n.value
         return ms.value, n.value
+
+    def time_passes(self, warmup: int, iters: int) -> tuple[float, float, int]:
+        """(count-pass ms, write-pass ms, rows) from HIP events on the query's stream."""
+        c, w = C.c_double(0), C.c_double(0)
+        n = C.c_ulonglong(0)
+        if lib.vg_query_time_passes(self.h, warmup, iters, C.byref(c), C.byref(w), C.byref(n)) != VIRG_SUCCESS:
+            raise VirginianError("vg_query_time_passes failed")
+        self.last_rows = n.value
+        return c.value, w.value, n.value
 
     def close(self) -> None:
         if self.h:

@@ -28,7 +28,7 @@ extern "C" {
 #include "vg_prog.h"
 }
 
-extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stream);
+extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stream, hipEvent_t mid);
 extern "C" uint32_t vg_kernel_tile_rows(void);
 extern "C" uint32_t vg_kernel_grid(const vg_prog *P, uint32_t num_tiles);
 extern "C" int vg_kernel_check(const vg_prog *P);
@@ -295,10 +295,13 @@ static int build_plan(vgd_device *d, const std::vector<const virg_tablet_meta *>
 }
 
 static int launch(const vg_prog &P, const plan &p, launch_res &lr, void *const *out,
-	uint64_t capacity, hipStream_t stream)
+	uint64_t capacity, hipStream_t stream, hipEvent_t mid = nullptr)
 {
-	if (p.num_tiles == 0)
+	if (p.num_tiles == 0) {
+		if (mid)
+			HCHK(hipEventRecord(mid, stream), "event");
 		return VIRG_SUCCESS;
+	}
 	if (ensure_lr(lr, p.num_tiles, vg_kernel_grid(&P, p.num_tiles), stream) == VIRG_FAIL)
 		return VIRG_FAIL;
 	vg_launch L;
@@ -313,7 +316,7 @@ static int launch(const vg_prog &P, const plan &p, launch_res &lr, void *const *
 	L.wgcount = lr.wgcount;
 	for (unsigned j = 0; j < P.nout; j++)
 		L.out[j] = out[j];
-	int rc = vg_kernel_launch(&P, &L, stream);
+	int rc = vg_kernel_launch(&P, &L, stream, mid);
 	if (rc != 0) {
 		fprintf(stderr, "::: Virginian: kernel launch failed (%d)\n", rc);
 		return VIRG_FAIL;
@@ -651,6 +654,41 @@ extern "C" int vg_query_time(vg_query *q, int warmup, int iters, double *ms_per_
 	HCHK(hipEventElapsedTime(&ms, q->e0, q->e1), "event time");
 	ms_per_launch[0] = iters > 0 ? ms / iters : 0.0;
 	return VIRG_SUCCESS;
+}
+
+/* Per-pass times: `iters` launches after `warmup`, each bracketed by
+ * events on the query's stream (start, between the count and the write
+ * pass, end) and waited for before the next; means in milliseconds. */
+extern "C" int vg_query_time_passes(vg_query *q, int warmup, int iters, double *ms_count,
+	double *ms_write, unsigned long long *rows)
+{
+	hipEvent_t mid = nullptr;
+	HCHK(hipEventCreate(&mid), "event");
+	double c = 0, w = 0;
+	int rc = VIRG_SUCCESS;
+	for (int i = 0; i < warmup + iters && rc == VIRG_SUCCESS; i++) {
+		if (hipEventRecord(q->e0, q->stream) != hipSuccess ||
+			launch(q->P, q->p, q->lr, q->out, q->capacity, q->stream, mid) == VIRG_FAIL ||
+			hipEventRecord(q->e1, q->stream) != hipSuccess ||
+			vg_query_sync(q, rows) == VIRG_FAIL) {
+			rc = VIRG_FAIL;
+			break;
+		}
+		float a = 0, b = 0;
+		if (hipEventElapsedTime(&a, q->e0, mid) != hipSuccess ||
+			hipEventElapsedTime(&b, mid, q->e1) != hipSuccess) {
+			rc = VIRG_FAIL;
+			break;
+		}
+		if (i >= warmup) {
+			c += a;
+			w += b;
+		}
+	}
+	(void)hipEventDestroy(mid);
+	ms_count[0] = iters > 0 ? c / iters : 0.0;
+	ms_write[0] = iters > 0 ? w / iters : 0.0;
+	return rc;
 }
 
 /* Shape of a prepared query: rows scanned, tiles, referenced-column bytes

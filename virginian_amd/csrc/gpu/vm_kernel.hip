@@ -935,7 +935,9 @@ extern "C" int vg_kernel_check(const vg_prog *P)
 	return vg_kernel_lds_bytes(P) > VG_LDS_LIMIT ? -3 : 0;
 }
 
-extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stream)
+/* Count pass then write pass on `stream`; `mid`, if given, is recorded
+ * between them (per-pass timing). */
+extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stream, hipEvent_t mid)
 {
 	if (L->num_tiles == 0)
 		return 0;
@@ -946,6 +948,8 @@ extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stre
 	const uint32_t grid = (L->num_tiles + L->tiles_per_wg - 1) / L->tiles_per_wg;
 	hipLaunchKernelGGL(vg_count_kernel, dim3(grid), dim3(256), count_lds_bytes(P), stream, *P, *L);
 	if (hipGetLastError() != hipSuccess)
+		return -2;
+	if (mid && hipEventRecord(mid, stream) != hipSuccess)
 		return -2;
 	hipLaunchKernelGGL(vg_write_kernel, dim3(grid), dim3(256), write_lds_bytes(P), stream, *P, *L);
 	return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -14,6 +14,8 @@ int virg_table_create(virginian *v, const char *name, virg_t key_type)
 	virg_db *db = &v->db;
 	unsigned t;
 	for (t = 0; t < VIRG_MAX_TABLES; t++)
+		VIRG_CHECK(db->table_status[t] && strcmp(db->tables[t], name) == 0, "Table exists");
+	for (t = 0; t < VIRG_MAX_TABLES; t++)
 		if (db->table_status[t] == 0)
 			break;
 	VIRG_CHECK(t == VIRG_MAX_TABLES, "Too many tables");
