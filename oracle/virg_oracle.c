@@ -200,11 +200,26 @@ static double rd(const ctx *C, int r, unsigned i, long long *li)
 
 static void cast(ctx *C, sink *s, const virg_op *o, unsigned pc, unsigned n)
 {
+	/* Cast p2 to type p1 in place, row by row, for the rows at this op.
+	 * The register block is a union of per-type arrays, so the converted
+	 * block is built apart and copied back: converting in place in row
+	 * order would let a widened row overwrite rows not yet read.  (The
+	 * reference's Cast, virginia.c:682-862, reads p1/p2 left over from the
+	 * previous op and is never generated; this is the clean definition.) */
 	int to = o->p1, r = o->p2;
 	int from = C->type[r];
+	regblock out = C->reg[r];
+	const int resized = tsize(to) != tsize(from);
+	if (resized)
+		memset(&out, 0, sizeof out);
 	for (unsigned i = 0; i < n; i++) {
-		if (!ACTIVE(i))
+		if (!ACTIVE(i)) {
+			/* a row that skips a cast changing the width has no value of
+			 * the new type: reading it later is ill-formed */
+			if (resized)
+				C->written[r] &= ~(1ull << i);
 			continue;
+		}
 		if (!((C->written[r] >> i) & 1))
 			s->illformed = 1;
 		long long li;
@@ -212,16 +227,17 @@ static void cast(ctx *C, sink *s, const virg_op *o, unsigned pc, unsigned n)
 		int isint = from == VIRG_INT || from == VIRG_INT64 || from == VIRG_CHAR;
 		float f = from == VIRG_FLOAT ? C->reg[r].f[i] : (from == VIRG_DOUBLE ? (float)C->reg[r].d[i] : (float)li);
 		switch (to) {
-		case VIRG_INT: C->reg[r].i[i] = isint ? (int)li : (from == VIRG_FLOAT ? (int)C->reg[r].f[i] : (int)d); break;
-		case VIRG_INT64: C->reg[r].li[i] = isint ? li : (from == VIRG_FLOAT ? (long long)C->reg[r].f[i] : (long long)d); break;
-		case VIRG_FLOAT: C->reg[r].f[i] = f; break;
-		case VIRG_DOUBLE: C->reg[r].d[i] = isint ? (double)li : d; break;
-		case VIRG_CHAR: C->reg[r].c[i] = isint ? (char)li : (char)(int)d; break;
+		case VIRG_INT: out.i[i] = isint ? (int)li : (from == VIRG_FLOAT ? (int)C->reg[r].f[i] : (int)d); break;
+		case VIRG_INT64: out.li[i] = isint ? li : (from == VIRG_FLOAT ? (long long)C->reg[r].f[i] : (long long)d); break;
+		case VIRG_FLOAT: out.f[i] = f; break;
+		case VIRG_DOUBLE: out.d[i] = isint ? (double)li : d; break;
+		case VIRG_CHAR: out.c[i] = isint ? (char)li : (char)(int)d; break;
 		default: fail(s, "op %u: bad cast", pc); return;
 		}
 		C->row_pc[i]++;
 		C->written[r] |= 1ull << i;
 	}
+	C->reg[r] = out;
 	C->type[r] = (virg_t)to;
 	C->stride[r] = (size_t)tsize(to);
 }

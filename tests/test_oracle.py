@@ -211,3 +211,26 @@ def test_type_mismatch_is_an_error(db):
     db.gen_table("test", 100, 0, seed=1)
     with pytest.raises(oracle.OracleError):
         oracle.select(db, "select id from test where uniformi < 2.0")
+
+
+def test_typed_program_brute_force(db):
+    """char / double / int64 arithmetic, Cast, Not/And/Or/Neq keep-branches,
+    a masked redefinition and integer division (truncating toward zero),
+    written directly in numpy."""
+    from tests.helpers import typed_program, typed_table
+    c, d, l, i, f = typed_table(db, 5000, 3)
+    got, ill = oracle.select(db, typed_program())
+    assert ill == 0
+    m = i <= 7
+    with np.errstate(over="ignore"):
+        want = {
+            "c0": (c[m].astype(np.int32) * 2).astype(np.int8),
+            "c1": d[m] * d[m],
+            "c2": l[m],
+            "c3": np.where(i[m] == 3, -1, 3).astype(np.int32),
+            "c4": np.where(i[m] == 3, -3, np.trunc(i[m] / 3)).astype(np.int32),
+            "c5": f[m].astype(np.float64),
+            "c6": f[m].astype(np.float64) + d[m],
+            "c7": (np.arange(5000, dtype=np.int64) * 3)[m],
+        }
+    assert_same(got, want, "typed program")

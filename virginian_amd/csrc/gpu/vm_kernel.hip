@@ -633,61 +633,73 @@ __device__ __forceinline__ void load_cols(const vg_prog &P, const cursor &C, uin
 }
 
 /* ----------------------------------------------------------- count pass */
-/* Each wave evaluates one whole tile of T = 1024 rows: lane l owns strips
- * q*256 + 4l .. +3 (q < 4), so every load of the wave is 1 KB contiguous
- * and four of them per column are in flight.  The row set of count lane l,
- * strip q, is exactly that of write lane l of wave q, so the validity byte
- * the write pass reads for thread 64q + l is bits [4q, 4q+4) here, and the
- * write pass's wave q count is the strip-q count.  Workgroup g covers
- * tiles [g*tiles_per_wg, (g+1)*tiles_per_wg), its waves taking every
- * fourth tile, and stores the workgroup's total.
- * LDS: one register file of nslot_count * T words per wave. */
+/* Validity leaves the count pass as one byte per write-pass lane: byte
+ * tile*256 + 64q + l holds rows q*256 + 4l .. +3 of the tile (bit j = row
+ * 4l + j), exactly the rows write-pass lane l of wave q owns;
+ * tcount[tile*4 + q] counts the selected rows of strip q and wgcount[g]
+ * those of workgroup g, which covers tiles [g*tiles_per_wg,
+ * (g+1)*tiles_per_wg).
+ *   NS = 4 (a filter needing at most VG_COUNT_WIDE_SLOTS register slots):
+ *     each wave takes a whole tile, lane l owning rows q*256 + 4l .. +3 of
+ *     strip q < 4, so four 1 KB loads per column are in flight per wave;
+ *     the waves take every fourth tile; a register file of
+ *     nslot_count * T words per wave.
+ *   NS = 1 (more slots): the 4 waves share each tile, wave q taking strip
+ *     q as in the write pass; one register file of nslot_count * T words
+ *     (a wave only touches its own rows of it). */
+template <int NS>
 __global__ __launch_bounds__(256) void vg_count_kernel(const vg_prog P, const vg_launch L)
 {
 	constexpr uint32_t T = VG_T;
 	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
 	__shared__ uint32_t s_wcount[4];
 	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
-	uint32_t *regs = lds + wave * P.nslot_count * T;
-	const strips<4> S{0u, 256u, lane};
+	uint32_t *regs = NS == 4 ? lds + wave * P.nslot_count * T : lds;
+	const strips<NS> S{NS == 4 ? 0u : wave * 256u, 256u, lane};
 	const uint32_t g = blockIdx.x;
 	const uint32_t t0 = g * L.tiles_per_wg;
 	const uint32_t t1 = min(t0 + L.tiles_per_wg, L.num_tiles);
-	uint16_t *vmask = (uint16_t *)L.vbits;
 	uint32_t count = 0;
 	cursor C;
 	cursor_init(C, P, lane);
-	for (uint32_t tile = t0 + wave; tile < t1; tile += 4) {
+	for (uint32_t tile = t0 + (NS == 4 ? wave : 0u); tile < t1; tile += (NS == 4 ? 4u : 1u)) {
 		cursor_seek(C, P, L, tile, lane);
 		const uint32_t row0 = (tile - C.first) * T;
 		const uint32_t left = C.rows - row0;
 		const uint32_t nrows = left < T ? left : T;
-		uint32_t live = 0xFFFFu;
+		uint32_t live = (1u << (4 * NS)) - 1u;
 		if (nrows < T) {
 			live = 0;
 #pragma unroll
-			for (int q = 0; q < 4; ++q)
+			for (int q = 0; q < NS; ++q)
 #pragma unroll
 				for (int j = 0; j < 4; ++j)
 					live |= S.row(q) + j < nrows ? 1u << (4 * q + j) : 0u;
 		}
 		rowstate RS;
 		rows_init(RS, live);
-		/* a lane only ever touches its own rows of the register file */
-		load_cols<4, T>(P, C, row0, S, nrows, false, regs);
-		run_uops<4, T>(P, 0, P.nfilter, ~0ull, regs, S, RS);
-		/* validity of write-pass lane (wave q, lane l) = bits [4q, 4q+4) of
-		 * this lane's mask; per-strip counts summed over the wave, two
-		 * 16-bit counts per word */
+		load_cols<NS, T>(P, C, row0, S, nrows, false, regs);
+		run_uops<NS, T>(P, 0, P.nfilter, ~0ull, regs, S, RS);
 		const uint32_t v = RS.valid;
-		vmask[(size_t)tile * 64 + lane] = (uint16_t)v;
-		const uint32_t n01 = wave_sum((uint32_t)__builtin_popcount(v & 0xFu) |
-			((uint32_t)__builtin_popcount(v & 0xF0u) << 16));
-		const uint32_t n23 = wave_sum((uint32_t)__builtin_popcount(v & 0xF00u) |
-			((uint32_t)__builtin_popcount(v & 0xF000u) << 16));
-		if (lane < 4)
-			L.tcount[(size_t)tile * 4 + lane] = ((lane < 2 ? n01 : n23) >> (16 * (lane & 1))) & 0xFFFFu;
-		count += (n01 & 0xFFFFu) + (n01 >> 16) + (n23 & 0xFFFFu) + (n23 >> 16);
+		if constexpr (NS == 4) {
+#pragma unroll
+			for (int q = 0; q < 4; ++q)
+				L.vbits[(size_t)tile * 256 + 64 * q + lane] = (uint8_t)((v >> (4 * q)) & 0xFu);
+			/* per-strip counts summed over the wave, two 16-bit counts a word */
+			const uint32_t n01 = wave_sum((uint32_t)__builtin_popcount(v & 0xFu) |
+				((uint32_t)__builtin_popcount(v & 0xF0u) << 16));
+			const uint32_t n23 = wave_sum((uint32_t)__builtin_popcount(v & 0xF00u) |
+				((uint32_t)__builtin_popcount(v & 0xF000u) << 16));
+			if (lane < 4)
+				L.tcount[(size_t)tile * 4 + lane] = ((lane < 2 ? n01 : n23) >> (16 * (lane & 1))) & 0xFFFFu;
+			count += (n01 & 0xFFFFu) + (n01 >> 16) + (n23 & 0xFFFFu) + (n23 >> 16);
+		} else {
+			L.vbits[(size_t)tile * 256 + 64 * wave + lane] = (uint8_t)v;
+			const uint32_t n = wave_sum((uint32_t)__builtin_popcount(v));
+			if (lane == 0)
+				L.tcount[(size_t)tile * 4 + wave] = n;
+			count += n;
+		}
 	}
 	if (lane == 0)
 		s_wcount[wave] = count;
@@ -736,7 +748,6 @@ __global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg
 	}
 	__syncthreads();
 	uint64_t chunk_base = s_base;
-	const uint16_t *vmask = (const uint16_t *)L.vbits;
 	cursor C;
 	cursor_init(C, P, lane);
 
@@ -768,7 +779,7 @@ __global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg
 				continue;
 			const uint32_t wave_off = (wave > 0 ? c0w : 0u) + (wave > 1 ? c1w : 0u) + (wave > 2 ? c2w : 0u);
 			const uint32_t tile = c0 + i;
-			const uint32_t mybits = (vmask[(size_t)tile * 64 + lane] >> (4 * wave)) & 0xFu;
+			const uint32_t mybits = L.vbits[(size_t)tile * 256 + tid];
 			cursor_seek(C, P, L, tile, lane);
 			const uint32_t row0 = (tile - C.first) * T;
 			const uint32_t left = C.rows - row0;
@@ -862,9 +873,18 @@ __global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg
 }
 
 /* --------------------------------------------------------------- launch */
+/* the count pass takes a whole tile per wave when the filter's register
+ * file is this small */
+#define VG_COUNT_WIDE_SLOTS 2u
+
+static bool count_wide(const vg_prog *P)
+{
+	return P->nslot_count <= VG_COUNT_WIDE_SLOTS;
+}
+
 static size_t count_lds_bytes(const vg_prog *P)
 {
-	return (size_t)4 * P->nslot_count * VG_T * 4;
+	return (size_t)(count_wide(P) ? 4 : 1) * P->nslot_count * VG_T * 4;
 }
 
 static size_t write_lds_bytes(const vg_prog *P)
@@ -899,7 +919,8 @@ static void allow_lds()
 {
 	static bool done = false;
 	if (!done) {
-		(void)hipFuncSetAttribute((const void *)vg_count_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, VG_LDS_LIMIT);
+		(void)hipFuncSetAttribute((const void *)vg_count_kernel<4>, hipFuncAttributeMaxDynamicSharedMemorySize, VG_LDS_LIMIT);
+		(void)hipFuncSetAttribute((const void *)vg_count_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, VG_LDS_LIMIT);
 		(void)hipFuncSetAttribute((const void *)vg_write_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, VG_LDS_LIMIT);
 		done = true;
 	}
@@ -946,7 +967,10 @@ extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stre
 	const uint32_t g = grid_for(P, L->num_tiles);
 	L->tiles_per_wg = (L->num_tiles + g - 1) / g;
 	const uint32_t grid = (L->num_tiles + L->tiles_per_wg - 1) / L->tiles_per_wg;
-	hipLaunchKernelGGL(vg_count_kernel, dim3(grid), dim3(256), count_lds_bytes(P), stream, *P, *L);
+	if (count_wide(P))
+		hipLaunchKernelGGL(vg_count_kernel<4>, dim3(grid), dim3(256), count_lds_bytes(P), stream, *P, *L);
+	else
+		hipLaunchKernelGGL(vg_count_kernel<1>, dim3(grid), dim3(256), count_lds_bytes(P), stream, *P, *L);
 	if (hipGetLastError() != hipSuccess)
 		return -2;
 	if (mid && hipEventRecord(mid, stream) != hipSuccess)
