@@ -18,10 +18,10 @@ import math
 
 import numpy as np
 
-from . import (OP_Add, OP_Column, OP_Converge, OP_Div, OP_Eq, OP_Finish, OP_Float, OP_Ge,
-               OP_Gt, OP_Integer, OP_Invalid, OP_Le, OP_Lt, OP_Mul, OP_Neq, OP_Parallel,
-               OP_Result, OP_ResultColumn, OP_Rowid, OP_Sub, OP_Table, VIRG_FLOAT, VIRG_INT,
-               VIRG_INT64, VIRG_OPS)
+from . import (OP_Add, OP_And, OP_Column, OP_Converge, OP_Div, OP_Eq, OP_Finish, OP_Float,
+               OP_Ge, OP_Gt, OP_Integer, OP_Invalid, OP_Le, OP_Lt, OP_Mul, OP_Neq, OP_Not, OP_Or,
+               OP_Parallel, OP_Result, OP_ResultColumn, OP_Rowid, OP_Sub, OP_Table, VIRG_FLOAT,
+               VIRG_INT, VIRG_INT64, VIRG_OPS)
 
 # benchmark schema: column index -> (name, type)
 SCHEMA = [("uniformi", VIRG_INT), ("normali5", VIRG_INT), ("normali20", VIRG_INT),
@@ -156,6 +156,19 @@ def _try_program(rng: np.random.Generator, selectivity: float | None, table: int
         for i in group_fix:
             op, p1, p2, _, p4 = b.ops[i]
             b.ops[i] = (op, p1, p2, end, p4)
+    if rng.random() < 0.25:
+        # logic ops (never emitted by the reference's code generator, but
+        # part of its VM): Not drops rows whose value is zero, And / Or
+        # keep-branch to the next op
+        typ = int(rng.choice([VIRG_INT, VIRG_FLOAT]))
+        a = b.value(typ, int(rng.integers(0, 2)))
+        c = b.value(typ, 0) if rng.random() < 0.5 else b.const(typ)
+        k = int(rng.choice([OP_And, OP_Or, OP_Not]))
+        if k == OP_Not:
+            fixups_result.append(b.emit(OP_Not, a, 0, 0, 0))
+        else:
+            nxt = len(b.ops) + 1
+            b.emit(k, a, c, nxt, 1)
     if rng.random() < 0.15:
         # the codegen's trailing pattern: a keep-branch over an Invalid
         typ = int(rng.choice([VIRG_INT, VIRG_FLOAT]))
