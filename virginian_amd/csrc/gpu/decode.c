@@ -672,5 +672,15 @@ int vg_decode(const virg_op *stmt, unsigned num_ops, unsigned pc,
 				used = 1;
 		col->wload = (uint8_t)used;
 	}
+	P->wsafe = 1;
+	for (unsigned j = 0; j < nres; j++) {
+		const vg_out *o = &P->out[j];
+		if (o->kind != VG_OUT_SLOT)
+			continue;
+		for (unsigned c = 0; c < P->nload; c++)
+			if (P->col[c].wload && (o->slot == P->col[c].slot || o->slot == P->col[c].slot_hi ||
+					o->slot_hi == P->col[c].slot || o->slot_hi == P->col[c].slot_hi))
+				P->wsafe = 0;
+	}
 	return VIRG_SUCCESS;
 }

@@ -105,7 +105,8 @@ typedef struct {
 	uint32_t wfull;     /* the write pass reruns every micro-op (masked writes) */
 	uint64_t wneed;     /* micro-ops the write pass runs (bit k = micro-op k) */
 	uint32_t nslot_count; /* slots [0, nslot_count) hold everything the count pass uses */
-	uint32_t pad2;
+	uint32_t wsafe;     /* no output reads a loaded column's slot: the write pass
+	                     * may load the next tile while writing this one */
 	vg_col   col[VG_MAX_COLS + 1];
 	vg_out   out[VG_MAX_OUT];
 	vg_uop   uop[VG_MAX_UOPS];
@@ -128,19 +129,22 @@ typedef struct {
 	uint32_t pad[5];
 } vg_ctrl;
 
-/* kernel arguments besides the program.  The count pass and the write
- * pass run with the same grid: workgroup g owns tiles
- * [g * tiles_per_wg, (g + 1) * tiles_per_wg). */
+/* kernel arguments besides the program.  Count-pass workgroup g owns
+ * tiles [g * tiles_per_wg, (g + 1) * tiles_per_wg), write-pass workgroup g
+ * tiles [g * wtiles_per_wg, (g + 1) * wtiles_per_wg); each pass has the
+ * grid its own occupancy allows. */
 typedef struct {
 	const vg_tabdesc *tabs;
 	const uint32_t *tile_tab;
 	uint32_t num_tiles;
 	uint32_t tiles_per_wg;
+	uint32_t wtiles_per_wg;
+	uint32_t pad;
 	uint64_t capacity;
 	vg_ctrl *ctrl;
 	uint8_t *vbits;          /* one validity byte per write-pass lane per tile */
 	uint32_t *tcount;        /* rows selected per tile and wave: 4 per tile */
-	uint32_t *wgcount;       /* rows selected per workgroup */
+	uint32_t *wgcount;       /* rows selected per count-pass workgroup */
 	void *out[VG_MAX_OUT];
 } vg_launch;
 

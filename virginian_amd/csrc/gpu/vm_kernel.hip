@@ -617,7 +617,7 @@ __device__ __forceinline__ const char *cursor_base(const cursor &C, uint32_t k)
 
 template <int NS, uint32_t TR>
 __device__ __forceinline__ void load_cols(const vg_prog &P, const cursor &C, uint32_t row0,
-	const strips<NS> &S, uint32_t nrows, bool write_pass, uint32_t *regs)
+	const strips<NS> &S, uint32_t nrows, bool write_pass, uint32_t *regs, bool wait = true)
 {
 	const bool aligned = C.aligned != 0;
 	for (uint32_t k = 0; k < P.nload; ++k) {
@@ -628,8 +628,10 @@ __device__ __forceinline__ void load_cols(const vg_prog &P, const cursor &C, uin
 		load_col<NS, TR>(c, base, S, nrows, aligned, regs);
 	}
 	/* The compiler does not count global_load_lds as a write of the LDS
-	 * it later reads, so it inserts no wait before those reads. */
-	wait_vmem();
+	 * it later reads, so it inserts no wait before those reads: wait here
+	 * unless the caller waits later. */
+	if (wait)
+		wait_vmem();
 }
 
 /* ----------------------------------------------------------- count pass */
@@ -709,17 +711,67 @@ __global__ __launch_bounds__(256) void vg_count_kernel(const vg_prog P, const vg
 }
 
 /* ----------------------------------------------------------- write pass */
-/* Workgroup g starts at the sum of the counts of workgroups < g.  Its
- * tiles' per-wave counts are staged in LDS 64 tiles at a time with their
- * running offsets, so empty tiles cost nothing.  For a tile with selected
- * rows, each wave with selected rows loads what the result needs (only in
- * lanes holding a selected row), runs the micro-ops computing result
- * registers, ranks its rows with ballot + mbcnt and writes its contiguous
- * slice of every result column; waves never wait for each other inside a
- * tile.  LDS: nslot * T words of registers, then 256 row numbers per
- * wave. */
+/* A workgroup of 4/WS waves works on one tile at a time; wave w takes
+ * strips WS*w .. WS*w+WS-1 of it (lane l owns rows 256s + 4l .. +3 of
+ * strip s).  Workgroup g owns tiles [g*wtiles_per_wg, (g+1)*wtiles_per_wg)
+ * and starts its output after the rows selected before its first tile.
+ * The per-strip counts of 64 tiles at a time are staged in LDS with their
+ * running offsets; a wave visits only the tiles where it has selected
+ * rows, loads what the result needs (only in lanes holding a selected row),
+ * runs the micro-ops computing result registers, ranks its rows (strip by
+ * strip, each in row order) and writes its contiguous slice of every
+ * result column.  LDS: nslot * T words of registers, then 256*WS row
+ * numbers per wave. */
 #define VG_CHUNK 64
-__global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg_launch L)
+#define VG_WS 1                               /* strips per write-pass wave */
+#define VG_WRITE_THREADS (256 / VG_WS)
+
+/* rows selected in this wave's strips of staged tile i */
+__device__ __forceinline__ uint32_t wave_count(const uint32_t (*cnt)[4], uint32_t i, uint32_t wave)
+{
+	uint32_t n = 0;
+#pragma unroll
+	for (int s = 0; s < VG_WS; ++s)
+		n += cnt[i][VG_WS * wave + s];
+	return uni(n);
+}
+
+/* first tile index >= i of the chunk where wave `wave` has selected rows */
+__device__ __forceinline__ uint32_t next_work(const uint32_t (*cnt)[4], uint32_t i, uint32_t nt,
+	uint32_t wave)
+{
+	while (i < nt && wave_count(cnt, i, wave) == 0)
+		++i;
+	return i;
+}
+
+/* this lane's selected-row bits of a tile: strip WS*w+s in bits 4s..4s+3 */
+__device__ __forceinline__ uint32_t load_bits(const vg_launch &L, uint32_t tile, uint32_t wave,
+	uint32_t lane)
+{
+	const uint8_t *v = L.vbits + (size_t)tile * 256 + 64 * VG_WS * wave + lane;
+	uint32_t b = 0;
+#pragma unroll
+	for (int s = 0; s < VG_WS; ++s)
+		b |= (uint32_t)v[64 * s] << (4 * s);
+	return b;
+}
+
+/* Point the cursor at `tile` and start loading what the write pass needs
+ * for this lane's rows with selected bits; the caller waits (wait_vmem)
+ * before the register file is read. */
+__device__ __forceinline__ void issue_tile(const vg_prog &P, const vg_launch &L, cursor &C,
+	uint32_t tile, uint32_t lane, const strips<VG_WS> &S, uint32_t bits, uint32_t *regs)
+{
+	cursor_seek(C, P, L, tile, lane);
+	const uint32_t row0 = (tile - C.first) * VG_T;
+	const uint32_t left = C.rows - row0;
+	const uint32_t nrows = left < VG_T ? left : VG_T;
+	if (bits)
+		load_cols<VG_WS, VG_T>(P, C, row0, S, nrows, true, regs, false);
+}
+
+__global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_prog P, const vg_launch L)
 {
 	constexpr uint32_t T = VG_T;
 	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
@@ -728,18 +780,23 @@ __global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg
 	__shared__ uint64_t s_base;
 	uint32_t *regs = lds;
 	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
-	uint16_t *srcrow = (uint16_t *)(lds + P.nslot * T) + wave * 256;
-	const strips<1> S{wave * 256u, 0u, lane};
-	const uint32_t lrow = S.row(0);
+	uint16_t *srcrow = (uint16_t *)(lds + P.nslot * T) + wave * 256 * VG_WS;
+	const strips<VG_WS> S{wave * 256u * VG_WS, 256u, lane};
 	const uint32_t g = blockIdx.x;
-	const uint32_t t0 = g * L.tiles_per_wg;
-	const uint32_t t1 = min(t0 + L.tiles_per_wg, L.num_tiles);
+	const uint32_t t0 = g * L.wtiles_per_wg;
+	const uint32_t t1 = min(t0 + L.wtiles_per_wg, L.num_tiles);
 
 	if (wave == 0) {
-		/* offset of this workgroup: sum of the counts before it */
+		/* offset of this workgroup: the rows selected before tile t0 --
+		 * whole count-pass workgroups, then the tiles of the one t0 is in */
+		const uint32_t cw = t0 / L.tiles_per_wg;
 		uint64_t base = 0;
-		for (uint32_t i = lane; i < g; i += 64)
-			base += L.wgcount[i];
+		for (uint32_t k = lane; k < cw; k += 64)
+			base += L.wgcount[k];
+		for (uint32_t t = cw * L.tiles_per_wg + lane; t < t0; t += 64) {
+			const u32x4 v = *(const u32x4 *)(L.tcount + (size_t)t * 4);
+			base += (uint64_t)v.x + v.y + v.z + v.w;
+		}
 #pragma unroll
 		for (int off = 32; off > 0; off >>= 1)
 			base += __shfl_xor(base, off, 64);
@@ -772,39 +829,62 @@ __global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg
 			s_off[lane] = chunk_base + (incl - tot);
 		}
 		__syncthreads();
-		for (uint32_t i = 0; i < nt; ++i) {
-			const uint32_t c0w = s_cnt[i][0], c1w = s_cnt[i][1], c2w = s_cnt[i][2], c3w = s_cnt[i][3];
-			const uint32_t wave_tot = wave == 0 ? c0w : wave == 1 ? c1w : wave == 2 ? c2w : c3w;
-			if (wave_tot == 0)
-				continue;
-			const uint32_t wave_off = (wave > 0 ? c0w : 0u) + (wave > 1 ? c1w : 0u) + (wave > 2 ? c2w : 0u);
+		/* The tiles of the chunk where this wave has selected rows, in
+		 * order.  The next one's validity bits are fetched while this one
+		 * waits for its columns, and, when no output reads a loaded
+		 * column's slot, the next one's columns load while this one's
+		 * results are written. */
+		uint32_t i = next_work(s_cnt, 0, nt, wave);
+		uint32_t nbits = 0;
+		if (i < nt) {
+			nbits = load_bits(L, c0 + i, wave, lane);
+			issue_tile(P, L, C, c0 + i, lane, S, nbits, regs);
+		}
+		while (i < nt) {
+			uint32_t scnt[VG_WS], wave_off = 0;
+#pragma unroll
+			for (int s = 0; s < VG_WS; ++s)
+				scnt[s] = uni(s_cnt[i][VG_WS * wave + s]);
+			for (uint32_t q = 0; q < VG_WS * wave; ++q)
+				wave_off += s_cnt[i][q];
+			wave_off = uni(wave_off);
+			const uint32_t wave_tot = wave_count(s_cnt, i, wave);
 			const uint32_t tile = c0 + i;
-			const uint32_t mybits = L.vbits[(size_t)tile * 256 + tid];
-			cursor_seek(C, P, L, tile, lane);
-			const uint32_t row0 = (tile - C.first) * T;
-			const uint32_t left = C.rows - row0;
-			const uint32_t nrows = left < T ? left : T;
+			const uint32_t mybits = nbits;
+			const cursor Cc = C;            /* this tile's tablet, for the gathers */
+			const uint32_t inext = next_work(s_cnt, i + 1, nt, wave);
+			if (inext < nt)
+				nbits = load_bits(L, c0 + inext, wave, lane);
+			wait_vmem();                    /* this tile's columns, the next bits */
+			const uint32_t row0 = (tile - Cc.first) * T;
 			rowstate RS;
 			rows_init(RS, mybits);
-			if (mybits)
-				load_cols<1, T>(P, C, row0, S, nrows, true, regs);
 			if (P.wfull) {
 				/* rerun everything: the selected rows take the same
 				 * branches as in the count pass, so masked writes see the
 				 * same active rows */
-				run_uops<1, T>(P, 0, P.nuop, ~0ull, regs, S, RS);
+				run_uops<VG_WS, T>(P, 0, P.nuop, ~0ull, regs, S, RS);
 			} else if (P.wneed) {
-				run_uops<1, T>(P, 0, P.nuop, P.wneed, regs, S, RS);
+				run_uops<VG_WS, T>(P, 0, P.nuop, P.wneed, regs, S, RS);
 			}
-			/* ranks inside the wave, row numbers in rank order */
-			uint32_t pos = 0;
+			/* the loaded columns' slots are free from here on */
+			const bool early = P.wsafe && inext < nt;
+			if (early)
+				issue_tile(P, L, C, c0 + inext, lane, S, nbits, regs);
+			/* ranks: strip by strip, each in row order */
+			uint32_t sbase = 0;
 #pragma unroll
-			for (int r = 0; r < 4; ++r)
-				pos += rank_below(__ballot((mybits >> r) & 1u));
+			for (int s = 0; s < VG_WS; ++s) {
+				uint32_t pos = sbase;
 #pragma unroll
-			for (int r = 0; r < 4; ++r)
-				if ((mybits >> r) & 1u)
-					srcrow[pos++] = (uint16_t)(lrow + r);
+				for (int r = 0; r < 4; ++r)
+					pos += rank_below(__ballot((mybits >> (4 * s + r)) & 1u));
+#pragma unroll
+				for (int r = 0; r < 4; ++r)
+					if ((mybits >> (4 * s + r)) & 1u)
+						srcrow[pos++] = (uint16_t)(S.row(s) + r);
+				sbase += scnt[s];
+			}
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 			__builtin_amdgcn_wave_barrier();
 			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -832,8 +912,8 @@ __global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg
 							((uint8_t *)dst)[at + k] = (uint8_t)lo[srcrow[k]];
 					}
 				} else if (o.kind == VG_OUT_COLUMN) {
-					const vg_col cc = cursor_col(C, o.col);
-					const char *src = cursor_base(C, o.col) + (size_t)row0 * cc.width;
+					const vg_col cc = cursor_col(Cc, o.col);
+					const char *src = cursor_base(Cc, o.col) + (size_t)row0 * cc.width;
 					if (o.width == 8) {
 						for (uint32_t k = lane; k < n; k += 64)
 							((uint64_t *)dst)[at + k] = gptr<uint64_t>(src)[srcrow[k]];
@@ -861,6 +941,9 @@ __global__ __launch_bounds__(256) void vg_write_kernel(const vg_prog P, const vg
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 			__builtin_amdgcn_wave_barrier();
 			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+			if (!early && inext < nt)
+				issue_tile(P, L, C, c0 + inext, lane, S, nbits, regs);
+			i = inext;
 		}
 		{
 			const uint32_t lastc = s_cnt[nt - 1][0] + s_cnt[nt - 1][1] + s_cnt[nt - 1][2] + s_cnt[nt - 1][3];
@@ -926,28 +1009,48 @@ static void allow_lds()
 	}
 }
 
-/* Workgroups per pass: enough to fill every CU at the occupancy the write
- * pass's LDS footprint allows, never more than there are tiles. */
-static uint32_t grid_for(const vg_prog *P, uint32_t num_tiles)
+/* Workgroups per pass: every CU filled at the pass's occupancy, never
+ * more than there are tiles. */
+static uint32_t count_grid(const vg_prog *P, uint32_t num_tiles)
 {
 	allow_lds();
-	int per_cu = 0;
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, vg_write_kernel, 256,
-			write_lds_bytes(P)) != hipSuccess || per_cu < 1)
-		per_cu = 1;
-	uint32_t g = (uint32_t)num_cus() * (uint32_t)per_cu;
+	int ct = 0;
+	const hipError_t e = count_wide(P)
+		? hipOccupancyMaxActiveBlocksPerMultiprocessor(&ct, vg_count_kernel<4>, 256, count_lds_bytes(P))
+		: hipOccupancyMaxActiveBlocksPerMultiprocessor(&ct, vg_count_kernel<1>, 256, count_lds_bytes(P));
+	if (e != hipSuccess || ct < 1)
+		ct = 1;
+	const uint32_t g = (uint32_t)num_cus() * (uint32_t)ct;
 	return g < num_tiles ? g : num_tiles;
 }
 
-/* Number of workgroups a launch uses (the size of the per-workgroup count
- * array). */
+static uint32_t write_grid(const vg_prog *P, uint32_t num_tiles)
+{
+	allow_lds();
+	int wr = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&wr, vg_write_kernel, VG_WRITE_THREADS,
+			write_lds_bytes(P)) != hipSuccess || wr < 1)
+		wr = 1;
+	const uint32_t g = (uint32_t)num_cus() * (uint32_t)wr;
+	return g < num_tiles ? g : num_tiles;
+}
+
+/* split num_tiles into runs of *per tiles for at most `want` workgroups;
+ * returns the number of workgroups */
+static uint32_t split(uint32_t num_tiles, uint32_t want, uint32_t *per)
+{
+	*per = (num_tiles + want - 1) / want;
+	return (num_tiles + *per - 1) / *per;
+}
+
+/* Number of count-pass workgroups of a launch (the size of the
+ * per-workgroup count array). */
 extern "C" uint32_t vg_kernel_grid(const vg_prog *P, uint32_t num_tiles)
 {
 	if (num_tiles == 0)
 		return 0;
-	const uint32_t g = grid_for(P, num_tiles);
-	const uint32_t per = (num_tiles + g - 1) / g;
-	return (num_tiles + per - 1) / per;
+	uint32_t per;
+	return split(num_tiles, count_grid(P, num_tiles), &per);
 }
 
 /* 0 if the program's register file fits the LDS of both passes */
@@ -964,9 +1067,8 @@ extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stre
 		return 0;
 	if (vg_kernel_check(P) != 0)
 		return -3;
-	const uint32_t g = grid_for(P, L->num_tiles);
-	L->tiles_per_wg = (L->num_tiles + g - 1) / g;
-	const uint32_t grid = (L->num_tiles + L->tiles_per_wg - 1) / L->tiles_per_wg;
+	const uint32_t grid = split(L->num_tiles, count_grid(P, L->num_tiles), &L->tiles_per_wg);
+	const uint32_t wgrid = split(L->num_tiles, write_grid(P, L->num_tiles), &L->wtiles_per_wg);
 	if (count_wide(P))
 		hipLaunchKernelGGL(vg_count_kernel<4>, dim3(grid), dim3(256), count_lds_bytes(P), stream, *P, *L);
 	else
@@ -975,7 +1077,7 @@ extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stre
 		return -2;
 	if (mid && hipEventRecord(mid, stream) != hipSuccess)
 		return -2;
-	hipLaunchKernelGGL(vg_write_kernel, dim3(grid), dim3(256), write_lds_bytes(P), stream, *P, *L);
+	hipLaunchKernelGGL(vg_write_kernel, dim3(wgrid), dim3(VG_WRITE_THREADS), write_lds_bytes(P), stream, *P, *L);
 	return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
