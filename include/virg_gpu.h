@@ -64,6 +64,33 @@ int vg_query_time_passes(vg_query *q, int warmup, int iters, double *ms_count,
 int vg_query_info(vg_query *q, unsigned long long *rows, unsigned *tiles,
 	unsigned *read_bytes_per_row, unsigned *write_bytes_per_row, unsigned *nout);
 
+/* Out-of-core select: the table stays in host memory; tablets are taken in
+ * batches of about batch_rows rows (0: 8M) and only the columns the
+ * program loads are copied to HBM, on two streams in turn, overlapping
+ * copies with kernels; result rows are appended to out[j] (one host array
+ * per result column, room for `capacity` rows) in single-core row order.
+ * The same path serves virg_vm_gpu when virginian.use_stream is set
+ * (reference: src/vm/vm_gpu.cu:249-440). */
+typedef struct {
+	unsigned long long rows_scanned;
+	unsigned long long rows_out;
+	unsigned long long h2d_bytes;     /* column bytes copied to the device */
+	unsigned long long d2h_bytes;     /* result bytes copied back */
+	double seconds;                   /* wall time of the whole scan */
+	unsigned batches;
+} vg_stream_stats;
+int vg_stream_select(virginian *v, const virg_op *ops, unsigned num_ops, void **out,
+	unsigned long long capacity, unsigned long long batch_rows, vg_stream_stats *st,
+	char *err, size_t errlen);
+
+/* Pinned host memory (results copy into it at link speed). */
+void *vg_host_alloc(size_t bytes);
+void vg_host_free(void *p);
+
+/* Best host-to-device and device-to-host copy rates (GB/s) of one pinned
+ * buffer of `bytes` over `reps` copies each. */
+int vg_link_probe(virginian *v, size_t bytes, int reps, double *h2d_gbs, double *d2h_gbs);
+
 /* Architecture name and compute-unit count of the device in use
  * (VIRG_FAIL and *cus = 0 without one). */
 int vg_device_info(virginian *v, char *name, size_t n, int *cus);

@@ -123,6 +123,13 @@ class VirginianError(RuntimeError):
     pass
 
 
+class StreamStats(C.Structure):
+    """vg_stream_stats (include/virg_gpu.h)."""
+    _fields_ = [("rows_scanned", C.c_ulonglong), ("rows_out", C.c_ulonglong),
+                ("h2d_bytes", C.c_ulonglong), ("d2h_bytes", C.c_ulonglong),
+                ("seconds", C.c_double), ("batches", C.c_uint)]
+
+
 def _load() -> C.CDLL:
     if not _LIB_PATH.exists():
         raise VirginianError(
@@ -174,6 +181,11 @@ def _load() -> C.CDLL:
         "vg_query_info": (C.c_int, [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint), C.POINTER(C.c_uint),
                                     C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
         "vg_device_info": (C.c_int, [V, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
+        "vg_stream_select": (C.c_int, [V, C.POINTER(Op), C.c_uint, C.POINTER(P), C.c_ulonglong,
+                                       C.c_ulonglong, C.POINTER(StreamStats), C.c_char_p, C.c_size_t]),
+        "vg_host_alloc": (P, [C.c_size_t]),
+        "vg_host_free": (None, [P]),
+        "vg_link_probe": (C.c_int, [V, C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "vg_decode_program": (C.c_int, [C.POINTER(Op), C.c_uint, C.c_uint, C.POINTER(TabletMeta),
                                         P, C.c_size_t, C.c_char_p, C.c_size_t]),
     }
@@ -233,6 +245,16 @@ class Virginian:
     @use_gpu.setter
     def use_gpu(self, on: bool) -> None:
         self._st.use_gpu = 1 if on else 0
+
+    @property
+    def use_stream(self) -> bool:
+        """virginian.use_stream: virg_vm_gpu streams tablets from host
+        memory instead of keeping them resident in HBM."""
+        return bool(self._st.use_stream)
+
+    @use_stream.setter
+    def use_stream(self, on: bool) -> None:
+        self._st.use_stream = 1 if on else 0
 
     @property
     def state(self) -> State:
@@ -339,6 +361,61 @@ class Virginian:
             return self.execute(vm)
         finally:
             self.free_vm(vm)
+
+    def stream_select(self, sql_or_ops, capacity: int | None = None, batch_rows: int = 0,
+                      pinned: bool = True) -> tuple[dict, StreamStats]:
+        """Out-of-core select (vg_stream_select): the table stays in host
+        memory and only referenced columns stream through HBM.  Results go
+        to pinned host buffers unless pinned=False."""
+        if isinstance(sql_or_ops, str):
+            vm = self.compile(sql_or_ops)
+            try:
+                arr, n = ops_array(vm)
+                names, types = result_schema(vm)
+                arr = (Op * n)(*arr)
+            finally:
+                self.free_vm(vm)
+            keep = None
+        else:
+            arr, n, keep = make_ops(sql_or_ops)
+            names = [o[4] for o in sql_or_ops if o[0] == OP_ResultColumn]
+            types = [o[1] for o in sql_or_ops if o[0] == OP_ResultColumn]
+        tid = None
+        for i in range(n):
+            if arr[i].op == OP_Table:
+                tid = arr[i].p1
+                break
+        cap = capacity if capacity is not None else self.numrows(tid)
+        bufs, outs = [], []
+        try:
+            for t in types:
+                dt = np.dtype(NP_TYPE[t])
+                nbytes = max(cap * dt.itemsize, 1)
+                if pinned:
+                    ptr = lib.vg_host_alloc(nbytes)
+                    if not ptr:
+                        raise VirginianError("vg_host_alloc failed")
+                    bufs.append(ptr)
+                    outs.append(np.ctypeslib.as_array((C.c_uint8 * nbytes).from_address(ptr)).view(dt))
+                else:
+                    outs.append(np.empty(max(cap, 1), dtype=dt))
+            ptrs = (C.c_void_p * max(len(outs), 1))(*[o.ctypes.data for o in outs])
+            st = StreamStats()
+            err = C.create_string_buffer(256)
+            if lib.vg_stream_select(self.ptr, arr, n, ptrs, cap, batch_rows, C.byref(st), err, 256) != VIRG_SUCCESS:
+                raise VirginianError(err.value.decode() or "vg_stream_select failed")
+            cols = {nm: o[:st.rows_out].copy() for nm, o in zip(names, outs)}
+        finally:
+            for ptr in bufs:
+                lib.vg_host_free(ptr)
+        return cols, st
+
+    def link_probe(self, nbytes: int = 1 << 30, reps: int = 3) -> tuple[float, float]:
+        """(host-to-device, device-to-host) GB/s of pinned copies."""
+        h, d = C.c_double(0), C.c_double(0)
+        if lib.vg_link_probe(self.ptr, nbytes, reps, C.byref(h), C.byref(d)) != VIRG_SUCCESS:
+            raise VirginianError("vg_link_probe failed")
+        return h.value, d.value
 
     def prepare(self, sql_or_ops, capacity: int = 0, rows_per_lane: int = 0) -> "DeviceQuery":
         return DeviceQuery(self, sql_or_ops, capacity, rows_per_lane)
