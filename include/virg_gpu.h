@@ -64,6 +64,21 @@ int vg_query_time_passes(vg_query *q, int warmup, int iters, double *ms_count,
 int vg_query_info(vg_query *q, unsigned long long *rows, unsigned *tiles,
 	unsigned *read_bytes_per_row, unsigned *write_bytes_per_row, unsigned *nout);
 
+/* A batch of prepared queries (distinct, on one device) replayed as one
+ * HIP graph: each query's launch pair on its own forked stream, so small
+ * queries run side by side and the host submits one graph per batch.
+ * Results stay with each query (vg_query_fetch after vg_batch_sync).
+ * The queries must not be launched by other means while the batch is in
+ * flight. */
+typedef struct vg_batch vg_batch;
+vg_batch *vg_batch_create(vg_query *const *qs, unsigned n, char *err, size_t errlen);
+void vg_batch_free(vg_batch *b);
+int vg_batch_launch(vg_batch *b);
+/* Wait for the batch; rows[i] (when rows is not NULL) = rows query i selected. */
+int vg_batch_sync(vg_batch *b, unsigned long long *rows);
+/* Mean milliseconds per replay after `warmup` untimed ones (HIP events). */
+int vg_batch_time(vg_batch *b, int warmup, int iters, double *ms);
+
 /* Out-of-core select: the table stays in host memory; tablets are taken in
  * batches of about batch_rows rows (0: 8M) and only the columns the
  * program loads are copied to HBM, on two streams in turn, overlapping

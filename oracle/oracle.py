@@ -59,7 +59,8 @@ def _ops(db: va.Virginian, prog):
 
 
 def select(db: va.Virginian, prog, table: str | None = None, threads: int = 0,
-           ordered: bool = True, count_only: bool = False) -> tuple[dict, int]:
+           ordered: bool = True, count_only: bool = False,
+           max_tablets: int | None = None) -> tuple[dict, int]:
     """Run the program's data-parallel section on the CPU oracle over the
     table named by its Table op (or `table`).  Returns (columns, illformed).
     threads = 0 runs the single-threaded reference-order VM."""
@@ -76,6 +77,8 @@ def select(db: va.Virginian, prog, table: str | None = None, threads: int = 0,
     if tid is None or par is None:
         raise OracleError("program has no Table/Parallel op")
     tabs = db.tablets(tid)
+    if max_tablets is not None:
+        tabs = tabs[:max_tablets]
     tarr = (C.POINTER(va.TabletMeta) * len(tabs))(*tabs)
     cap = sum(t.contents.rows for t in tabs)
     outs = [] if count_only else [np.empty(cap, dtype=va.NP_TYPE[t]) for t in types]

@@ -183,6 +183,11 @@ def _load() -> C.CDLL:
         "vg_device_info": (C.c_int, [V, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
         "vg_stream_select": (C.c_int, [V, C.POINTER(Op), C.c_uint, C.POINTER(P), C.c_ulonglong,
                                        C.c_ulonglong, C.POINTER(StreamStats), C.c_char_p, C.c_size_t]),
+        "vg_batch_create": (P, [C.POINTER(P), C.c_uint, C.c_char_p, C.c_size_t]),
+        "vg_batch_free": (None, [P]),
+        "vg_batch_launch": (C.c_int, [P]),
+        "vg_batch_sync": (C.c_int, [P, C.POINTER(C.c_ulonglong)]),
+        "vg_batch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "vg_host_alloc": (P, [C.c_size_t]),
         "vg_host_free": (None, [P]),
         "vg_link_probe": (C.c_int, [V, C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -537,6 +542,45 @@ class DeviceQuery:
     def close(self) -> None:
         if self.h:
             lib.vg_query_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GraphBatch:
+    """Prepared queries replayed together as one HIP graph (vg_batch_* in
+    include/virg_gpu.h); each query keeps its own results."""
+
+    def __init__(self, queries: list[DeviceQuery]):
+        self.queries = list(queries)
+        arr = (C.c_void_p * max(len(self.queries), 1))(*[q.h for q in self.queries])
+        err = C.create_string_buffer(256)
+        self.h = lib.vg_batch_create(arr, len(self.queries), err, 256)
+        if not self.h:
+            raise VirginianError(err.value.decode() or "vg_batch_create failed")
+
+    def run(self) -> list[int]:
+        rows = (C.c_ulonglong * max(len(self.queries), 1))()
+        if lib.vg_batch_launch(self.h) != VIRG_SUCCESS or lib.vg_batch_sync(self.h, rows) != VIRG_SUCCESS:
+            raise VirginianError("batch run failed")
+        out = list(rows)[:len(self.queries)]
+        for q, n in zip(self.queries, out):
+            q.last_rows = n
+        return out
+
+    def time(self, warmup: int, iters: int) -> float:
+        ms = C.c_double(0)
+        if lib.vg_batch_time(self.h, warmup, iters, C.byref(ms)) != VIRG_SUCCESS:
+            raise VirginianError("vg_batch_time failed")
+        return ms.value
+
+    def close(self) -> None:
+        if self.h:
+            lib.vg_batch_free(self.h)
             self.h = None
 
     def __del__(self):

@@ -157,9 +157,14 @@ extern "C" void *vgd_host_alloc(size_t bytes)
 	return p;
 }
 
+/* Frees slot memory from vgd_host_alloc, or from the plain allocation the
+ * host library falls back to when pinning fails. */
 extern "C" void vgd_host_free(void *p)
 {
-	(void)hipHostFree(p);
+	if (hipHostFree(p) != hipSuccess) {
+		(void)hipGetLastError();
+		free(p);
+	}
 }
 
 /* Make a tablet resident in HBM; returns its device copy. */
@@ -1145,4 +1150,141 @@ extern "C" int vg_decode_program(const virg_op *ops, unsigned num_ops, unsigned 
 	if (out_bytes < sizeof(vg_prog))
 		return qerr(err, errlen, "output buffer smaller than vg_prog (%zu bytes)", sizeof(vg_prog));
 	return vg_decode(ops, num_ops, pc, tab, nullptr, (vg_prog *)out, err, errlen);
+}
+
+/* ------------------------------------------------------------- vg_batch */
+/* Many prepared queries replayed as one HIP graph.  Each query's launch
+ * pair is captured on its own stream, forked from and joined back to the
+ * batch's stream, so the graph runs queries concurrently (a query over a
+ * small table fills only a few compute units) and the host submits one
+ * graph instead of two launches per query.  Everything a capture must not
+ * contain (allocation, memset) happens before capturing: each query's
+ * launch resources are sized when the batch is built. */
+struct vg_batch {
+	std::vector<vg_query *> qs;
+	hipStream_t stream = nullptr;
+	hipEvent_t fork = nullptr;
+	std::vector<hipEvent_t> joins;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+};
+
+extern "C" void vg_batch_free(vg_batch *b)
+{
+	if (!b)
+		return;
+	if (b->stream)
+		(void)hipStreamSynchronize(b->stream);
+	if (b->exec) (void)hipGraphExecDestroy(b->exec);
+	if (b->graph) (void)hipGraphDestroy(b->graph);
+	for (hipEvent_t e : b->joins)
+		(void)hipEventDestroy(e);
+	if (b->fork) (void)hipEventDestroy(b->fork);
+	if (b->stream) (void)hipStreamDestroy(b->stream);
+	delete b;
+}
+
+extern "C" vg_batch *vg_batch_create(vg_query *const *qs, unsigned n, char *err, size_t errlen)
+{
+	vg_batch *b = new vg_batch();
+	b->qs.assign(qs, qs + n);
+	for (unsigned i = 0; i < n; i++) {
+		if (!qs[i] || qs[i]->d != qs[0]->d) {
+			qerr(err, errlen, "batch queries must be prepared on one device");
+			vg_batch_free(b);
+			return nullptr;
+		}
+		for (unsigned j = 0; j < i; j++)
+			if (qs[j] == qs[i]) {
+				/* one set of launch resources per query */
+				qerr(err, errlen, "query %u appears twice in the batch", i);
+				vg_batch_free(b);
+				return nullptr;
+			}
+	}
+#define BCHK(call, what) do { if (herr((call), (what))) { qerr(err, errlen, "%s failed", what); \
+	vg_batch_free(b); return nullptr; } } while (0)
+	if (n)
+		BCHK(hipSetDevice(qs[0]->d->device), "set device");
+	BCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
+	BCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "event");
+	for (unsigned i = 0; i < n; i++) {
+		vg_query *q = qs[i];
+		if (q->p.num_tiles && ensure_lr(q->lr, q->p.num_tiles, vg_kernel_grid(&q->P, q->p.num_tiles),
+				q->stream) == VIRG_FAIL) {
+			qerr(err, errlen, "cannot set up query %u", i);
+			vg_batch_free(b);
+			return nullptr;
+		}
+		BCHK(hipStreamSynchronize(q->stream), "query setup");
+		hipEvent_t e;
+		BCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+		b->joins.push_back(e);
+	}
+	BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+	int ok = hipEventRecord(b->fork, b->stream) == hipSuccess;
+	for (unsigned i = 0; i < n && ok; i++) {
+		vg_query *q = qs[i];
+		ok = hipStreamWaitEvent(q->stream, b->fork, 0) == hipSuccess &&
+			launch(q->P, q->p, q->lr, q->out, q->capacity, q->stream) == VIRG_SUCCESS &&
+			hipEventRecord(b->joins[i], q->stream) == hipSuccess &&
+			hipStreamWaitEvent(b->stream, b->joins[i], 0) == hipSuccess;
+	}
+	const hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
+	if (!ok || ec != hipSuccess) {
+		(void)hipGetLastError();
+		qerr(err, errlen, "graph capture failed");
+		vg_batch_free(b);
+		return nullptr;
+	}
+	BCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+#undef BCHK
+	return b;
+}
+
+/* Submit the whole batch (asynchronous). */
+extern "C" int vg_batch_launch(vg_batch *b)
+{
+	HCHK(hipGraphLaunch(b->exec, b->stream), "graph launch");
+	return VIRG_SUCCESS;
+}
+
+/* Wait for the batch; rows[i] = rows selected by query i. */
+extern "C" int vg_batch_sync(vg_batch *b, unsigned long long *rows)
+{
+	HCHK(hipStreamSynchronize(b->stream), "batch");
+	for (size_t i = 0; i < b->qs.size(); i++) {
+		unsigned long long n = 0;
+		if (vg_query_sync(b->qs[i], &n) == VIRG_FAIL)
+			return VIRG_FAIL;
+		if (rows)
+			rows[i] = n;
+	}
+	return VIRG_SUCCESS;
+}
+
+/* Mean milliseconds per batch over `iters` replays after `warmup`, timed
+ * with HIP events on the batch's stream. */
+extern "C" int vg_batch_time(vg_batch *b, int warmup, int iters, double *ms)
+{
+	hipEvent_t e0, e1;
+	HCHK(hipEventCreate(&e0), "event");
+	HCHK(hipEventCreate(&e1), "event");
+	int ok = 1;
+	for (int i = 0; i < warmup && ok; i++)
+		ok = hipGraphLaunch(b->exec, b->stream) == hipSuccess;
+	ok = ok && hipEventRecord(e0, b->stream) == hipSuccess;
+	for (int i = 0; i < iters && ok; i++)
+		ok = hipGraphLaunch(b->exec, b->stream) == hipSuccess;
+	ok = ok && hipEventRecord(e1, b->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+	float t = 0;
+	ok = ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (!ok) {
+		herr(hipGetLastError(), "batch timing");
+		return VIRG_FAIL;
+	}
+	*ms = iters > 0 ? t / iters : 0.0;
+	return VIRG_SUCCESS;
 }
