@@ -102,12 +102,18 @@ def bench_outofcore(a) -> int:
     if cus == 0:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
-    t0 = time.perf_counter()
-    db.gen_table("test", rows, a.tablet_rows, seed=a.seed, threads=16)
-    print(f"# generated {rows} rows in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     h2d_peak, d2h_peak = db.link_probe(1 << 30, 3)
     print(f"# pinned copy: {h2d_peak:.1f} GB/s host-to-device, {d2h_peak:.1f} GB/s device-to-host",
           file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    os.environ["VIRG_GEN_PROGRESS"] = str(1 << 28)
+    # the system pins about 64 GiB at most: keep some of it for the staging
+    # and result buffers; tablets beyond the limit are staged by host threads
+    db.pin_limit(a.pin_limit_gib << 30)
+    db.gen_table("test", rows, a.tablet_rows, seed=a.seed, threads=16)
+    pinned, pageable = db.host_memory()
+    print(f"# generated {rows} rows in {time.perf_counter() - t0:.1f}s; tablets: {pinned / 2**30:.1f} GiB "
+          f"pinned, {pageable / 2**30:.1f} GiB pageable", file=sys.stderr, flush=True)
     cap = rows // 8
     for _ in range(a.warmup):
         db.stream_select(CONFIG3_SQL, capacity=cap, batch_rows=a.batch_rows)
@@ -145,7 +151,8 @@ def bench_outofcore(a) -> int:
                                "memory (1M-row tablets), referenced columns streamed through HBM",
                    "rows": rows, "tablet_rows": a.tablet_rows, "selected": n,
                    "link_bytes_per_step": round(link / a.steps), "step_ms": per,
-                   "pinned_copy_gbs": {"h2d": round(h2d_peak, 1), "d2h": round(d2h_peak, 1)}},
+                   "pinned_copy_gbs": {"h2d": round(h2d_peak, 1), "d2h": round(d2h_peak, 1)},
+                   "tablet_bytes": {"pinned": pinned, "pageable": pageable}},
         "roofline": {"bound": "link", "achieved": round(value, 2), "peak": round(h2d_peak, 1),
                      "unit": "GB/s", "frac": round(value / h2d_peak, 4), "traffic": None},
         "cpu_baseline": cpu,
@@ -173,6 +180,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pin-limit-gib", type=int, default=52,
+                    help="out-of-core: pin at most this much tablet memory")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--workload", choices=["resident", "outofcore"], default="resident")
     ap.add_argument("--batch-rows", type=int, default=8 << 20)

@@ -98,6 +98,14 @@ int vg_stream_select(virginian *v, const virg_op *ops, unsigned num_ops, void **
 	unsigned long long capacity, unsigned long long batch_rows, vg_stream_stats *st,
 	char *err, size_t errlen);
 
+/* Tablet slot memory allocated since virg_init: pinned (hipHostMalloc)
+ * and pageable (no device, or the system refused to pin more). */
+int vg_host_memory(virginian *v, unsigned long long *pinned, unsigned long long *pageable);
+/* Pin at most `bytes` of tablet slot memory from now on (0: no limit);
+ * slots beyond it are pageable and reach the device through host-thread
+ * copies into a pinned staging buffer. */
+int vg_host_pin_limit(virginian *v, unsigned long long bytes);
+
 /* Pinned host memory (results copy into it at link speed). */
 void *vg_host_alloc(size_t bytes);
 void vg_host_free(void *p);

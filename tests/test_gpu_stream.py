@@ -57,3 +57,18 @@ def test_stream_capacity_error(gpu_db):
     gpu_db.gen_table("test", 100_000, 1 << 15, seed=24)
     with pytest.raises(va.VirginianError):
         gpu_db.stream_select("select id from test where uniformi < 50", capacity=10)
+
+
+def test_stream_pageable_tablets(gpu_db):
+    """Tablets beyond the pin limit are pageable and go through host-thread
+    staging; batches mixing pinned and pageable tablets give the same
+    result."""
+    gpu_db.pin_limit(3 * (40 << 20))
+    gpu_db.gen_table("test", (6 << 20) + 5, 1 << 20, seed=25)
+    pinned, pageable = gpu_db.host_memory()
+    assert pinned > 0 and pageable > 0
+    want, _ = oracle.select(gpu_db, CONFIG3_SQL, threads=8)
+    for batch_rows in (1 << 20, 5 << 20):
+        got, st = gpu_db.stream_select(CONFIG3_SQL, batch_rows=batch_rows)
+        assert st.h2d_bytes == ((6 << 20) + 5) * 28
+        assert_same(got, want, f"pageable batch_rows={batch_rows}")

@@ -188,6 +188,8 @@ def _load() -> C.CDLL:
         "vg_batch_launch": (C.c_int, [P]),
         "vg_batch_sync": (C.c_int, [P, C.POINTER(C.c_ulonglong)]),
         "vg_batch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "vg_host_memory": (C.c_int, [V, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
+        "vg_host_pin_limit": (C.c_int, [V, C.c_ulonglong]),
         "vg_host_alloc": (P, [C.c_size_t]),
         "vg_host_free": (None, [P]),
         "vg_link_probe": (C.c_int, [V, C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -414,6 +416,18 @@ class Virginian:
             for ptr in bufs:
                 lib.vg_host_free(ptr)
         return cols, st
+
+    def host_memory(self) -> tuple[int, int]:
+        """(pinned, pageable) bytes of tablet slot memory allocated so far."""
+        a, b = C.c_ulonglong(0), C.c_ulonglong(0)
+        if lib.vg_host_memory(self.ptr, C.byref(a), C.byref(b)) != VIRG_SUCCESS:
+            raise VirginianError("vg_host_memory failed")
+        return a.value, b.value
+
+    def pin_limit(self, nbytes: int) -> None:
+        """Pin at most nbytes of tablet memory from now on (0: no limit)."""
+        if lib.vg_host_pin_limit(self.ptr, nbytes) != VIRG_SUCCESS:
+            raise VirginianError("vg_host_pin_limit failed")
 
     def link_probe(self, nbytes: int = 1 << 30, reps: int = 3) -> tuple[float, float]:
         """(host-to-device, device-to-host) GB/s of pinned copies."""

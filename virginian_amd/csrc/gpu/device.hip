@@ -20,7 +20,9 @@
 #include <string.h>
 #include <stdlib.h>
 #include <stdarg.h>
+#include <atomic>
 #include <chrono>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -363,6 +365,7 @@ struct sbuf {
 	hipEvent_t freed = nullptr;     /* the batch's results copied out */
 	char *stage = nullptr;
 	size_t stage_cap = 0;
+	char *h_stage = nullptr;        /* pinned host copy of `stage` for pageable tablets */
 	char *out = nullptr;
 	size_t out_cap = 0;
 	vg_tabdesc *d_desc = nullptr, *h_desc = nullptr;
@@ -378,6 +381,7 @@ static void free_sbuf(sbuf &b)
 {
 	if (b.stream) (void)hipStreamSynchronize(b.stream);
 	if (b.stage) (void)hipFree(b.stage);
+	if (b.h_stage) (void)hipHostFree(b.h_stage);
 	if (b.out) (void)hipFree(b.out);
 	if (b.d_desc) (void)hipFree(b.d_desc);
 	if (b.d_tile) (void)hipFree(b.d_tile);
@@ -409,6 +413,48 @@ static const char *host_column(const virg_tablet_meta *t, unsigned src)
 {
 	return src == VG_KEY_SRC ? (const char *)t + t->key_block
 	                         : (const char *)t + t->fixed_block + t->fixed_offset[src];
+}
+
+/* true when host memory at p is pinned (page-locked, device-visible) */
+static bool host_pinned(const void *p)
+{
+	hipPointerAttribute_t a;
+	if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+		(void)hipGetLastError();
+		return false;
+	}
+	return a.type == hipMemoryTypeHost;
+}
+
+/* host-to-host copies done by a pool of threads, in 4 MB pieces */
+struct host_copy {
+	char *dst;
+	const char *src;
+	size_t bytes;
+};
+
+static void parallel_copy(const std::vector<host_copy> &jobs)
+{
+	const size_t piece = 4u << 20;
+	std::vector<host_copy> pieces;
+	for (const host_copy &j : jobs)
+		for (size_t o = 0; o < j.bytes; o += piece)
+			pieces.push_back({j.dst + o, j.src + o, j.bytes - o < piece ? j.bytes - o : piece});
+	unsigned nt = std::thread::hardware_concurrency();
+	nt = nt < 1 ? 1 : nt > 16 ? 16 : nt;
+	if (pieces.size() < nt)
+		nt = (unsigned)pieces.size();
+	std::atomic<size_t> next{0};
+	auto work = [&]() {
+		for (size_t i; (i = next.fetch_add(1)) < pieces.size();)
+			memcpy(pieces[i].dst, pieces[i].src, pieces[i].bytes);
+	};
+	std::vector<std::thread> th;
+	for (unsigned t = 1; t < nt; t++)
+		th.emplace_back(work);
+	work();
+	for (std::thread &t : th)
+		t.join();
 }
 
 static int stream_finish(vgd_device *d, const vg_prog &P, sbuf &b, stream_sink &sink,
@@ -463,6 +509,14 @@ static int stream_select(vgd_device *d, const vg_prog &P, const std::vector<cons
 		maxtiles = t > maxtiles ? t : maxtiles;
 		maxtabs = cut[k + 1] - cut[k] > maxtabs ? cut[k + 1] - cut[k] : maxtabs;
 	}
+	/* Tablets the system would not pin are copied by host threads into a
+	 * pinned staging buffer first (a whole batch at once). */
+	std::vector<uint8_t> pageable(tabs.size());
+	bool any_pageable = false;
+	for (size_t i = 0; i < tabs.size(); i++) {
+		pageable[i] = !host_pinned(tabs[i]);
+		any_pageable = any_pageable || pageable[i];
+	}
 	sbuf B[2];
 	hipStream_t down = nullptr;
 	int rc = VIRG_SUCCESS;
@@ -481,6 +535,9 @@ static int stream_select(vgd_device *d, const vg_prog &P, const std::vector<cons
 		SCHK(hipEventCreateWithFlags(&b.done, hipEventDisableTiming), "event");
 		SCHK(hipEventCreateWithFlags(&b.freed, hipEventDisableTiming), "event");
 		SCHK(hipMalloc((void **)&b.stage, b.stage_cap), "stream staging");
+		if (any_pageable)
+			SCHK(hipHostMalloc((void **)&b.h_stage, b.stage_cap, hipHostMallocDefault),
+				"pinned staging for pageable tablets");
 		SCHK(hipMalloc((void **)&b.out, b.out_cap), "stream output");
 		SCHK(hipMalloc((void **)&b.d_desc, b.desc_cap * sizeof(vg_tabdesc)), "descriptors");
 		SCHK(hipMalloc((void **)&b.d_tile, b.tile_cap * sizeof(uint32_t)), "tile table");
@@ -504,6 +561,10 @@ static int stream_select(vgd_device *d, const vg_prog &P, const std::vector<cons
 		uint32_t ntile = 0;
 		plan p;
 		b.rows = 0;
+		bool staged = false;
+		for (size_t i = cut[k]; i < cut[k + 1]; i++)
+			staged = staged || pageable[i];
+		std::vector<host_copy> jobs;
 		for (size_t i = cut[k]; i < cut[k + 1]; i++) {
 			const virg_tablet_meta *t = tabs[i];
 			vg_tabdesc &td = b.h_desc[i - cut[k]];
@@ -512,7 +573,9 @@ static int stream_select(vgd_device *d, const vg_prog &P, const std::vector<cons
 				const vg_col &col = P.col[c];
 				const size_t bytes = (size_t)t->rows * col.width;
 				td.col[col.src] = (uint64_t)(b.stage + off);
-				if (bytes)
+				if (staged)
+					jobs.push_back({b.h_stage + off, host_column(t, col.src), bytes});
+				else if (bytes)
 					SCHK(hipMemcpyAsync(b.stage + off, host_column(t, col.src), bytes,
 						hipMemcpyHostToDevice, b.stream), "stream columns");
 				h2d += bytes;
@@ -528,6 +591,13 @@ static int stream_select(vgd_device *d, const vg_prog &P, const std::vector<cons
 			b.rows += t->rows;
 		}
 		scanned += b.rows;
+		if (staged) {
+			/* the buffer's previous batch has finished (stream_finish), so
+			 * its staging copy is no longer being read */
+			parallel_copy(jobs);
+			SCHK(hipMemcpyAsync(b.stage, b.h_stage, off, hipMemcpyHostToDevice, b.stream),
+				"stream columns");
+		}
 		SCHK(hipMemcpyAsync(b.d_desc, b.h_desc, (cut[k + 1] - cut[k]) * sizeof(vg_tabdesc),
 			hipMemcpyHostToDevice, b.stream), "descriptors");
 		if (ntile)

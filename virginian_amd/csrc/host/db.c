@@ -57,14 +57,18 @@ size_t *virg_slot_bytes(virginian *v, unsigned i)
 void *virg_host_alloc(virginian *v, size_t bytes)
 {
 	virg_state *s = virg_state_of(v);
-	if (s && s->pinned) {
+	if (s && s->pinned && (!s->pin_limit || s->pinned_bytes + bytes <= s->pin_limit)) {
 		void *p = vgd_host_alloc(bytes);
-		if (p)
+		if (p) {
+			s->pinned_bytes += bytes;
 			return p;
+		}
 	}
 	void *p = NULL;
 	if (posix_memalign(&p, 4096, bytes) != 0)
 		return NULL;
+	if (s)
+		s->pageable_bytes += bytes;
 	return p;
 }
 
@@ -294,5 +298,27 @@ int virg_close(virginian *v)
 	free(s);
 	v->gpu_slots = NULL;
 	pthread_mutex_destroy(&v->slot_lock);
+	return VIRG_SUCCESS;
+}
+
+/* Slot memory allocated since virg_init: pinned, and pageable (no device,
+ * or pinning failed). */
+int vg_host_memory(virginian *v, unsigned long long *pinned, unsigned long long *pageable)
+{
+	virg_state *s = virg_state_of(v);
+	VIRG_CHECK(s == NULL, "database not open");
+	*pinned = s->pinned_bytes;
+	*pageable = s->pageable_bytes;
+	return VIRG_SUCCESS;
+}
+
+/* Pin at most `bytes` of tablet slot memory (0: as much as the system
+ * allows); later slots are pageable.  Leaves pinned memory for staging and
+ * result buffers when a table is larger than what the system will pin. */
+int vg_host_pin_limit(virginian *v, unsigned long long bytes)
+{
+	virg_state *s = virg_state_of(v);
+	VIRG_CHECK(s == NULL, "database not open");
+	s->pin_limit = bytes;
 	return VIRG_SUCCESS;
 }

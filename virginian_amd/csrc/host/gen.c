@@ -210,9 +210,17 @@ int virg_gen_table(virginian *v, const char *name, uint64_t rows, unsigned table
 	int32_t *ui = (int32_t *)malloc(chunk * 4), *ni5 = (int32_t *)malloc(chunk * 4), *ni20 = (int32_t *)malloc(chunk * 4);
 	float *uf = (float *)malloc(chunk * 4), *nf5 = (float *)malloc(chunk * 4), *nf20 = (float *)malloc(chunk * 4);
 	int ok = id && ui && ni5 && ni20 && uf && nf5 && nf20;
+	/* VIRG_GEN_PROGRESS=n: a line on stderr every n rows (long tables) */
+	const char *pe = getenv("VIRG_GEN_PROGRESS");
+	const uint64_t every = pe ? strtoull(pe, NULL, 10) : 0;
 	for (uint64_t r0 = 0; ok && r0 < rows; r0 += chunk) {
 		uint64_t n = rows - r0 < chunk ? rows - r0 : chunk;
 		virg_gen_rows_mt(seed, r0, n, id, ui, ni5, ni20, uf, nf5, nf20, nthreads);
+		if (every && (r0 + n) / every != r0 / every) {
+			fprintf(stderr, "# generated %llu of %llu rows\n", (unsigned long long)(r0 + n),
+				(unsigned long long)rows);
+			fflush(stderr);
+		}
 		if (tablet_rows) {
 			const void *cols[6] = { ui, ni5, ni20, uf, nf5, nf20 };
 			unsigned cap = (tablet_rows + 15u) & ~15u;

@@ -38,6 +38,9 @@ typedef struct {
 	virg_extslots  ext;
 	vgd_device    *dev;       /* NULL when no GPU is usable */
 	int            pinned;    /* slot memory comes from hipHostMalloc */
+	unsigned long long pinned_bytes;    /* slot memory allocated pinned ... */
+	unsigned long long pageable_bytes;  /* ... and pageable (pinning failed) */
+	unsigned long long pin_limit;       /* pin no slot memory beyond this (0: no limit) */
 } virg_state;
 
 #define VIRG_STATE_MAGIC 0x56474d49u
