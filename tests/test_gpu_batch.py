@@ -64,3 +64,88 @@ def test_graph_batch_rejects_duplicates(gpu_db):
             va.GraphBatch([q, q])
     finally:
         q.close()
+
+
+@pytest.fixture
+def mix(gpu_db):
+    progs = _mix(gpu_db, 64, 9, [1000, 4097, 65_536, 300_001, 1_000_000, 2_500_000, 17, 200_000])
+    want = []
+    for p in progs:
+        w, ill = oracle.select(gpu_db, p)
+        assert ill == 0
+        want.append(w)
+    qs = [gpu_db.prepare(p) for p in progs]
+    yield qs, want
+    for q in qs:
+        q.close()
+
+
+def _check(qs, want, rows, what):
+    for i, (q, w) in enumerate(zip(qs, want)):
+        assert rows[i] == len(next(iter(w.values()))), f"query {i} {what}"
+        assert_same(q.fetch(), w, f"query {i} {what}")
+
+
+@pytest.mark.parametrize("tpi,order,frac", [(1, 0, 1), (8, 1, 1), (64, 2, 3), (3, 1, 7)])
+def test_persistent_batch(mix, tpi, order, frac):
+    """Two persistent launches over item lists give every query its own
+    result, for item sizes, list orders and grids that split the work
+    differently, on every replay, with eager launches in between."""
+    qs, want = mix
+    b = va.PersistentBatch(qs, tiles_per_item=tpi, order=order)
+    try:
+        if frac > 1:
+            cfg = b.config()
+            b.configure(tpi, order, max(1, cfg["count_wgs"] // frac), max(1, cfg["write_wgs"] // frac))
+        for replay in range(2):
+            print(f"tpi {tpi} order {order} replay {replay}", flush=True)
+            rows, lat = b.run()
+            _check(qs, want, rows, f"tpi {tpi} replay {replay}")
+            assert all(x >= 0 for x in lat) and max(lat) > 0
+            for i in range(0, len(qs), 9):
+                qs[i].run()
+                assert_same(qs[i].fetch(), want[i], f"query {i} eager")
+    finally:
+        b.close()
+
+
+def test_tuned_batch_and_plain(mix):
+    """The autotuner's pick gives the same results, and so do the plain
+    per-query launches it is compared with."""
+    qs, want = mix
+    b = va.PersistentBatch(qs)
+    try:
+        cfg, ms = b.tune(iters=1)
+        print("tuned", cfg, ms, flush=True)
+        assert ms > 0
+        rows, lat = b.run()
+        _check(qs, want, rows, "tuned")
+    finally:
+        b.close()
+    ms, lat = va.plain_time(qs, 1, 2)
+    assert ms > 0 and len(lat) == len(qs) and lat[-1] > 0
+    rows = [q.sync() for q in qs]
+    _check(qs, want, rows, "plain")
+
+
+def test_persistent_batch_overflow(gpu_db):
+    gpu_db.gen_table("test", 100_000, 1 << 15, seed=4)
+    q1 = gpu_db.prepare("select id from test where uniformi < 50", capacity=1000)
+    q2 = gpu_db.prepare("select id from test where uniformi < 5")
+    b = va.PersistentBatch([q1, q2])
+    try:
+        with pytest.raises(va.VirginianError):
+            b.run()
+        # the next run of a batch without the overflowing query is fine
+    finally:
+        b.close()
+    b = va.PersistentBatch([q2])
+    try:
+        rows, _ = b.run()
+        want, _ = oracle.select(gpu_db, "select id from test where uniformi < 5")
+        assert rows[0] == len(want["id"])
+        assert_same(q2.fetch(), want, "after overflow")
+    finally:
+        b.close()
+        q1.close()
+        q2.close()

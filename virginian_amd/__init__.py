@@ -130,6 +130,15 @@ class StreamStats(C.Structure):
                 ("seconds", C.c_double), ("batches", C.c_uint)]
 
 
+class PBatchConfig(C.Structure):
+    """vg_pbatch_config (include/virg_gpu.h)."""
+    _fields_ = [("tiles_per_item", C.c_uint), ("count_wgs", C.c_uint), ("write_wgs", C.c_uint),
+                ("order", C.c_uint)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 def _load() -> C.CDLL:
     if not _LIB_PATH.exists():
         raise VirginianError(
@@ -190,6 +199,17 @@ def _load() -> C.CDLL:
         "vg_batch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "vg_host_memory": (C.c_int, [V, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
         "vg_host_pin_limit": (C.c_int, [V, C.c_ulonglong]),
+        "vg_pbatch_create": (P, [C.POINTER(P), C.c_uint, C.POINTER(PBatchConfig), C.c_char_p, C.c_size_t]),
+        "vg_pbatch_free": (None, [P]),
+        "vg_pbatch_configure": (C.c_int, [P, C.POINTER(PBatchConfig), C.c_char_p, C.c_size_t]),
+        "vg_pbatch_get_config": (C.c_int, [P, C.POINTER(PBatchConfig)]),
+        "vg_pbatch_launch": (C.c_int, [P]),
+        "vg_pbatch_sync": (C.c_int, [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_double)]),
+        "vg_pbatch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "vg_pbatch_tune": (C.c_int, [P, C.c_int, C.POINTER(PBatchConfig), C.POINTER(C.c_double),
+                                     C.c_char_p, C.c_size_t]),
+        "vg_plain_time": (C.c_int, [C.POINTER(P), C.c_uint, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
         "vg_host_alloc": (P, [C.c_size_t]),
         "vg_host_free": (None, [P]),
         "vg_link_probe": (C.c_int, [V, C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -529,6 +549,14 @@ class DeviceQuery:
         self.last_rows = n.value
         return n.value
 
+    def sync(self) -> int:
+        """Wait for the last launch of this query; returns its rows."""
+        n = C.c_ulonglong(0)
+        if lib.vg_query_sync(self.h, C.byref(n)) != VIRG_SUCCESS:
+            raise VirginianError("vg_query_sync failed")
+        self.last_rows = n.value
+        return n.value
+
     def fetch(self) -> dict:
         outs = [np.empty(self.last_rows, dtype=NP_TYPE[t]) for t in self.types]
         ptrs = (C.c_void_p * max(len(outs), 1))(*[o.ctypes.data for o in outs])
@@ -602,3 +630,80 @@ class GraphBatch:
             self.close()
         except Exception:
             pass
+
+
+def _handles(queries) -> C.Array:
+    return (C.c_void_p * max(len(queries), 1))(*[q.h for q in queries])
+
+
+class PersistentBatch:
+    """Prepared queries run by two persistent launches over device item
+    lists, captured as one HIP graph (vg_pbatch_* in include/virg_gpu.h)."""
+
+    def __init__(self, queries: list[DeviceQuery], tiles_per_item: int = 0, order: int = 0,
+                 count_wgs: int = 0, write_wgs: int = 0):
+        self.queries = list(queries)
+        cfg = PBatchConfig(tiles_per_item, count_wgs, write_wgs, order)
+        err = C.create_string_buffer(256)
+        self.h = lib.vg_pbatch_create(_handles(self.queries), len(self.queries), C.byref(cfg), err, 256)
+        if not self.h:
+            raise VirginianError(err.value.decode() or "vg_pbatch_create failed")
+
+    def configure(self, tiles_per_item: int = 0, order: int = 0, count_wgs: int = 0,
+                  write_wgs: int = 0) -> None:
+        cfg = PBatchConfig(tiles_per_item, count_wgs, write_wgs, order)
+        err = C.create_string_buffer(256)
+        if lib.vg_pbatch_configure(self.h, C.byref(cfg), err, 256) != VIRG_SUCCESS:
+            raise VirginianError(err.value.decode() or "vg_pbatch_configure failed")
+
+    def config(self) -> dict:
+        cfg = PBatchConfig()
+        lib.vg_pbatch_get_config(self.h, C.byref(cfg))
+        return cfg.as_dict()
+
+    def run(self) -> tuple[list[int], list[float]]:
+        """(rows per query, latency ms per query) of one run."""
+        n = len(self.queries)
+        rows = (C.c_ulonglong * max(n, 1))()
+        lat = (C.c_double * max(n, 1))()
+        if lib.vg_pbatch_launch(self.h) != VIRG_SUCCESS or lib.vg_pbatch_sync(self.h, rows, lat) != VIRG_SUCCESS:
+            raise VirginianError("batch run failed")
+        for q, r in zip(self.queries, rows):
+            q.last_rows = r
+        return list(rows)[:n], list(lat)[:n]
+
+    def time(self, warmup: int, iters: int) -> float:
+        ms = C.c_double(0)
+        if lib.vg_pbatch_time(self.h, warmup, iters, C.byref(ms)) != VIRG_SUCCESS:
+            raise VirginianError("vg_pbatch_time failed")
+        return ms.value
+
+    def tune(self, iters: int = 5) -> tuple[dict, float]:
+        best = PBatchConfig()
+        ms = C.c_double(0)
+        err = C.create_string_buffer(256)
+        if lib.vg_pbatch_tune(self.h, iters, C.byref(best), C.byref(ms), err, 256) != VIRG_SUCCESS:
+            raise VirginianError(err.value.decode() or "vg_pbatch_tune failed")
+        return best.as_dict(), ms.value
+
+    def close(self) -> None:
+        if self.h:
+            lib.vg_pbatch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plain_time(queries: list[DeviceQuery], warmup: int, iters: int) -> tuple[float, list[float]]:
+    """(ms per round, per-query latency ms of the last round) launching the
+    queries one after another on one stream."""
+    n = len(queries)
+    ms = C.c_double(0)
+    lat = (C.c_double * max(n, 1))()
+    if lib.vg_plain_time(_handles(queries), n, warmup, iters, C.byref(ms), lat) != VIRG_SUCCESS:
+        raise VirginianError("vg_plain_time failed")
+    return ms.value, list(lat)[:n]

@@ -20,6 +20,7 @@
 #include <string.h>
 #include <stdlib.h>
 #include <stdarg.h>
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <thread>
@@ -1356,5 +1357,402 @@ extern "C" int vg_batch_time(vg_batch *b, int warmup, int iters, double *ms)
 		return VIRG_FAIL;
 	}
 	*ms = iters > 0 ? t / iters : 0.0;
+	return VIRG_SUCCESS;
+}
+
+/* ------------------------------------------------------------ vg_pbatch */
+/* Many prepared queries run by two persistent launches (count items, then
+ * write items) that work through device-resident item lists, the three
+ * launches (reset, count, write) captured once as a HIP graph.  Item size,
+ * list order and grid sizes are the tunables (vg_pbatch_tune). */
+extern "C" size_t vg_kernel_batch_lds(int pass, const vg_prog *P);
+extern "C" uint32_t vg_kernel_batch_grid(int pass, size_t lds);
+extern "C" int vg_kernel_batch_launch(const vg_batch_args *A, uint32_t cgrid, size_t clds, uint32_t wgrid,
+	size_t wlds, hipStream_t stream);
+
+struct vg_pbatch {
+	std::vector<vg_query *> qs;
+	vgd_device *d = nullptr;
+	hipStream_t stream = nullptr;
+	vg_prog *d_progs = nullptr;
+	vg_launch *d_launch = nullptr;
+	vg_bqinfo *d_info = nullptr;
+	vg_bquery *d_q = nullptr;
+	vg_bhead *d_head = nullptr;
+	vg_item *d_items = nullptr, *d_witems = nullptr;
+	uint32_t *d_icount = nullptr;
+	uint64_t *d_ibase = nullptr;
+	size_t items_cap = 0;
+	vg_batch_args A;
+	vg_pbatch_config cfg;
+	size_t clds = 0, wlds = 0;
+	uint32_t cgrid = 0, wgrid = 0;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	double clock_khz = 0;
+	std::vector<vg_bquery> h_q;
+};
+
+static void pbatch_drop_graph(vg_pbatch *b)
+{
+	if (b->exec) (void)hipGraphExecDestroy(b->exec);
+	if (b->graph) (void)hipGraphDestroy(b->graph);
+	b->exec = nullptr;
+	b->graph = nullptr;
+}
+
+extern "C" void vg_pbatch_free(vg_pbatch *b)
+{
+	if (!b)
+		return;
+	if (b->stream)
+		(void)hipStreamSynchronize(b->stream);
+	pbatch_drop_graph(b);
+	void *bufs[] = { b->d_progs, b->d_launch, b->d_info, b->d_q, b->d_head, b->d_items, b->d_witems,
+		b->d_icount, b->d_ibase };
+	for (void *p : bufs)
+		if (p)
+			(void)hipFree(p);
+	if (b->stream)
+		(void)hipStreamDestroy(b->stream);
+	delete b;
+}
+
+/* the kernel arguments of one query's launch (as launch() builds them) */
+static void fill_launch(const vg_query *q, vg_launch &L)
+{
+	memset(&L, 0, sizeof L);
+	L.tabs = q->p.d_desc;
+	L.tile_tab = q->p.d_tile;
+	L.num_tiles = q->p.num_tiles;
+	L.capacity = q->capacity;
+	L.ctrl = q->lr.ctrl;
+	L.vbits = q->lr.vbits;
+	L.tcount = q->lr.tcount;
+	L.wgcount = q->lr.wgcount;
+	for (unsigned j = 0; j < q->P.nout; j++)
+		L.out[j] = q->out[j];
+}
+
+/* Build the item lists for b->cfg, upload them and capture the graph. */
+static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
+{
+	const unsigned n = (unsigned)b->qs.size();
+	const uint32_t tpi = b->cfg.tiles_per_item ? b->cfg.tiles_per_item : 8;
+	std::vector<vg_bqinfo> info(n);
+	uint32_t ioff = 0;
+	for (unsigned i = 0; i < n; i++) {
+		const uint32_t nt = b->qs[i]->p.num_tiles;
+		info[i].nitem = (nt + tpi - 1) / tpi;
+		info[i].ioff = ioff;
+		ioff += info[i].nitem;
+	}
+	/* count list: largest query first (the small items come last and
+	 * even out the end); write list: in the configured order */
+	std::vector<unsigned> by_size(n), worder(n);
+	for (unsigned i = 0; i < n; i++)
+		by_size[i] = worder[i] = i;
+	std::stable_sort(by_size.begin(), by_size.end(), [&](unsigned x, unsigned y) {
+		return b->qs[x]->p.num_tiles > b->qs[y]->p.num_tiles; });
+	if (b->cfg.order == 1)
+		worder.assign(by_size.rbegin(), by_size.rend());
+	else if (b->cfg.order == 2)
+		worder = by_size;
+	std::vector<vg_item> items, witems;
+	auto add = [&](std::vector<vg_item> &v, unsigned q) {
+		const uint32_t nt = b->qs[q]->p.num_tiles;
+		for (uint32_t k = 0; k < info[q].nitem; k++)
+			v.push_back({q, k * tpi, (k + 1) * tpi < nt ? (k + 1) * tpi : nt, k});
+	};
+	for (unsigned q : by_size)
+		add(items, q);
+	for (unsigned q : worder)
+		add(witems, q);
+	const size_t ni = items.size();
+	if (ni > b->items_cap) {
+		if (b->d_items) (void)hipFree(b->d_items);
+		if (b->d_witems) (void)hipFree(b->d_witems);
+		if (b->d_icount) (void)hipFree(b->d_icount);
+		if (b->d_ibase) (void)hipFree(b->d_ibase);
+		b->d_items = b->d_witems = nullptr;
+		b->d_icount = nullptr;
+		b->d_ibase = nullptr;
+		b->items_cap = 0;
+		if (herr(hipMalloc((void **)&b->d_items, ni * sizeof(vg_item)), "batch items") ||
+			herr(hipMalloc((void **)&b->d_witems, ni * sizeof(vg_item)), "batch items") ||
+			herr(hipMalloc((void **)&b->d_icount, ni * sizeof(uint32_t)), "batch counts") ||
+			herr(hipMalloc((void **)&b->d_ibase, ni * sizeof(uint64_t)), "batch offsets"))
+			return qerr(err, errlen, "cannot allocate the batch's item lists");
+		b->items_cap = ni;
+	}
+	if ((ni && (herr(hipMemcpy(b->d_items, items.data(), ni * sizeof(vg_item), hipMemcpyHostToDevice),
+			"batch items") ||
+		herr(hipMemcpy(b->d_witems, witems.data(), ni * sizeof(vg_item), hipMemcpyHostToDevice),
+			"batch items"))) ||
+		herr(hipMemcpy(b->d_info, info.data(), n * sizeof(vg_bqinfo), hipMemcpyHostToDevice), "batch info"))
+		return qerr(err, errlen, "cannot upload the batch's item lists");
+	vg_batch_args &A = b->A;
+	memset(&A, 0, sizeof A);
+	A.progs = b->d_progs;
+	A.launch = b->d_launch;
+	A.info = b->d_info;
+	A.items = b->d_items;
+	A.witems = b->d_witems;
+	A.nitems = (uint32_t)ni;
+	A.nquery = n;
+	A.q = b->d_q;
+	A.icount = b->d_icount;
+	A.ibase = b->d_ibase;
+	A.head = b->d_head;
+	const uint32_t cmax = vg_kernel_batch_grid(0, b->clds), wmax = vg_kernel_batch_grid(1, b->wlds);
+	b->cgrid = b->cfg.count_wgs && b->cfg.count_wgs < cmax ? b->cfg.count_wgs : cmax;
+	b->wgrid = b->cfg.write_wgs && b->cfg.write_wgs < wmax ? b->cfg.write_wgs : wmax;
+	const uint32_t most = ni ? (uint32_t)ni : 1;
+	b->cgrid = b->cgrid < most ? b->cgrid : most;
+	b->wgrid = b->wgrid < most ? b->wgrid : most;
+	pbatch_drop_graph(b);
+	if (herr(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture"))
+		return qerr(err, errlen, "graph capture failed");
+	const int lrc = vg_kernel_batch_launch(&A, b->cgrid, b->clds, b->wgrid, b->wlds, b->stream);
+	const hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
+	if (lrc != 0 || ec != hipSuccess) {
+		(void)hipGetLastError();
+		return qerr(err, errlen, "graph capture failed (%d)", lrc);
+	}
+	if (herr(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate"))
+		return qerr(err, errlen, "graph instantiate failed");
+	return VIRG_SUCCESS;
+}
+
+extern "C" vg_pbatch *vg_pbatch_create(vg_query *const *qs, unsigned n, const vg_pbatch_config *cfg,
+	char *err, size_t errlen)
+{
+	if (n == 0) {
+		qerr(err, errlen, "empty batch");
+		return nullptr;
+	}
+	vg_pbatch *b = new vg_pbatch();
+	b->qs.assign(qs, qs + n);
+	memset(&b->cfg, 0, sizeof b->cfg);
+	if (cfg)
+		b->cfg = *cfg;
+#define PCHK(cond, ...) do { if (cond) { (void)hipGetLastError(); qerr(err, errlen, __VA_ARGS__); \
+	vg_pbatch_free(b); return nullptr; } } while (0)
+	for (unsigned i = 0; i < n; i++) {
+		PCHK(!qs[i] || qs[i]->d != qs[0]->d, "batch queries must be prepared on one device");
+		for (unsigned j = 0; j < i; j++)
+			PCHK(qs[j] == qs[i], "query %u appears twice in the batch", i);
+	}
+	b->d = qs[0]->d;
+	PCHK(hipSetDevice(b->d->device) != hipSuccess, "set device");
+	PCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess, "stream");
+	int khz = 0;
+	PCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->d->device) != hipSuccess || khz <= 0,
+		"device clock rate");
+	b->clock_khz = khz;
+	std::vector<vg_prog> progs(n);
+	std::vector<vg_launch> launches(n);
+	for (unsigned i = 0; i < n; i++) {
+		vg_query *q = qs[i];
+		PCHK(ensure_lr(q->lr, q->p.num_tiles, vg_kernel_grid(&q->P, q->p.num_tiles ? q->p.num_tiles : 1),
+			q->stream) == VIRG_FAIL, "cannot set up query %u", i);
+		PCHK(hipStreamSynchronize(q->stream) != hipSuccess, "query setup");
+		progs[i] = q->P;
+		fill_launch(q, launches[i]);
+		const size_t c = vg_kernel_batch_lds(0, &q->P), w = vg_kernel_batch_lds(1, &q->P);
+		b->clds = c > b->clds ? c : b->clds;
+		b->wlds = w > b->wlds ? w : b->wlds;
+	}
+	PCHK(hipMalloc((void **)&b->d_progs, n * sizeof(vg_prog)) != hipSuccess ||
+		hipMalloc((void **)&b->d_launch, n * sizeof(vg_launch)) != hipSuccess ||
+		hipMalloc((void **)&b->d_info, n * sizeof(vg_bqinfo)) != hipSuccess ||
+		hipMalloc((void **)&b->d_q, n * sizeof(vg_bquery)) != hipSuccess ||
+		hipMalloc((void **)&b->d_head, sizeof(vg_bhead)) != hipSuccess, "cannot allocate the batch");
+	PCHK(hipMemcpy(b->d_progs, progs.data(), n * sizeof(vg_prog), hipMemcpyHostToDevice) != hipSuccess ||
+		hipMemcpy(b->d_launch, launches.data(), n * sizeof(vg_launch), hipMemcpyHostToDevice) != hipSuccess,
+		"cannot upload the batch");
+#undef PCHK
+	b->h_q.resize(n);
+	if (pbatch_build(b, err, errlen) == VIRG_FAIL) {
+		vg_pbatch_free(b);
+		return nullptr;
+	}
+	return b;
+}
+
+extern "C" int vg_pbatch_configure(vg_pbatch *b, const vg_pbatch_config *cfg, char *err, size_t errlen)
+{
+	HCHK(hipStreamSynchronize(b->stream), "batch");
+	b->cfg = *cfg;
+	return pbatch_build(b, err, errlen);
+}
+
+extern "C" int vg_pbatch_get_config(vg_pbatch *b, vg_pbatch_config *cfg)
+{
+	*cfg = b->cfg;
+	cfg->count_wgs = b->cgrid;
+	cfg->write_wgs = b->wgrid;
+	return VIRG_SUCCESS;
+}
+
+extern "C" int vg_pbatch_launch(vg_pbatch *b)
+{
+	HCHK(hipGraphLaunch(b->exec, b->stream), "graph launch");
+	return VIRG_SUCCESS;
+}
+
+/* Wait for the run; rows[i] = rows query i selected, latency_ms[i] = its
+ * results complete, from the start of the run (device clock); either may
+ * be NULL.  Fails when a query's result exceeded its capacity. */
+extern "C" int vg_pbatch_sync(vg_pbatch *b, unsigned long long *rows, double *latency_ms)
+{
+	HCHK(hipStreamSynchronize(b->stream), "batch");
+	const size_t n = b->qs.size();
+	if (latency_ms) {
+		vg_bhead h;
+		HCHK(hipMemcpy(b->h_q.data(), b->d_q, n * sizeof(vg_bquery), hipMemcpyDeviceToHost), "batch state");
+		HCHK(hipMemcpy(&h, b->d_head, sizeof h, hipMemcpyDeviceToHost), "batch state");
+		for (size_t i = 0; i < n; i++)
+			latency_ms[i] = b->h_q[i].t_done > h.t_start
+				? (double)(b->h_q[i].t_done - h.t_start) / b->clock_khz : 0.0;
+	}
+	for (size_t i = 0; i < n; i++) {
+		vg_query *q = b->qs[i];
+		uint64_t r = 0;
+		if (q->p.num_tiles && read_ctrl(q->lr, q->stream, &r) == VIRG_FAIL)
+			return VIRG_FAIL;
+		q->last_rows = r;
+		if (rows)
+			rows[i] = r;
+	}
+	return VIRG_SUCCESS;
+}
+
+/* mean ms per run of `iters` graph replays after `warmup`, HIP events */
+static int graph_time(hipGraphExec_t exec, hipStream_t s, int warmup, int iters, double *ms)
+{
+	hipEvent_t e0, e1;
+	HCHK(hipEventCreate(&e0), "event");
+	if (herr(hipEventCreate(&e1), "event")) {
+		(void)hipEventDestroy(e0);
+		return VIRG_FAIL;
+	}
+	int ok = 1;
+	for (int i = 0; i < warmup && ok; i++)
+		ok = hipGraphLaunch(exec, s) == hipSuccess;
+	ok = ok && hipEventRecord(e0, s) == hipSuccess;
+	for (int i = 0; i < iters && ok; i++)
+		ok = hipGraphLaunch(exec, s) == hipSuccess;
+	ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+	float t = 0;
+	ok = ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (!ok) {
+		herr(hipGetLastError(), "graph timing");
+		return VIRG_FAIL;
+	}
+	*ms = iters > 0 ? t / iters : 0.0;
+	return VIRG_SUCCESS;
+}
+
+extern "C" int vg_pbatch_time(vg_pbatch *b, int warmup, int iters, double *ms)
+{
+	return graph_time(b->exec, b->stream, warmup, iters, ms);
+}
+
+/* Autotuner: time every candidate configuration (tiles per item x write
+ * order x grid size) over `iters` replays on this batch and keep the
+ * fastest; *best (optional) receives it, *best_ms its time. */
+extern "C" int vg_pbatch_tune(vg_pbatch *b, int iters, vg_pbatch_config *best, double *best_ms,
+	char *err, size_t errlen)
+{
+	static const unsigned tpis[] = { 1, 2, 4, 8, 16, 32, 64 };
+	static const unsigned orders[] = { 0, 1, 2 };
+	const uint32_t cmax = vg_kernel_batch_grid(0, b->clds), wmax = vg_kernel_batch_grid(1, b->wlds);
+	const unsigned fr[] = { 1, 2 };          /* grid = max / fr */
+	vg_pbatch_config top;
+	memset(&top, 0, sizeof top);
+	double top_ms = 1e30;
+	for (unsigned t : tpis)
+		for (unsigned o : orders)
+			for (unsigned f : fr) {
+				vg_pbatch_config c;
+				memset(&c, 0, sizeof c);
+				c.tiles_per_item = t;
+				c.order = o;
+				c.count_wgs = cmax / f;
+				c.write_wgs = wmax / f;
+				double ms = 0;
+				if (vg_pbatch_configure(b, &c, err, errlen) == VIRG_FAIL ||
+					vg_pbatch_time(b, 1, iters, &ms) == VIRG_FAIL)
+					return VIRG_FAIL;
+				if (ms < top_ms) {
+					top_ms = ms;
+					top = c;
+				}
+			}
+	if (vg_pbatch_configure(b, &top, err, errlen) == VIRG_FAIL)
+		return VIRG_FAIL;
+	if (best)
+		*best = top;
+	if (best_ms)
+		*best_ms = top_ms;
+	return VIRG_SUCCESS;
+}
+
+/* The plain way: each query's count and write launches, one query after
+ * another on one stream.  Mean ms per round over `iters` rounds after
+ * `warmup`; latency_ms[i] (optional) = query i's results complete, from
+ * the start of the last round (HIP events). */
+extern "C" int vg_plain_time(vg_query *const *qs, unsigned n, int warmup, int iters, double *ms,
+	double *latency_ms)
+{
+	if (n == 0)
+		return VIRG_FAIL;
+	HCHK(hipSetDevice(qs[0]->d->device), "set device");
+	hipStream_t s;
+	HCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
+	std::vector<hipEvent_t> ev(n + 2, nullptr);
+	int ok = 1;
+	for (auto &e : ev)
+		ok = ok && hipEventCreate(&e) == hipSuccess;
+	for (unsigned i = 0; i < n && ok; i++)
+		ok = ensure_lr(qs[i]->lr, qs[i]->p.num_tiles, vg_kernel_grid(&qs[i]->P, qs[i]->p.num_tiles ?
+			qs[i]->p.num_tiles : 1), qs[i]->stream) == VIRG_SUCCESS &&
+			hipStreamSynchronize(qs[i]->stream) == hipSuccess;
+	auto round = [&](bool mark) {
+		for (unsigned i = 0; i < n && ok; i++) {
+			ok = launch(qs[i]->P, qs[i]->p, qs[i]->lr, qs[i]->out, qs[i]->capacity, s) == VIRG_SUCCESS;
+			if (mark)
+				ok = ok && hipEventRecord(ev[i + 2], s) == hipSuccess;
+		}
+	};
+	for (int w = 0; w < warmup && ok; w++)
+		round(false);
+	ok = ok && hipEventRecord(ev[0], s) == hipSuccess;
+	for (int it = 0; it + 1 < iters && ok; it++)
+		round(false);
+	ok = ok && hipEventRecord(ev[1], s) == hipSuccess;   /* start of the last round */
+	round(true);
+	ok = ok && hipStreamSynchronize(s) == hipSuccess;
+	float t = 0;
+	ok = ok && hipEventElapsedTime(&t, ev[0], ev[n + 1]) == hipSuccess;
+	if (ok) {
+		*ms = iters > 0 ? t / iters : 0.0;
+		for (unsigned i = 0; i < n && ok && latency_ms; i++) {
+			float l = 0;
+			ok = hipEventElapsedTime(&l, ev[1], ev[i + 2]) == hipSuccess;
+			latency_ms[i] = l;
+		}
+	}
+	for (auto &e : ev)
+		if (e)
+			(void)hipEventDestroy(e);
+	(void)hipStreamDestroy(s);
+	if (!ok) {
+		herr(hipGetLastError(), "plain launches");
+		return VIRG_FAIL;
+	}
 	return VIRG_SUCCESS;
 }

@@ -148,4 +148,51 @@ typedef struct {
 	void *out[VG_MAX_OUT];
 } vg_launch;
 
+/* ---- batched execution: two persistent launches run many queries ----
+ * Work items are (query, tile range).  Every query's tiles are cut into
+ * items of at most tiles_per_item tiles; count item k and write item k of
+ * a query cover the same tiles.  The count launch works through the count
+ * list and the write launch through the write list (any order each), every
+ * workgroup taking the next item from an atomic counter. */
+typedef struct {
+	uint32_t query;
+	uint32_t t0, t1;         /* tiles [t0, t1) of the query's launch */
+	uint32_t k;              /* item index within the query */
+} vg_item;
+
+typedef struct {
+	uint32_t nitem;          /* items of the query in each pass */
+	uint32_t ioff;           /* its first entry of icount / ibase */
+} vg_bqinfo;
+
+/* per-query state, cleared before every run */
+typedef struct {
+	uint32_t done;           /* count items finished */
+	uint32_t pad0;
+	uint32_t wdone;          /* write items finished */
+	uint32_t pad;
+	uint64_t t_done;         /* device clock when its last write item finished */
+	uint64_t pad2;
+} vg_bquery;
+
+typedef struct {
+	uint32_t next;           /* next count item to hand out */
+	uint32_t wnext;          /* next write item to hand out */
+	uint64_t t_start;        /* device clock when the run started */
+} vg_bhead;
+
+typedef struct {
+	const vg_prog *progs;    /* one per query */
+	const vg_launch *launch; /* one per query */
+	const vg_bqinfo *info;
+	const vg_item *items;    /* count list */
+	const vg_item *witems;   /* write list (the same items, maybe reordered) */
+	uint32_t nitems;
+	uint32_t nquery;
+	vg_bquery *q;
+	uint32_t *icount;        /* rows selected by each count item */
+	uint64_t *ibase;         /* output row of each item's first selected row */
+	vg_bhead *head;
+} vg_batch_args;
+
 #endif

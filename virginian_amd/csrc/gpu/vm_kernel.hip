@@ -568,7 +568,7 @@ __device__ __forceinline__ void cursor_init(cursor &C, const vg_prog &P, uint32_
 {
 	C.first = C.end = C.rows = C.aligned = 0;
 	C.cd_lo = C.cd_hi = C.cb_lo = C.cb_hi = 0;
-	if (lane < P.ncol) {
+	if (lane < ldc(&P.ncol)) {
 		const uint32_t *w = (const uint32_t *)&P.col[lane];
 		C.cd_lo = w[0];
 		C.cd_hi = w[1];
@@ -581,14 +581,14 @@ __device__ __forceinline__ void cursor_seek(cursor &C, const vg_prog &P, const v
 {
 	if (tile < C.end)
 		return;
-	const uint32_t tb = ldc(&L.tile_tab[tile]);
-	const vg_tabdesc *D = &L.tabs[tb];
+	const uint32_t tb = ldc(ldc(&L.tile_tab) + tile);
+	const vg_tabdesc *D = ldc(&L.tabs) + tb;
 	C.first = ldc(&D->first_tile);
 	C.rows = ldc(&D->rows);
 	C.end = C.first + (C.rows + VG_T - 1) / VG_T;
 	C.aligned = ldc(&D->aligned);
 	uint64_t b = 0;
-	if (lane < P.ncol)
+	if (lane < ldc(&P.ncol))
 		b = gptr<uint64_t>(&D->col[0])[C.cd_lo & 0xFFu];   /* vg_col.src is byte 0 */
 	C.cb_lo = (uint32_t)b;
 	C.cb_hi = (uint32_t)(b >> 32);
@@ -620,7 +620,7 @@ __device__ __forceinline__ void load_cols(const vg_prog &P, const cursor &C, uin
 	const strips<NS> &S, uint32_t nrows, bool write_pass, uint32_t *regs, bool wait = true)
 {
 	const bool aligned = C.aligned != 0;
-	for (uint32_t k = 0; k < P.nload; ++k) {
+	for (uint32_t k = 0; k < ldc(&P.nload); ++k) {
 		const vg_col c = cursor_col(C, k);
 		if (write_pass ? !c.wload : c.phase != VG_PHASE_FILTER)
 			continue;
@@ -649,18 +649,17 @@ __device__ __forceinline__ void load_cols(const vg_prog &P, const cursor &C, uin
  *   NS = 1 (more slots): the 4 waves share each tile, wave q taking strip
  *     q as in the write pass; one register file of nslot_count * T words
  *     (a wave only touches its own rows of it). */
+/* Count tiles [t0, t1) of a launch with the 4 waves of the workgroup;
+ * returns the rows selected in them (in every thread). */
 template <int NS>
-__global__ __launch_bounds__(256) void vg_count_kernel(const vg_prog P, const vg_launch L)
+__device__ __forceinline__ uint32_t count_tiles(const vg_prog &P, const vg_launch &L, uint32_t t0,
+	uint32_t t1, uint32_t *lds)
 {
 	constexpr uint32_t T = VG_T;
-	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
 	__shared__ uint32_t s_wcount[4];
 	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
-	uint32_t *regs = NS == 4 ? lds + wave * P.nslot_count * T : lds;
+	uint32_t *regs = NS == 4 ? lds + wave * ldc(&P.nslot_count) * T : lds;
 	const strips<NS> S{NS == 4 ? 0u : wave * 256u, 256u, lane};
-	const uint32_t g = blockIdx.x;
-	const uint32_t t0 = g * L.tiles_per_wg;
-	const uint32_t t1 = min(t0 + L.tiles_per_wg, L.num_tiles);
 	uint32_t count = 0;
 	cursor C;
 	cursor_init(C, P, lane);
@@ -681,33 +680,46 @@ __global__ __launch_bounds__(256) void vg_count_kernel(const vg_prog P, const vg
 		rowstate RS;
 		rows_init(RS, live);
 		load_cols<NS, T>(P, C, row0, S, nrows, false, regs);
-		run_uops<NS, T>(P, 0, P.nfilter, ~0ull, regs, S, RS);
+		run_uops<NS, T>(P, 0, ldc(&P.nfilter), ~0ull, regs, S, RS);
 		const uint32_t v = RS.valid;
 		if constexpr (NS == 4) {
 #pragma unroll
 			for (int q = 0; q < 4; ++q)
-				L.vbits[(size_t)tile * 256 + 64 * q + lane] = (uint8_t)((v >> (4 * q)) & 0xFu);
+				ldc(&L.vbits)[(size_t)tile * 256 + 64 * q + lane] = (uint8_t)((v >> (4 * q)) & 0xFu);
 			/* per-strip counts summed over the wave, two 16-bit counts a word */
 			const uint32_t n01 = wave_sum((uint32_t)__builtin_popcount(v & 0xFu) |
 				((uint32_t)__builtin_popcount(v & 0xF0u) << 16));
 			const uint32_t n23 = wave_sum((uint32_t)__builtin_popcount(v & 0xF00u) |
 				((uint32_t)__builtin_popcount(v & 0xF000u) << 16));
 			if (lane < 4)
-				L.tcount[(size_t)tile * 4 + lane] = ((lane < 2 ? n01 : n23) >> (16 * (lane & 1))) & 0xFFFFu;
+				ldc(&L.tcount)[(size_t)tile * 4 + lane] = ((lane < 2 ? n01 : n23) >> (16 * (lane & 1))) & 0xFFFFu;
 			count += (n01 & 0xFFFFu) + (n01 >> 16) + (n23 & 0xFFFFu) + (n23 >> 16);
 		} else {
-			L.vbits[(size_t)tile * 256 + 64 * wave + lane] = (uint8_t)v;
+			ldc(&L.vbits)[(size_t)tile * 256 + 64 * wave + lane] = (uint8_t)v;
 			const uint32_t n = wave_sum((uint32_t)__builtin_popcount(v));
 			if (lane == 0)
-				L.tcount[(size_t)tile * 4 + wave] = n;
+				ldc(&L.tcount)[(size_t)tile * 4 + wave] = n;
 			count += n;
 		}
 	}
 	if (lane == 0)
 		s_wcount[wave] = count;
 	__syncthreads();
-	if (tid == 0)
-		L.wgcount[g] = s_wcount[0] + s_wcount[1] + s_wcount[2] + s_wcount[3];
+	const uint32_t n = s_wcount[0] + s_wcount[1] + s_wcount[2] + s_wcount[3];
+	__syncthreads();                /* s_wcount is reused by the next call */
+	return n;
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void vg_count_kernel(const vg_prog P, const vg_launch L)
+{
+	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+	const uint32_t g = blockIdx.x;
+	const uint32_t t0 = g * L.tiles_per_wg;
+	const uint32_t t1 = min(t0 + L.tiles_per_wg, L.num_tiles);
+	const uint32_t n = count_tiles<NS>(P, L, t0, t1, lds);
+	if (threadIdx.x == 0)
+		L.wgcount[g] = n;
 }
 
 /* ----------------------------------------------------------- write pass */
@@ -749,7 +761,7 @@ __device__ __forceinline__ uint32_t next_work(const uint32_t (*cnt)[4], uint32_t
 __device__ __forceinline__ uint32_t load_bits(const vg_launch &L, uint32_t tile, uint32_t wave,
 	uint32_t lane)
 {
-	const uint8_t *v = L.vbits + (size_t)tile * 256 + 64 * VG_WS * wave + lane;
+	const uint8_t *v = ldc(&L.vbits) + (size_t)tile * 256 + 64 * VG_WS * wave + lane;
 	uint32_t b = 0;
 #pragma unroll
 	for (int s = 0; s < VG_WS; ++s)
@@ -771,40 +783,19 @@ __device__ __forceinline__ void issue_tile(const vg_prog &P, const vg_launch &L,
 		load_cols<VG_WS, VG_T>(P, C, row0, S, nrows, true, regs, false);
 }
 
-__global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_prog P, const vg_launch L)
+/* Write the selected rows of tiles [t0, t1) of a launch, the first at
+ * output row `base`; returns the output row after the last. */
+__device__ __forceinline__ uint64_t write_tiles(const vg_prog &P, const vg_launch &L, uint32_t t0,
+	uint32_t t1, uint64_t base, uint32_t *lds)
 {
 	constexpr uint32_t T = VG_T;
-	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
 	__shared__ uint32_t s_cnt[VG_CHUNK][4];
 	__shared__ uint64_t s_off[VG_CHUNK];
-	__shared__ uint64_t s_base;
 	uint32_t *regs = lds;
 	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
-	uint16_t *srcrow = (uint16_t *)(lds + P.nslot * T) + wave * 256 * VG_WS;
+	uint16_t *srcrow = (uint16_t *)(lds + ldc(&P.nslot) * T) + wave * 256 * VG_WS;
 	const strips<VG_WS> S{wave * 256u * VG_WS, 256u, lane};
-	const uint32_t g = blockIdx.x;
-	const uint32_t t0 = g * L.wtiles_per_wg;
-	const uint32_t t1 = min(t0 + L.wtiles_per_wg, L.num_tiles);
-
-	if (wave == 0) {
-		/* offset of this workgroup: the rows selected before tile t0 --
-		 * whole count-pass workgroups, then the tiles of the one t0 is in */
-		const uint32_t cw = t0 / L.tiles_per_wg;
-		uint64_t base = 0;
-		for (uint32_t k = lane; k < cw; k += 64)
-			base += L.wgcount[k];
-		for (uint32_t t = cw * L.tiles_per_wg + lane; t < t0; t += 64) {
-			const u32x4 v = *(const u32x4 *)(L.tcount + (size_t)t * 4);
-			base += (uint64_t)v.x + v.y + v.z + v.w;
-		}
-#pragma unroll
-		for (int off = 32; off > 0; off >>= 1)
-			base += __shfl_xor(base, off, 64);
-		if (lane == 0)
-			s_base = base;
-	}
-	__syncthreads();
-	uint64_t chunk_base = s_base;
+	uint64_t chunk_base = base;
 	cursor C;
 	cursor_init(C, P, lane);
 
@@ -814,7 +805,7 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_pro
 			/* lane i: counts of tile c0+i and its offset inside the chunk */
 			uint32_t c[4] = {0, 0, 0, 0};
 			if (lane < nt) {
-				const u32x4 v = *(const u32x4 *)(L.tcount + (size_t)(c0 + lane) * 4);
+				const u32x4 v = *(const u32x4 *)(ldc(&L.tcount) + (size_t)(c0 + lane) * 4);
 				c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w;
 			}
 			const uint32_t tot = c[0] + c[1] + c[2] + c[3];
@@ -859,16 +850,16 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_pro
 			const uint32_t row0 = (tile - Cc.first) * T;
 			rowstate RS;
 			rows_init(RS, mybits);
-			if (P.wfull) {
+			if (ldc(&P.wfull)) {
 				/* rerun everything: the selected rows take the same
 				 * branches as in the count pass, so masked writes see the
 				 * same active rows */
-				run_uops<VG_WS, T>(P, 0, P.nuop, ~0ull, regs, S, RS);
-			} else if (P.wneed) {
-				run_uops<VG_WS, T>(P, 0, P.nuop, P.wneed, regs, S, RS);
+				run_uops<VG_WS, T>(P, 0, ldc(&P.nuop), ~0ull, regs, S, RS);
+			} else if (ldc(&P.wneed)) {
+				run_uops<VG_WS, T>(P, 0, ldc(&P.nuop), ldc(&P.wneed), regs, S, RS);
 			}
 			/* the loaded columns' slots are free from here on */
-			const bool early = P.wsafe && inext < nt;
+			const bool early = ldc(&P.wsafe) && inext < nt;
 			if (early)
 				issue_tile(P, L, C, c0 + inext, lane, S, nbits, regs);
 			/* ranks: strip by strip, each in row order */
@@ -890,13 +881,13 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_pro
 			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
 			const uint64_t at = s_off[i] + wave_off;
-			const uint64_t room = L.capacity > at ? L.capacity - at : 0;
+			const uint64_t room = ldc(&L.capacity) > at ? ldc(&L.capacity) - at : 0;
 			const uint32_t n = (uint64_t)wave_tot < room ? wave_tot : (uint32_t)room;
 			if (n < wave_tot && lane == 0)
-				atomicOr(&L.ctrl->overflow, 1u);
-			for (uint32_t j = 0; j < P.nout; ++j) {
+				atomicOr(&ldc(&L.ctrl)->overflow, 1u);
+			for (uint32_t j = 0; j < ldc(&P.nout); ++j) {
 				const vg_out o = ldc_struct(&P.out[j]);
-				char *dst = (char *)L.out[j];
+				char *dst = (char *)ldc(&L.out[j]);
 				if (o.kind == VG_OUT_SLOT) {
 					const uint32_t *lo = regs + o.slot * T, *hi = regs + o.slot_hi * T;
 					if (o.width == 8) {
@@ -951,8 +942,158 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_pro
 		}
 		__syncthreads();
 	}
+	return chunk_base;
+}
+
+__global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_prog P, const vg_launch L)
+{
+	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+	__shared__ uint64_t s_base;
+	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
+	const uint32_t g = blockIdx.x;
+	const uint32_t t0 = g * L.wtiles_per_wg;
+	const uint32_t t1 = min(t0 + L.wtiles_per_wg, L.num_tiles);
+
+	if (wave == 0) {
+		/* offset of this workgroup: the rows selected before tile t0 --
+		 * whole count-pass workgroups, then the tiles of the one t0 is in */
+		const uint32_t cw = t0 / L.tiles_per_wg;
+		uint64_t base = 0;
+		for (uint32_t k = lane; k < cw; k += 64)
+			base += L.wgcount[k];
+		for (uint32_t t = cw * L.tiles_per_wg + lane; t < t0; t += 64) {
+			const u32x4 v = *(const u32x4 *)(L.tcount + (size_t)t * 4);
+			base += (uint64_t)v.x + v.y + v.z + v.w;
+		}
+#pragma unroll
+		for (int off = 32; off > 0; off >>= 1)
+			base += __shfl_xor(base, off, 64);
+		if (lane == 0)
+			s_base = base;
+	}
+	__syncthreads();
+	const uint64_t end = write_tiles(P, L, t0, t1, s_base, lds);
 	if (g == gridDim.x - 1 && tid == 0)
-		L.ctrl->total = chunk_base;
+		L.ctrl->total = end;
+}
+
+/* ------------------------------------------------------ batched queries */
+/* Two persistent launches run the count and write items of many queries
+ * (each with its own program, tablets and outputs).  In each, a workgroup
+ * takes the next item of its list from an atomic counter until the list
+ * is empty.  A count item counts its tiles like the count pass (4 waves
+ * share a tile); the last count item of a query to finish turns its items'
+ * counts into output offsets.  After the count launch, a write item
+ * writes its tiles like the write pass, from its item's offset. */
+
+/* clear the per-query state and the item counters; stamp the start */
+__global__ __launch_bounds__(256) void vg_batch_reset(const vg_batch_args A)
+{
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	if (i < A.nquery) {
+		vg_bquery z;
+		memset(&z, 0, sizeof z);
+		A.q[i] = z;
+		vg_ctrl *c = A.launch[i].ctrl;
+		c->total = 0;
+		c->overflow = 0;
+	}
+	if (i == 0) {
+		A.head->next = 0;
+		A.head->wnext = 0;
+		A.head->t_start = wall_clock64();
+	}
+}
+
+/* the next item of a list, in every thread (query ~0u: none left) */
+__device__ __forceinline__ void next_item(const vg_item *items, uint32_t nitems, uint32_t *counter,
+	vg_item *s_item, uint32_t &q, uint32_t &t0, uint32_t &t1, uint32_t &k)
+{
+	if (threadIdx.x == 0) {
+		const uint32_t i = atomicAdd(counter, 1u);
+		uint32_t iq = ~0u, i0 = 0, i1 = 0, ik = 0;
+		if (i < nitems) {
+			const u32x4 v = *(const u32x4 *)(items + i);
+			iq = v.x; i0 = v.y; i1 = v.z; ik = v.w;
+		}
+		s_item->query = iq;
+		s_item->t0 = i0;
+		s_item->t1 = i1;
+		s_item->k = ik;
+	}
+	__syncthreads();
+	q = uni(s_item->query);
+	t0 = uni(s_item->t0);
+	t1 = uni(s_item->t1);
+	k = uni(s_item->k);
+	__syncthreads();                /* s_item is rewritten by the next fetch */
+}
+
+__global__ __launch_bounds__(256) void vg_batch_count_kernel(const vg_batch_args A)
+{
+	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+	__shared__ vg_item s_item;
+	__shared__ uint32_t s_last;
+	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
+	for (;;) {
+		uint32_t q, t0, t1, k;
+		next_item(A.items, A.nitems, &A.head->next, &s_item, q, t0, t1, k);
+		if (q == ~0u)
+			break;
+		const vg_prog &P = A.progs[q];
+		const vg_launch &L = A.launch[q];
+		const uint32_t nitem = ldc(&A.info[q].nitem), ioff = ldc(&A.info[q].ioff);
+		const uint32_t n = count_tiles<1>(P, L, t0, t1, lds);
+		if (tid == 0)
+			A.icount[ioff + k] = n;
+		/* the item's count is visible device-wide before the query's
+		 * counter moves (every thread's validity bytes and tile counts
+		 * only need to be visible after this launch) */
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+		__syncthreads();
+		if (tid == 0)
+			s_last = __hip_atomic_fetch_add(&A.q[q].done, 1u, __ATOMIC_ACQ_REL,
+				__HIP_MEMORY_SCOPE_AGENT) + 1 == nitem;
+		__syncthreads();
+		if (uni(s_last) && wave == 0) {
+			/* last count item of the query: exclusive scan of its items'
+			 * counts into their output offsets, and the query's total */
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+			uint64_t carry = 0;
+			for (uint32_t c = 0; c < nitem; c += 64) {
+				const uint32_t v = c + lane < nitem ? A.icount[ioff + c + lane] : 0u;
+				uint64_t incl = v;
+#pragma unroll
+				for (int off = 1; off < 64; off <<= 1) {
+					const uint64_t y = __shfl_up(incl, off, 64);
+					if (lane >= (uint32_t)off)
+						incl += y;
+				}
+				if (c + lane < nitem)
+					A.ibase[ioff + c + lane] = carry + incl - v;
+				carry += __shfl(incl, 63, 64);
+			}
+			if (lane == 0)
+				ldc(&L.ctrl)->total = carry;
+		}
+		__syncthreads();            /* s_last is rewritten by the next item */
+	}
+}
+
+__global__ __launch_bounds__(VG_WRITE_THREADS) void vg_batch_write_kernel(const vg_batch_args A)
+{
+	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+	__shared__ vg_item s_item;
+	for (;;) {
+		uint32_t q, t0, t1, k;
+		next_item(A.witems, A.nitems, &A.head->wnext, &s_item, q, t0, t1, k);
+		if (q == ~0u)
+			break;
+		const uint32_t nitem = ldc(&A.info[q].nitem), ioff = ldc(&A.info[q].ioff);
+		write_tiles(A.progs[q], A.launch[q], t0, t1, A.ibase[ioff + k], lds);
+		if (threadIdx.x == 0 && atomicAdd(&A.q[q].wdone, 1u) + 1 == nitem)
+			A.q[q].t_done = wall_clock64();
+	}
 }
 
 /* --------------------------------------------------------------- launch */
@@ -1084,4 +1225,55 @@ extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stre
 extern "C" uint32_t vg_kernel_tile_rows(void)
 {
 	return VG_T;
+}
+
+/* dynamic LDS of a program's batch count items (pass 0: the tile's
+ * register file shared by the 4 waves) or write items (pass 1) */
+extern "C" size_t vg_kernel_batch_lds(int pass, const vg_prog *P)
+{
+	return pass == 0 ? (size_t)P->nslot_count * VG_T * 4 : write_lds_bytes(P);
+}
+
+static void batch_attrs()
+{
+	static bool done = false;
+	if (!done) {
+		(void)hipFuncSetAttribute((const void *)vg_batch_count_kernel,
+			hipFuncAttributeMaxDynamicSharedMemorySize, VG_LDS_LIMIT);
+		(void)hipFuncSetAttribute((const void *)vg_batch_write_kernel,
+			hipFuncAttributeMaxDynamicSharedMemorySize, VG_LDS_LIMIT);
+		done = true;
+	}
+}
+
+/* Workgroups of the batch count (pass 0) or write (pass 1) kernel that
+ * fit on the device at once with `lds` bytes of dynamic LDS each. */
+extern "C" uint32_t vg_kernel_batch_grid(int pass, size_t lds)
+{
+	batch_attrs();
+	int per = 0;
+	const hipError_t e = pass == 0
+		? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, vg_batch_count_kernel, 256, lds)
+		: hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, vg_batch_write_kernel, VG_WRITE_THREADS, lds);
+	if (e != hipSuccess || per < 1)
+		per = 1;
+	return (uint32_t)num_cus() * (uint32_t)per;
+}
+
+/* Reset, the count launch and the write launch, on `stream`. */
+extern "C" int vg_kernel_batch_launch(const vg_batch_args *A, uint32_t cgrid, size_t clds, uint32_t wgrid,
+	size_t wlds, hipStream_t stream)
+{
+	if (clds > VG_LDS_LIMIT || wlds > VG_LDS_LIMIT)
+		return -3;
+	batch_attrs();
+	const uint32_t rgrid = (A->nquery + 255) / 256;
+	hipLaunchKernelGGL(vg_batch_reset, dim3(rgrid ? rgrid : 1), dim3(256), 0, stream, *A);
+	if (hipGetLastError() != hipSuccess)
+		return -2;
+	hipLaunchKernelGGL(vg_batch_count_kernel, dim3(cgrid), dim3(256), clds, stream, *A);
+	if (hipGetLastError() != hipSuccess)
+		return -2;
+	hipLaunchKernelGGL(vg_batch_write_kernel, dim3(wgrid), dim3(VG_WRITE_THREADS), wlds, stream, *A);
+	return hipGetLastError() == hipSuccess ? 0 : -2;
 }
