@@ -1093,6 +1093,12 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_batch_write_kernel(const 
 		write_tiles(A.progs[q], A.launch[q], t0, t1, A.ibase[ioff + k], lds);
 		if (threadIdx.x == 0 && atomicAdd(&A.q[q].wdone, 1u) + 1 == nitem)
 			A.q[q].t_done = wall_clock64();
+		/* Keep this barrier: without it the compiler merges the block
+		 * above with the next item's fetch (both run by thread 0 only)
+		 * across the loop's back edge, and the other threads then reach
+		 * the fetch barrier ahead of the fetch and reread the old item,
+		 * forever. */
+		__syncthreads();
 	}
 }
 
