@@ -220,6 +220,7 @@ def bench_batched(a) -> int:
     b = va.PersistentBatch(qs)
     t0 = time.perf_counter()
     cfg, tune_ms = b.tune(iters=3)
+    cfg = b.config()
     print(f"# tuned in {time.perf_counter() - t0:.1f}s: {cfg} {tune_ms:.3f} ms", file=sys.stderr, flush=True)
     batch_ms = b.time(a.warmup, a.steps)
     lats = []

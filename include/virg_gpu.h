@@ -79,23 +79,24 @@ int vg_batch_sync(vg_batch *b, unsigned long long *rows);
 /* Mean milliseconds per replay after `warmup` untimed ones (HIP events). */
 int vg_batch_time(vg_batch *b, int warmup, int iters, double *ms);
 
-/* A batch of prepared queries (distinct, on one device) run by two
- * persistent launches: the count launch works through a device-resident
- * list of (query, tile range) items, every workgroup taking the next item
- * from an atomic counter, and the last item of each query to finish turns
- * its items' counts into output offsets; the write launch then works
- * through the write items the same way.  Reset, count and write launches
- * are captured as one HIP graph, so a run is one graph launch whatever
- * the number of queries.  Each query keeps its own results, exactly as a
- * vg_query_run would leave them (vg_query_fetch after vg_pbatch_sync).
- * The queries must not be launched by other means while a run is in
- * flight. */
+/* A batch of prepared queries (distinct, on one device) run by persistent
+ * launches: a count launch works through a device-resident list of
+ * (query, tile range) items, every workgroup taking the next item from an
+ * atomic counter, and the last item of each query to finish turns its
+ * items' counts into output offsets; a write launch then works through the
+ * write items the same way.  Queries are grouped by the occupancy their
+ * register files allow, each group with its own pair of launches on its
+ * own branch; the reset launch and every group's pair are captured as one
+ * HIP graph, so a run is one graph launch whatever the number of queries.
+ * Each query keeps its own results, exactly as a vg_query_run would leave
+ * them (vg_query_fetch after vg_pbatch_sync).  The queries must not be
+ * launched by other means while a run is in flight. */
 typedef struct {
 	unsigned tiles_per_item;   /* tiles (1024 rows) per work item; 0: 8 */
-	unsigned count_wgs;        /* workgroups of the count launch; 0: as many as fit */
-	unsigned write_wgs;        /* workgroups of the write launch; 0: as many as fit */
+	unsigned grid_div;         /* launch 1/grid_div of the workgroups that fit; 0: 1 */
+	unsigned split;            /* 0: group the queries by occupancy, 1: one group */
 	unsigned order;            /* write list: 0 batch order, 1 smallest query first,
-	                            * 2 largest first (the count list is largest first) */
+	                            * 2 largest first (count lists are largest first) */
 } vg_pbatch_config;
 typedef struct vg_pbatch vg_pbatch;
 vg_pbatch *vg_pbatch_create(vg_query *const *qs, unsigned n, const vg_pbatch_config *cfg,
@@ -103,8 +104,8 @@ vg_pbatch *vg_pbatch_create(vg_query *const *qs, unsigned n, const vg_pbatch_con
 void vg_pbatch_free(vg_pbatch *b);
 /* Rebuild the item lists and the graph for another configuration. */
 int vg_pbatch_configure(vg_pbatch *b, const vg_pbatch_config *cfg, char *err, size_t errlen);
-/* The configuration in use, with the grid sizes actually launched. */
-int vg_pbatch_get_config(vg_pbatch *b, vg_pbatch_config *cfg);
+/* The configuration in use and the number of query groups. */
+int vg_pbatch_get_config(vg_pbatch *b, vg_pbatch_config *cfg, unsigned *ngroups);
 int vg_pbatch_launch(vg_pbatch *b);
 /* Wait for the run.  rows[i]: rows query i selected; latency_ms[i]: when
  * its last write item finished, from the start of the run, by the device
@@ -113,8 +114,8 @@ int vg_pbatch_sync(vg_pbatch *b, unsigned long long *rows, double *latency_ms);
 /* Mean milliseconds per run over `iters` graph replays after `warmup`. */
 int vg_pbatch_time(vg_pbatch *b, int warmup, int iters, double *ms);
 /* Autotuner: times every candidate configuration (tiles per item 1..64,
- * the three write orders, full and half grids) on this batch over `iters`
- * replays each and keeps the fastest. */
+ * the three write orders, full and half grids, grouped or not) on this
+ * batch over `iters` replays each and keeps the fastest. */
 int vg_pbatch_tune(vg_pbatch *b, int iters, vg_pbatch_config *best, double *best_ms,
 	char *err, size_t errlen);
 /* The plain way, for comparison: each query's count and write launches,

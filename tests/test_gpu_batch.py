@@ -86,19 +86,21 @@ def _check(qs, want, rows, what):
         assert_same(q.fetch(), w, f"query {i} {what}")
 
 
-@pytest.mark.parametrize("tpi,order,frac", [(1, 0, 1), (8, 1, 1), (64, 2, 3), (3, 1, 7)])
-def test_persistent_batch(mix, tpi, order, frac):
-    """Two persistent launches over item lists give every query its own
-    result, for item sizes, list orders and grids that split the work
-    differently, on every replay, with eager launches in between."""
+@pytest.mark.parametrize("tpi,order,div,split", [(1, 0, 1, 0), (8, 1, 1, 0), (64, 2, 3, 0), (3, 1, 7, 1),
+                                                  (16, 0, 2, 1)])
+def test_persistent_batch(mix, tpi, order, div, split):
+    """Persistent launches over item lists give every query its own
+    result, for item sizes, list orders, grids and query groupings that
+    split the work differently, on every replay, with eager launches in
+    between."""
     qs, want = mix
     b = va.PersistentBatch(qs, tiles_per_item=tpi, order=order)
     try:
-        if frac > 1:
-            cfg = b.config()
-            b.configure(tpi, order, max(1, cfg["count_wgs"] // frac), max(1, cfg["write_wgs"] // frac))
+        b.configure(tpi, order, div, split)
+        cfg = b.config()
+        assert cfg["groups"] >= 1 and (split == 0 or cfg["groups"] == 1)
         for replay in range(2):
-            print(f"tpi {tpi} order {order} replay {replay}", flush=True)
+            print(f"tpi {tpi} order {order} div {div} split {split} groups {cfg['groups']} replay {replay}", flush=True)
             rows, lat = b.run()
             _check(qs, want, rows, f"tpi {tpi} replay {replay}")
             assert all(x >= 0 for x in lat) and max(lat) > 0

@@ -132,7 +132,7 @@ class StreamStats(C.Structure):
 
 class PBatchConfig(C.Structure):
     """vg_pbatch_config (include/virg_gpu.h)."""
-    _fields_ = [("tiles_per_item", C.c_uint), ("count_wgs", C.c_uint), ("write_wgs", C.c_uint),
+    _fields_ = [("tiles_per_item", C.c_uint), ("grid_div", C.c_uint), ("split", C.c_uint),
                 ("order", C.c_uint)]
 
     def as_dict(self) -> dict:
@@ -202,7 +202,7 @@ def _load() -> C.CDLL:
         "vg_pbatch_create": (P, [C.POINTER(P), C.c_uint, C.POINTER(PBatchConfig), C.c_char_p, C.c_size_t]),
         "vg_pbatch_free": (None, [P]),
         "vg_pbatch_configure": (C.c_int, [P, C.POINTER(PBatchConfig), C.c_char_p, C.c_size_t]),
-        "vg_pbatch_get_config": (C.c_int, [P, C.POINTER(PBatchConfig)]),
+        "vg_pbatch_get_config": (C.c_int, [P, C.POINTER(PBatchConfig), C.POINTER(C.c_uint)]),
         "vg_pbatch_launch": (C.c_int, [P]),
         "vg_pbatch_sync": (C.c_int, [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_double)]),
         "vg_pbatch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
@@ -641,25 +641,28 @@ class PersistentBatch:
     lists, captured as one HIP graph (vg_pbatch_* in include/virg_gpu.h)."""
 
     def __init__(self, queries: list[DeviceQuery], tiles_per_item: int = 0, order: int = 0,
-                 count_wgs: int = 0, write_wgs: int = 0):
+                 grid_div: int = 0, split: int = 0):
         self.queries = list(queries)
-        cfg = PBatchConfig(tiles_per_item, count_wgs, write_wgs, order)
+        cfg = PBatchConfig(tiles_per_item, grid_div, split, order)
         err = C.create_string_buffer(256)
         self.h = lib.vg_pbatch_create(_handles(self.queries), len(self.queries), C.byref(cfg), err, 256)
         if not self.h:
             raise VirginianError(err.value.decode() or "vg_pbatch_create failed")
 
-    def configure(self, tiles_per_item: int = 0, order: int = 0, count_wgs: int = 0,
-                  write_wgs: int = 0) -> None:
-        cfg = PBatchConfig(tiles_per_item, count_wgs, write_wgs, order)
+    def configure(self, tiles_per_item: int = 0, order: int = 0, grid_div: int = 0, split: int = 0) -> None:
+        cfg = PBatchConfig(tiles_per_item, grid_div, split, order)
         err = C.create_string_buffer(256)
         if lib.vg_pbatch_configure(self.h, C.byref(cfg), err, 256) != VIRG_SUCCESS:
             raise VirginianError(err.value.decode() or "vg_pbatch_configure failed")
 
     def config(self) -> dict:
+        """The configuration in use, with the number of query groups."""
         cfg = PBatchConfig()
-        lib.vg_pbatch_get_config(self.h, C.byref(cfg))
-        return cfg.as_dict()
+        ng = C.c_uint(0)
+        lib.vg_pbatch_get_config(self.h, C.byref(cfg), C.byref(ng))
+        d = cfg.as_dict()
+        d["groups"] = ng.value
+        return d
 
     def run(self) -> tuple[list[int], list[float]]:
         """(rows per query, latency ms per query) of one run."""

@@ -1361,32 +1361,48 @@ extern "C" int vg_batch_time(vg_batch *b, int warmup, int iters, double *ms)
 }
 
 /* ------------------------------------------------------------ vg_pbatch */
-/* Many prepared queries run by two persistent launches (count items, then
- * write items) that work through device-resident item lists, the three
- * launches (reset, count, write) captured once as a HIP graph.  Item size,
- * list order and grid sizes are the tunables (vg_pbatch_tune). */
+/* Many prepared queries run by persistent launches that work through
+ * device-resident item lists.  A launch's LDS per workgroup is that of the
+ * largest register file among its queries, which sets how many workgroups
+ * fit on a CU; so the queries are split into groups by the occupancy their
+ * own register files allow, and each group gets its own count and write
+ * launches, sized to that occupancy.  The reset launch and every group's
+ * pair, each group on its own branch, are captured once as a HIP graph.
+ * Item size, write order, grid size and the grouping are the tunables
+ * (vg_pbatch_tune). */
 extern "C" size_t vg_kernel_batch_lds(int pass, const vg_prog *P);
 extern "C" uint32_t vg_kernel_batch_grid(int pass, size_t lds);
-extern "C" int vg_kernel_batch_launch(const vg_batch_args *A, uint32_t cgrid, size_t clds, uint32_t wgrid,
+extern "C" int vg_kernel_batch_reset(const vg_batch_args *A, hipStream_t stream);
+extern "C" int vg_kernel_batch_passes(const vg_batch_args *A, uint32_t cgrid, size_t clds, uint32_t wgrid,
 	size_t wlds, hipStream_t stream);
+
+/* one group of queries with its own launch pair */
+struct pgroup {
+	std::vector<unsigned> qids;
+	size_t clds = 0, wlds = 0;
+	uint32_t cgrid = 0, wgrid = 0;
+	vg_batch_args A;
+};
 
 struct vg_pbatch {
 	std::vector<vg_query *> qs;
 	vgd_device *d = nullptr;
 	hipStream_t stream = nullptr;
+	std::vector<hipStream_t> branch;        /* one per group (graph capture) */
+	std::vector<hipEvent_t> join;
+	hipEvent_t fork = nullptr;
 	vg_prog *d_progs = nullptr;
 	vg_launch *d_launch = nullptr;
 	vg_bqinfo *d_info = nullptr;
 	vg_bquery *d_q = nullptr;
-	vg_bhead *d_head = nullptr;
+	vg_bhead *d_head = nullptr;             /* one per group */
+	size_t head_cap = 0;
 	vg_item *d_items = nullptr, *d_witems = nullptr;
 	uint32_t *d_icount = nullptr;
 	uint64_t *d_ibase = nullptr;
 	size_t items_cap = 0;
-	vg_batch_args A;
+	std::vector<pgroup> groups;
 	vg_pbatch_config cfg;
-	size_t clds = 0, wlds = 0;
-	uint32_t cgrid = 0, wgrid = 0;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	double clock_khz = 0;
@@ -1413,6 +1429,12 @@ extern "C" void vg_pbatch_free(vg_pbatch *b)
 	for (void *p : bufs)
 		if (p)
 			(void)hipFree(p);
+	for (hipEvent_t e : b->join)
+		(void)hipEventDestroy(e);
+	for (hipStream_t s : b->branch)
+		(void)hipStreamDestroy(s);
+	if (b->fork)
+		(void)hipEventDestroy(b->fork);
 	if (b->stream)
 		(void)hipStreamDestroy(b->stream);
 	delete b;
@@ -1434,11 +1456,27 @@ static void fill_launch(const vg_query *q, vg_launch &L)
 		L.out[j] = q->out[j];
 }
 
-/* Build the item lists for b->cfg, upload them and capture the graph. */
+/* Group the queries (b->cfg.split 0: by the write launch's occupancy,
+ * 1: all together), build the item lists, upload them and capture the
+ * graph. */
 static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 {
 	const unsigned n = (unsigned)b->qs.size();
 	const uint32_t tpi = b->cfg.tiles_per_item ? b->cfg.tiles_per_item : 8;
+	/* groups, keyed by the write launch's workgroups per device */
+	std::vector<uint32_t> key(n, 0);
+	if (b->cfg.split == 0)
+		for (unsigned i = 0; i < n; i++)
+			key[i] = vg_kernel_batch_grid(1, vg_kernel_batch_lds(1, &b->qs[i]->P));
+	std::vector<uint32_t> keys(key);
+	std::sort(keys.begin(), keys.end());
+	keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+	std::reverse(keys.begin(), keys.end());              /* highest occupancy first */
+	b->groups.assign(keys.size(), pgroup());
+	for (unsigned i = 0; i < n; i++) {
+		const size_t g = std::find(keys.begin(), keys.end(), key[i]) - keys.begin();
+		b->groups[g].qids.push_back(i);
+	}
 	std::vector<vg_bqinfo> info(n);
 	uint32_t ioff = 0;
 	for (unsigned i = 0; i < n; i++) {
@@ -1447,27 +1485,36 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		info[i].ioff = ioff;
 		ioff += info[i].nitem;
 	}
-	/* count list: largest query first (the small items come last and
-	 * even out the end); write list: in the configured order */
-	std::vector<unsigned> by_size(n), worder(n);
-	for (unsigned i = 0; i < n; i++)
-		by_size[i] = worder[i] = i;
-	std::stable_sort(by_size.begin(), by_size.end(), [&](unsigned x, unsigned y) {
-		return b->qs[x]->p.num_tiles > b->qs[y]->p.num_tiles; });
-	if (b->cfg.order == 1)
-		worder.assign(by_size.rbegin(), by_size.rend());
-	else if (b->cfg.order == 2)
-		worder = by_size;
+	auto tiles = [&](unsigned q) { return b->qs[q]->p.num_tiles; };
 	std::vector<vg_item> items, witems;
 	auto add = [&](std::vector<vg_item> &v, unsigned q) {
-		const uint32_t nt = b->qs[q]->p.num_tiles;
+		const uint32_t nt = tiles(q);
 		for (uint32_t k = 0; k < info[q].nitem; k++)
 			v.push_back({q, k * tpi, (k + 1) * tpi < nt ? (k + 1) * tpi : nt, k});
 	};
-	for (unsigned q : by_size)
-		add(items, q);
-	for (unsigned q : worder)
-		add(witems, q);
+	std::vector<size_t> goff;
+	for (pgroup &g : b->groups) {
+		/* count list: largest query first (small items last even out the
+		 * end); write list in the configured order */
+		std::vector<unsigned> by_size(g.qids), worder(g.qids);
+		std::stable_sort(by_size.begin(), by_size.end(), [&](unsigned x, unsigned y) {
+			return tiles(x) > tiles(y); });
+		if (b->cfg.order == 1)
+			worder.assign(by_size.rbegin(), by_size.rend());
+		else if (b->cfg.order == 2)
+			worder = by_size;
+		goff.push_back(items.size());
+		for (unsigned q : by_size)
+			add(items, q);
+		for (unsigned q : worder)
+			add(witems, q);
+		for (unsigned q : g.qids) {
+			const size_t c = vg_kernel_batch_lds(0, &b->qs[q]->P), w = vg_kernel_batch_lds(1, &b->qs[q]->P);
+			g.clds = c > g.clds ? c : g.clds;
+			g.wlds = w > g.wlds ? w : g.wlds;
+		}
+	}
+	goff.push_back(items.size());
 	const size_t ni = items.size();
 	if (ni > b->items_cap) {
 		if (b->d_items) (void)hipFree(b->d_items);
@@ -1485,39 +1532,75 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 			return qerr(err, errlen, "cannot allocate the batch's item lists");
 		b->items_cap = ni;
 	}
+	const size_t ng = b->groups.size();
+	if (ng > b->head_cap) {
+		if (b->d_head) (void)hipFree(b->d_head);
+		b->d_head = nullptr;
+		b->head_cap = 0;
+		if (herr(hipMalloc((void **)&b->d_head, ng * sizeof(vg_bhead)), "batch heads"))
+			return qerr(err, errlen, "cannot allocate the batch's list heads");
+		b->head_cap = ng;
+	}
+	while (b->branch.size() < ng) {
+		hipStream_t s;
+		hipEvent_t e;
+		if (herr(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream"))
+			return qerr(err, errlen, "cannot create the batch's streams");
+		b->branch.push_back(s);
+		if (herr(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event"))
+			return qerr(err, errlen, "cannot create the batch's events");
+		b->join.push_back(e);
+	}
 	if ((ni && (herr(hipMemcpy(b->d_items, items.data(), ni * sizeof(vg_item), hipMemcpyHostToDevice),
 			"batch items") ||
 		herr(hipMemcpy(b->d_witems, witems.data(), ni * sizeof(vg_item), hipMemcpyHostToDevice),
 			"batch items"))) ||
 		herr(hipMemcpy(b->d_info, info.data(), n * sizeof(vg_bqinfo), hipMemcpyHostToDevice), "batch info"))
 		return qerr(err, errlen, "cannot upload the batch's item lists");
-	vg_batch_args &A = b->A;
-	memset(&A, 0, sizeof A);
-	A.progs = b->d_progs;
-	A.launch = b->d_launch;
-	A.info = b->d_info;
-	A.items = b->d_items;
-	A.witems = b->d_witems;
-	A.nitems = (uint32_t)ni;
-	A.nquery = n;
-	A.q = b->d_q;
-	A.icount = b->d_icount;
-	A.ibase = b->d_ibase;
-	A.head = b->d_head;
-	const uint32_t cmax = vg_kernel_batch_grid(0, b->clds), wmax = vg_kernel_batch_grid(1, b->wlds);
-	b->cgrid = b->cfg.count_wgs && b->cfg.count_wgs < cmax ? b->cfg.count_wgs : cmax;
-	b->wgrid = b->cfg.write_wgs && b->cfg.write_wgs < wmax ? b->cfg.write_wgs : wmax;
-	const uint32_t most = ni ? (uint32_t)ni : 1;
-	b->cgrid = b->cgrid < most ? b->cgrid : most;
-	b->wgrid = b->wgrid < most ? b->wgrid : most;
+	for (size_t gi = 0; gi < ng; gi++) {
+		pgroup &g = b->groups[gi];
+		vg_batch_args &A = g.A;
+		memset(&A, 0, sizeof A);
+		A.progs = b->d_progs;
+		A.launch = b->d_launch;
+		A.info = b->d_info;
+		A.items = b->d_items + goff[gi];
+		A.witems = b->d_witems + goff[gi];
+		A.nitems = (uint32_t)(goff[gi + 1] - goff[gi]);
+		A.nquery = n;
+		A.q = b->d_q;
+		A.icount = b->d_icount;
+		A.ibase = b->d_ibase;
+		A.head = b->d_head + gi;
+		A.nhead = 1;
+		const uint32_t cmax = vg_kernel_batch_grid(0, g.clds), wmax = vg_kernel_batch_grid(1, g.wlds);
+		const unsigned fr = b->cfg.grid_div ? b->cfg.grid_div : 1;
+		g.cgrid = cmax / fr ? cmax / fr : 1;
+		g.wgrid = wmax / fr ? wmax / fr : 1;
+		const uint32_t most = A.nitems ? A.nitems : 1;
+		g.cgrid = g.cgrid < most ? g.cgrid : most;
+		g.wgrid = g.wgrid < most ? g.wgrid : most;
+	}
+	/* the reset clears every query and every group's head */
+	vg_batch_args R = b->groups[0].A;
+	R.head = b->d_head;
+	R.nhead = (uint32_t)ng;
 	pbatch_drop_graph(b);
 	if (herr(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture"))
 		return qerr(err, errlen, "graph capture failed");
-	const int lrc = vg_kernel_batch_launch(&A, b->cgrid, b->clds, b->wgrid, b->wlds, b->stream);
+	int rc = vg_kernel_batch_reset(&R, b->stream) == 0 && hipEventRecord(b->fork, b->stream) == hipSuccess;
+	for (size_t gi = 0; gi < ng && rc; gi++) {
+		pgroup &g = b->groups[gi];
+		hipStream_t s = ng == 1 ? b->stream : b->branch[gi];
+		rc = (ng == 1 || hipStreamWaitEvent(s, b->fork, 0) == hipSuccess) &&
+			vg_kernel_batch_passes(&g.A, g.cgrid, g.clds, g.wgrid, g.wlds, s) == 0 &&
+			(ng == 1 || (hipEventRecord(b->join[gi], s) == hipSuccess &&
+				hipStreamWaitEvent(b->stream, b->join[gi], 0) == hipSuccess));
+	}
 	const hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
-	if (lrc != 0 || ec != hipSuccess) {
+	if (!rc || ec != hipSuccess) {
 		(void)hipGetLastError();
-		return qerr(err, errlen, "graph capture failed (%d)", lrc);
+		return qerr(err, errlen, "graph capture failed");
 	}
 	if (herr(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate"))
 		return qerr(err, errlen, "graph instantiate failed");
@@ -1546,6 +1629,7 @@ extern "C" vg_pbatch *vg_pbatch_create(vg_query *const *qs, unsigned n, const vg
 	b->d = qs[0]->d;
 	PCHK(hipSetDevice(b->d->device) != hipSuccess, "set device");
 	PCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess, "stream");
+	PCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess, "event");
 	int khz = 0;
 	PCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->d->device) != hipSuccess || khz <= 0,
 		"device clock rate");
@@ -1559,15 +1643,11 @@ extern "C" vg_pbatch *vg_pbatch_create(vg_query *const *qs, unsigned n, const vg
 		PCHK(hipStreamSynchronize(q->stream) != hipSuccess, "query setup");
 		progs[i] = q->P;
 		fill_launch(q, launches[i]);
-		const size_t c = vg_kernel_batch_lds(0, &q->P), w = vg_kernel_batch_lds(1, &q->P);
-		b->clds = c > b->clds ? c : b->clds;
-		b->wlds = w > b->wlds ? w : b->wlds;
 	}
 	PCHK(hipMalloc((void **)&b->d_progs, n * sizeof(vg_prog)) != hipSuccess ||
 		hipMalloc((void **)&b->d_launch, n * sizeof(vg_launch)) != hipSuccess ||
 		hipMalloc((void **)&b->d_info, n * sizeof(vg_bqinfo)) != hipSuccess ||
-		hipMalloc((void **)&b->d_q, n * sizeof(vg_bquery)) != hipSuccess ||
-		hipMalloc((void **)&b->d_head, sizeof(vg_bhead)) != hipSuccess, "cannot allocate the batch");
+		hipMalloc((void **)&b->d_q, n * sizeof(vg_bquery)) != hipSuccess, "cannot allocate the batch");
 	PCHK(hipMemcpy(b->d_progs, progs.data(), n * sizeof(vg_prog), hipMemcpyHostToDevice) != hipSuccess ||
 		hipMemcpy(b->d_launch, launches.data(), n * sizeof(vg_launch), hipMemcpyHostToDevice) != hipSuccess,
 		"cannot upload the batch");
@@ -1587,11 +1667,11 @@ extern "C" int vg_pbatch_configure(vg_pbatch *b, const vg_pbatch_config *cfg, ch
 	return pbatch_build(b, err, errlen);
 }
 
-extern "C" int vg_pbatch_get_config(vg_pbatch *b, vg_pbatch_config *cfg)
+extern "C" int vg_pbatch_get_config(vg_pbatch *b, vg_pbatch_config *cfg, unsigned *ngroups)
 {
 	*cfg = b->cfg;
-	cfg->count_wgs = b->cgrid;
-	cfg->write_wgs = b->wgrid;
+	if (ngroups)
+		*ngroups = (unsigned)b->groups.size();
 	return VIRG_SUCCESS;
 }
 
@@ -1669,29 +1749,30 @@ extern "C" int vg_pbatch_tune(vg_pbatch *b, int iters, vg_pbatch_config *best, d
 {
 	static const unsigned tpis[] = { 1, 2, 4, 8, 16, 32, 64 };
 	static const unsigned orders[] = { 0, 1, 2 };
-	const uint32_t cmax = vg_kernel_batch_grid(0, b->clds), wmax = vg_kernel_batch_grid(1, b->wlds);
-	const unsigned fr[] = { 1, 2 };          /* grid = max / fr */
+	static const unsigned divs[] = { 1, 2 };
+	static const unsigned splits[] = { 0, 1 };
 	vg_pbatch_config top;
 	memset(&top, 0, sizeof top);
 	double top_ms = 1e30;
-	for (unsigned t : tpis)
-		for (unsigned o : orders)
-			for (unsigned f : fr) {
-				vg_pbatch_config c;
-				memset(&c, 0, sizeof c);
-				c.tiles_per_item = t;
-				c.order = o;
-				c.count_wgs = cmax / f;
-				c.write_wgs = wmax / f;
-				double ms = 0;
-				if (vg_pbatch_configure(b, &c, err, errlen) == VIRG_FAIL ||
-					vg_pbatch_time(b, 1, iters, &ms) == VIRG_FAIL)
-					return VIRG_FAIL;
-				if (ms < top_ms) {
-					top_ms = ms;
-					top = c;
+	for (unsigned sp : splits)
+		for (unsigned t : tpis)
+			for (unsigned o : orders)
+				for (unsigned f : divs) {
+					vg_pbatch_config c;
+					memset(&c, 0, sizeof c);
+					c.tiles_per_item = t;
+					c.order = o;
+					c.grid_div = f;
+					c.split = sp;
+					double ms = 0;
+					if (vg_pbatch_configure(b, &c, err, errlen) == VIRG_FAIL ||
+						vg_pbatch_time(b, 1, iters, &ms) == VIRG_FAIL)
+						return VIRG_FAIL;
+					if (ms < top_ms) {
+						top_ms = ms;
+						top = c;
+					}
 				}
-			}
 	if (vg_pbatch_configure(b, &top, err, errlen) == VIRG_FAIL)
 		return VIRG_FAIL;
 	if (best)

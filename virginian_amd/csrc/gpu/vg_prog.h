@@ -192,7 +192,9 @@ typedef struct {
 	vg_bquery *q;
 	uint32_t *icount;        /* rows selected by each count item */
 	uint64_t *ibase;         /* output row of each item's first selected row */
-	vg_bhead *head;
+	vg_bhead *head;          /* this list's head (the reset clears nhead of them) */
+	uint32_t nhead;
+	uint32_t pad;
 } vg_batch_args;
 
 #endif

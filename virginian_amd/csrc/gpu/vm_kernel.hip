@@ -998,10 +998,10 @@ __global__ __launch_bounds__(256) void vg_batch_reset(const vg_batch_args A)
 		c->total = 0;
 		c->overflow = 0;
 	}
-	if (i == 0) {
-		A.head->next = 0;
-		A.head->wnext = 0;
-		A.head->t_start = wall_clock64();
+	if (i < A.nhead) {
+		A.head[i].next = 0;
+		A.head[i].wnext = 0;
+		A.head[i].t_start = wall_clock64();
 	}
 }
 
@@ -1266,17 +1266,22 @@ extern "C" uint32_t vg_kernel_batch_grid(int pass, size_t lds)
 	return (uint32_t)num_cus() * (uint32_t)per;
 }
 
-/* Reset, the count launch and the write launch, on `stream`. */
-extern "C" int vg_kernel_batch_launch(const vg_batch_args *A, uint32_t cgrid, size_t clds, uint32_t wgrid,
+/* Clear the state of every query and of A->nhead item-list heads. */
+extern "C" int vg_kernel_batch_reset(const vg_batch_args *A, hipStream_t stream)
+{
+	const uint32_t m = A->nquery > A->nhead ? A->nquery : A->nhead;
+	const uint32_t rgrid = (m + 255) / 256;
+	hipLaunchKernelGGL(vg_batch_reset, dim3(rgrid ? rgrid : 1), dim3(256), 0, stream, *A);
+	return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+/* The count launch, then the write launch, over A's item lists. */
+extern "C" int vg_kernel_batch_passes(const vg_batch_args *A, uint32_t cgrid, size_t clds, uint32_t wgrid,
 	size_t wlds, hipStream_t stream)
 {
 	if (clds > VG_LDS_LIMIT || wlds > VG_LDS_LIMIT)
 		return -3;
 	batch_attrs();
-	const uint32_t rgrid = (A->nquery + 255) / 256;
-	hipLaunchKernelGGL(vg_batch_reset, dim3(rgrid ? rgrid : 1), dim3(256), 0, stream, *A);
-	if (hipGetLastError() != hipSuccess)
-		return -2;
 	hipLaunchKernelGGL(vg_batch_count_kernel, dim3(cgrid), dim3(256), clds, stream, *A);
 	if (hipGetLastError() != hipSuccess)
 		return -2;
