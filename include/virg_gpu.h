@@ -97,6 +97,11 @@ typedef struct {
 	unsigned split;            /* 0: group the queries by occupancy, 1: one group */
 	unsigned order;            /* write list: 0 batch order, 1 smallest query first,
 	                            * 2 largest first (count lists are largest first) */
+	unsigned single;           /* 1: one pass into each query's capacity as given (an
+	                            * estimate): every tile counts, takes its output offset
+	                            * by decoupled look-back over the query's earlier tiles
+	                            * and writes its rows there; a query whose result does
+	                            * not fit reports it (vg_pbatch_results) */
 } vg_pbatch_config;
 typedef struct vg_pbatch vg_pbatch;
 vg_pbatch *vg_pbatch_create(vg_query *const *qs, unsigned n, const vg_pbatch_config *cfg,
@@ -111,6 +116,11 @@ int vg_pbatch_launch(vg_pbatch *b);
  * its last write item finished, from the start of the run, by the device
  * clock.  Either may be NULL.  Fails when a result exceeded its capacity. */
 int vg_pbatch_sync(vg_pbatch *b, unsigned long long *rows, double *latency_ms);
+/* Wait for the run without failing on a query that did not fit: rows[i]
+ * (rows query i selected, or at least its capacity), latency_ms[i] as
+ * above, flags[i] = 0 when the result is complete, else 1 (capacity
+ * exceeded) and/or 2 (a single-pass wait gave up).  Any may be NULL. */
+int vg_pbatch_results(vg_pbatch *b, unsigned long long *rows, double *latency_ms, unsigned *flags);
 /* Mean milliseconds per run over `iters` graph replays after `warmup`. */
 int vg_pbatch_time(vg_pbatch *b, int warmup, int iters, double *ms);
 /* Autotuner: times every candidate configuration (tiles per item 1..64,

@@ -151,3 +151,49 @@ def test_persistent_batch_overflow(gpu_db):
         b.close()
         q1.close()
         q2.close()
+
+
+@pytest.mark.parametrize("tpi,split", [(1, 0), (4, 1), (16, 0)])
+def test_single_pass(mix, tpi, split):
+    """One pass with look-back offsets: every query's result in row order,
+    on every replay, when the capacity is large enough."""
+    qs, want = mix
+    b = va.PersistentBatch(qs, tiles_per_item=tpi, split=split, single=1)
+    try:
+        for replay in range(2):
+            print(f"single tpi {tpi} split {split} replay {replay}", flush=True)
+            rows, lat, flags = b.results()
+            assert not any(flags)
+            _check(qs, want, rows, f"single tpi {tpi} replay {replay}")
+            assert max(lat) > 0
+    finally:
+        b.close()
+
+
+def test_single_pass_capacity(gpu_db):
+    """A result that does not fit is reported, with its exact row count,
+    and never written past the capacity; one that fits exactly is whole."""
+    gpu_db.gen_table("test", 300_001, 1 << 16, seed=7)
+    sqls = ["select id, uniformf from test where uniformi < 30",
+            "select normalf5 from test where uniformf < 0.5",
+            "select id from test where normali20 > 100"]
+    exact = []
+    for sql in sqls:
+        w, _ = oracle.select(gpu_db, sql)
+        exact.append((w, len(next(iter(w.values())))))
+    assert exact[2][1] == 0
+    qs = [gpu_db.prepare(sqls[0], capacity=exact[0][1]),          # fits exactly
+          gpu_db.prepare(sqls[1], capacity=max(exact[1][1] - 1, 1)),  # one short
+          gpu_db.prepare(sqls[0], capacity=1000),                  # far short
+          gpu_db.prepare(sqls[2], capacity=1)]                     # empty result
+    b = va.PersistentBatch(qs, tiles_per_item=2, single=1)
+    try:
+        rows, _, flags = b.results()
+        assert flags == [0, 1, 1, 0]
+        assert rows[:3] == [exact[0][1], exact[1][1], exact[0][1]]
+        assert rows[3] == 0
+        assert_same(qs[0].fetch(), exact[0][0], "exact fit")
+    finally:
+        b.close()
+        for q in qs:
+            q.close()

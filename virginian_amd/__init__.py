@@ -133,7 +133,7 @@ class StreamStats(C.Structure):
 class PBatchConfig(C.Structure):
     """vg_pbatch_config (include/virg_gpu.h)."""
     _fields_ = [("tiles_per_item", C.c_uint), ("grid_div", C.c_uint), ("split", C.c_uint),
-                ("order", C.c_uint)]
+                ("order", C.c_uint), ("single", C.c_uint)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -206,6 +206,7 @@ def _load() -> C.CDLL:
         "vg_pbatch_launch": (C.c_int, [P]),
         "vg_pbatch_sync": (C.c_int, [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_double)]),
         "vg_pbatch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "vg_pbatch_results": (C.c_int, [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
         "vg_pbatch_tune": (C.c_int, [P, C.c_int, C.POINTER(PBatchConfig), C.POINTER(C.c_double),
                                      C.c_char_p, C.c_size_t]),
         "vg_plain_time": (C.c_int, [C.POINTER(P), C.c_uint, C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -641,16 +642,17 @@ class PersistentBatch:
     lists, captured as one HIP graph (vg_pbatch_* in include/virg_gpu.h)."""
 
     def __init__(self, queries: list[DeviceQuery], tiles_per_item: int = 0, order: int = 0,
-                 grid_div: int = 0, split: int = 0):
+                 grid_div: int = 0, split: int = 0, single: int = 0):
         self.queries = list(queries)
-        cfg = PBatchConfig(tiles_per_item, grid_div, split, order)
+        cfg = PBatchConfig(tiles_per_item, grid_div, split, order, single)
         err = C.create_string_buffer(256)
         self.h = lib.vg_pbatch_create(_handles(self.queries), len(self.queries), C.byref(cfg), err, 256)
         if not self.h:
             raise VirginianError(err.value.decode() or "vg_pbatch_create failed")
 
-    def configure(self, tiles_per_item: int = 0, order: int = 0, grid_div: int = 0, split: int = 0) -> None:
-        cfg = PBatchConfig(tiles_per_item, grid_div, split, order)
+    def configure(self, tiles_per_item: int = 0, order: int = 0, grid_div: int = 0, split: int = 0,
+                  single: int = 0) -> None:
+        cfg = PBatchConfig(tiles_per_item, grid_div, split, order, single)
         err = C.create_string_buffer(256)
         if lib.vg_pbatch_configure(self.h, C.byref(cfg), err, 256) != VIRG_SUCCESS:
             raise VirginianError(err.value.decode() or "vg_pbatch_configure failed")
@@ -674,6 +676,20 @@ class PersistentBatch:
         for q, r in zip(self.queries, rows):
             q.last_rows = r
         return list(rows)[:n], list(lat)[:n]
+
+    def results(self) -> tuple[list[int], list[float], list[int]]:
+        """Launch, wait, and return (rows, latency ms, overflow flags) per
+        query without raising on a query whose result did not fit."""
+        n = len(self.queries)
+        rows = (C.c_ulonglong * max(n, 1))()
+        lat = (C.c_double * max(n, 1))()
+        flags = (C.c_uint * max(n, 1))()
+        if lib.vg_pbatch_launch(self.h) != VIRG_SUCCESS or \
+                lib.vg_pbatch_results(self.h, rows, lat, flags) != VIRG_SUCCESS:
+            raise VirginianError("batch run failed")
+        for q, r, f in zip(self.queries, rows, flags):
+            q.last_rows = 0 if f else r
+        return list(rows)[:n], list(lat)[:n], list(flags)[:n]
 
     def time(self, warmup: int, iters: int) -> float:
         ms = C.c_double(0)

@@ -1375,6 +1375,11 @@ extern "C" uint32_t vg_kernel_batch_grid(int pass, size_t lds);
 extern "C" int vg_kernel_batch_reset(const vg_batch_args *A, hipStream_t stream);
 extern "C" int vg_kernel_batch_passes(const vg_batch_args *A, uint32_t cgrid, size_t clds, uint32_t wgrid,
 	size_t wlds, hipStream_t stream);
+extern "C" int vg_kernel_batch_single(const vg_batch_args *A, uint32_t grid, size_t lds, hipStream_t stream);
+
+/* polls of an unpublished predecessor tile before a single-pass item gives
+ * up (the query then reports VG_OVF_WAIT and must be run in two passes) */
+#define VG_SPIN_LIMIT (1u << 22)
 
 /* one group of queries with its own launch pair */
 struct pgroup {
@@ -1401,6 +1406,8 @@ struct vg_pbatch {
 	uint32_t *d_icount = nullptr;
 	uint64_t *d_ibase = nullptr;
 	size_t items_cap = 0;
+	uint64_t *d_status = nullptr;           /* single pass: a word per tile */
+	size_t status_cap = 0;
 	std::vector<pgroup> groups;
 	vg_pbatch_config cfg;
 	hipGraph_t graph = nullptr;
@@ -1425,7 +1432,7 @@ extern "C" void vg_pbatch_free(vg_pbatch *b)
 		(void)hipStreamSynchronize(b->stream);
 	pbatch_drop_graph(b);
 	void *bufs[] = { b->d_progs, b->d_launch, b->d_info, b->d_q, b->d_head, b->d_items, b->d_witems,
-		b->d_icount, b->d_ibase };
+		b->d_icount, b->d_ibase, b->d_status };
 	for (void *p : bufs)
 		if (p)
 			(void)hipFree(p);
@@ -1478,12 +1485,24 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		b->groups[g].qids.push_back(i);
 	}
 	std::vector<vg_bqinfo> info(n);
-	uint32_t ioff = 0;
+	uint32_t ioff = 0, soff = 0;
 	for (unsigned i = 0; i < n; i++) {
 		const uint32_t nt = b->qs[i]->p.num_tiles;
 		info[i].nitem = (nt + tpi - 1) / tpi;
 		info[i].ioff = ioff;
+		info[i].soff = soff;
+		info[i].pad = 0;
 		ioff += info[i].nitem;
+		soff += nt;
+	}
+	const bool single = b->cfg.single != 0;
+	if (single && soff > b->status_cap) {
+		if (b->d_status) (void)hipFree(b->d_status);
+		b->d_status = nullptr;
+		b->status_cap = 0;
+		if (herr(hipMalloc((void **)&b->d_status, (size_t)soff * sizeof(uint64_t)), "tile status"))
+			return qerr(err, errlen, "cannot allocate the batch's tile status words");
+		b->status_cap = soff;
 	}
 	auto tiles = [&](unsigned q) { return b->qs[q]->p.num_tiles; };
 	std::vector<vg_item> items, witems;
@@ -1513,6 +1532,8 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 			g.clds = c > g.clds ? c : g.clds;
 			g.wlds = w > g.wlds ? w : g.wlds;
 		}
+		if (single)
+			g.clds = 0;
 	}
 	goff.push_back(items.size());
 	const size_t ni = items.size();
@@ -1573,7 +1594,10 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		A.ibase = b->d_ibase;
 		A.head = b->d_head + gi;
 		A.nhead = 1;
-		const uint32_t cmax = vg_kernel_batch_grid(0, g.clds), wmax = vg_kernel_batch_grid(1, g.wlds);
+		A.spin_limit = VG_SPIN_LIMIT;
+		A.status = single ? b->d_status : nullptr;
+		A.nstatus = 0;
+		const uint32_t cmax = vg_kernel_batch_grid(0, g.clds), wmax = vg_kernel_batch_grid(single ? 2 : 1, g.wlds);
 		const unsigned fr = b->cfg.grid_div ? b->cfg.grid_div : 1;
 		g.cgrid = cmax / fr ? cmax / fr : 1;
 		g.wgrid = wmax / fr ? wmax / fr : 1;
@@ -1585,6 +1609,8 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 	vg_batch_args R = b->groups[0].A;
 	R.head = b->d_head;
 	R.nhead = (uint32_t)ng;
+	R.status = single ? b->d_status : nullptr;
+	R.nstatus = single ? soff : 0;
 	pbatch_drop_graph(b);
 	if (herr(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture"))
 		return qerr(err, errlen, "graph capture failed");
@@ -1593,7 +1619,8 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		pgroup &g = b->groups[gi];
 		hipStream_t s = ng == 1 ? b->stream : b->branch[gi];
 		rc = (ng == 1 || hipStreamWaitEvent(s, b->fork, 0) == hipSuccess) &&
-			vg_kernel_batch_passes(&g.A, g.cgrid, g.clds, g.wgrid, g.wlds, s) == 0 &&
+			(single ? vg_kernel_batch_single(&g.A, g.wgrid, g.wlds, s)
+			        : vg_kernel_batch_passes(&g.A, g.cgrid, g.clds, g.wgrid, g.wlds, s)) == 0 &&
 			(ng == 1 || (hipEventRecord(b->join[gi], s) == hipSuccess &&
 				hipStreamWaitEvent(b->stream, b->join[gi], 0) == hipSuccess));
 	}
@@ -1704,6 +1731,36 @@ extern "C" int vg_pbatch_sync(vg_pbatch *b, unsigned long long *rows, double *la
 		q->last_rows = r;
 		if (rows)
 			rows[i] = r;
+	}
+	return VIRG_SUCCESS;
+}
+
+/* Wait for the run; per query: rows selected (valid when flags[i] is 0),
+ * latency, and flags[i] = the VG_OVF_* bits (the result is then
+ * incomplete and the query must be run again with exact sizing). */
+extern "C" int vg_pbatch_results(vg_pbatch *b, unsigned long long *rows, double *latency_ms, unsigned *flags)
+{
+	HCHK(hipStreamSynchronize(b->stream), "batch");
+	const size_t n = b->qs.size();
+	if (latency_ms) {
+		vg_bhead h;
+		HCHK(hipMemcpy(b->h_q.data(), b->d_q, n * sizeof(vg_bquery), hipMemcpyDeviceToHost), "batch state");
+		HCHK(hipMemcpy(&h, b->d_head, sizeof h, hipMemcpyDeviceToHost), "batch state");
+		for (size_t i = 0; i < n; i++)
+			latency_ms[i] = b->h_q[i].t_done > h.t_start
+				? (double)(b->h_q[i].t_done - h.t_start) / b->clock_khz : 0.0;
+	}
+	for (size_t i = 0; i < n; i++) {
+		vg_query *q = b->qs[i];
+		vg_ctrl c;
+		memset(&c, 0, sizeof c);
+		if (q->p.num_tiles)
+			HCHK(hipMemcpy(&c, q->lr.ctrl, sizeof c, hipMemcpyDeviceToHost), "control block");
+		q->last_rows = c.overflow ? 0 : c.total;
+		if (rows)
+			rows[i] = c.total;
+		if (flags)
+			flags[i] = c.overflow;
 	}
 	return VIRG_SUCCESS;
 }

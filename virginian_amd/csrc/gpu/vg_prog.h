@@ -163,7 +163,20 @@ typedef struct {
 typedef struct {
 	uint32_t nitem;          /* items of the query in each pass */
 	uint32_t ioff;           /* its first entry of icount / ibase */
+	uint32_t soff;           /* its first tile status word (single pass) */
+	uint32_t pad;
 } vg_bqinfo;
+
+/* Single pass: one status word per tile, published by the workgroup that
+ * holds the tile: bits 63:62 = 0 not yet, 1 the tile's own count, 2 the
+ * count of the query's rows up to and including the tile; bits 61:0 the
+ * count.  Cleared before every run. */
+#define VG_ST_AGG  (1ull << 62)
+#define VG_ST_PRE  (2ull << 62)
+#define VG_ST_VAL  ((1ull << 62) - 1)
+/* vg_ctrl.overflow bits */
+#define VG_OVF_CAPACITY 1u       /* output capacity exceeded */
+#define VG_OVF_WAIT     2u       /* a single-pass wait gave up: result incomplete */
 
 /* per-query state, cleared before every run */
 typedef struct {
@@ -194,7 +207,9 @@ typedef struct {
 	uint64_t *ibase;         /* output row of each item's first selected row */
 	vg_bhead *head;          /* this list's head (the reset clears nhead of them) */
 	uint32_t nhead;
-	uint32_t pad;
+	uint32_t spin_limit;     /* single pass: polls of a predecessor before giving up */
+	uint64_t *status;        /* single pass: tile status words of every query */
+	uint64_t nstatus;
 } vg_batch_args;
 
 #endif

@@ -615,14 +615,17 @@ __device__ __forceinline__ const char *cursor_base(const cursor &C, uint32_t k)
 	return (const char *)(((uint64_t)hi << 32) | (uint64_t)lo);
 }
 
+/* which columns load_cols loads */
+enum { VG_LOAD_FILTER = 0, VG_LOAD_WRITE = 1, VG_LOAD_ALL = 2 };
+
 template <int NS, uint32_t TR>
 __device__ __forceinline__ void load_cols(const vg_prog &P, const cursor &C, uint32_t row0,
-	const strips<NS> &S, uint32_t nrows, bool write_pass, uint32_t *regs, bool wait = true)
+	const strips<NS> &S, uint32_t nrows, int mode, uint32_t *regs, bool wait = true)
 {
 	const bool aligned = C.aligned != 0;
 	for (uint32_t k = 0; k < ldc(&P.nload); ++k) {
 		const vg_col c = cursor_col(C, k);
-		if (write_pass ? !c.wload : c.phase != VG_PHASE_FILTER)
+		if (mode == VG_LOAD_WRITE ? !c.wload : mode == VG_LOAD_FILTER && c.phase != VG_PHASE_FILTER)
 			continue;
 		const char *base = cursor_base(C, k) + (size_t)row0 * c.width;
 		load_col<NS, TR>(c, base, S, nrows, aligned, regs);
@@ -679,7 +682,7 @@ __device__ __forceinline__ uint32_t count_tiles(const vg_prog &P, const vg_launc
 		}
 		rowstate RS;
 		rows_init(RS, live);
-		load_cols<NS, T>(P, C, row0, S, nrows, false, regs);
+		load_cols<NS, T>(P, C, row0, S, nrows, VG_LOAD_FILTER, regs);
 		run_uops<NS, T>(P, 0, ldc(&P.nfilter), ~0ull, regs, S, RS);
 		const uint32_t v = RS.valid;
 		if constexpr (NS == 4) {
@@ -780,7 +783,59 @@ __device__ __forceinline__ void issue_tile(const vg_prog &P, const vg_launch &L,
 	const uint32_t left = C.rows - row0;
 	const uint32_t nrows = left < VG_T ? left : VG_T;
 	if (bits)
-		load_cols<VG_WS, VG_T>(P, C, row0, S, nrows, true, regs, false);
+		load_cols<VG_WS, VG_T>(P, C, row0, S, nrows, VG_LOAD_WRITE, regs, false);
+}
+
+/* Write n result rows of one wave: result row at+k comes from tile row
+ * srcrow[k] (register slots, a column of the tile's tablet, or a
+ * constant), lanes taking every 64th row. */
+__device__ __forceinline__ void write_rows(const vg_prog &P, const vg_launch &L, const cursor &Cc,
+	uint32_t row0, const uint32_t *regs, const uint16_t *srcrow, uint64_t at, uint32_t n, uint32_t lane)
+{
+	constexpr uint32_t T = VG_T;
+	for (uint32_t j = 0; j < ldc(&P.nout); ++j) {
+		const vg_out o = ldc_struct(&P.out[j]);
+		char *dst = (char *)ldc(&L.out[j]);
+		if (o.kind == VG_OUT_SLOT) {
+			const uint32_t *lo = regs + o.slot * T, *hi = regs + o.slot_hi * T;
+			if (o.width == 8) {
+				for (uint32_t k = lane; k < n; k += 64) {
+					const uint32_t s = srcrow[k];
+					((uint64_t *)dst)[at + k] = ((uint64_t)hi[s] << 32) | lo[s];
+				}
+			} else if (o.width == 4) {
+				for (uint32_t k = lane; k < n; k += 64)
+					((uint32_t *)dst)[at + k] = lo[srcrow[k]];
+			} else {
+				for (uint32_t k = lane; k < n; k += 64)
+					((uint8_t *)dst)[at + k] = (uint8_t)lo[srcrow[k]];
+			}
+		} else if (o.kind == VG_OUT_COLUMN) {
+			const vg_col cc = cursor_col(Cc, o.col);
+			const char *src = cursor_base(Cc, o.col) + (size_t)row0 * cc.width;
+			if (o.width == 8) {
+				for (uint32_t k = lane; k < n; k += 64)
+					((uint64_t *)dst)[at + k] = gptr<uint64_t>(src)[srcrow[k]];
+			} else if (o.width == 4) {
+				for (uint32_t k = lane; k < n; k += 64)
+					((uint32_t *)dst)[at + k] = gptr<uint32_t>(src)[srcrow[k]];
+			} else {
+				for (uint32_t k = lane; k < n; k += 64)
+					((uint8_t *)dst)[at + k] = gptr<uint8_t>(src)[srcrow[k]];
+			}
+		} else {
+			if (o.width == 8) {
+				for (uint32_t k = lane; k < n; k += 64)
+					((uint64_t *)dst)[at + k] = o.imm;
+			} else if (o.width == 4) {
+				for (uint32_t k = lane; k < n; k += 64)
+					((uint32_t *)dst)[at + k] = (uint32_t)o.imm;
+			} else {
+				for (uint32_t k = lane; k < n; k += 64)
+					((uint8_t *)dst)[at + k] = (uint8_t)o.imm;
+			}
+		}
+	}
 }
 
 /* Write the selected rows of tiles [t0, t1) of a launch, the first at
@@ -885,49 +940,7 @@ __device__ __forceinline__ uint64_t write_tiles(const vg_prog &P, const vg_launc
 			const uint32_t n = (uint64_t)wave_tot < room ? wave_tot : (uint32_t)room;
 			if (n < wave_tot && lane == 0)
 				atomicOr(&ldc(&L.ctrl)->overflow, 1u);
-			for (uint32_t j = 0; j < ldc(&P.nout); ++j) {
-				const vg_out o = ldc_struct(&P.out[j]);
-				char *dst = (char *)ldc(&L.out[j]);
-				if (o.kind == VG_OUT_SLOT) {
-					const uint32_t *lo = regs + o.slot * T, *hi = regs + o.slot_hi * T;
-					if (o.width == 8) {
-						for (uint32_t k = lane; k < n; k += 64) {
-							const uint32_t s = srcrow[k];
-							((uint64_t *)dst)[at + k] = ((uint64_t)hi[s] << 32) | lo[s];
-						}
-					} else if (o.width == 4) {
-						for (uint32_t k = lane; k < n; k += 64)
-							((uint32_t *)dst)[at + k] = lo[srcrow[k]];
-					} else {
-						for (uint32_t k = lane; k < n; k += 64)
-							((uint8_t *)dst)[at + k] = (uint8_t)lo[srcrow[k]];
-					}
-				} else if (o.kind == VG_OUT_COLUMN) {
-					const vg_col cc = cursor_col(Cc, o.col);
-					const char *src = cursor_base(Cc, o.col) + (size_t)row0 * cc.width;
-					if (o.width == 8) {
-						for (uint32_t k = lane; k < n; k += 64)
-							((uint64_t *)dst)[at + k] = gptr<uint64_t>(src)[srcrow[k]];
-					} else if (o.width == 4) {
-						for (uint32_t k = lane; k < n; k += 64)
-							((uint32_t *)dst)[at + k] = gptr<uint32_t>(src)[srcrow[k]];
-					} else {
-						for (uint32_t k = lane; k < n; k += 64)
-							((uint8_t *)dst)[at + k] = gptr<uint8_t>(src)[srcrow[k]];
-					}
-				} else {
-					if (o.width == 8) {
-						for (uint32_t k = lane; k < n; k += 64)
-							((uint64_t *)dst)[at + k] = o.imm;
-					} else if (o.width == 4) {
-						for (uint32_t k = lane; k < n; k += 64)
-							((uint32_t *)dst)[at + k] = (uint32_t)o.imm;
-					} else {
-						for (uint32_t k = lane; k < n; k += 64)
-							((uint8_t *)dst)[at + k] = (uint8_t)o.imm;
-					}
-				}
-			}
+			write_rows(P, L, Cc, row0, regs, srcrow, at, n, lane);
 			/* srcrow and the register rows are rewritten by the next tile */
 			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 			__builtin_amdgcn_wave_barrier();
@@ -1003,6 +1016,8 @@ __global__ __launch_bounds__(256) void vg_batch_reset(const vg_batch_args A)
 		A.head[i].wnext = 0;
 		A.head[i].t_start = wall_clock64();
 	}
+	for (uint64_t j = i; j < A.nstatus; j += (uint64_t)gridDim.x * 256)
+		A.status[j] = 0;
 }
 
 /* the next item of a list, in every thread (query ~0u: none left) */
@@ -1099,6 +1114,150 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_batch_write_kernel(const 
 		 * the fetch barrier ahead of the fetch and reread the old item,
 		 * forever. */
 		__syncthreads();
+	}
+}
+
+/* 64-bit sum over the wave */
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t x)
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1)
+		x += __shfl_xor(x, off, 64);
+	return x;
+}
+
+/* Decoupled look-back (run by one whole wave): publish tile `tile`'s count
+ * n, add up the counts of the tiles before it -- the nearest published
+ * running total plus the single counts after it -- and publish the running
+ * total through this tile.  Returns the rows selected before the tile, or
+ * ~0 when a predecessor stayed unpublished for spin_limit polls. */
+__device__ __forceinline__ uint64_t lookback(uint64_t *st, uint32_t tile, uint32_t n, uint32_t lane,
+	uint32_t spin_limit)
+{
+	if (tile == 0) {
+		if (lane == 0)
+			__hip_atomic_store(&st[0], VG_ST_PRE | n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		return 0;
+	}
+	if (lane == 0)
+		__hip_atomic_store(&st[tile], VG_ST_AGG | n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	uint64_t sum = 0;
+	int64_t j0 = (int64_t)tile - 1;         /* lane l looks at tile j0 - l */
+	uint32_t polls = 0;
+	for (;;) {
+		const int64_t j = j0 - (int64_t)lane;
+		const uint64_t v = j >= 0 ? __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+		                          : VG_ST_PRE;          /* before tile 0: total 0 */
+		const uint32_t f = (uint32_t)(v >> 62);
+		const uint64_t pm = __ballot(f == 2u), nm = __ballot(f == 0u);
+		const uint64_t first = pm & (~pm + 1);          /* nearest running total */
+		const uint64_t upto = pm ? (first | (first - 1)) : ~0ull;
+		if (nm & upto) {
+			/* a nearer tile has not published yet */
+			if (++polls >= spin_limit)
+				return ~0ull;
+			__builtin_amdgcn_s_sleep(1);
+			continue;
+		}
+		sum += wave_sum64(((upto >> lane) & 1ull) ? (v & VG_ST_VAL) : 0ull);
+		if (pm)
+			break;
+		j0 -= 64;
+	}
+	if (lane == 0)
+		__hip_atomic_store(&st[tile], VG_ST_PRE | (sum + n), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	return sum;
+}
+
+/* Single pass over tile items (each query's tiles listed in order): the 4
+ * waves load every column the program uses for their strips of the tile,
+ * run the whole program once, count, take the tile's output offset by
+ * look-back, and write the selected rows there.  The output capacity is
+ * the caller's estimate; past it rows are dropped and VG_OVF_CAPACITY set,
+ * and the caller runs the exact two passes instead. */
+__global__ __launch_bounds__(256) void vg_batch_single_kernel(const vg_batch_args A)
+{
+	constexpr uint32_t T = VG_T;
+	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+	__shared__ vg_item s_item;
+	__shared__ uint32_t s_wc[4];
+	__shared__ uint64_t s_pre;
+	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
+	const strips<1> S{wave * 256u, 256u, lane};
+	for (;;) {
+		uint32_t q, t0, t1, k;
+		next_item(A.witems, A.nitems, &A.head->wnext, &s_item, q, t0, t1, k);
+		if (q == ~0u)
+			break;
+		const vg_prog &P = A.progs[q];
+		const vg_launch &L = A.launch[q];
+		uint64_t *st = A.status + ldc(&A.info[q].soff);
+		uint32_t *regs = lds;
+		uint16_t *srcrow = (uint16_t *)(lds + ldc(&P.nslot) * T) + wave * 256;
+		cursor C;
+		cursor_init(C, P, lane);
+		for (uint32_t tile = t0; tile < t1; ++tile) {
+			cursor_seek(C, P, L, tile, lane);
+			const uint32_t row0 = (tile - C.first) * T;
+			const uint32_t left = C.rows - row0;
+			const uint32_t nrows = left < T ? left : T;
+			uint32_t live = 0xFu;
+			if (nrows < T) {
+				live = 0;
+#pragma unroll
+				for (int j = 0; j < 4; ++j)
+					live |= S.row(0) + j < nrows ? 1u << j : 0u;
+			}
+			rowstate RS;
+			rows_init(RS, live);
+			load_cols<1, T>(P, C, row0, S, nrows, VG_LOAD_ALL, regs);
+			run_uops<1, T>(P, 0, ldc(&P.nuop), ~0ull, regs, S, RS);
+			const uint32_t mybits = RS.valid & 0xFu;
+			const uint32_t wc = wave_sum((uint32_t)__builtin_popcount(mybits));
+			if (lane == 0)
+				s_wc[wave] = wc;
+			__syncthreads();
+			const uint32_t n = s_wc[0] + s_wc[1] + s_wc[2] + s_wc[3];
+			if (wave == 0) {
+				const uint64_t pre = lookback(st, tile, n, lane, A.spin_limit);
+				if (lane == 0)
+					s_pre = pre;
+			}
+			__syncthreads();
+			const uint64_t pre = s_pre;
+			if (pre == ~0ull) {
+				if (tid == 0)
+					atomicOr(&ldc(&L.ctrl)->overflow, VG_OVF_WAIT);
+			} else {
+				uint32_t wave_off = 0;
+				for (uint32_t w = 0; w < wave; ++w)
+					wave_off += s_wc[w];
+				uint32_t pos = 0;
+#pragma unroll
+				for (int r = 0; r < 4; ++r)
+					pos += rank_below(__ballot((mybits >> r) & 1u));
+#pragma unroll
+				for (int r = 0; r < 4; ++r)
+					if ((mybits >> r) & 1u)
+						srcrow[pos++] = (uint16_t)(S.row(0) + r);
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+				const uint64_t at = pre + uni(wave_off);
+				const uint64_t cap = ldc(&L.capacity);
+				const uint64_t room = cap > at ? cap - at : 0;
+				const uint32_t m = (uint64_t)wc < room ? wc : (uint32_t)room;
+				if (m < wc && lane == 0)
+					atomicOr(&ldc(&L.ctrl)->overflow, VG_OVF_CAPACITY);
+				write_rows(P, L, C, row0, regs, srcrow, at, m, lane);
+				if (tile + 1 == ldc(&L.num_tiles) && tid == 0)
+					ldc(&L.ctrl)->total = pre + n;
+			}
+			__syncthreads();        /* registers, srcrow, s_wc, s_pre are reused */
+		}
+		if (tid == 0 && atomicAdd(&A.q[q].wdone, t1 - t0) + (t1 - t0) == ldc(&L.num_tiles))
+			A.q[q].t_done = wall_clock64();
+		__syncthreads();            /* see the write kernel: keeps the fetch in the loop */
 	}
 }
 
@@ -1234,7 +1393,8 @@ extern "C" uint32_t vg_kernel_tile_rows(void)
 }
 
 /* dynamic LDS of a program's batch count items (pass 0: the tile's
- * register file shared by the 4 waves) or write items (pass 1) */
+ * register file shared by the 4 waves), write items (pass 1) or
+ * single-pass items (pass 2) */
 extern "C" size_t vg_kernel_batch_lds(int pass, const vg_prog *P)
 {
 	return pass == 0 ? (size_t)P->nslot_count * VG_T * 4 : write_lds_bytes(P);
@@ -1248,6 +1408,8 @@ static void batch_attrs()
 			hipFuncAttributeMaxDynamicSharedMemorySize, VG_LDS_LIMIT);
 		(void)hipFuncSetAttribute((const void *)vg_batch_write_kernel,
 			hipFuncAttributeMaxDynamicSharedMemorySize, VG_LDS_LIMIT);
+		(void)hipFuncSetAttribute((const void *)vg_batch_single_kernel,
+			hipFuncAttributeMaxDynamicSharedMemorySize, VG_LDS_LIMIT);
 		done = true;
 	}
 }
@@ -1260,7 +1422,9 @@ extern "C" uint32_t vg_kernel_batch_grid(int pass, size_t lds)
 	int per = 0;
 	const hipError_t e = pass == 0
 		? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, vg_batch_count_kernel, 256, lds)
-		: hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, vg_batch_write_kernel, VG_WRITE_THREADS, lds);
+		: pass == 1
+		? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, vg_batch_write_kernel, VG_WRITE_THREADS, lds)
+		: hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, vg_batch_single_kernel, 256, lds);
 	if (e != hipSuccess || per < 1)
 		per = 1;
 	return (uint32_t)num_cus() * (uint32_t)per;
@@ -1269,8 +1433,9 @@ extern "C" uint32_t vg_kernel_batch_grid(int pass, size_t lds)
 /* Clear the state of every query and of A->nhead item-list heads. */
 extern "C" int vg_kernel_batch_reset(const vg_batch_args *A, hipStream_t stream)
 {
-	const uint32_t m = A->nquery > A->nhead ? A->nquery : A->nhead;
-	const uint32_t rgrid = (m + 255) / 256;
+	uint64_t m = A->nquery > A->nhead ? A->nquery : A->nhead;
+	m = m > A->nstatus ? m : (A->nstatus < 256u * 1024u ? A->nstatus : 256u * 1024u);
+	const uint32_t rgrid = (uint32_t)((m + 255) / 256);
 	hipLaunchKernelGGL(vg_batch_reset, dim3(rgrid ? rgrid : 1), dim3(256), 0, stream, *A);
 	return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -1286,5 +1451,15 @@ extern "C" int vg_kernel_batch_passes(const vg_batch_args *A, uint32_t cgrid, si
 	if (hipGetLastError() != hipSuccess)
 		return -2;
 	hipLaunchKernelGGL(vg_batch_write_kernel, dim3(wgrid), dim3(VG_WRITE_THREADS), wlds, stream, *A);
+	return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+/* The single-pass launch over A's (write) item list. */
+extern "C" int vg_kernel_batch_single(const vg_batch_args *A, uint32_t grid, size_t lds, hipStream_t stream)
+{
+	if (lds > VG_LDS_LIMIT)
+		return -3;
+	batch_attrs();
+	hipLaunchKernelGGL(vg_batch_single_kernel, dim3(grid), dim3(256), lds, stream, *A);
 	return hipGetLastError() == hipSuccess ? 0 : -2;
 }
