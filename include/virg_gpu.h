@@ -33,11 +33,26 @@ typedef struct vg_query vg_query;
  * Result, Converge, Finish: what virg_sql or virg_vm_addop produce) for
  * repeated execution over the table its Table op names.  The table's
  * tablets are copied to HBM once.  Output capacity is `capacity` rows
- * (0 = the table's row count).  `reserved` must be 0.  Returns NULL with a
- * message in err when the program cannot run (no device, unsupported op,
- * operand types that differ, too many registers). */
+ * (0 = the table's row count).  Returns NULL with a message in err when
+ * the program cannot run (no device, unsupported op, operand types that
+ * differ, too many registers).  flags (0 or an OR of):
+ *   VG_QUERY_LIGHT      for serving: the table's tablet descriptors are
+ *                       shared with other queries, the query uses the
+ *                       device's stream, its control block comes from a
+ *                       pool and the two-pass arrays are made on first use;
+ *   VG_QUERY_NO_OUTPUT  no output memory; give it with vg_query_set_outputs. */
+#define VG_QUERY_LIGHT     1
+#define VG_QUERY_NO_OUTPUT 2
 vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned num_ops,
-	unsigned long long capacity, int reserved, char *err, size_t errlen);
+	unsigned long long capacity, int flags, char *err, size_t errlen);
+/* Results go to caller device memory from now on: out[j] has room for
+ * `capacity` rows of result column j. */
+int vg_query_set_outputs(vg_query *q, void *const *out, unsigned long long capacity);
+/* Bytes per row of each result column; returns the number of columns. */
+unsigned vg_query_out_widths(vg_query *q, unsigned *widths);
+/* Device memory for caller-managed outputs (a serving arena). */
+void *vg_device_alloc(virginian *v, size_t bytes);
+void vg_device_free(virginian *v, void *p);
 void vg_query_free(vg_query *q);
 
 /* Launch the section once on the query's stream (asynchronous). */
@@ -97,6 +112,9 @@ typedef struct {
 	unsigned split;            /* 0: group the queries by occupancy, 1: one group */
 	unsigned order;            /* write list: 0 batch order, 1 smallest query first,
 	                            * 2 largest first (count lists are largest first) */
+	unsigned direct;           /* 1: launch directly instead of replaying a captured graph
+	                            * (for batches whose queries change: vg_pbatch_reload) */
+	virginian *db;             /* direct batches created empty: the database */
 	unsigned single;           /* 1: one pass into each query's capacity as given (an
 	                            * estimate): every tile counts, takes its output offset
 	                            * by decoupled look-back over the query's earlier tiles
@@ -107,6 +125,10 @@ typedef struct vg_pbatch vg_pbatch;
 vg_pbatch *vg_pbatch_create(vg_query *const *qs, unsigned n, const vg_pbatch_config *cfg,
 	char *err, size_t errlen);
 void vg_pbatch_free(vg_pbatch *b);
+/* Replace the batch's queries (after its last run). */
+int vg_pbatch_reload(vg_pbatch *b, vg_query *const *qs, unsigned n, char *err, size_t errlen);
+/* 1 when the batch's last run has finished, else 0 (does not wait). */
+int vg_pbatch_done(vg_pbatch *b);
 /* Rebuild the item lists and the graph for another configuration. */
 int vg_pbatch_configure(vg_pbatch *b, const vg_pbatch_config *cfg, char *err, size_t errlen);
 /* The configuration in use and the number of query groups. */
@@ -170,6 +192,28 @@ void vg_host_free(void *p);
 /* Best host-to-device and device-to-host copy rates (GB/s) of one pinned
  * buffer of `bytes` over `reps` copies each. */
 int vg_link_probe(virginian *v, size_t bytes, int reps, double *h2d_gbs, double *d2h_gbs);
+
+/* Result-size predictor (no counterpart in the reference): a 3-layer
+ * perceptron nin -> nh -> nh -> 1 (ReLU, ReLU, sigmoid; f32) predicting
+ * the fraction of a tablet's rows a program selects, trained on the GPU
+ * by its own forward, backward and Adam kernels.  Parameters, flat f32:
+ * W1[nin][nh] b1[nh] W2[nh][nh] b2[nh] W3[nh] b3; loss = mean (y - t)^2.
+ * Arrays are host memory, examples row-major (n x nin). */
+typedef struct vg_mlp vg_mlp;
+vg_mlp *vg_mlp_create(unsigned nin, unsigned nh, unsigned long long seed);
+void vg_mlp_free(vg_mlp *m);
+unsigned vg_mlp_num_params(const vg_mlp *m);
+int vg_mlp_get_params(const vg_mlp *m, float *out);
+int vg_mlp_set_params(vg_mlp *m, const float *in);
+int vg_mlp_forward(vg_mlp *m, const float *x, unsigned n, float *y);
+int vg_mlp_gradient(vg_mlp *m, const float *x, const float *t, unsigned n, float *loss, float *grad);
+int vg_mlp_train(vg_mlp *m, const float *x, const float *t, unsigned n, unsigned epochs, unsigned batch,
+	float lr, unsigned long long seed, float *loss);
+
+/* Rows the last launch of q selected in each of its tablets (n entries at
+ * most, in table order; the per-tablet counts the predictor learns from);
+ * returns the number of tablets. */
+unsigned vg_query_tablet_rows(vg_query *q, unsigned long long *rows, unsigned n);
 
 /* Architecture name and compute-unit count of the device in use
  * (VIRG_FAIL and *cus = 0 without one). */

@@ -1,0 +1,68 @@
+"""CPU restatement of the result-size predictor (test infrastructure: only
+tests/ and bench.py's CPU leg use it; the product trains and runs the
+network with its own HIP kernels, virginian_amd/csrc/gpu/mlp.hip).
+
+The network has no counterpart in the reference, so there is no reference
+file to follow.  Its definition is the textbook one:
+    h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), y = sigmoid(h2 W3 + b3)
+    loss = mean((y - t)^2)
+with parameters packed flat as W1[nin][nh] b1[nh] W2[nh][nh] b2[nh] W3[nh] b3.
+
+Pinned by: central finite differences in float64 against `gradient`
+(tests/test_mlp.py), which a wrong sign, a dropped ReLU mask or a
+transposed weight fails; and a closed form for nh = 1 with zero weights.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def layout(nin: int, nh: int) -> dict:
+    o = {}
+    k = 0
+    for name, size in (("W1", nin * nh), ("b1", nh), ("W2", nh * nh), ("b2", nh), ("W3", nh), ("b3", 1)):
+        o[name] = (k, k + size)
+        k += size
+    o["n"] = k
+    return o
+
+
+def unpack(p: np.ndarray, nin: int, nh: int) -> dict:
+    L = layout(nin, nh)
+    g = {k: p[v[0]:v[1]] for k, v in L.items() if k != "n"}
+    return {"W1": g["W1"].reshape(nin, nh), "b1": g["b1"], "W2": g["W2"].reshape(nh, nh), "b2": g["b2"],
+            "W3": g["W3"].reshape(nh, 1), "b3": g["b3"]}
+
+
+def forward(p: np.ndarray, x: np.ndarray, nin: int, nh: int):
+    w = unpack(p, nin, nh)
+    z1 = x @ w["W1"] + w["b1"]
+    h1 = np.maximum(z1, 0)
+    z2 = h1 @ w["W2"] + w["b2"]
+    h2 = np.maximum(z2, 0)
+    z3 = h2 @ w["W3"] + w["b3"]
+    y = 1.0 / (1.0 + np.exp(-z3[:, 0]))
+    return y, (h1, h2)
+
+
+def loss(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int) -> float:
+    y, _ = forward(p, x, nin, nh)
+    return float(np.mean((y - t) ** 2))
+
+
+def gradient(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int):
+    """(loss, gradient packed like p) by backpropagation."""
+    w = unpack(p, nin, nh)
+    y, (h1, h2) = forward(p, x, nin, nh)
+    B = x.shape[0]
+    d3 = (2.0 * (y - t) / B * y * (1.0 - y))[:, None]     # dL/dz3
+    gW3 = h2.T @ d3
+    gb3 = d3.sum(axis=0)
+    d2 = (d3 @ w["W3"].T) * (h2 > 0)
+    gW2 = h1.T @ d2
+    gb2 = d2.sum(axis=0)
+    d1 = (d2 @ w["W2"].T) * (h1 > 0)
+    gW1 = x.T @ d1
+    gb1 = d1.sum(axis=0)
+    g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
+    return float(np.mean((y - t) ** 2)), g

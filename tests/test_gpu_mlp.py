@@ -1,0 +1,142 @@
+"""The result-size predictor on the GPU: the HIP forward and backward
+kernels against the CPU oracle (tolerance for f32 sums of up to a few
+hundred terms: 1e-4 relative), training on the executor's own per-tablet
+counts, and what the trained network gains over sizing without it."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import virginian_amd as va
+from oracle import mlp as M
+from oracle import oracle
+from tests.helpers import assert_same
+from virginian_amd import predict as PR
+from virginian_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_forward_and_gradient_match_oracle(gpu_db):
+    for nin, nh, B, seed in [(7, 5, 9, 1), (PR.NFEATURES, 32, 300, 2), (40, 32, 1, 3)]:
+        net = PR.MLP(nin, nh, seed=seed)
+        rng = np.random.default_rng(seed)
+        # biases away from zero so that ReLUs are on and off
+        p = net.params().astype(np.float64)
+        p += rng.normal(0, 0.05, len(p))
+        net.set_params(p.astype(np.float32))
+        p = net.params().astype(np.float64)
+        x = rng.normal(0, 1, (B, nin)).astype(np.float32)
+        t = rng.uniform(0, 1, B).astype(np.float32)
+        y = net.forward(x)
+        y_ref, _ = M.forward(p, x.astype(np.float64), nin, nh)
+        assert np.allclose(y, y_ref, rtol=1e-4, atol=1e-6), (nin, np.abs(y - y_ref).max())
+        loss, g = net.gradient(x, t)
+        loss_ref, g_ref = M.gradient(p, x.astype(np.float64), t.astype(np.float64), nin, nh)
+        assert np.isclose(loss, loss_ref, rtol=1e-4)
+        scale = np.abs(g_ref).max()
+        assert np.allclose(g, g_ref, rtol=1e-4, atol=1e-5 * scale), (nin, np.abs(g - g_ref).max(), scale)
+        net.close()
+
+
+def test_training_fits_a_known_function(gpu_db):
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, (2000, 6)).astype(np.float32)
+    t = (1 / (1 + np.exp(-(2 * x[:, 0] - x[:, 1] * x[:, 2])))).astype(np.float32)
+    net = PR.MLP(6, 32, seed=3)
+    before = M.loss(net.params().astype(np.float64), x, t, 6, 32)
+    after = net.train(x, t, epochs=40, batch=128, lr=3e-3)
+    assert after < before / 10 and after < 2e-3, (before, after)
+    # the loss the GPU reports is the oracle's loss of the trained weights
+    assert np.isclose(after, M.loss(net.params().astype(np.float64), x, t, 6, 32), rtol=1e-3)
+    net.close()
+
+
+@pytest.fixture
+def learned(gpu_db):
+    """A trained predictor over three tables and held-out programs."""
+    rng = np.random.default_rng(17)
+    sizes = [200_000, 700_000, 1_500_000]
+    stats = {}
+    for k, rows in enumerate(sizes):
+        gpu_db.gen_table(f"t{k}", rows, 1 << 17, seed=40 + k)
+        tid = gpu_db.table_id(f"t{k}")
+        stats[tid] = PR.TableStats(gpu_db, tid)
+    tids = list(stats)
+
+    def progs(n):
+        out = []
+        for _ in range(n):
+            sel = float(rng.uniform(0.0001, 0.8)) if rng.random() < 0.8 else None
+            out.append(W.random_program(rng, max_ops=40, selectivity=sel, table=int(rng.choice(tids))))
+        return out
+
+    train, held = progs(240), progs(60)
+    x, t = PR.collect(gpu_db, train, stats)
+    net = PR.MLP(seed=7)
+    net.train(x, t, epochs=80, batch=256, lr=2e-3)
+    yield gpu_db, net, stats, train, held, (x, t)
+    net.close()
+
+
+def test_predictor_gains_over_sizing_without_it(learned):
+    """At the same overflow rate on held-out programs, buffers sized by
+    the network reserve less than buffers sized from the mean selectivity
+    of the training runs (the method without it), and far less than the
+    table (what a single pass needs with no estimate at all)."""
+    db, net, stats, train, held, (x, t) = learned
+    hx, ht = PR.collect(db, held, stats)
+    mae_net = float(np.mean(np.abs(net.forward(hx) - ht)))
+    mae_const = float(np.mean(np.abs(np.mean(t) - ht)))
+    print(f"held-out MAE per tablet: network {mae_net:.4f}, mean selectivity {mae_const:.4f}")
+    assert mae_net < 0.5 * mae_const
+    pred_net, pred_const, true_rows, rows = [], [], [], []
+    for ops in held:
+        st = stats[PR.table_of(ops)]
+        f = PR.program_features(ops, st)
+        pred_net.append(net.forward(f))
+        pred_const.append(np.full(len(st.rows), float(np.mean(t))))
+        rows.append(st.rows)
+    k = 0
+    for ops, r in zip(held, rows):
+        true_rows.append(float((ht[k:k + len(r)] * r).sum()))
+        k += len(r)
+    m_net = PR.calibrate(pred_net, true_rows, rows, 0.05)
+    m_const = PR.calibrate(pred_const, true_rows, rows, 0.05)
+    res_net = sum(PR.capacity(p, r, m_net) for p, r in zip(pred_net, rows))
+    res_const = sum(PR.capacity(p, r, m_const) for p, r in zip(pred_const, rows))
+    res_all = sum(r.sum() for r in rows)
+    print(f"reserved rows at <=5% overflow: network {res_net} (margin {m_net:.3f}), "
+          f"mean selectivity {res_const} (margin {m_const:.3f}), whole tables {res_all:.0f}, "
+          f"selected {sum(true_rows):.0f}")
+    assert res_net < 0.7 * res_const
+
+
+def test_single_pass_with_predicted_sizes(learned):
+    """Single pass into network-sized buffers; a query that did not fit is
+    rerun in two passes; every result equals the oracle's."""
+    db, net, stats, train, held, _ = learned
+    qs, want = [], []
+    for ops in held[:40]:
+        st = stats[PR.table_of(ops)]
+        cap = PR.capacity(net.forward(PR.program_features(ops, st)), st.rows, 0.02)
+        qs.append(db.prepare(ops, capacity=cap, flags=va.VG_QUERY_LIGHT))
+        want.append(oracle.select(db, ops)[0])
+    b = va.PersistentBatch(qs, tiles_per_item=4, single=1)
+    try:
+        rows, _, flags = b.results()
+    finally:
+        b.close()
+    print(f"{sum(1 for f in flags if f)} of {len(qs)} did not fit", flush=True)
+    for i, (q, w, f) in enumerate(zip(qs, want, flags)):
+        n = len(next(iter(w.values())))
+        assert rows[i] == n
+        if f:
+            q2 = db.prepare(held[i], capacity=n, flags=va.VG_QUERY_LIGHT)
+            q2.run()
+            assert_same(q2.fetch(), w, f"query {i} rerun")
+            q2.close()
+        else:
+            assert_same(q.fetch(), w, f"query {i} single pass")
+    for q in qs:
+        q.close()

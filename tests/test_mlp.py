@@ -1,0 +1,96 @@
+"""The result-size predictor on the CPU: the oracle's backpropagation is
+pinned by central finite differences (and a planted mistake shows the
+check has teeth); features and sizing helpers on a small table."""
+from __future__ import annotations
+
+import numpy as np
+
+import virginian_amd as va
+from oracle import mlp as M
+from tests.helpers import CONFIG1_SQL
+from virginian_amd import predict as PR
+
+
+def _fd(p, x, t, nin, nh, eps=1e-6):
+    g = np.zeros_like(p)
+    for i in range(len(p)):
+        a, b = p.copy(), p.copy()
+        a[i] += eps
+        b[i] -= eps
+        g[i] = (M.loss(a, x, t, nin, nh) - M.loss(b, x, t, nin, nh)) / (2 * eps)
+    return g
+
+
+def _case(seed, nin=5, nh=4, B=7):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.7, M.layout(nin, nh)["n"])
+    x = rng.normal(0, 1, (B, nin))
+    t = rng.uniform(0, 1, B)
+    return p, x, t, nin, nh
+
+
+def test_oracle_gradient_matches_finite_differences():
+    for seed in range(4):
+        p, x, t, nin, nh = _case(seed)
+        _, g = M.gradient(p, x, t, nin, nh)
+        fd = _fd(p, x, t, nin, nh)
+        assert np.allclose(g, fd, rtol=1e-5, atol=1e-8), (seed, np.abs(g - fd).max())
+
+
+def test_finite_difference_check_catches_a_dropped_relu_mask():
+    p, x, t, nin, nh = _case(11)
+    w = M.unpack(p, nin, nh)
+    y, (h1, h2) = M.forward(p, x, nin, nh)
+    d3 = (2.0 * (y - t) / len(t) * y * (1 - y))[:, None]
+    d2_wrong = d3 @ w["W3"].T                  # the ReLU mask left out
+    gW2_wrong = h1.T @ d2_wrong
+    L = M.layout(nin, nh)
+    fd = _fd(p, x, t, nin, nh)[L["W2"][0]:L["W2"][1]]
+    assert not np.allclose(gW2_wrong.ravel(), fd, rtol=1e-3, atol=1e-6)
+
+
+def test_oracle_closed_form_at_zero_weights():
+    nin, nh = 3, 2
+    p = np.zeros(M.layout(nin, nh)["n"])
+    x = np.ones((4, nin))
+    t = np.array([0.0, 1.0, 0.25, 0.5])
+    y, _ = M.forward(p, x, nin, nh)
+    assert np.allclose(y, 0.5)                 # sigmoid(0)
+    loss, g = M.gradient(p, x, t, nin, nh)
+    assert np.isclose(loss, np.mean((0.5 - t) ** 2))
+    L = M.layout(nin, nh)
+    # only b3 has a gradient: mean 2 (y - t) y (1 - y)
+    assert np.isclose(g[L["b3"][0]], np.mean(2 * (0.5 - t) * 0.25))
+    assert np.count_nonzero(g) == 1
+
+
+def test_features_and_estimates(db):
+    db.gen_table("test", 300_000, 1 << 16, seed=3)
+    tid = db.table_id("test")
+    st = PR.TableStats(db, tid)
+    assert len(st.rows) == 5 and st.rows.sum() == 300_000
+    # uniformi < 50: half the rows on every tablet
+    est = st.passing(0, va.OP_Lt, 50.0)
+    assert np.all(np.abs(est - 0.5) < 0.02)
+    # uniformf > 90 (uniform on [0, 100))
+    est = st.passing(3, va.OP_Gt, 90.0)
+    assert np.all(np.abs(est - 0.1) < 0.02)
+    vm = db.compile(CONFIG1_SQL)
+    try:
+        ops = va.program(vm)
+    finally:
+        db.free_vm(vm)
+    f = PR.program_features(ops, st)
+    assert f.shape == (5, PR.NFEATURES) and f.dtype == np.float32
+    assert np.all(np.isfinite(f))
+    # two compares found, each on a bare column against a constant
+    assert f[0, 0] == 1 and f[0, PR.TERM_F] == 1 and f[0, 2 * PR.TERM_F] == 0
+
+
+def test_capacity_and_calibration():
+    rows = np.array([1000.0, 1000.0])
+    assert PR.capacity(np.array([0.1, 0.2]), rows, 0.0) == 300 + 64
+    assert PR.capacity(np.array([0.9, 0.9]), rows, 0.5) == 2000 + 64
+    preds = [np.array([0.1]), np.array([0.1])]
+    m = PR.calibrate(preds, [100.0, 300.0], [np.array([1000.0])] * 2, 0.0)
+    assert PR.capacity(preds[1], np.array([1000.0]), m) >= 300

@@ -18,6 +18,7 @@ _LIB_PATH = Path(os.environ.get("VIRGINIAN_AMD_LIB") or
                  Path(__file__).resolve().parent / "lib" / "libvirginian_amd.so")
 
 VIRG_FAIL, VIRG_SUCCESS = 0, 1
+VG_QUERY_LIGHT, VG_QUERY_NO_OUTPUT = 1, 2
 VIRG_MAX_COLUMNS = 16
 VIRG_MAX_COLUMN_NAME = 16
 VIRG_MAX_TABLES = 16
@@ -133,10 +134,10 @@ class StreamStats(C.Structure):
 class PBatchConfig(C.Structure):
     """vg_pbatch_config (include/virg_gpu.h)."""
     _fields_ = [("tiles_per_item", C.c_uint), ("grid_div", C.c_uint), ("split", C.c_uint),
-                ("order", C.c_uint), ("single", C.c_uint)]
+                ("order", C.c_uint), ("direct", C.c_uint), ("db", C.c_void_p), ("single", C.c_uint)]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "db"}
 
 
 def _load() -> C.CDLL:
@@ -207,6 +208,12 @@ def _load() -> C.CDLL:
         "vg_pbatch_sync": (C.c_int, [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_double)]),
         "vg_pbatch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "vg_pbatch_results": (C.c_int, [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+        "vg_pbatch_reload": (C.c_int, [P, C.POINTER(P), C.c_uint, C.c_char_p, C.c_size_t]),
+        "vg_pbatch_done": (C.c_int, [P]),
+        "vg_query_set_outputs": (C.c_int, [P, C.POINTER(P), C.c_ulonglong]),
+        "vg_query_out_widths": (C.c_uint, [P, C.POINTER(C.c_uint)]),
+        "vg_device_alloc": (P, [V, C.c_size_t]),
+        "vg_device_free": (None, [V, P]),
         "vg_pbatch_tune": (C.c_int, [P, C.c_int, C.POINTER(PBatchConfig), C.POINTER(C.c_double),
                                      C.c_char_p, C.c_size_t]),
         "vg_plain_time": (C.c_int, [C.POINTER(P), C.c_uint, C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -457,8 +464,18 @@ class Virginian:
             raise VirginianError("vg_link_probe failed")
         return h.value, d.value
 
-    def prepare(self, sql_or_ops, capacity: int = 0, rows_per_lane: int = 0) -> "DeviceQuery":
-        return DeviceQuery(self, sql_or_ops, capacity, rows_per_lane)
+    def prepare(self, sql_or_ops, capacity: int = 0, flags: int = 0) -> "DeviceQuery":
+        """flags: VG_QUERY_LIGHT / VG_QUERY_NO_OUTPUT (include/virg_gpu.h)."""
+        return DeviceQuery(self, sql_or_ops, capacity, flags)
+
+    def device_alloc(self, nbytes: int) -> int:
+        ptr = lib.vg_device_alloc(self.ptr, nbytes)
+        if not ptr:
+            raise VirginianError(f"cannot allocate {nbytes} bytes of device memory")
+        return ptr
+
+    def device_free(self, ptr: int) -> None:
+        lib.vg_device_free(self.ptr, ptr)
 
 
 def program(vm) -> list[tuple]:
@@ -517,7 +534,7 @@ class DeviceQuery:
     """A query prepared for repeated execution with its table resident in
     HBM (vg_query_* in include/virg_gpu.h)."""
 
-    def __init__(self, db: Virginian, sql_or_ops, capacity: int = 0, rows_per_lane: int = 0):
+    def __init__(self, db: Virginian, sql_or_ops, capacity: int = 0, flags: int = 0):
         self.db = db
         if isinstance(sql_or_ops, str):
             vm = db.compile(sql_or_ops)
@@ -534,7 +551,7 @@ class DeviceQuery:
             self.types = [o[1] for o in sql_or_ops if o[0] == OP_ResultColumn]
         self._ops = arr
         err = C.create_string_buffer(256)
-        self.h = lib.vg_query_create(db.ptr, arr, n, capacity, rows_per_lane, err, 256)
+        self.h = lib.vg_query_create(db.ptr, arr, n, capacity, flags, err, 256)
         if not self.h:
             raise VirginianError(err.value.decode() or "vg_query_create failed")
         rows, tiles, rb, wb, nout = (C.c_ulonglong(), C.c_uint(), C.c_uint(), C.c_uint(), C.c_uint())
@@ -549,6 +566,16 @@ class DeviceQuery:
             raise VirginianError("vg_query_run failed")
         self.last_rows = n.value
         return n.value
+
+    def out_widths(self) -> list[int]:
+        w = (C.c_uint * VIRG_MAX_COLUMNS)()
+        n = lib.vg_query_out_widths(self.h, w)
+        return list(w)[:n]
+
+    def set_outputs(self, ptrs: list[int], capacity: int) -> None:
+        arr = (C.c_void_p * max(len(ptrs), 1))(*ptrs)
+        lib.vg_query_set_outputs(self.h, arr, capacity)
+        self.capacity = capacity
 
     def sync(self) -> int:
         """Wait for the last launch of this query; returns its rows."""
@@ -642,17 +669,17 @@ class PersistentBatch:
     lists, captured as one HIP graph (vg_pbatch_* in include/virg_gpu.h)."""
 
     def __init__(self, queries: list[DeviceQuery], tiles_per_item: int = 0, order: int = 0,
-                 grid_div: int = 0, split: int = 0, single: int = 0):
+                 grid_div: int = 0, split: int = 0, single: int = 0, direct: int = 0, db=None):
         self.queries = list(queries)
-        cfg = PBatchConfig(tiles_per_item, grid_div, split, order, single)
+        cfg = PBatchConfig(tiles_per_item, grid_div, split, order, direct, db.ptr if db else None, single)
         err = C.create_string_buffer(256)
         self.h = lib.vg_pbatch_create(_handles(self.queries), len(self.queries), C.byref(cfg), err, 256)
         if not self.h:
             raise VirginianError(err.value.decode() or "vg_pbatch_create failed")
 
     def configure(self, tiles_per_item: int = 0, order: int = 0, grid_div: int = 0, split: int = 0,
-                  single: int = 0) -> None:
-        cfg = PBatchConfig(tiles_per_item, grid_div, split, order, single)
+                  single: int = 0, direct: int = 0) -> None:
+        cfg = PBatchConfig(tiles_per_item, grid_div, split, order, direct, None, single)
         err = C.create_string_buffer(256)
         if lib.vg_pbatch_configure(self.h, C.byref(cfg), err, 256) != VIRG_SUCCESS:
             raise VirginianError(err.value.decode() or "vg_pbatch_configure failed")
@@ -676,6 +703,32 @@ class PersistentBatch:
         for q, r in zip(self.queries, rows):
             q.last_rows = r
         return list(rows)[:n], list(lat)[:n]
+
+    def reload(self, queries: list[DeviceQuery]) -> None:
+        """Run another set of queries from now on (after the last run)."""
+        err = C.create_string_buffer(256)
+        if lib.vg_pbatch_reload(self.h, _handles(queries), len(queries), err, 256) != VIRG_SUCCESS:
+            raise VirginianError(err.value.decode() or "vg_pbatch_reload failed")
+        self.queries = list(queries)
+
+    def launch(self) -> None:
+        if lib.vg_pbatch_launch(self.h) != VIRG_SUCCESS:
+            raise VirginianError("batch launch failed")
+
+    def done(self) -> bool:
+        return lib.vg_pbatch_done(self.h) == 1
+
+    def collect(self) -> tuple[list[int], list[float], list[int]]:
+        """Wait for the launched run: (rows, latency ms, overflow flags)."""
+        n = len(self.queries)
+        rows = (C.c_ulonglong * max(n, 1))()
+        lat = (C.c_double * max(n, 1))()
+        flags = (C.c_uint * max(n, 1))()
+        if lib.vg_pbatch_results(self.h, rows, lat, flags) != VIRG_SUCCESS:
+            raise VirginianError("batch run failed")
+        for q, r, f in zip(self.queries, rows, flags):
+            q.last_rows = 0 if f else r
+        return list(rows)[:n], list(lat)[:n], list(flags)[:n]
 
     def results(self) -> tuple[list[int], list[float], list[int]]:
         """Launch, wait, and return (rows, latency ms, overflow flags) per

@@ -62,6 +62,22 @@ struct launch_res {
 	size_t wgcount_cap = 0;
 };
 
+/* device description of a list of resident tablets */
+struct plan {
+	std::vector<vg_tabdesc> desc;
+	std::vector<uint32_t> tile_tab;
+	vg_tabdesc *d_desc = nullptr;
+	uint32_t *d_tile = nullptr;
+	uint32_t num_tiles = 0;
+	uint64_t rows = 0;
+};
+
+/* a plan shared by the queries over one table (keyed by its tablets) */
+struct shared_plan {
+	plan p;
+	std::vector<std::pair<unsigned, unsigned>> key;    /* (tablet id, rows) */
+};
+
 struct vgd_device {
 	int device = 0;
 	hipStream_t stream = nullptr;
@@ -70,7 +86,15 @@ struct vgd_device {
 	void *outbuf = nullptr;
 	size_t outbuf_bytes = 0;
 	hipDeviceProp_t prop;
+	std::unordered_map<unsigned, shared_plan *> plans;  /* by table id */
+	std::vector<shared_plan *> retired;                 /* freed at close */
+	std::vector<vg_ctrl *> ctrl_free;                   /* pooled control blocks */
+	std::vector<vg_ctrl *> ctrl_chunks;
 };
+
+static void free_plan(plan &p);
+static int build_plan(vgd_device *d, const std::vector<const virg_tablet_meta *> &tabs,
+	hipStream_t stream, plan &p);
 
 static int herr(hipError_t e, const char *what)
 {
@@ -132,6 +156,14 @@ extern "C" void vgd_close(virginian *v)
 	free_lr(d->lr);
 	if (d->outbuf)
 		(void)hipFree(d->outbuf);
+	for (auto &kv : d->plans)
+		d->retired.push_back(kv.second);
+	for (shared_plan *sp : d->retired) {
+		free_plan(sp->p);
+		delete sp;
+	}
+	for (vg_ctrl *c : d->ctrl_chunks)
+		(void)hipFree(c);
 	(void)hipStreamDestroy(d->stream);
 	delete d;
 	s->dev = nullptr;
@@ -145,6 +177,10 @@ extern "C" void vgd_forget_tablet(virginian *v, unsigned id)
 	auto it = d->tablets.find(id);
 	if (it == d->tablets.end())
 		return;
+	/* shared plans may hold the tablet's address: build them again */
+	for (auto &kv : d->plans)
+		d->retired.push_back(kv.second);
+	d->plans.clear();
 	(void)hipDeviceSynchronize();
 	(void)hipFree(it->second.dptr);
 	d->tablets.erase(it);
@@ -233,16 +269,6 @@ static int ensure_lr(launch_res &lr, uint32_t num_tiles, uint32_t grid, hipStrea
 	}
 	return VIRG_SUCCESS;
 }
-
-/* device description of a list of resident tablets */
-struct plan {
-	std::vector<vg_tabdesc> desc;
-	std::vector<uint32_t> tile_tab;
-	vg_tabdesc *d_desc = nullptr;
-	uint32_t *d_tile = nullptr;
-	uint32_t num_tiles = 0;
-	uint64_t rows = 0;
-};
 
 static void free_plan(plan &p)
 {
@@ -948,14 +974,68 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 	return VIRG_SUCCESS;
 }
 
+/* The shared plan of a table's current tablets, built on first use. */
+static plan *shared_plan_of(vgd_device *d, unsigned table, const std::vector<const virg_tablet_meta *> &tabs)
+{
+	std::vector<std::pair<unsigned, unsigned>> key;
+	for (const virg_tablet_meta *t : tabs)
+		key.push_back({t->id, t->rows});
+	auto it = d->plans.find(table);
+	if (it != d->plans.end()) {
+		if (it->second->key == key)
+			return &it->second->p;
+		d->retired.push_back(it->second);
+		d->plans.erase(it);
+	}
+	shared_plan *sp = new shared_plan();
+	sp->key = key;
+	if (build_plan(d, tabs, d->stream, sp->p) == VIRG_FAIL || hipStreamSynchronize(d->stream) != hipSuccess) {
+		free_plan(sp->p);
+		delete sp;
+		return nullptr;
+	}
+	d->plans[table] = sp;
+	return &sp->p;
+}
+
+/* a zeroed control block from the device's pool */
+static vg_ctrl *pool_ctrl(vgd_device *d)
+{
+	if (d->ctrl_free.empty()) {
+		const unsigned chunk = 256;
+		vg_ctrl *c = nullptr;
+		if (hipMalloc((void **)&c, chunk * sizeof(vg_ctrl)) != hipSuccess) {
+			(void)hipGetLastError();
+			return nullptr;
+		}
+		d->ctrl_chunks.push_back(c);
+		for (unsigned i = 0; i < chunk; i++)
+			d->ctrl_free.push_back(c + chunk - 1 - i);
+	}
+	vg_ctrl *c = d->ctrl_free.back();
+	if (hipMemset(c, 0, sizeof(vg_ctrl)) != hipSuccess) {
+		(void)hipGetLastError();
+		return nullptr;
+	}
+	d->ctrl_free.pop_back();
+	return c;
+}
+
 /* -------------------------------------------------------- vg_query ABI */
+struct vg_query;
+static int ensure_events(vg_query *q);
+
 
 struct vg_query {
 	virginian *v = nullptr;
 	vgd_device *d = nullptr;
 	vg_prog P;
 	std::vector<const virg_tablet_meta *> tabs;
-	plan p;
+	plan own;
+	plan *p = &own;          /* own, or the device's shared plan of the table */
+	bool own_stream = true;  /* else the device's stream */
+	bool own_out = true;     /* else the outputs are the caller's memory */
+	bool pooled_ctrl = false;
 	launch_res lr;
 	void *out[VG_MAX_OUT];
 	size_t width[VG_MAX_OUT];
@@ -965,6 +1045,16 @@ struct vg_query {
 	uint64_t last_rows = 0;
 	uint64_t read_bytes_per_row = 0;
 };
+
+/* timing events, made on first use (light queries have none) */
+static int ensure_events(vg_query *q)
+{
+	if (!q->e0)
+		HCHK(hipEventCreate(&q->e0), "event");
+	if (!q->e1)
+		HCHK(hipEventCreate(&q->e1), "event");
+	return VIRG_SUCCESS;
+}
 
 static int qerr(char *err, size_t n, const char *fmt, ...)
 {
@@ -983,14 +1073,18 @@ extern "C" void vg_query_free(vg_query *q)
 		return;
 	if (q->stream)
 		(void)hipStreamSynchronize(q->stream);
-	free_plan(q->p);
+	free_plan(q->own);
+	if (q->pooled_ctrl && q->lr.ctrl) {
+		q->d->ctrl_free.push_back(q->lr.ctrl);
+		q->lr.ctrl = nullptr;
+	}
 	free_lr(q->lr);
-	for (unsigned j = 0; j < VG_MAX_OUT; j++)
+	for (unsigned j = 0; j < VG_MAX_OUT && q->own_out; j++)
 		if (q->out[j])
 			(void)hipFree(q->out[j]);
 	if (q->e0) (void)hipEventDestroy(q->e0);
 	if (q->e1) (void)hipEventDestroy(q->e1);
-	if (q->stream) (void)hipStreamDestroy(q->stream);
+	if (q->stream && q->own_stream) (void)hipStreamDestroy(q->stream);
 	delete q;
 }
 
@@ -1000,9 +1094,8 @@ extern "C" void vg_query_free(vg_query *q)
  * table resident in HBM.  Output capacity is `capacity` rows (0 = every
  * row of the table).  `reserved` must be 0. */
 extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned num_ops,
-	unsigned long long capacity, int reserved, char *err, size_t errlen)
+	unsigned long long capacity, int flags, char *err, size_t errlen)
 {
-	(void)reserved;
 	vgd_device *d = dev_of(v);
 	if (!d) {
 		qerr(err, errlen, "no usable GPU: the data-parallel VM needs a HIP device");
@@ -1057,31 +1150,57 @@ extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned 
 		delete q;
 		return nullptr;
 	}
-	if (hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess ||
-		hipEventCreate(&q->e0) != hipSuccess || hipEventCreate(&q->e1) != hipSuccess) {
-		qerr(err, errlen, "cannot create stream/events");
-		vg_query_free(q);
-		return nullptr;
-	}
-	if (build_plan(d, q->tabs, q->stream, q->p) == VIRG_FAIL) {
-		qerr(err, errlen, "cannot make the table resident");
-		vg_query_free(q);
-		return nullptr;
-	}
-	q->capacity = capacity ? capacity : q->p.rows;
-	for (unsigned j = 0; j < q->P.nout; j++) {
-		q->width[j] = q->P.out[j].width;
-		if (hipMalloc(&q->out[j], q->capacity * q->width[j] + 256) != hipSuccess) {
-			qerr(err, errlen, "cannot allocate %llu-row output", (unsigned long long)q->capacity);
+	const bool light = (flags & VG_QUERY_LIGHT) != 0;
+	if (light) {
+		q->own_stream = false;
+		q->stream = d->stream;
+		q->p = shared_plan_of(d, (unsigned)table, q->tabs);
+		if (!q->p) {
+			q->p = &q->own;
+			qerr(err, errlen, "cannot make the table resident");
 			vg_query_free(q);
 			return nullptr;
 		}
+	} else {
+		if (hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess ||
+			hipEventCreate(&q->e0) != hipSuccess || hipEventCreate(&q->e1) != hipSuccess) {
+			qerr(err, errlen, "cannot create stream/events");
+			vg_query_free(q);
+			return nullptr;
+		}
+		if (build_plan(d, q->tabs, q->stream, q->own) == VIRG_FAIL) {
+			qerr(err, errlen, "cannot make the table resident");
+			vg_query_free(q);
+			return nullptr;
+		}
+	}
+	q->capacity = capacity ? capacity : q->p->rows;
+	for (unsigned j = 0; j < q->P.nout; j++)
+		q->width[j] = q->P.out[j].width;
+	if (flags & VG_QUERY_NO_OUTPUT) {
+		q->own_out = false;
+	} else {
+		for (unsigned j = 0; j < q->P.nout; j++)
+			if (hipMalloc(&q->out[j], q->capacity * q->width[j] + 256) != hipSuccess) {
+				qerr(err, errlen, "cannot allocate %llu-row output", (unsigned long long)q->capacity);
+				vg_query_free(q);
+				return nullptr;
+			}
 	}
 	uint64_t rb = 0;
 	for (unsigned c = 0; c < q->P.ncol; c++)
 		rb += q->P.col[c].width;
 	q->read_bytes_per_row = rb;
-	if (ensure_lr(q->lr, q->p.num_tiles, vg_kernel_grid(&q->P, q->p.num_tiles), q->stream) == VIRG_FAIL ||
+	if (light) {
+		/* the control block only; the two-pass arrays on first use */
+		q->lr.ctrl = pool_ctrl(d);
+		q->pooled_ctrl = q->lr.ctrl != nullptr;
+		if (!q->lr.ctrl) {
+			qerr(err, errlen, "cannot set up the launch");
+			vg_query_free(q);
+			return nullptr;
+		}
+	} else if (ensure_lr(q->lr, q->p->num_tiles, vg_kernel_grid(&q->P, q->p->num_tiles), q->stream) == VIRG_FAIL ||
 		hipStreamSynchronize(q->stream) != hipSuccess) {
 		qerr(err, errlen, "cannot set up the launch");
 		vg_query_free(q);
@@ -1093,14 +1212,14 @@ extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned 
 /* Launch once, asynchronously, on the query's stream. */
 extern "C" int vg_query_launch(vg_query *q)
 {
-	return launch(q->P, q->p, q->lr, q->out, q->capacity, q->stream);
+	return launch(q->P, *q->p, q->lr, q->out, q->capacity, q->stream);
 }
 
 /* Wait for the last launch; returns the selected row count. */
 extern "C" int vg_query_sync(vg_query *q, unsigned long long *rows)
 {
 	uint64_t n = 0;
-	if (q->p.num_tiles && read_ctrl(q->lr, q->stream, &n) == VIRG_FAIL)
+	if (q->p->num_tiles && read_ctrl(q->lr, q->stream, &n) == VIRG_FAIL)
 		return VIRG_FAIL;
 	q->last_rows = n;
 	rows[0] = n;
@@ -1136,6 +1255,8 @@ extern "C" int vg_query_time(vg_query *q, int warmup, int iters, double *ms_per_
 			return VIRG_FAIL;
 	if (vg_query_sync(q, rows) == VIRG_FAIL)
 		return VIRG_FAIL;
+	if (ensure_events(q) == VIRG_FAIL)
+		return VIRG_FAIL;
 	HCHK(hipEventRecord(q->e0, q->stream), "event");
 	for (int i = 0; i < iters; i++)
 		if (vg_query_launch(q) == VIRG_FAIL)
@@ -1155,13 +1276,15 @@ extern "C" int vg_query_time(vg_query *q, int warmup, int iters, double *ms_per_
 extern "C" int vg_query_time_passes(vg_query *q, int warmup, int iters, double *ms_count,
 	double *ms_write, unsigned long long *rows)
 {
+	if (ensure_events(q) == VIRG_FAIL)
+		return VIRG_FAIL;
 	hipEvent_t mid = nullptr;
 	HCHK(hipEventCreate(&mid), "event");
 	double c = 0, w = 0;
 	int rc = VIRG_SUCCESS;
 	for (int i = 0; i < warmup + iters && rc == VIRG_SUCCESS; i++) {
 		if (hipEventRecord(q->e0, q->stream) != hipSuccess ||
-			launch(q->P, q->p, q->lr, q->out, q->capacity, q->stream, mid) == VIRG_FAIL ||
+			launch(q->P, *q->p, q->lr, q->out, q->capacity, q->stream, mid) == VIRG_FAIL ||
 			hipEventRecord(q->e1, q->stream) != hipSuccess ||
 			vg_query_sync(q, rows) == VIRG_FAIL) {
 			rc = VIRG_FAIL;
@@ -1189,8 +1312,8 @@ extern "C" int vg_query_time_passes(vg_query *q, int warmup, int iters, double *
 extern "C" int vg_query_info(vg_query *q, unsigned long long *rows, unsigned *tiles,
 	unsigned *read_bytes_per_row, unsigned *write_bytes_per_row, unsigned *nout)
 {
-	rows[0] = q->p.rows;
-	tiles[0] = q->p.num_tiles;
+	rows[0] = q->p->rows;
+	tiles[0] = q->p->num_tiles;
 	read_bytes_per_row[0] = (unsigned)q->read_bytes_per_row;
 	unsigned w = 0;
 	for (unsigned j = 0; j < q->P.nout; j++)
@@ -1281,7 +1404,7 @@ extern "C" vg_batch *vg_batch_create(vg_query *const *qs, unsigned n, char *err,
 	BCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "event");
 	for (unsigned i = 0; i < n; i++) {
 		vg_query *q = qs[i];
-		if (q->p.num_tiles && ensure_lr(q->lr, q->p.num_tiles, vg_kernel_grid(&q->P, q->p.num_tiles),
+		if (q->p->num_tiles && ensure_lr(q->lr, q->p->num_tiles, vg_kernel_grid(&q->P, q->p->num_tiles),
 				q->stream) == VIRG_FAIL) {
 			qerr(err, errlen, "cannot set up query %u", i);
 			vg_batch_free(b);
@@ -1297,7 +1420,7 @@ extern "C" vg_batch *vg_batch_create(vg_query *const *qs, unsigned n, char *err,
 	for (unsigned i = 0; i < n && ok; i++) {
 		vg_query *q = qs[i];
 		ok = hipStreamWaitEvent(q->stream, b->fork, 0) == hipSuccess &&
-			launch(q->P, q->p, q->lr, q->out, q->capacity, q->stream) == VIRG_SUCCESS &&
+			launch(q->P, *q->p, q->lr, q->out, q->capacity, q->stream) == VIRG_SUCCESS &&
 			hipEventRecord(b->joins[i], q->stream) == hipSuccess &&
 			hipStreamWaitEvent(b->stream, b->joins[i], 0) == hipSuccess;
 	}
@@ -1409,6 +1532,8 @@ struct vg_pbatch {
 	uint64_t *d_status = nullptr;           /* single pass: a word per tile */
 	size_t status_cap = 0;
 	std::vector<pgroup> groups;
+	vg_batch_args R;                        /* the reset's arguments */
+	size_t qcap = 0;                        /* queries the per-query arrays hold */
 	vg_pbatch_config cfg;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -1451,9 +1576,9 @@ extern "C" void vg_pbatch_free(vg_pbatch *b)
 static void fill_launch(const vg_query *q, vg_launch &L)
 {
 	memset(&L, 0, sizeof L);
-	L.tabs = q->p.d_desc;
-	L.tile_tab = q->p.d_tile;
-	L.num_tiles = q->p.num_tiles;
+	L.tabs = q->p->d_desc;
+	L.tile_tab = q->p->d_tile;
+	L.num_tiles = q->p->num_tiles;
 	L.capacity = q->capacity;
 	L.ctrl = q->lr.ctrl;
 	L.vbits = q->lr.vbits;
@@ -1461,6 +1586,27 @@ static void fill_launch(const vg_query *q, vg_launch &L)
 	L.wgcount = q->lr.wgcount;
 	for (unsigned j = 0; j < q->P.nout; j++)
 		L.out[j] = q->out[j];
+}
+
+/* Issue a run on b->stream: the reset, then every group's launches, each
+ * group on its own branch when there are several. */
+static int pbatch_record(vg_pbatch *b)
+{
+	const size_t ng = b->groups.size();
+	const bool single = b->cfg.single != 0;
+	int rc = vg_kernel_batch_reset(&b->R, b->stream) == 0 && hipEventRecord(b->fork, b->stream) == hipSuccess;
+	for (size_t gi = 0; gi < ng && rc; gi++) {
+		pgroup &g = b->groups[gi];
+		if (g.A.nitems == 0)
+			continue;
+		hipStream_t s = ng == 1 ? b->stream : b->branch[gi];
+		rc = (ng == 1 || hipStreamWaitEvent(s, b->fork, 0) == hipSuccess) &&
+			(single ? vg_kernel_batch_single(&g.A, g.wgrid, g.wlds, s)
+			        : vg_kernel_batch_passes(&g.A, g.cgrid, g.clds, g.wgrid, g.wlds, s)) == 0 &&
+			(ng == 1 || (hipEventRecord(b->join[gi], s) == hipSuccess &&
+				hipStreamWaitEvent(b->stream, b->join[gi], 0) == hipSuccess));
+	}
+	return rc;
 }
 
 /* Group the queries (b->cfg.split 0: by the write launch's occupancy,
@@ -1487,7 +1633,7 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 	std::vector<vg_bqinfo> info(n);
 	uint32_t ioff = 0, soff = 0;
 	for (unsigned i = 0; i < n; i++) {
-		const uint32_t nt = b->qs[i]->p.num_tiles;
+		const uint32_t nt = b->qs[i]->p->num_tiles;
 		info[i].nitem = (nt + tpi - 1) / tpi;
 		info[i].ioff = ioff;
 		info[i].soff = soff;
@@ -1504,7 +1650,7 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 			return qerr(err, errlen, "cannot allocate the batch's tile status words");
 		b->status_cap = soff;
 	}
-	auto tiles = [&](unsigned q) { return b->qs[q]->p.num_tiles; };
+	auto tiles = [&](unsigned q) { return b->qs[q]->p->num_tiles; };
 	std::vector<vg_item> items, witems;
 	auto add = [&](std::vector<vg_item> &v, unsigned q) {
 		const uint32_t nt = tiles(q);
@@ -1606,24 +1752,17 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		g.wgrid = g.wgrid < most ? g.wgrid : most;
 	}
 	/* the reset clears every query and every group's head */
-	vg_batch_args R = b->groups[0].A;
-	R.head = b->d_head;
-	R.nhead = (uint32_t)ng;
-	R.status = single ? b->d_status : nullptr;
-	R.nstatus = single ? soff : 0;
+	b->R = b->groups[0].A;
+	b->R.head = b->d_head;
+	b->R.nhead = (uint32_t)ng;
+	b->R.status = single ? b->d_status : nullptr;
+	b->R.nstatus = single ? soff : 0;
 	pbatch_drop_graph(b);
+	if (b->cfg.direct)
+		return VIRG_SUCCESS;
 	if (herr(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture"))
 		return qerr(err, errlen, "graph capture failed");
-	int rc = vg_kernel_batch_reset(&R, b->stream) == 0 && hipEventRecord(b->fork, b->stream) == hipSuccess;
-	for (size_t gi = 0; gi < ng && rc; gi++) {
-		pgroup &g = b->groups[gi];
-		hipStream_t s = ng == 1 ? b->stream : b->branch[gi];
-		rc = (ng == 1 || hipStreamWaitEvent(s, b->fork, 0) == hipSuccess) &&
-			(single ? vg_kernel_batch_single(&g.A, g.wgrid, g.wlds, s)
-			        : vg_kernel_batch_passes(&g.A, g.cgrid, g.clds, g.wgrid, g.wlds, s)) == 0 &&
-			(ng == 1 || (hipEventRecord(b->join[gi], s) == hipSuccess &&
-				hipStreamWaitEvent(b->stream, b->join[gi], 0) == hipSuccess));
-	}
+	const int rc = pbatch_record(b);
 	const hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
 	if (!rc || ec != hipSuccess) {
 		(void)hipGetLastError();
@@ -1634,26 +1773,82 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 	return VIRG_SUCCESS;
 }
 
+/* Take a new set of queries: check them, make sure each has what the
+ * configured mode needs, and upload their programs and launch arguments. */
+static int pbatch_set_queries(vg_pbatch *b, vg_query *const *qs, unsigned n, char *err, size_t errlen)
+{
+	for (unsigned i = 0; i < n; i++) {
+		if (!qs[i] || qs[i]->d != b->d)
+			return qerr(err, errlen, "batch queries must be prepared on one device");
+		for (unsigned j = 0; j < i; j++)
+			if (qs[j] == qs[i])
+				return qerr(err, errlen, "query %u appears twice in the batch", i);
+	}
+	b->qs.assign(qs, qs + n);
+	std::vector<vg_prog> progs(n);
+	std::vector<vg_launch> launches(n);
+	for (unsigned i = 0; i < n; i++) {
+		vg_query *q = qs[i];
+		const uint32_t nt = q->p->num_tiles;
+		if (b->cfg.single) {
+			if (!q->lr.ctrl && ensure_lr(q->lr, nt, vg_kernel_grid(&q->P, nt ? nt : 1), q->stream) == VIRG_FAIL)
+				return qerr(err, errlen, "cannot set up query %u", i);
+		} else if (ensure_lr(q->lr, nt, vg_kernel_grid(&q->P, nt ? nt : 1), q->stream) == VIRG_FAIL) {
+			return qerr(err, errlen, "cannot set up query %u", i);
+		}
+		progs[i] = q->P;
+		fill_launch(q, launches[i]);
+	}
+	for (unsigned i = 0; i < n; i++)
+		if (herr(hipStreamSynchronize(qs[i]->stream), "query setup"))
+			return qerr(err, errlen, "query setup failed");
+	if (n > b->qcap) {
+		void *bufs[] = { b->d_progs, b->d_launch, b->d_info, b->d_q };
+		for (void *p : bufs)
+			if (p)
+				(void)hipFree(p);
+		b->d_progs = nullptr;
+		b->d_launch = nullptr;
+		b->d_info = nullptr;
+		b->d_q = nullptr;
+		b->qcap = 0;
+		const size_t cap = n < 64 ? 64 : n;
+		if (herr(hipMalloc((void **)&b->d_progs, cap * sizeof(vg_prog)), "batch") ||
+			herr(hipMalloc((void **)&b->d_launch, cap * sizeof(vg_launch)), "batch") ||
+			herr(hipMalloc((void **)&b->d_info, cap * sizeof(vg_bqinfo)), "batch") ||
+			herr(hipMalloc((void **)&b->d_q, cap * sizeof(vg_bquery)), "batch"))
+			return qerr(err, errlen, "cannot allocate the batch");
+		b->qcap = cap;
+	}
+	if (n && (herr(hipMemcpy(b->d_progs, progs.data(), n * sizeof(vg_prog), hipMemcpyHostToDevice), "batch") ||
+		herr(hipMemcpy(b->d_launch, launches.data(), n * sizeof(vg_launch), hipMemcpyHostToDevice), "batch")))
+		return qerr(err, errlen, "cannot upload the batch");
+	b->h_q.resize(n);
+	return VIRG_SUCCESS;
+}
+
 extern "C" vg_pbatch *vg_pbatch_create(vg_query *const *qs, unsigned n, const vg_pbatch_config *cfg,
 	char *err, size_t errlen)
 {
-	if (n == 0) {
+	vg_pbatch_config c0;
+	memset(&c0, 0, sizeof c0);
+	if (cfg)
+		c0 = *cfg;
+	if (n == 0 && !c0.direct) {
 		qerr(err, errlen, "empty batch");
 		return nullptr;
 	}
 	vg_pbatch *b = new vg_pbatch();
-	b->qs.assign(qs, qs + n);
-	memset(&b->cfg, 0, sizeof b->cfg);
-	if (cfg)
-		b->cfg = *cfg;
+	b->cfg = c0;
 #define PCHK(cond, ...) do { if (cond) { (void)hipGetLastError(); qerr(err, errlen, __VA_ARGS__); \
 	vg_pbatch_free(b); return nullptr; } } while (0)
-	for (unsigned i = 0; i < n; i++) {
-		PCHK(!qs[i] || qs[i]->d != qs[0]->d, "batch queries must be prepared on one device");
-		for (unsigned j = 0; j < i; j++)
-			PCHK(qs[j] == qs[i], "query %u appears twice in the batch", i);
+	if (n) {
+		b->d = qs[0] ? qs[0]->d : nullptr;
+		PCHK(!b->d, "batch queries must be prepared on one device");
+	} else {
+		b->d = dev_of(c0.db);
+		PCHK(!b->d, "no usable GPU");
 	}
-	b->d = qs[0]->d;
 	PCHK(hipSetDevice(b->d->device) != hipSuccess, "set device");
 	PCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess, "stream");
 	PCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess, "event");
@@ -1661,30 +1856,38 @@ extern "C" vg_pbatch *vg_pbatch_create(vg_query *const *qs, unsigned n, const vg
 	PCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->d->device) != hipSuccess || khz <= 0,
 		"device clock rate");
 	b->clock_khz = khz;
-	std::vector<vg_prog> progs(n);
-	std::vector<vg_launch> launches(n);
-	for (unsigned i = 0; i < n; i++) {
-		vg_query *q = qs[i];
-		PCHK(ensure_lr(q->lr, q->p.num_tiles, vg_kernel_grid(&q->P, q->p.num_tiles ? q->p.num_tiles : 1),
-			q->stream) == VIRG_FAIL, "cannot set up query %u", i);
-		PCHK(hipStreamSynchronize(q->stream) != hipSuccess, "query setup");
-		progs[i] = q->P;
-		fill_launch(q, launches[i]);
-	}
-	PCHK(hipMalloc((void **)&b->d_progs, n * sizeof(vg_prog)) != hipSuccess ||
-		hipMalloc((void **)&b->d_launch, n * sizeof(vg_launch)) != hipSuccess ||
-		hipMalloc((void **)&b->d_info, n * sizeof(vg_bqinfo)) != hipSuccess ||
-		hipMalloc((void **)&b->d_q, n * sizeof(vg_bquery)) != hipSuccess, "cannot allocate the batch");
-	PCHK(hipMemcpy(b->d_progs, progs.data(), n * sizeof(vg_prog), hipMemcpyHostToDevice) != hipSuccess ||
-		hipMemcpy(b->d_launch, launches.data(), n * sizeof(vg_launch), hipMemcpyHostToDevice) != hipSuccess,
-		"cannot upload the batch");
 #undef PCHK
-	b->h_q.resize(n);
-	if (pbatch_build(b, err, errlen) == VIRG_FAIL) {
+	if (pbatch_set_queries(b, qs, n, err, errlen) == VIRG_FAIL ||
+		(n && pbatch_build(b, err, errlen) == VIRG_FAIL)) {
 		vg_pbatch_free(b);
 		return nullptr;
 	}
 	return b;
+}
+
+/* Replace the batch's queries (waits for the batch's last run first). */
+extern "C" int vg_pbatch_reload(vg_pbatch *b, vg_query *const *qs, unsigned n, char *err, size_t errlen)
+{
+	HCHK(hipStreamSynchronize(b->stream), "batch");
+	if (pbatch_set_queries(b, qs, n, err, errlen) == VIRG_FAIL)
+		return VIRG_FAIL;
+	if (n == 0) {
+		b->groups.clear();
+		pbatch_drop_graph(b);
+		return VIRG_SUCCESS;
+	}
+	return pbatch_build(b, err, errlen);
+}
+
+/* 1 when the batch's last run has finished (does not wait). */
+extern "C" int vg_pbatch_done(vg_pbatch *b)
+{
+	const hipError_t e = hipStreamQuery(b->stream);
+	if (e == hipSuccess)
+		return 1;
+	if (e != hipErrorNotReady)
+		(void)hipGetLastError();
+	return 0;
 }
 
 extern "C" int vg_pbatch_configure(vg_pbatch *b, const vg_pbatch_config *cfg, char *err, size_t errlen)
@@ -1704,6 +1907,15 @@ extern "C" int vg_pbatch_get_config(vg_pbatch *b, vg_pbatch_config *cfg, unsigne
 
 extern "C" int vg_pbatch_launch(vg_pbatch *b)
 {
+	if (b->qs.empty())
+		return VIRG_SUCCESS;
+	if (b->cfg.direct) {
+		if (!pbatch_record(b)) {
+			herr(hipGetLastError(), "batch launch");
+			return VIRG_FAIL;
+		}
+		return VIRG_SUCCESS;
+	}
 	HCHK(hipGraphLaunch(b->exec, b->stream), "graph launch");
 	return VIRG_SUCCESS;
 }
@@ -1726,7 +1938,7 @@ extern "C" int vg_pbatch_sync(vg_pbatch *b, unsigned long long *rows, double *la
 	for (size_t i = 0; i < n; i++) {
 		vg_query *q = b->qs[i];
 		uint64_t r = 0;
-		if (q->p.num_tiles && read_ctrl(q->lr, q->stream, &r) == VIRG_FAIL)
+		if (q->p->num_tiles && read_ctrl(q->lr, q->stream, &r) == VIRG_FAIL)
 			return VIRG_FAIL;
 		q->last_rows = r;
 		if (rows)
@@ -1754,7 +1966,7 @@ extern "C" int vg_pbatch_results(vg_pbatch *b, unsigned long long *rows, double 
 		vg_query *q = b->qs[i];
 		vg_ctrl c;
 		memset(&c, 0, sizeof c);
-		if (q->p.num_tiles)
+		if (q->p->num_tiles)
 			HCHK(hipMemcpy(&c, q->lr.ctrl, sizeof c, hipMemcpyDeviceToHost), "control block");
 		q->last_rows = c.overflow ? 0 : c.total;
 		if (rows)
@@ -1795,6 +2007,8 @@ static int graph_time(hipGraphExec_t exec, hipStream_t s, int warmup, int iters,
 
 extern "C" int vg_pbatch_time(vg_pbatch *b, int warmup, int iters, double *ms)
 {
+	if (b->cfg.direct || b->qs.empty())
+		return VIRG_FAIL;
 	return graph_time(b->exec, b->stream, warmup, iters, ms);
 }
 
@@ -1856,12 +2070,12 @@ extern "C" int vg_plain_time(vg_query *const *qs, unsigned n, int warmup, int it
 	for (auto &e : ev)
 		ok = ok && hipEventCreate(&e) == hipSuccess;
 	for (unsigned i = 0; i < n && ok; i++)
-		ok = ensure_lr(qs[i]->lr, qs[i]->p.num_tiles, vg_kernel_grid(&qs[i]->P, qs[i]->p.num_tiles ?
-			qs[i]->p.num_tiles : 1), qs[i]->stream) == VIRG_SUCCESS &&
+		ok = ensure_lr(qs[i]->lr, qs[i]->p->num_tiles, vg_kernel_grid(&qs[i]->P, qs[i]->p->num_tiles ?
+			qs[i]->p->num_tiles : 1), qs[i]->stream) == VIRG_SUCCESS &&
 			hipStreamSynchronize(qs[i]->stream) == hipSuccess;
 	auto round = [&](bool mark) {
 		for (unsigned i = 0; i < n && ok; i++) {
-			ok = launch(qs[i]->P, qs[i]->p, qs[i]->lr, qs[i]->out, qs[i]->capacity, s) == VIRG_SUCCESS;
+			ok = launch(qs[i]->P, *qs[i]->p, qs[i]->lr, qs[i]->out, qs[i]->capacity, s) == VIRG_SUCCESS;
 			if (mark)
 				ok = ok && hipEventRecord(ev[i + 2], s) == hipSuccess;
 		}
@@ -1893,4 +2107,76 @@ extern "C" int vg_plain_time(vg_query *const *qs, unsigned n, int warmup, int it
 		return VIRG_FAIL;
 	}
 	return VIRG_SUCCESS;
+}
+
+/* Results of `q` go to caller device memory from now on: out[j] holds
+ * room for `capacity` rows of result column j (its width in bytes). */
+extern "C" int vg_query_set_outputs(vg_query *q, void *const *out, unsigned long long capacity)
+{
+	if (q->own_out)
+		for (unsigned j = 0; j < VG_MAX_OUT; j++)
+			if (q->out[j]) {
+				(void)hipStreamSynchronize(q->stream);
+				(void)hipFree(q->out[j]);
+			}
+	q->own_out = false;
+	for (unsigned j = 0; j < q->P.nout; j++)
+		q->out[j] = out[j];
+	q->capacity = capacity;
+	return VIRG_SUCCESS;
+}
+
+/* Bytes per row of each result column, in order; returns their count. */
+extern "C" unsigned vg_query_out_widths(vg_query *q, unsigned *widths)
+{
+	for (unsigned j = 0; j < q->P.nout; j++)
+		widths[j] = (unsigned)q->P.out[j].width;
+	return q->P.nout;
+}
+
+/* Device memory for caller-managed outputs (e.g. a serving arena). */
+extern "C" void *vg_device_alloc(virginian *v, size_t bytes)
+{
+	vgd_device *d = dev_of(v);
+	void *p = nullptr;
+	if (!d || hipSetDevice(d->device) != hipSuccess || hipMalloc(&p, bytes) != hipSuccess) {
+		(void)hipGetLastError();
+		return nullptr;
+	}
+	return p;
+}
+
+extern "C" void vg_device_free(virginian *v, void *p)
+{
+	(void)v;
+	if (p)
+		(void)hipFree(p);
+}
+
+/* Rows the last (two-pass) launch of q selected per tablet, from the count
+ * pass's per-tile counts. */
+extern "C" unsigned vg_query_tablet_rows(vg_query *q, unsigned long long *rows, unsigned n)
+{
+	const plan &p = *q->p;
+	const size_t nt = p.num_tiles;
+	if (!q->lr.tcount || nt == 0) {
+		for (unsigned i = 0; i < n && i < p.desc.size(); i++)
+			rows[i] = 0;
+		return (unsigned)p.desc.size();
+	}
+	std::vector<uint32_t> tc(nt * 4);
+	if (hipStreamSynchronize(q->stream) != hipSuccess ||
+		hipMemcpy(tc.data(), q->lr.tcount, nt * 4 * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) {
+		(void)hipGetLastError();
+		return 0;
+	}
+	const uint32_t T = vg_kernel_tile_rows();
+	for (size_t i = 0; i < p.desc.size() && i < n; i++) {
+		const uint32_t f = p.desc[i].first_tile, k = (p.desc[i].rows + T - 1) / T;
+		unsigned long long s = 0;
+		for (uint32_t t = f; t < f + k; t++)
+			s += (unsigned long long)tc[t * 4] + tc[t * 4 + 1] + tc[t * 4 + 2] + tc[t * 4 + 3];
+		rows[i] = s;
+	}
+	return (unsigned)p.desc.size();
 }

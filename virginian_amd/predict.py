@@ -1,0 +1,303 @@
+"""Learned result sizing for single-pass execution.
+
+A query run in one pass (vg_pbatch single mode) writes its rows straight
+into output buffers that must be sized before it runs.  The predictor
+estimates, per tablet, the fraction of rows a program selects; the buffer
+is the sum over tablets of rows * min(1, prediction + margin).  A run whose
+result does not fit reports it and is run again by the exact two-pass
+executor, so results never depend on the prediction: a better predictor
+only means smaller buffers (more queries admitted into HBM) and fewer
+reruns.
+
+Features of a (program, tablet) pair:
+  - the filter's compares in program order (up to MAX_TERMS): operator,
+    whether the branch keeps validity, whether each side is a bare column
+    (which one) or a constant (its value scaled to the column's range), and
+    the fraction of the tablet's rows passing `column op constant` read off
+    the tablet's histogram;
+  - program shape: compares, Invalid and logic ops, program length;
+  - the tablet: rows, and per column its min, max and a 16-bin histogram
+    over the table's range.
+The network (include/virg_gpu.h, vg_mlp_*) is trained on the GPU by the
+project's own kernels on per-tablet counts the executor reports
+(vg_query_tablet_rows).  No counterpart in the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import (NP_TYPE, OP_And, OP_Column, OP_Eq, OP_Float, OP_Ge, OP_Gt, OP_Integer, OP_Invalid, OP_Le,
+               OP_Lt, OP_Neq, OP_Not, OP_Or, OP_Parallel, OP_Result, OP_Rowid, OP_Table, VIRG_SUCCESS,
+               VirginianError, lib)
+
+MAX_TERMS = 10
+NCOLS = 6
+NBINS = 16
+CMP_OPS = [OP_Lt, OP_Le, OP_Gt, OP_Ge, OP_Eq, OP_Neq]
+MIRROR = {OP_Lt: OP_Gt, OP_Le: OP_Ge, OP_Gt: OP_Lt, OP_Ge: OP_Le, OP_Eq: OP_Eq, OP_Neq: OP_Neq}
+TERM_F = 1 + 6 + 1 + 1 + NCOLS + 1 + 1 + 1
+NFEATURES = MAX_TERMS * TERM_F + 4 + 1 + NCOLS * (2 + NBINS)
+
+
+# ------------------------------------------------------------ tablet stats
+def _column(meta, c: int) -> np.ndarray:
+    dt = np.dtype(NP_TYPE[meta.fixed_type[c]])
+    base = C.addressof(meta) + meta.fixed_block + meta.fixed_offset[c]
+    buf = (C.c_char * (meta.rows * dt.itemsize)).from_address(base)
+    return np.frombuffer(buf, dtype=dt)
+
+
+class TableStats:
+    """Per-tablet min, max and histogram of each fixed column, over the
+    table's own range (computed once when the table is loaded)."""
+
+    def __init__(self, db, table_id: int):
+        tabs = db.tablets(table_id)
+        self.rows = np.array([t.contents.rows for t in tabs], dtype=np.float64)
+        ncol = min(NCOLS, tabs[0].contents.fixed_columns)
+        self.is_int = [np.issubdtype(NP_TYPE[tabs[0].contents.fixed_type[c]], np.integer) for c in range(ncol)]
+        mins = np.full((len(tabs), NCOLS), 0.0)
+        maxs = np.full((len(tabs), NCOLS), 0.0)
+        for i, t in enumerate(tabs):
+            for c in range(ncol):
+                col = _column(t.contents, c)
+                if len(col):
+                    mins[i, c], maxs[i, c] = float(col.min()), float(col.max())
+        self.lo = mins.min(axis=0)
+        self.hi = np.maximum(maxs.max(axis=0), self.lo + 1e-6)
+        self.edges = [np.linspace(self.lo[c], self.hi[c], NBINS + 1) for c in range(NCOLS)]
+        self.hist = np.zeros((len(tabs), NCOLS, NBINS))
+        for i, t in enumerate(tabs):
+            for c in range(ncol):
+                col = _column(t.contents, c)
+                if len(col):
+                    h, _ = np.histogram(col, bins=self.edges[c])
+                    self.hist[i, c] = h / len(col)
+        span = self.hi - self.lo
+        self.tablet_features = np.concatenate([
+            ((mins - self.lo) / span)[:, :, None], ((maxs - self.lo) / span)[:, :, None], self.hist],
+            axis=2).reshape(len(tabs), -1)
+
+    def cdf(self, c: int, v: float) -> np.ndarray:
+        """Fraction of each tablet's rows with column c < v (histogram,
+        linear inside a bin)."""
+        e = self.edges[c]
+        if v <= e[0]:
+            return np.zeros(len(self.rows))
+        if v >= e[-1]:
+            return np.ones(len(self.rows))
+        k = min(int(np.searchsorted(e, v, side="right")) - 1, NBINS - 1)
+        part = (v - e[k]) / (e[k + 1] - e[k])
+        return self.hist[:, c, :k].sum(axis=1) + part * self.hist[:, c, k]
+
+    def passing(self, c: int, op: int, v: float) -> np.ndarray:
+        """Estimated fraction of each tablet's rows with (column c op v)."""
+        if self.is_int[c]:
+            lt, le = self.cdf(c, math.ceil(v)), self.cdf(c, math.floor(v) + 1)
+            eq = le - self.cdf(c, math.floor(v)) if float(v).is_integer() else np.zeros(len(self.rows))
+        else:
+            lt = le = self.cdf(c, v)
+            eq = np.zeros(len(self.rows))
+        return {OP_Lt: lt, OP_Le: le, OP_Gt: 1 - le, OP_Ge: 1 - lt, OP_Eq: eq, OP_Neq: 1 - eq}[op]
+
+
+# ------------------------------------------------------- program features
+def _value(op) -> float:
+    return float(op[4]) if op[0] == OP_Float else float(op[2])
+
+
+def program_features(ops: list[tuple], stats: TableStats) -> np.ndarray:
+    """Feature rows (one per tablet) of a program over the table `stats`
+    describes."""
+    ntab = len(stats.rows)
+    terms = np.zeros((ntab, MAX_TERMS, TERM_F))
+    regdef: dict[int, tuple] = {}
+    start = next((i + 1 for i, o in enumerate(ops) if o[0] == OP_Parallel), 0)
+    nterm = ninvalid = nlogic = 0
+    for o in ops[start:]:
+        k = o[0]
+        if k == OP_Result:
+            break
+        if k == OP_Column:
+            regdef[o[1]] = ("col", o[2])
+        elif k in (OP_Integer, OP_Float):
+            regdef[o[1]] = ("const", _value(o))
+        elif k == OP_Rowid:
+            regdef[o[1]] = ("rowid",)
+        elif k == OP_Invalid:
+            ninvalid += 1
+        elif k in (OP_And, OP_Or, OP_Not):
+            nlogic += 1
+        elif k in CMP_OPS:
+            if nterm < MAX_TERMS:
+                a, b = regdef.get(o[1], ("expr",)), regdef.get(o[2], ("expr",))
+                op = k
+                if a[0] == "const" and b[0] == "col":
+                    a, b, op = b, a, MIRROR[k]
+                f = terms[:, nterm]
+                f[:, 0] = 1.0
+                f[:, 1 + CMP_OPS.index(op)] = 1.0
+                f[:, 7] = 1.0 if (o[4] or 0) else 0.0
+                est = np.full(ntab, 0.5)
+                if a[0] == "col" and a[1] < NCOLS:
+                    f[:, 8] = 1.0
+                    f[:, 9 + a[1]] = 1.0
+                    if b[0] == "const":
+                        c = a[1]
+                        f[:, 9 + NCOLS] = 1.0
+                        f[:, 10 + NCOLS] = np.clip((b[1] - stats.lo[c]) / (stats.hi[c] - stats.lo[c]), -1, 2)
+                        est = stats.passing(c, op, b[1])
+                f[:, 11 + NCOLS] = est
+            nterm += 1
+        else:
+            # arithmetic and everything else: the destination is an expression
+            if len(o) > 1 and k not in (OP_Table,):
+                regdef[o[1]] = ("expr",)
+    shape = np.array([nterm / MAX_TERMS, ninvalid, nlogic, len(ops) / 40.0])
+    return np.concatenate([terms.reshape(ntab, -1), np.tile(shape, (ntab, 1)),
+                           (stats.rows / float(1 << 20))[:, None], stats.tablet_features],
+                          axis=1).astype(np.float32)
+
+
+# ------------------------------------------------------------------- MLP
+class MLP:
+    """The predictor network on the GPU (vg_mlp_* in include/virg_gpu.h)."""
+
+    def __init__(self, nin: int = NFEATURES, nh: int = 32, seed: int = 1):
+        _bind()
+        self.nin, self.nh = nin, nh
+        self.h = lib.vg_mlp_create(nin, nh, seed)
+        if not self.h:
+            raise VirginianError("vg_mlp_create failed (no device?)")
+        self.nparams = lib.vg_mlp_num_params(self.h)
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.nparams, dtype=np.float32)
+        lib.vg_mlp_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if lib.vg_mlp_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))) != VIRG_SUCCESS:
+            raise VirginianError("vg_mlp_set_params failed")
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty(len(x), dtype=np.float32)
+        if len(x) and lib.vg_mlp_forward(self.h, x.ctypes.data_as(C.POINTER(C.c_float)), len(x),
+                                         y.ctypes.data_as(C.POINTER(C.c_float))) != VIRG_SUCCESS:
+            raise VirginianError("vg_mlp_forward failed")
+        return y
+
+    def gradient(self, x: np.ndarray, t: np.ndarray) -> tuple[float, np.ndarray]:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = C.c_float(0)
+        if lib.vg_mlp_gradient(self.h, x.ctypes.data_as(C.POINTER(C.c_float)),
+                               t.ctypes.data_as(C.POINTER(C.c_float)), len(x), C.byref(loss),
+                               g.ctypes.data_as(C.POINTER(C.c_float))) != VIRG_SUCCESS:
+            raise VirginianError("vg_mlp_gradient failed")
+        return loss.value, g
+
+    def train(self, x: np.ndarray, t: np.ndarray, epochs: int = 60, batch: int = 256, lr: float = 2e-3,
+              seed: int = 1) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_float(0)
+        if lib.vg_mlp_train(self.h, x.ctypes.data_as(C.POINTER(C.c_float)),
+                            t.ctypes.data_as(C.POINTER(C.c_float)), len(x), epochs, batch, lr, seed,
+                            C.byref(loss)) != VIRG_SUCCESS:
+            raise VirginianError("vg_mlp_train failed")
+        return loss.value
+
+    def close(self) -> None:
+        if self.h:
+            lib.vg_mlp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_BOUND = False
+
+
+def _bind() -> None:
+    global _BOUND
+    if _BOUND:
+        return
+    F = C.POINTER(C.c_float)
+    P = C.c_void_p
+    sig = {
+        "vg_mlp_create": (P, [C.c_uint, C.c_uint, C.c_ulonglong]),
+        "vg_mlp_free": (None, [P]),
+        "vg_mlp_num_params": (C.c_uint, [P]),
+        "vg_mlp_get_params": (C.c_int, [P, F]),
+        "vg_mlp_set_params": (C.c_int, [P, F]),
+        "vg_mlp_forward": (C.c_int, [P, F, C.c_uint, F]),
+        "vg_mlp_gradient": (C.c_int, [P, F, F, C.c_uint, C.POINTER(C.c_float), F]),
+        "vg_mlp_train": (C.c_int, [P, F, F, C.c_uint, C.c_uint, C.c_uint, C.c_float, C.c_ulonglong,
+                                   C.POINTER(C.c_float)]),
+        "vg_query_tablet_rows": (C.c_uint, [P, C.POINTER(C.c_ulonglong), C.c_uint]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _BOUND = True
+
+
+# ----------------------------------------------------- training and sizing
+def table_of(ops: list[tuple]) -> int:
+    return next(o[1] for o in ops if o[0] == OP_Table)
+
+
+def tablet_rows_out(q) -> np.ndarray:
+    """Rows q's last two-pass launch selected in each of its tablets."""
+    _bind()
+    n = 1 << 16
+    buf = (C.c_ulonglong * n)()
+    k = lib.vg_query_tablet_rows(q.h, buf, n)
+    return np.array(buf[:k], dtype=np.float64)
+
+
+def collect(db, programs: list[list[tuple]], stats: dict) -> tuple[np.ndarray, np.ndarray]:
+    """Training examples from the executor's own counts: run each program
+    in two passes and pair each tablet's features with the fraction of
+    its rows selected."""
+    xs, ts = [], []
+    for ops in programs:
+        st = stats[table_of(ops)]
+        q = db.prepare(ops, flags=1)
+        try:
+            q.run()
+            sel = tablet_rows_out(q) / np.maximum(st.rows, 1)
+        finally:
+            q.close()
+        xs.append(program_features(ops, st))
+        ts.append(sel)
+    return np.concatenate(xs).astype(np.float32), np.concatenate(ts).astype(np.float32)
+
+
+def capacity(pred: np.ndarray, rows: np.ndarray, margin: float) -> int:
+    """Output rows to reserve: per tablet rows * min(1, pred + margin),
+    plus a little slack."""
+    return int(np.ceil(rows * np.minimum(1.0, np.maximum(pred, 0.0) + margin)).sum()) + 64
+
+
+def calibrate(pred_sel: list[np.ndarray], true_rows: list[float], rows: list[np.ndarray],
+              max_overflow: float) -> float:
+    """Smallest margin (on a grid) with which at most max_overflow of the
+    queries would not fit."""
+    for m in np.linspace(0.0, 1.0, 201):
+        over = sum(capacity(p, r, m) < t for p, t, r in zip(pred_sel, true_rows, rows))
+        if over <= max_overflow * len(true_rows):
+            return float(m)
+    return 1.0
