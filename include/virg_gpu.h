@@ -198,9 +198,12 @@ int vg_link_probe(virginian *v, size_t bytes, int reps, double *h2d_gbs, double 
  * the fraction of a tablet's rows a program selects, trained on the GPU
  * by its own forward, backward and Adam kernels.  Parameters, flat f32:
  * W1[nin][nh] b1[nh] W2[nh][nh] b2[nh] W3[nh] b3; loss = mean (y - t)^2.
+ * With skip != 0 the output pre-activation also gets skip * x[0] (no
+ * parameter) and W3 starts at 0, so the network corrects an estimate
+ * whose logit / skip is fed as feature 0.
  * Arrays are host memory, examples row-major (n x nin). */
 typedef struct vg_mlp vg_mlp;
-vg_mlp *vg_mlp_create(unsigned nin, unsigned nh, unsigned long long seed);
+vg_mlp *vg_mlp_create(unsigned nin, unsigned nh, float skip, unsigned long long seed);
 void vg_mlp_free(vg_mlp *m);
 unsigned vg_mlp_num_params(const vg_mlp *m);
 int vg_mlp_get_params(const vg_mlp *m, float *out);

@@ -6,7 +6,10 @@ The network has no counterpart in the reference, so there is no reference
 file to follow.  Its definition is the textbook one:
     h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), y = sigmoid(h2 W3 + b3)
     loss = mean((y - t)^2)
-with parameters packed flat as W1[nin][nh] b1[nh] W2[nh][nh] b2[nh] W3[nh] b3.
+with parameters packed flat as W1[nin][nh] b1[nh] W2[nh][nh] b2[nh] W3[nh] b3,
+and optionally `skip` times the first input added to the output's
+pre-activation (the network then learns a correction to an estimate given
+as that input's logit; the skip has no parameter).
 
 Pinned by: central finite differences in float64 against `gradient`
 (tests/test_mlp.py), which a wrong sign, a dropped ReLU mask or a
@@ -34,26 +37,26 @@ def unpack(p: np.ndarray, nin: int, nh: int) -> dict:
             "W3": g["W3"].reshape(nh, 1), "b3": g["b3"]}
 
 
-def forward(p: np.ndarray, x: np.ndarray, nin: int, nh: int):
+def forward(p: np.ndarray, x: np.ndarray, nin: int, nh: int, skip: float = 0.0):
     w = unpack(p, nin, nh)
     z1 = x @ w["W1"] + w["b1"]
     h1 = np.maximum(z1, 0)
     z2 = h1 @ w["W2"] + w["b2"]
     h2 = np.maximum(z2, 0)
-    z3 = h2 @ w["W3"] + w["b3"]
+    z3 = h2 @ w["W3"] + w["b3"] + skip * x[:, :1]
     y = 1.0 / (1.0 + np.exp(-z3[:, 0]))
     return y, (h1, h2)
 
 
-def loss(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int) -> float:
-    y, _ = forward(p, x, nin, nh)
+def loss(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int, skip: float = 0.0) -> float:
+    y, _ = forward(p, x, nin, nh, skip)
     return float(np.mean((y - t) ** 2))
 
 
-def gradient(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int):
+def gradient(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int, skip: float = 0.0):
     """(loss, gradient packed like p) by backpropagation."""
     w = unpack(p, nin, nh)
-    y, (h1, h2) = forward(p, x, nin, nh)
+    y, (h1, h2) = forward(p, x, nin, nh, skip)
     B = x.shape[0]
     d3 = (2.0 * (y - t) / B * y * (1.0 - y))[:, None]     # dL/dz3
     gW3 = h2.T @ d3

@@ -18,8 +18,9 @@ pytestmark = pytest.mark.gpu
 
 
 def test_forward_and_gradient_match_oracle(gpu_db):
-    for nin, nh, B, seed in [(7, 5, 9, 1), (PR.NFEATURES, 32, 300, 2), (40, 32, 1, 3)]:
-        net = PR.MLP(nin, nh, seed=seed)
+    for nin, nh, B, seed, skip in [(7, 5, 9, 1, 0.0), (PR.NFEATURES, 32, 300, 2, PR.SKIP), (40, 32, 1, 3, 0.0),
+                                   (12, 32, 64, 4, 2.5)]:
+        net = PR.MLP(nin, nh, seed=seed, skip=skip)
         rng = np.random.default_rng(seed)
         # biases away from zero so that ReLUs are on and off
         p = net.params().astype(np.float64)
@@ -29,10 +30,10 @@ def test_forward_and_gradient_match_oracle(gpu_db):
         x = rng.normal(0, 1, (B, nin)).astype(np.float32)
         t = rng.uniform(0, 1, B).astype(np.float32)
         y = net.forward(x)
-        y_ref, _ = M.forward(p, x.astype(np.float64), nin, nh)
+        y_ref, _ = M.forward(p, x.astype(np.float64), nin, nh, skip)
         assert np.allclose(y, y_ref, rtol=1e-4, atol=1e-6), (nin, np.abs(y - y_ref).max())
         loss, g = net.gradient(x, t)
-        loss_ref, g_ref = M.gradient(p, x.astype(np.float64), t.astype(np.float64), nin, nh)
+        loss_ref, g_ref = M.gradient(p, x.astype(np.float64), t.astype(np.float64), nin, nh, skip)
         assert np.isclose(loss, loss_ref, rtol=1e-4)
         scale = np.abs(g_ref).max()
         assert np.allclose(g, g_ref, rtol=1e-4, atol=1e-5 * scale), (nin, np.abs(g - g_ref).max(), scale)
@@ -43,7 +44,7 @@ def test_training_fits_a_known_function(gpu_db):
     rng = np.random.default_rng(5)
     x = rng.uniform(-1, 1, (2000, 6)).astype(np.float32)
     t = (1 / (1 + np.exp(-(2 * x[:, 0] - x[:, 1] * x[:, 2])))).astype(np.float32)
-    net = PR.MLP(6, 32, seed=3)
+    net = PR.MLP(6, 32, seed=3, skip=0.0)
     before = M.loss(net.params().astype(np.float64), x, t, 6, 32)
     after = net.train(x, t, epochs=40, batch=128, lr=3e-3)
     assert after < before / 10 and after < 2e-3, (before, after)
@@ -74,7 +75,7 @@ def learned(gpu_db):
     train, held = progs(240), progs(60)
     x, t = PR.collect(gpu_db, train, stats)
     net = PR.MLP(seed=7)
-    net.train(x, t, epochs=80, batch=256, lr=2e-3)
+    net.train(x, t)
     yield gpu_db, net, stats, train, held, (x, t)
     net.close()
 
@@ -89,6 +90,8 @@ def test_predictor_gains_over_sizing_without_it(learned):
     mae_net = float(np.mean(np.abs(net.forward(hx) - ht)))
     mae_const = float(np.mean(np.abs(np.mean(t) - ht)))
     print(f"held-out MAE per tablet: network {mae_net:.4f}, mean selectivity {mae_const:.4f}")
+    mae_est = float(np.mean(np.abs(hx[:, 3] - ht)))
+    print(f"  (the independence estimate alone: {mae_est:.4f})")
     assert mae_net < 0.5 * mae_const
     pred_net, pred_const, true_rows, rows = [], [], [], []
     for ops in held:

@@ -83,8 +83,10 @@ def test_features_and_estimates(db):
     f = PR.program_features(ops, st)
     assert f.shape == (5, PR.NFEATURES) and f.dtype == np.float32
     assert np.all(np.isfinite(f))
-    # two compares found, each on a bare column against a constant
-    assert f[0, 0] == 1 and f[0, PR.TERM_F] == 1 and f[0, 2 * PR.TERM_F] == 0
+    # two compares, both on a bare column against a constant
+    assert f[0, 4] == 2 / 4 and f[0, 5] == 0
+    # uniformi < 50 and normali5 > 0: 0.5 * P(N(0,5) rounds above 0) = 0.23
+    assert np.all(np.abs(f[:, 3] - 0.23) < 0.03), f[:, 3]
 
 
 def test_capacity_and_calibration():
@@ -94,3 +96,16 @@ def test_capacity_and_calibration():
     preds = [np.array([0.1]), np.array([0.1])]
     m = PR.calibrate(preds, [100.0, 300.0], [np.array([1000.0])] * 2, 0.0)
     assert PR.capacity(preds[1], np.array([1000.0]), m) >= 300
+
+
+def test_oracle_skip_gradient_matches_finite_differences():
+    p, x, t, nin, nh = _case(21)
+    _, g = M.gradient(p, x, t, nin, nh, 2.5)
+    eps = 1e-6
+    fd = np.zeros_like(p)
+    for i in range(len(p)):
+        a, b = p.copy(), p.copy()
+        a[i] += eps
+        b[i] -= eps
+        fd[i] = (M.loss(a, x, t, nin, nh, 2.5) - M.loss(b, x, t, nin, nh, 2.5)) / (2 * eps)
+    assert np.allclose(g, fd, rtol=1e-5, atol=1e-8)

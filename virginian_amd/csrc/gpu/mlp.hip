@@ -9,7 +9,9 @@
  *
  * No counterpart in the reference.  Parameters, flat f32:
  *   W1[nin][H] b1[H] W2[H][H] b2[H] W3[H] b3
- * Loss: mean over examples of (y - t)^2.
+ * plus, optionally, a fixed skip: skip * x[0] added to the output's
+ * pre-activation (x[0] carries the logit of a classical estimate, which
+ * the network then corrects).  Loss: mean over examples of (y - t)^2.
  *
  * The layers are small (H = 32) and the batches a few hundred examples,
  * so each kernel gives one thread one output value; nothing here is worth
@@ -27,6 +29,7 @@ extern "C" {
 
 struct vg_mlp {
 	unsigned nin = 0, nh = 0, np = 0;
+	float skip = 0.0f;           /* output pre-activation += skip * x[0] */
 	float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
 	unsigned long long step = 0;
 };
@@ -49,16 +52,18 @@ static mlp_layout layout(unsigned nin, unsigned h)
 	return L;
 }
 
-/* out[b][j] = act(bias[j] + sum_i in[b][i] * W[i][j]); act 0 = ReLU,
- * 1 = sigmoid */
+/* out[b][j] = act(bias[j] + skip * xs[b * xs_stride] + sum_i in[b][i] *
+ * W[i][j]); act 0 = ReLU, 1 = sigmoid */
 __global__ void mlp_layer_fwd(const float *in, unsigned nin, const float *W, const float *bias, float *out,
-	unsigned nout, unsigned B, int act)
+	unsigned nout, unsigned B, int act, const float *xs, unsigned xs_stride, float skip)
 {
 	const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= B * nout)
 		return;
 	const unsigned b = t / nout, j = t % nout;
 	float z = bias[j];
+	if (skip != 0.0f)
+		z += skip * xs[(size_t)b * xs_stride];
 	const float *x = in + (size_t)b * nin;
 	for (unsigned i = 0; i < nin; i++)
 		z += x[i] * W[(size_t)i * nout + j];
@@ -187,11 +192,11 @@ static void forward(const vg_mlp *m, mlp_work &w, unsigned B, const float *x)
 	const mlp_layout L = layout(m->nin, m->nh);
 	const float *P = m->d_p;
 	hipLaunchKernelGGL(mlp_layer_fwd, dim3(blocks((size_t)B * m->nh)), dim3(256), 0, 0,
-		x, m->nin, P + L.w1, P + L.b1, w.h1, m->nh, B, 0);
+		x, m->nin, P + L.w1, P + L.b1, w.h1, m->nh, B, 0, x, m->nin, 0.0f);
 	hipLaunchKernelGGL(mlp_layer_fwd, dim3(blocks((size_t)B * m->nh)), dim3(256), 0, 0,
-		w.h1, m->nh, P + L.w2, P + L.b2, w.h2, m->nh, B, 0);
+		w.h1, m->nh, P + L.w2, P + L.b2, w.h2, m->nh, B, 0, x, m->nin, 0.0f);
 	hipLaunchKernelGGL(mlp_layer_fwd, dim3(blocks(B)), dim3(256), 0, 0,
-		w.h2, m->nh, P + L.w3, P + L.b3, w.y, 1u, B, 1);
+		w.h2, m->nh, P + L.w3, P + L.b3, w.y, 1u, B, 1, x, m->nin, m->skip);
 }
 
 /* backward after forward: gradient of the mean loss into m->d_g */
@@ -240,14 +245,18 @@ extern "C" void vg_mlp_free(vg_mlp *m)
 }
 
 /* A network for `nin` features and `nh` hidden units per layer, weights
- * drawn uniformly in +-sqrt(6 / fan_in) from `seed`, biases 0. */
-extern "C" vg_mlp *vg_mlp_create(unsigned nin, unsigned nh, unsigned long long seed)
+ * drawn uniformly in +-sqrt(6 / fan_in) from `seed`, biases 0.  With
+ * skip != 0 the output's pre-activation also gets skip * x[0] and W3
+ * starts at 0: the network starts as sigmoid(skip * x[0]) and learns a
+ * correction to it. */
+extern "C" vg_mlp *vg_mlp_create(unsigned nin, unsigned nh, float skip, unsigned long long seed)
 {
 	if (nin == 0 || nh == 0)
 		return nullptr;
 	vg_mlp *m = new vg_mlp();
 	m->nin = nin;
 	m->nh = nh;
+	m->skip = skip;
 	const mlp_layout L = layout(nin, nh);
 	m->np = (unsigned)L.n;
 	float **bufs[] = { &m->d_p, &m->d_g, &m->d_m, &m->d_v };
@@ -267,7 +276,8 @@ extern "C" vg_mlp *vg_mlp_create(unsigned nin, unsigned nh, unsigned long long s
 	};
 	fill(L.w1, (size_t)nin * nh, nin);
 	fill(L.w2, (size_t)nh * nh, nh);
-	fill(L.w3, nh, nh);
+	if (skip == 0.0f)
+		fill(L.w3, nh, nh);
 	if (hipMemcpy(m->d_p, p.data(), L.n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
 		(void)hipGetLastError();
 		vg_mlp_free(m);

@@ -10,12 +10,12 @@ only means smaller buffers (more queries admitted into HBM) and fewer
 reruns.
 
 Features of a (program, tablet) pair:
-  - the filter's compares in program order (up to MAX_TERMS): operator,
-    whether the branch keeps validity, whether each side is a bare column
-    (which one) or a constant (its value scaled to the column's range), and
-    the fraction of the tablet's rows passing `column op constant` read off
-    the tablet's histogram;
-  - program shape: compares, Invalid and logic ops, program length;
+  - the filter evaluated under independence (structural_estimate): each
+    `column op constant` compare passing at its histogram estimate on the
+    tablet, and any other compare at 1/2 -- and at 1/4 and 3/4, which
+    brackets what the compares the histograms cannot judge may do;
+  - program shape: compares the histograms judge and those they cannot,
+    keep and drop branches, Invalid and logic ops, program length;
   - the tablet: rows, and per column its min, max and a 16-bin histogram
     over the table's range.
 The network (include/virg_gpu.h, vg_mlp_*) is trained on the GPU by the
@@ -33,13 +33,14 @@ from . import (NP_TYPE, OP_And, OP_Column, OP_Eq, OP_Float, OP_Ge, OP_Gt, OP_Int
                OP_Lt, OP_Neq, OP_Not, OP_Or, OP_Parallel, OP_Result, OP_Rowid, OP_Table, VIRG_SUCCESS,
                VirginianError, lib)
 
-MAX_TERMS = 10
 NCOLS = 6
 NBINS = 16
 CMP_OPS = [OP_Lt, OP_Le, OP_Gt, OP_Ge, OP_Eq, OP_Neq]
 MIRROR = {OP_Lt: OP_Gt, OP_Le: OP_Ge, OP_Gt: OP_Lt, OP_Ge: OP_Le, OP_Eq: OP_Eq, OP_Neq: OP_Neq}
-TERM_F = 1 + 6 + 1 + 1 + NCOLS + 1 + 1 + 1
-NFEATURES = MAX_TERMS * TERM_F + 4 + 1 + NCOLS * (2 + NBINS)
+NFEATURES = 4 + 7 + 1 + NCOLS * (2 + NBINS)
+# feature 0 is logit(estimate) / SKIP; the network adds SKIP * feature 0 to
+# its output's pre-activation and so learns a correction to the estimate
+SKIP = 4.0
 
 
 # ------------------------------------------------------------ tablet stats
@@ -109,14 +110,55 @@ def _value(op) -> float:
     return float(op[4]) if op[0] == OP_Float else float(op[2])
 
 
+def structural_estimate(ops: list[tuple], stats: TableStats, unknown: float = 0.5) -> np.ndarray:
+    """Per tablet: the fraction of rows the filter keeps if its compares
+    were independent, each `column op constant` passing at its histogram
+    estimate and any other compare at `unknown`.  Follows the forward-jump
+    semantics: a taken compare jumps to p3, keeping validity when p4 is set
+    and dropping the row otherwise; Invalid drops the rows reaching it."""
+    ntab = len(stats.rows)
+    start = next((i + 1 for i, o in enumerate(ops) if o[0] == OP_Parallel), 0)
+    res = next((i for i, o in enumerate(ops) if o[0] == OP_Result and i >= start), len(ops))
+    arrive = {start: np.ones(ntab)}          # valid rows reaching each op
+    regdef: dict[int, tuple] = {}
+    for pc in range(start, res):
+        o = ops[pc]
+        m = arrive.pop(pc, None)
+        m = np.zeros(ntab) if m is None else m
+        k = o[0]
+        nxt = m
+        if k == OP_Column:
+            regdef[o[1]] = ("col", o[2])
+        elif k in (OP_Integer, OP_Float):
+            regdef[o[1]] = ("const", _value(o))
+        elif k == OP_Invalid:
+            nxt = np.zeros(ntab)
+        elif k in CMP_OPS or k in (OP_And, OP_Or, OP_Not):
+            p = np.full(ntab, unknown)
+            if k in CMP_OPS:
+                a, b = regdef.get(o[1], ("expr",)), regdef.get(o[2], ("expr",))
+                op = k
+                if a[0] == "const" and b[0] == "col":
+                    a, b, op = b, a, MIRROR[k]
+                if a[0] == "col" and a[1] < NCOLS and b[0] == "const":
+                    p = np.clip(stats.passing(a[1], op, b[1]), 0.0, 1.0)
+            target = o[3]
+            if (o[4] or 0) and target > pc:
+                arrive[target] = arrive.get(target, 0) + m * p
+            nxt = m * (1 - p)
+        elif len(o) > 1 and k != OP_Table:
+            regdef[o[1]] = ("expr",)
+        arrive[pc + 1] = arrive.get(pc + 1, 0) + nxt
+    return np.clip(arrive.get(res, np.zeros(ntab)), 0.0, 1.0)
+
+
 def program_features(ops: list[tuple], stats: TableStats) -> np.ndarray:
     """Feature rows (one per tablet) of a program over the table `stats`
     describes."""
     ntab = len(stats.rows)
-    terms = np.zeros((ntab, MAX_TERMS, TERM_F))
     regdef: dict[int, tuple] = {}
     start = next((i + 1 for i, o in enumerate(ops) if o[0] == OP_Parallel), 0)
-    nterm = ninvalid = nlogic = 0
+    known = unknown = keep = drop = ninvalid = nlogic = 0
     for o in ops[start:]:
         k = o[0]
         if k == OP_Result:
@@ -125,51 +167,48 @@ def program_features(ops: list[tuple], stats: TableStats) -> np.ndarray:
             regdef[o[1]] = ("col", o[2])
         elif k in (OP_Integer, OP_Float):
             regdef[o[1]] = ("const", _value(o))
-        elif k == OP_Rowid:
-            regdef[o[1]] = ("rowid",)
         elif k == OP_Invalid:
             ninvalid += 1
         elif k in (OP_And, OP_Or, OP_Not):
             nlogic += 1
         elif k in CMP_OPS:
-            if nterm < MAX_TERMS:
-                a, b = regdef.get(o[1], ("expr",)), regdef.get(o[2], ("expr",))
-                op = k
-                if a[0] == "const" and b[0] == "col":
-                    a, b, op = b, a, MIRROR[k]
-                f = terms[:, nterm]
-                f[:, 0] = 1.0
-                f[:, 1 + CMP_OPS.index(op)] = 1.0
-                f[:, 7] = 1.0 if (o[4] or 0) else 0.0
-                est = np.full(ntab, 0.5)
-                if a[0] == "col" and a[1] < NCOLS:
-                    f[:, 8] = 1.0
-                    f[:, 9 + a[1]] = 1.0
-                    if b[0] == "const":
-                        c = a[1]
-                        f[:, 9 + NCOLS] = 1.0
-                        f[:, 10 + NCOLS] = np.clip((b[1] - stats.lo[c]) / (stats.hi[c] - stats.lo[c]), -1, 2)
-                        est = stats.passing(c, op, b[1])
-                f[:, 11 + NCOLS] = est
-            nterm += 1
-        else:
-            # arithmetic and everything else: the destination is an expression
-            if len(o) > 1 and k not in (OP_Table,):
-                regdef[o[1]] = ("expr",)
-    shape = np.array([nterm / MAX_TERMS, ninvalid, nlogic, len(ops) / 40.0])
-    return np.concatenate([terms.reshape(ntab, -1), np.tile(shape, (ntab, 1)),
-                           (stats.rows / float(1 << 20))[:, None], stats.tablet_features],
-                          axis=1).astype(np.float32)
+            a, b = regdef.get(o[1], ("expr",)), regdef.get(o[2], ("expr",))
+            if a[0] == "const" and b[0] == "col":
+                a, b = b, a
+            if a[0] == "col" and a[1] < NCOLS and b[0] == "const":
+                known += 1
+            else:
+                unknown += 1
+            if o[4] or 0:
+                keep += 1
+            else:
+                drop += 1
+        elif len(o) > 1 and k != OP_Table:
+            regdef[o[1]] = ("expr",)
+    est = structural_estimate(ops, stats)
+    lo = structural_estimate(ops, stats, 0.25)
+    hi = structural_estimate(ops, stats, 0.75)
+    shape = np.array([known / 4, unknown / 4, keep / 4, drop / 4, ninvalid, nlogic, len(ops) / 40.0])
+    return np.concatenate([_logit(est)[:, None], _logit(lo)[:, None], _logit(hi)[:, None], est[:, None],
+                           np.tile(shape, (ntab, 1)), (stats.rows / float(1 << 20))[:, None],
+                           stats.tablet_features], axis=1).astype(np.float32)
+
+
+def _logit(p: np.ndarray) -> np.ndarray:
+    """log(p / (1 - p)) / SKIP, clipped: the network's output is a
+    sigmoid, so an estimate enters as (a scaled) logit."""
+    q = np.clip(p, 1e-4, 1 - 1e-4)
+    return np.log(q / (1 - q)) / SKIP
 
 
 # ------------------------------------------------------------------- MLP
 class MLP:
     """The predictor network on the GPU (vg_mlp_* in include/virg_gpu.h)."""
 
-    def __init__(self, nin: int = NFEATURES, nh: int = 32, seed: int = 1):
+    def __init__(self, nin: int = NFEATURES, nh: int = 32, seed: int = 1, skip: float = SKIP):
         _bind()
-        self.nin, self.nh = nin, nh
-        self.h = lib.vg_mlp_create(nin, nh, seed)
+        self.nin, self.nh, self.skip = nin, nh, skip
+        self.h = lib.vg_mlp_create(nin, nh, skip, seed)
         if not self.h:
             raise VirginianError("vg_mlp_create failed (no device?)")
         self.nparams = lib.vg_mlp_num_params(self.h)
@@ -203,7 +242,7 @@ class MLP:
             raise VirginianError("vg_mlp_gradient failed")
         return loss.value, g
 
-    def train(self, x: np.ndarray, t: np.ndarray, epochs: int = 60, batch: int = 256, lr: float = 2e-3,
+    def train(self, x: np.ndarray, t: np.ndarray, epochs: int = 10, batch: int = 256, lr: float = 2e-3,
               seed: int = 1) -> float:
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
@@ -236,7 +275,7 @@ def _bind() -> None:
     F = C.POINTER(C.c_float)
     P = C.c_void_p
     sig = {
-        "vg_mlp_create": (P, [C.c_uint, C.c_uint, C.c_ulonglong]),
+        "vg_mlp_create": (P, [C.c_uint, C.c_uint, C.c_float, C.c_ulonglong]),
         "vg_mlp_free": (None, [P]),
         "vg_mlp_num_params": (C.c_uint, [P]),
         "vg_mlp_get_params": (C.c_int, [P, F]),
