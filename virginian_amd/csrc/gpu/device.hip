@@ -90,6 +90,7 @@ struct vgd_device {
 	std::vector<shared_plan *> retired;                 /* freed at close */
 	std::vector<vg_ctrl *> ctrl_free;                   /* pooled control blocks */
 	std::vector<vg_ctrl *> ctrl_chunks;
+	std::vector<launch_res> lr_free;                    /* pooled two-pass arrays */
 };
 
 static void free_plan(plan &p);
@@ -164,6 +165,8 @@ extern "C" void vgd_close(virginian *v)
 	}
 	for (vg_ctrl *c : d->ctrl_chunks)
 		(void)hipFree(c);
+	for (launch_res &lr : d->lr_free)
+		free_lr(lr);
 	(void)hipStreamDestroy(d->stream);
 	delete d;
 	s->dev = nullptr;
@@ -1021,6 +1024,23 @@ static vg_ctrl *pool_ctrl(vgd_device *d)
 	return c;
 }
 
+/* A light query's two-pass arrays: pooled ones large enough, if any. */
+static void take_pooled_lr(vgd_device *d, launch_res &lr, uint32_t num_tiles, uint32_t grid)
+{
+	if (lr.vbits)
+		return;
+	for (size_t i = 0; i < d->lr_free.size(); i++) {
+		launch_res &f = d->lr_free[i];
+		if (f.tiles_cap >= num_tiles && f.wgcount_cap >= grid) {
+			vg_ctrl *c = lr.ctrl;
+			lr = f;
+			lr.ctrl = c;
+			d->lr_free.erase(d->lr_free.begin() + (long)i);
+			return;
+		}
+	}
+}
+
 /* -------------------------------------------------------- vg_query ABI */
 struct vg_query;
 static int ensure_events(vg_query *q);
@@ -1077,6 +1097,12 @@ extern "C" void vg_query_free(vg_query *q)
 	if (q->pooled_ctrl && q->lr.ctrl) {
 		q->d->ctrl_free.push_back(q->lr.ctrl);
 		q->lr.ctrl = nullptr;
+		if (q->lr.vbits) {
+			/* keep the two-pass arrays for the next light query (a free
+			 * would wait for the whole device) */
+			q->d->lr_free.push_back(q->lr);
+			q->lr = launch_res();
+		}
 	}
 	free_lr(q->lr);
 	for (unsigned j = 0; j < VG_MAX_OUT && q->own_out; j++)
@@ -1212,6 +1238,8 @@ extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned 
 /* Launch once, asynchronously, on the query's stream. */
 extern "C" int vg_query_launch(vg_query *q)
 {
+	if (q->pooled_ctrl)
+		take_pooled_lr(q->d, q->lr, q->p->num_tiles, vg_kernel_grid(&q->P, q->p->num_tiles));
 	return launch(q->P, *q->p, q->lr, q->out, q->capacity, q->stream);
 }
 
@@ -1790,6 +1818,8 @@ static int pbatch_set_queries(vg_pbatch *b, vg_query *const *qs, unsigned n, cha
 	for (unsigned i = 0; i < n; i++) {
 		vg_query *q = qs[i];
 		const uint32_t nt = q->p->num_tiles;
+		if (!b->cfg.single && q->pooled_ctrl)
+			take_pooled_lr(q->d, q->lr, nt, vg_kernel_grid(&q->P, nt ? nt : 1));
 		if (b->cfg.single) {
 			if (!q->lr.ctrl && ensure_lr(q->lr, nt, vg_kernel_grid(&q->P, nt ? nt : 1), q->stream) == VIRG_FAIL)
 				return qerr(err, errlen, "cannot set up query %u", i);

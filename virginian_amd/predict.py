@@ -110,21 +110,24 @@ def _value(op) -> float:
     return float(op[4]) if op[0] == OP_Float else float(op[2])
 
 
-def structural_estimate(ops: list[tuple], stats: TableStats, unknown: float = 0.5) -> np.ndarray:
-    """Per tablet: the fraction of rows the filter keeps if its compares
-    were independent, each `column op constant` passing at its histogram
-    estimate and any other compare at `unknown`.  Follows the forward-jump
-    semantics: a taken compare jumps to p3, keeping validity when p4 is set
-    and dropping the row otherwise; Invalid drops the rows reaching it."""
+def _walk(ops: list[tuple], stats: TableStats, unknown: np.ndarray):
+    """One pass over the filter: for each value in `unknown` (the pass rate
+    assumed for compares the histograms cannot judge), the fraction of each
+    tablet's rows reaching Result valid if the compares were independent,
+    each `column op constant` passing at its histogram estimate; a taken
+    compare jumps to p3, keeping validity when p4 is set and dropping the
+    row otherwise; Invalid drops the rows reaching it.  Also the program's
+    shape counts."""
     ntab = len(stats.rows)
+    u = np.asarray(unknown, dtype=np.float64)[:, None] * np.ones((1, ntab))
     start = next((i + 1 for i, o in enumerate(ops) if o[0] == OP_Parallel), 0)
     res = next((i for i, o in enumerate(ops) if o[0] == OP_Result and i >= start), len(ops))
-    arrive = {start: np.ones(ntab)}          # valid rows reaching each op
+    arrive = {start: np.ones_like(u)}
     regdef: dict[int, tuple] = {}
+    known = nunknown = keep = drop = ninvalid = nlogic = 0
     for pc in range(start, res):
         o = ops[pc]
         m = arrive.pop(pc, None)
-        m = np.zeros(ntab) if m is None else m
         k = o[0]
         nxt = m
         if k == OP_Column:
@@ -132,62 +135,52 @@ def structural_estimate(ops: list[tuple], stats: TableStats, unknown: float = 0.
         elif k in (OP_Integer, OP_Float):
             regdef[o[1]] = ("const", _value(o))
         elif k == OP_Invalid:
-            nxt = np.zeros(ntab)
+            ninvalid += 1
+            nxt = None
         elif k in CMP_OPS or k in (OP_And, OP_Or, OP_Not):
-            p = np.full(ntab, unknown)
+            p = u
             if k in CMP_OPS:
                 a, b = regdef.get(o[1], ("expr",)), regdef.get(o[2], ("expr",))
                 op = k
                 if a[0] == "const" and b[0] == "col":
                     a, b, op = b, a, MIRROR[k]
                 if a[0] == "col" and a[1] < NCOLS and b[0] == "const":
-                    p = np.clip(stats.passing(a[1], op, b[1]), 0.0, 1.0)
-            target = o[3]
-            if (o[4] or 0) and target > pc:
-                arrive[target] = arrive.get(target, 0) + m * p
-            nxt = m * (1 - p)
+                    p = np.clip(stats.passing(a[1], op, b[1]), 0.0, 1.0)[None, :]
+                    known += 1
+                else:
+                    nunknown += 1
+                if o[4] or 0:
+                    keep += 1
+                else:
+                    drop += 1
+            else:
+                nlogic += 1
+            if m is not None:
+                if (o[4] or 0) and o[3] > pc:
+                    arrive[o[3]] = arrive.get(o[3], 0) + m * p
+                nxt = m * (1 - p)
         elif len(o) > 1 and k != OP_Table:
             regdef[o[1]] = ("expr",)
-        arrive[pc + 1] = arrive.get(pc + 1, 0) + nxt
-    return np.clip(arrive.get(res, np.zeros(ntab)), 0.0, 1.0)
+        if nxt is not None:
+            arrive[pc + 1] = arrive.get(pc + 1, 0) + nxt
+    est = arrive.get(res)
+    est = np.zeros_like(u) if est is None else np.clip(est * np.ones_like(u), 0.0, 1.0)
+    return est, (known, nunknown, keep, drop, ninvalid, nlogic)
+
+
+def structural_estimate(ops: list[tuple], stats: TableStats, unknown: float = 0.5) -> np.ndarray:
+    """Per tablet: the fraction of rows the filter keeps if its compares
+    were independent (see _walk), compares the histograms cannot judge
+    passing at `unknown`."""
+    return _walk(ops, stats, np.array([unknown]))[0][0]
 
 
 def program_features(ops: list[tuple], stats: TableStats) -> np.ndarray:
     """Feature rows (one per tablet) of a program over the table `stats`
     describes."""
     ntab = len(stats.rows)
-    regdef: dict[int, tuple] = {}
-    start = next((i + 1 for i, o in enumerate(ops) if o[0] == OP_Parallel), 0)
-    known = unknown = keep = drop = ninvalid = nlogic = 0
-    for o in ops[start:]:
-        k = o[0]
-        if k == OP_Result:
-            break
-        if k == OP_Column:
-            regdef[o[1]] = ("col", o[2])
-        elif k in (OP_Integer, OP_Float):
-            regdef[o[1]] = ("const", _value(o))
-        elif k == OP_Invalid:
-            ninvalid += 1
-        elif k in (OP_And, OP_Or, OP_Not):
-            nlogic += 1
-        elif k in CMP_OPS:
-            a, b = regdef.get(o[1], ("expr",)), regdef.get(o[2], ("expr",))
-            if a[0] == "const" and b[0] == "col":
-                a, b = b, a
-            if a[0] == "col" and a[1] < NCOLS and b[0] == "const":
-                known += 1
-            else:
-                unknown += 1
-            if o[4] or 0:
-                keep += 1
-            else:
-                drop += 1
-        elif len(o) > 1 and k != OP_Table:
-            regdef[o[1]] = ("expr",)
-    est = structural_estimate(ops, stats)
-    lo = structural_estimate(ops, stats, 0.25)
-    hi = structural_estimate(ops, stats, 0.75)
+    ests, (known, unknown, keep, drop, ninvalid, nlogic) = _walk(ops, stats, np.array([0.5, 0.25, 0.75]))
+    est, lo, hi = ests
     shape = np.array([known / 4, unknown / 4, keep / 4, drop / 4, ninvalid, nlogic, len(ops) / 40.0])
     return np.concatenate([_logit(est)[:, None], _logit(lo)[:, None], _logit(hi)[:, None], est[:, None],
                            np.tile(shape, (ntab, 1)), (stats.rows / float(1 << 20))[:, None],

@@ -1,0 +1,224 @@
+"""Serving loop: queries arrive over time and are run with continuous
+batching against resident tables.
+
+Each step of the loop (one host thread):
+  1. arrived requests join the waiting line;
+  2. while a batch runner is free, the waiting queries are admitted in
+     arrival order as long as their output buffers fit in the HBM arena
+     (admission by footprint); each is prepared against its table's shared
+     tablet descriptors and given buffers sized by the result-size
+     predictor (single pass) or for every row of its table (two passes,
+     the method without a predictor); the admitted queries go to the GPU
+     as one batch on the runner's stream, while earlier batches are still
+     running on theirs, so their work overlaps on the device;
+  3. every finished batch is collected: results are copied to the host
+     (the request is then complete), a query whose result did not fit is
+     run again in two passes with the exact size the single pass counted,
+     and its buffers return to the arena.
+Latency is measured from a request's arrival to its result on the host.
+No counterpart in the reference (its queries run one at a time,
+src/vm/execute.c:28-102).
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import OP_ResultColumn, VG_QUERY_LIGHT, VG_QUERY_NO_OUTPUT, PersistentBatch, VirginianError
+from . import predict as PR
+
+
+class Arena:
+    """First-fit allocator over one block of device memory."""
+
+    def __init__(self, db, nbytes: int):
+        self.db = db
+        self.base = db.device_alloc(nbytes)
+        self.size = nbytes
+        self.free = [(0, nbytes)]          # (offset, size), sorted
+        self.used = 0
+
+    def alloc(self, nbytes: int) -> int | None:
+        n = (nbytes + 255) & ~255
+        for i, (off, sz) in enumerate(self.free):
+            if sz >= n:
+                if sz == n:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (off + n, sz - n)
+                self.used += n
+                return off
+        return None
+
+    def release(self, off: int, nbytes: int) -> None:
+        n = (nbytes + 255) & ~255
+        self.used -= n
+        self.free.append((off, n))
+        self.free.sort()
+        merged = []
+        for o, s in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == o:
+                merged[-1] = (merged[-1][0], merged[-1][1] + s)
+            else:
+                merged.append((o, s))
+        self.free = merged
+
+    def close(self) -> None:
+        if self.base:
+            self.db.device_free(self.base)
+            self.base = 0
+
+
+@dataclass
+class Request:
+    arrival: float                  # seconds after the start
+    ops: list
+    rid: int = 0
+    ready: float = -1.0             # seconds after the start
+    rows: int = 0
+    rerun: bool = False
+    cap: int = -1                   # output rows reserved (set at admission)
+    result: dict = field(default_factory=dict)
+
+
+# bytes per value of each virg_t (INT, INT64, FLOAT, DOUBLE, CHAR)
+_WIDTH = [4, 8, 4, 8, 1]
+
+
+def _widths(ops: list) -> list[int]:
+    return [_WIDTH[o[1]] for o in ops if o[0] == OP_ResultColumn]
+
+
+class Server:
+    """mode "single": predictor-sized single pass; "two-pass": buffers for
+    every row of the table, count and write launches."""
+
+    def __init__(self, db, table_ids: list[int], arena_bytes: int, mode: str = "single", predictor=None,
+                 margin: float = 0.05, runners: int = 3, max_batch: int = 64, tiles_per_item: int = 4,
+                 keep_results: bool = True):
+        self.db = db
+        self.mode = mode
+        self.stats = {t: PR.TableStats(db, t) for t in table_ids}
+        self.net = predictor
+        self.margin = margin
+        self.arena = Arena(db, arena_bytes)
+        single = 1 if mode == "single" else 0
+        self.runners = [PersistentBatch([], tiles_per_item=tiles_per_item, single=single, direct=1, db=db)
+                        for _ in range(runners)]
+        self.max_batch = max_batch
+        self.keep = keep_results
+        self.peak_bytes = 0
+
+    def close(self) -> None:
+        for r in self.runners:
+            r.close()
+        self.arena.close()
+
+    # -- admission -------------------------------------------------------
+    def _capacities(self, reqs: list[Request]) -> list[int]:
+        if self.mode != "single":
+            return [int(self.stats[PR.table_of(r.ops)].rows.sum()) for r in reqs]
+        feats = [PR.program_features(r.ops, self.stats[PR.table_of(r.ops)]) for r in reqs]
+        pred = self.net.forward(np.concatenate(feats)) if self.net is not None else None
+        caps, k = [], 0
+        for r, f in zip(reqs, feats):
+            rows = self.stats[PR.table_of(r.ops)].rows
+            p = pred[k:k + len(f)] if pred is not None else PR.structural_estimate(r.ops, self.stats[PR.table_of(r.ops)])
+            k += len(f)
+            caps.append(PR.capacity(p, rows, self.margin))
+        return caps
+
+    def _admit(self, waiting: deque) -> list:
+        """Queries for one batch, in arrival order while their buffers fit:
+        prepared, with arena buffers."""
+        cand = [waiting[i] for i in range(min(len(waiting), self.max_batch))]
+        todo = [r for r in cand if r.cap < 0]
+        for r, c in zip(todo, self._capacities(todo) if todo else []):
+            r.cap = c
+        batch = []
+        for r in cand:
+            offs = []
+            for w in _widths(r.ops):
+                off = self.arena.alloc(r.cap * w + 256)
+                if off is None:
+                    break
+                offs.append((off, r.cap * w + 256))
+            if len(offs) < len(_widths(r.ops)):
+                for off, n in offs:
+                    self.arena.release(off, n)
+                break                       # admission is in arrival order
+            q = self.db.prepare(r.ops, capacity=r.cap, flags=VG_QUERY_LIGHT | VG_QUERY_NO_OUTPUT)
+            q.set_outputs([self.arena.base + off for off, _ in offs], r.cap)
+            waiting.popleft()
+            batch.append((r, q, offs))
+        self.peak_bytes = max(self.peak_bytes, self.arena.used)
+        return batch
+
+    # -- completion ------------------------------------------------------
+    def _finish(self, runner, batch, t0: float) -> int:
+        rows, _, flags = runner.collect()
+        for (r, q, offs), n, f in zip(batch, rows, flags):
+            if f:
+                # did not fit: two passes with the exact size the single pass counted
+                q2 = self.db.prepare(r.ops, capacity=max(int(n), 1), flags=VG_QUERY_LIGHT)
+                try:
+                    q2.run()
+                    res = q2.fetch()
+                finally:
+                    q2.close()
+                r.rerun = True
+            else:
+                res = q.fetch()
+            r.ready = time.perf_counter() - t0
+            r.rows = int(n)
+            if self.keep:
+                r.result = res
+            for off, nb in offs:
+                self.arena.release(off, nb)
+            q.close()
+        runner.reload([])
+        return len(batch)
+
+    def run(self, reqs: list[Request]) -> dict:
+        """Serve the requests at their arrival times; returns a summary."""
+        pending = deque(sorted(reqs, key=lambda r: r.arrival))
+        waiting: deque = deque()
+        free = list(self.runners)
+        inflight = []
+        done = 0
+        t0 = time.perf_counter()
+        while done < len(reqs):
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            progressed = False
+            if waiting and free:
+                batch = self._admit(waiting)
+                if batch:
+                    runner = free.pop()
+                    runner.reload([q for _, q, _ in batch])
+                    runner.launch()
+                    inflight.append((runner, batch))
+                    progressed = True
+                elif not inflight:
+                    raise VirginianError("a request's output buffers do not fit in the arena")
+            still = []
+            for runner, batch in inflight:
+                if runner.done():
+                    done += self._finish(runner, batch, t0)
+                    free.append(runner)
+                    progressed = True
+                else:
+                    still.append((runner, batch))
+            inflight = still
+            if not progressed and inflight and not waiting:
+                time.sleep(0)
+        end = max(r.ready for r in reqs) if reqs else 0.0
+        lat = np.array([r.ready - r.arrival for r in reqs]) * 1e3
+        return {"requests": len(reqs), "seconds": end, "queries_per_s": len(reqs) / end if end else 0.0,
+                "p50_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
+                "p99_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0,
+                "reruns": sum(r.rerun for r in reqs), "peak_arena_bytes": self.peak_bytes}
