@@ -671,7 +671,8 @@ class PersistentBatch:
     def __init__(self, queries: list[DeviceQuery], tiles_per_item: int = 0, order: int = 0,
                  grid_div: int = 0, split: int = 0, single: int = 0, direct: int = 0, db=None):
         self.queries = list(queries)
-        cfg = PBatchConfig(tiles_per_item, grid_div, split, order, direct, db.ptr if db else None, single)
+        cfg = PBatchConfig(tiles_per_item, grid_div, split, order, direct,
+                           C.cast(db.ptr, C.c_void_p) if db else None, single)
         err = C.create_string_buffer(256)
         self.h = lib.vg_pbatch_create(_handles(self.queries), len(self.queries), C.byref(cfg), err, 256)
         if not self.h:

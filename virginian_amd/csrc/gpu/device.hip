@@ -1001,7 +1001,7 @@ static plan *shared_plan_of(vgd_device *d, unsigned table, const std::vector<con
 	return &sp->p;
 }
 
-/* a zeroed control block from the device's pool */
+/* a control block from the device's pool */
 static vg_ctrl *pool_ctrl(vgd_device *d)
 {
 	if (d->ctrl_free.empty()) {
@@ -1011,15 +1011,18 @@ static vg_ctrl *pool_ctrl(vgd_device *d)
 			(void)hipGetLastError();
 			return nullptr;
 		}
+		if (hipMemset(c, 0, chunk * sizeof(vg_ctrl)) != hipSuccess) {
+			(void)hipGetLastError();
+			(void)hipFree(c);
+			return nullptr;
+		}
 		d->ctrl_chunks.push_back(c);
 		for (unsigned i = 0; i < chunk; i++)
 			d->ctrl_free.push_back(c + chunk - 1 - i);
 	}
+	/* no clearing: a batch run's reset and a two-pass launch's count pass
+	 * clear what they use */
 	vg_ctrl *c = d->ctrl_free.back();
-	if (hipMemset(c, 0, sizeof(vg_ctrl)) != hipSuccess) {
-		(void)hipGetLastError();
-		return nullptr;
-	}
 	d->ctrl_free.pop_back();
 	return c;
 }
@@ -1091,7 +1094,8 @@ extern "C" void vg_query_free(vg_query *q)
 {
 	if (!q)
 		return;
-	if (q->stream)
+	/* a light query is freed after its results were collected */
+	if (q->stream && !q->pooled_ctrl)
 		(void)hipStreamSynchronize(q->stream);
 	free_plan(q->own);
 	if (q->pooled_ctrl && q->lr.ctrl) {
@@ -1532,6 +1536,38 @@ extern "C" int vg_kernel_batch_single(const vg_batch_args *A, uint32_t grid, siz
  * up (the query then reports VG_OVF_WAIT and must be run in two passes) */
 #define VG_SPIN_LIMIT (1u << 22)
 
+/* pinned host memory that grows (by doubling) but is never shrunk */
+struct pinned_buf {
+	void *p = nullptr;
+	size_t cap = 0;
+	void *get(size_t bytes)
+	{
+		if (bytes > cap) {
+			if (p)
+				(void)hipHostFree(p);
+			p = nullptr;
+			size_t c = cap ? cap : 4096;
+			while (c < bytes)
+				c *= 2;
+			if (hipHostMalloc(&p, c, hipHostMallocDefault) != hipSuccess) {
+				(void)hipGetLastError();
+				p = nullptr;
+				cap = 0;
+				return nullptr;
+			}
+			cap = c;
+		}
+		return p;
+	}
+	void release()
+	{
+		if (p)
+			(void)hipHostFree(p);
+		p = nullptr;
+		cap = 0;
+	}
+};
+
 /* one group of queries with its own launch pair */
 struct pgroup {
 	std::vector<unsigned> qids;
@@ -1566,7 +1602,10 @@ struct vg_pbatch {
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	double clock_khz = 0;
-	std::vector<vg_bquery> h_q;
+	/* pinned staging: uploads and the state readback are asynchronous on
+	 * the batch's stream (a synchronous copy would wait behind whatever
+	 * holds the compute units) */
+	pinned_buf h_progs, h_launch, h_info, h_items, h_witems, h_q, h_head;
 };
 
 static void pbatch_drop_graph(vg_pbatch *b)
@@ -1593,6 +1632,10 @@ extern "C" void vg_pbatch_free(vg_pbatch *b)
 		(void)hipEventDestroy(e);
 	for (hipStream_t s : b->branch)
 		(void)hipStreamDestroy(s);
+	pinned_buf *pins[] = { &b->h_progs, &b->h_launch, &b->h_info, &b->h_items, &b->h_witems, &b->h_q,
+		&b->h_head };
+	for (pinned_buf *pb : pins)
+		pb->release();
 	if (b->fork)
 		(void)hipEventDestroy(b->fork);
 	if (b->stream)
@@ -1643,7 +1686,9 @@ static int pbatch_record(vg_pbatch *b)
 static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 {
 	const unsigned n = (unsigned)b->qs.size();
-	const uint32_t tpi = b->cfg.tiles_per_item ? b->cfg.tiles_per_item : 8;
+	/* a single-pass item is one tile: a tile waits for the counts of the
+	 * tiles before it, which must not sit behind other tiles of one item */
+	const uint32_t tpi = b->cfg.single ? 1u : b->cfg.tiles_per_item ? b->cfg.tiles_per_item : 8;
 	/* groups, keyed by the write launch's workgroups per device */
 	std::vector<uint32_t> key(n, 0);
 	if (b->cfg.split == 0)
@@ -1671,12 +1716,13 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 	}
 	const bool single = b->cfg.single != 0;
 	if (single && soff > b->status_cap) {
+		const size_t want = soff > 2 * b->status_cap ? soff : 2 * b->status_cap;
 		if (b->d_status) (void)hipFree(b->d_status);
 		b->d_status = nullptr;
 		b->status_cap = 0;
-		if (herr(hipMalloc((void **)&b->d_status, (size_t)soff * sizeof(uint64_t)), "tile status"))
+		if (herr(hipMalloc((void **)&b->d_status, want * sizeof(uint64_t)), "tile status"))
 			return qerr(err, errlen, "cannot allocate the batch's tile status words");
-		b->status_cap = soff;
+		b->status_cap = want;
 	}
 	auto tiles = [&](unsigned q) { return b->qs[q]->p->num_tiles; };
 	std::vector<vg_item> items, witems;
@@ -1712,6 +1758,7 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 	goff.push_back(items.size());
 	const size_t ni = items.size();
 	if (ni > b->items_cap) {
+		const size_t want = ni > 2 * b->items_cap ? ni : 2 * b->items_cap;
 		if (b->d_items) (void)hipFree(b->d_items);
 		if (b->d_witems) (void)hipFree(b->d_witems);
 		if (b->d_icount) (void)hipFree(b->d_icount);
@@ -1720,12 +1767,12 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		b->d_icount = nullptr;
 		b->d_ibase = nullptr;
 		b->items_cap = 0;
-		if (herr(hipMalloc((void **)&b->d_items, ni * sizeof(vg_item)), "batch items") ||
-			herr(hipMalloc((void **)&b->d_witems, ni * sizeof(vg_item)), "batch items") ||
-			herr(hipMalloc((void **)&b->d_icount, ni * sizeof(uint32_t)), "batch counts") ||
-			herr(hipMalloc((void **)&b->d_ibase, ni * sizeof(uint64_t)), "batch offsets"))
+		if (herr(hipMalloc((void **)&b->d_items, want * sizeof(vg_item)), "batch items") ||
+			herr(hipMalloc((void **)&b->d_witems, want * sizeof(vg_item)), "batch items") ||
+			herr(hipMalloc((void **)&b->d_icount, want * sizeof(uint32_t)), "batch counts") ||
+			herr(hipMalloc((void **)&b->d_ibase, want * sizeof(uint64_t)), "batch offsets"))
 			return qerr(err, errlen, "cannot allocate the batch's item lists");
-		b->items_cap = ni;
+		b->items_cap = want;
 	}
 	const size_t ng = b->groups.size();
 	if (ng > b->head_cap) {
@@ -1746,12 +1793,25 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 			return qerr(err, errlen, "cannot create the batch's events");
 		b->join.push_back(e);
 	}
-	if ((ni && (herr(hipMemcpy(b->d_items, items.data(), ni * sizeof(vg_item), hipMemcpyHostToDevice),
-			"batch items") ||
-		herr(hipMemcpy(b->d_witems, witems.data(), ni * sizeof(vg_item), hipMemcpyHostToDevice),
-			"batch items"))) ||
-		herr(hipMemcpy(b->d_info, info.data(), n * sizeof(vg_bqinfo), hipMemcpyHostToDevice), "batch info"))
-		return qerr(err, errlen, "cannot upload the batch's item lists");
+	{
+		void *hi = b->h_items.get(ni * sizeof(vg_item)), *hw = b->h_witems.get(ni * sizeof(vg_item));
+		void *hf = b->h_info.get(n * sizeof(vg_bqinfo));
+		if (!hi || !hw || !hf)
+			return qerr(err, errlen, "cannot allocate pinned staging");
+		memcpy(hi, items.data(), ni * sizeof(vg_item));
+		memcpy(hw, witems.data(), ni * sizeof(vg_item));
+		memcpy(hf, info.data(), n * sizeof(vg_bqinfo));
+		if ((ni && (herr(hipMemcpyAsync(b->d_items, hi, ni * sizeof(vg_item), hipMemcpyHostToDevice, b->stream),
+				"batch items") ||
+			herr(hipMemcpyAsync(b->d_witems, hw, ni * sizeof(vg_item), hipMemcpyHostToDevice, b->stream),
+				"batch items"))) ||
+			herr(hipMemcpyAsync(b->d_info, hf, n * sizeof(vg_bqinfo), hipMemcpyHostToDevice, b->stream),
+				"batch info"))
+			return qerr(err, errlen, "cannot upload the batch's item lists");
+		/* graph replays read the lists later: they must have landed */
+		if (!b->cfg.direct && herr(hipStreamSynchronize(b->stream), "batch items"))
+			return qerr(err, errlen, "cannot upload the batch's item lists");
+	}
 	for (size_t gi = 0; gi < ng; gi++) {
 		pgroup &g = b->groups[gi];
 		vg_batch_args &A = g.A;
@@ -1830,7 +1890,7 @@ static int pbatch_set_queries(vg_pbatch *b, vg_query *const *qs, unsigned n, cha
 		fill_launch(q, launches[i]);
 	}
 	for (unsigned i = 0; i < n; i++)
-		if (herr(hipStreamSynchronize(qs[i]->stream), "query setup"))
+		if (!qs[i]->pooled_ctrl && herr(hipStreamSynchronize(qs[i]->stream), "query setup"))
 			return qerr(err, errlen, "query setup failed");
 	if (n > b->qcap) {
 		void *bufs[] = { b->d_progs, b->d_launch, b->d_info, b->d_q };
@@ -1842,7 +1902,7 @@ static int pbatch_set_queries(vg_pbatch *b, vg_query *const *qs, unsigned n, cha
 		b->d_info = nullptr;
 		b->d_q = nullptr;
 		b->qcap = 0;
-		const size_t cap = n < 64 ? 64 : n;
+		const size_t cap = n < 64 ? 64 : n > 2 * b->qcap ? n : 2 * b->qcap;
 		if (herr(hipMalloc((void **)&b->d_progs, cap * sizeof(vg_prog)), "batch") ||
 			herr(hipMalloc((void **)&b->d_launch, cap * sizeof(vg_launch)), "batch") ||
 			herr(hipMalloc((void **)&b->d_info, cap * sizeof(vg_bqinfo)), "batch") ||
@@ -1850,10 +1910,17 @@ static int pbatch_set_queries(vg_pbatch *b, vg_query *const *qs, unsigned n, cha
 			return qerr(err, errlen, "cannot allocate the batch");
 		b->qcap = cap;
 	}
-	if (n && (herr(hipMemcpy(b->d_progs, progs.data(), n * sizeof(vg_prog), hipMemcpyHostToDevice), "batch") ||
-		herr(hipMemcpy(b->d_launch, launches.data(), n * sizeof(vg_launch), hipMemcpyHostToDevice), "batch")))
-		return qerr(err, errlen, "cannot upload the batch");
-	b->h_q.resize(n);
+	if (n) {
+		void *hp = b->h_progs.get(n * sizeof(vg_prog)), *hl = b->h_launch.get(n * sizeof(vg_launch));
+		if (!hp || !hl || !b->h_q.get(n * sizeof(vg_bquery)) || !b->h_head.get(sizeof(vg_bhead)))
+			return qerr(err, errlen, "cannot allocate pinned staging");
+		memcpy(hp, progs.data(), n * sizeof(vg_prog));
+		memcpy(hl, launches.data(), n * sizeof(vg_launch));
+		if (herr(hipMemcpyAsync(b->d_progs, hp, n * sizeof(vg_prog), hipMemcpyHostToDevice, b->stream), "batch") ||
+			herr(hipMemcpyAsync(b->d_launch, hl, n * sizeof(vg_launch), hipMemcpyHostToDevice, b->stream),
+				"batch"))
+			return qerr(err, errlen, "cannot upload the batch");
+	}
 	return VIRG_SUCCESS;
 }
 
@@ -1950,30 +2017,42 @@ extern "C" int vg_pbatch_launch(vg_pbatch *b)
 	return VIRG_SUCCESS;
 }
 
+/* Copy the batch's per-query state back (after its run) and read it. */
+static int pbatch_state(vg_pbatch *b, unsigned long long *rows, double *latency_ms, unsigned *flags)
+{
+	const size_t n = b->qs.size();
+	vg_bquery *hq = (vg_bquery *)b->h_q.get(n * sizeof(vg_bquery));
+	vg_bhead *hh = (vg_bhead *)b->h_head.get(sizeof(vg_bhead));
+	if (!hq || !hh)
+		return VIRG_FAIL;
+	HCHK(hipMemcpyAsync(hq, b->d_q, n * sizeof(vg_bquery), hipMemcpyDeviceToHost, b->stream), "batch state");
+	HCHK(hipMemcpyAsync(hh, b->d_head, sizeof(vg_bhead), hipMemcpyDeviceToHost, b->stream), "batch state");
+	HCHK(hipStreamSynchronize(b->stream), "batch");
+	for (size_t i = 0; i < n; i++) {
+		if (latency_ms)
+			latency_ms[i] = hq[i].t_done > hh->t_start ? (double)(hq[i].t_done - hh->t_start) / b->clock_khz : 0.0;
+		if (rows)
+			rows[i] = hq[i].total;
+		if (flags)
+			flags[i] = hq[i].flags;
+		b->qs[i]->last_rows = hq[i].flags ? 0 : hq[i].total;
+	}
+	return VIRG_SUCCESS;
+}
+
 /* Wait for the run; rows[i] = rows query i selected, latency_ms[i] = its
  * results complete, from the start of the run (device clock); either may
  * be NULL.  Fails when a query's result exceeded its capacity. */
 extern "C" int vg_pbatch_sync(vg_pbatch *b, unsigned long long *rows, double *latency_ms)
 {
-	HCHK(hipStreamSynchronize(b->stream), "batch");
 	const size_t n = b->qs.size();
-	if (latency_ms) {
-		vg_bhead h;
-		HCHK(hipMemcpy(b->h_q.data(), b->d_q, n * sizeof(vg_bquery), hipMemcpyDeviceToHost), "batch state");
-		HCHK(hipMemcpy(&h, b->d_head, sizeof h, hipMemcpyDeviceToHost), "batch state");
-		for (size_t i = 0; i < n; i++)
-			latency_ms[i] = b->h_q[i].t_done > h.t_start
-				? (double)(b->h_q[i].t_done - h.t_start) / b->clock_khz : 0.0;
-	}
-	for (size_t i = 0; i < n; i++) {
-		vg_query *q = b->qs[i];
-		uint64_t r = 0;
-		if (q->p->num_tiles && read_ctrl(q->lr, q->stream, &r) == VIRG_FAIL)
-			return VIRG_FAIL;
-		q->last_rows = r;
-		if (rows)
-			rows[i] = r;
-	}
+	std::vector<unsigned> flags(n);
+	if (n == 0)
+		return VIRG_SUCCESS;
+	if (pbatch_state(b, rows, latency_ms, flags.data()) == VIRG_FAIL)
+		return VIRG_FAIL;
+	for (size_t i = 0; i < n; i++)
+		VIRG_CHECK(flags[i] != 0, "result exceeded the output capacity");
 	return VIRG_SUCCESS;
 }
 
@@ -1982,29 +2061,9 @@ extern "C" int vg_pbatch_sync(vg_pbatch *b, unsigned long long *rows, double *la
  * incomplete and the query must be run again with exact sizing). */
 extern "C" int vg_pbatch_results(vg_pbatch *b, unsigned long long *rows, double *latency_ms, unsigned *flags)
 {
-	HCHK(hipStreamSynchronize(b->stream), "batch");
-	const size_t n = b->qs.size();
-	if (latency_ms) {
-		vg_bhead h;
-		HCHK(hipMemcpy(b->h_q.data(), b->d_q, n * sizeof(vg_bquery), hipMemcpyDeviceToHost), "batch state");
-		HCHK(hipMemcpy(&h, b->d_head, sizeof h, hipMemcpyDeviceToHost), "batch state");
-		for (size_t i = 0; i < n; i++)
-			latency_ms[i] = b->h_q[i].t_done > h.t_start
-				? (double)(b->h_q[i].t_done - h.t_start) / b->clock_khz : 0.0;
-	}
-	for (size_t i = 0; i < n; i++) {
-		vg_query *q = b->qs[i];
-		vg_ctrl c;
-		memset(&c, 0, sizeof c);
-		if (q->p->num_tiles)
-			HCHK(hipMemcpy(&c, q->lr.ctrl, sizeof c, hipMemcpyDeviceToHost), "control block");
-		q->last_rows = c.overflow ? 0 : c.total;
-		if (rows)
-			rows[i] = c.total;
-		if (flags)
-			flags[i] = c.overflow;
-	}
-	return VIRG_SUCCESS;
+	if (b->qs.empty())
+		return VIRG_SUCCESS;
+	return pbatch_state(b, rows, latency_ms, flags);
 }
 
 /* mean ms per run of `iters` graph replays after `warmup`, HIP events */

@@ -181,11 +181,11 @@ typedef struct {
 /* per-query state, cleared before every run */
 typedef struct {
 	uint32_t done;           /* count items finished */
-	uint32_t pad0;
-	uint32_t wdone;          /* write items finished */
+	uint32_t flags;          /* the query's overflow bits, when finished */
+	uint32_t wdone;          /* write (or single-pass) items finished */
 	uint32_t pad;
-	uint64_t t_done;         /* device clock when its last write item finished */
-	uint64_t pad2;
+	uint64_t t_done;         /* device clock when its last item finished */
+	uint64_t total;          /* rows selected, when finished */
 } vg_bquery;
 
 typedef struct {

@@ -720,6 +720,8 @@ __global__ __launch_bounds__(256) void vg_count_kernel(const vg_prog P, const vg
 	const uint32_t g = blockIdx.x;
 	const uint32_t t0 = g * L.tiles_per_wg;
 	const uint32_t t1 = min(t0 + L.tiles_per_wg, L.num_tiles);
+	if (g == 0 && threadIdx.x == 0)
+		L.ctrl->overflow = 0;           /* the write pass that follows sets it */
 	const uint32_t n = count_tiles<NS>(P, L, t0, t1, lds);
 	if (threadIdx.x == 0)
 		L.wgcount[g] = n;
@@ -1020,6 +1022,17 @@ __global__ __launch_bounds__(256) void vg_batch_reset(const vg_batch_args A)
 		A.status[j] = 0;
 }
 
+/* Run by the workgroup finishing a query's last item: the query's row
+ * count and overflow bits into its batch state (one copy brings every
+ * query's back), and the time. */
+__device__ __forceinline__ void finish_query(const vg_batch_args &A, uint32_t q)
+{
+	vg_ctrl *c = ldc(&A.launch[q].ctrl);
+	A.q[q].total = __hip_atomic_load(&c->total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	A.q[q].flags = __hip_atomic_load(&c->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	A.q[q].t_done = wall_clock64();
+}
+
 /* the next item of a list, in every thread (query ~0u: none left) */
 __device__ __forceinline__ void next_item(const vg_item *items, uint32_t nitems, uint32_t *counter,
 	vg_item *s_item, uint32_t &q, uint32_t &t0, uint32_t &t1, uint32_t &k)
@@ -1106,8 +1119,9 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_batch_write_kernel(const 
 			break;
 		const uint32_t nitem = ldc(&A.info[q].nitem), ioff = ldc(&A.info[q].ioff);
 		write_tiles(A.progs[q], A.launch[q], t0, t1, A.ibase[ioff + k], lds);
-		if (threadIdx.x == 0 && atomicAdd(&A.q[q].wdone, 1u) + 1 == nitem)
-			A.q[q].t_done = wall_clock64();
+		if (threadIdx.x == 0 && __hip_atomic_fetch_add(&A.q[q].wdone, 1u, __ATOMIC_ACQ_REL,
+				__HIP_MEMORY_SCOPE_AGENT) + 1 == nitem)
+			finish_query(A, q);
 		/* Keep this barrier: without it the compiler merges the block
 		 * above with the next item's fetch (both run by thread 0 only)
 		 * across the loop's back edge, and the other threads then reach
@@ -1255,8 +1269,9 @@ __global__ __launch_bounds__(256) void vg_batch_single_kernel(const vg_batch_arg
 			}
 			__syncthreads();        /* registers, srcrow, s_wc, s_pre are reused */
 		}
-		if (tid == 0 && atomicAdd(&A.q[q].wdone, t1 - t0) + (t1 - t0) == ldc(&L.num_tiles))
-			A.q[q].t_done = wall_clock64();
+		if (tid == 0 && __hip_atomic_fetch_add(&A.q[q].wdone, t1 - t0, __ATOMIC_ACQ_REL,
+				__HIP_MEMORY_SCOPE_AGENT) + (t1 - t0) == ldc(&L.num_tiles))
+			finish_query(A, q);
 		__syncthreads();            /* see the write kernel: keeps the fetch in the loop */
 	}
 }

@@ -108,12 +108,14 @@ class Server:
         single = 1 if mode == "single" else 0
         self.runners = [PersistentBatch([], tiles_per_item=tiles_per_item, single=single, direct=1, db=db)
                         for _ in range(runners)]
+        # results that did not fit run again here, in two passes at their exact size
+        self.rerun_runner = PersistentBatch([], tiles_per_item=tiles_per_item, single=0, direct=1, db=db)
         self.max_batch = max_batch
         self.keep = keep_results
         self.peak_bytes = 0
 
     def close(self) -> None:
-        for r in self.runners:
+        for r in self.runners + [self.rerun_runner]:
             r.close()
         self.arena.close()
 
@@ -125,10 +127,10 @@ class Server:
         pred = self.net.forward(np.concatenate(feats)) if self.net is not None else None
         caps, k = [], 0
         for r, f in zip(reqs, feats):
-            rows = self.stats[PR.table_of(r.ops)].rows
-            p = pred[k:k + len(f)] if pred is not None else PR.structural_estimate(r.ops, self.stats[PR.table_of(r.ops)])
+            st = self.stats[PR.table_of(r.ops)]
+            p = pred[k:k + len(f)] if pred is not None else f[:, 3]
             k += len(f)
-            caps.append(PR.capacity(p, rows, self.margin))
+            caps.append(PR.capacity(p, st.rows, self.margin))
         return caps
 
     def _admit(self, waiting: deque) -> list:
@@ -140,53 +142,58 @@ class Server:
             r.cap = c
         batch = []
         for r in cand:
+            widths = _widths(r.ops)
             offs = []
-            for w in _widths(r.ops):
+            for w in widths:
                 off = self.arena.alloc(r.cap * w + 256)
                 if off is None:
                     break
                 offs.append((off, r.cap * w + 256))
-            if len(offs) < len(_widths(r.ops)):
+            if len(offs) < len(widths):
                 for off, n in offs:
                     self.arena.release(off, n)
                 break                       # admission is in arrival order
-            q = self.db.prepare(r.ops, capacity=r.cap, flags=VG_QUERY_LIGHT | VG_QUERY_NO_OUTPUT)
-            q.set_outputs([self.arena.base + off for off, _ in offs], r.cap)
+            q = self.db.prepare(r.ops, capacity=max(r.cap, 1), flags=VG_QUERY_LIGHT | VG_QUERY_NO_OUTPUT)
+            q.set_outputs([self.arena.base + off for off, _ in offs], max(r.cap, 1))
             waiting.popleft()
             batch.append((r, q, offs))
         self.peak_bytes = max(self.peak_bytes, self.arena.used)
         return batch
 
     # -- completion ------------------------------------------------------
-    def _finish(self, runner, batch, t0: float) -> int:
+    def _release(self, q, offs) -> None:
+        for off, nb in offs:
+            self.arena.release(off, nb)
+        q.close()
+
+    def _finish(self, runner, batch, t0: float, reruns: deque) -> int:
+        """Collect a finished batch; returns the requests completed."""
         rows, _, flags = runner.collect()
+        done = 0
         for (r, q, offs), n, f in zip(batch, rows, flags):
             if f:
-                # did not fit: two passes with the exact size the single pass counted
-                q2 = self.db.prepare(r.ops, capacity=max(int(n), 1), flags=VG_QUERY_LIGHT)
-                try:
-                    q2.run()
-                    res = q2.fetch()
-                finally:
-                    q2.close()
+                # did not fit: again, in two passes, at the exact size counted
+                self._release(q, offs)
+                r.cap = int(n)
                 r.rerun = True
-            else:
-                res = q.fetch()
+                reruns.append(r)
+                continue
+            if self.keep:
+                r.result = q.fetch()
             r.ready = time.perf_counter() - t0
             r.rows = int(n)
-            if self.keep:
-                r.result = res
-            for off, nb in offs:
-                self.arena.release(off, nb)
-            q.close()
+            self._release(q, offs)
+            done += 1
         runner.reload([])
-        return len(batch)
+        return done
 
     def run(self, reqs: list[Request]) -> dict:
         """Serve the requests at their arrival times; returns a summary."""
         pending = deque(sorted(reqs, key=lambda r: r.arrival))
         waiting: deque = deque()
+        reruns: deque = deque()
         free = list(self.runners)
+        rerun_free = [self.rerun_runner]
         inflight = []
         done = 0
         t0 = time.perf_counter()
@@ -195,26 +202,28 @@ class Server:
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
             progressed = False
-            if waiting and free:
-                batch = self._admit(waiting)
-                if batch:
-                    runner = free.pop()
-                    runner.reload([q for _, q, _ in batch])
-                    runner.launch()
-                    inflight.append((runner, batch))
-                    progressed = True
-                elif not inflight:
-                    raise VirginianError("a request's output buffers do not fit in the arena")
+            # reruns first (they arrived earlier), then new requests
+            for queue, pool in ((reruns, rerun_free), (waiting, free)):
+                if queue and pool:
+                    batch = self._admit(queue)
+                    if batch:
+                        runner = pool.pop()
+                        runner.reload([q for _, q, _ in batch])
+                        runner.launch()
+                        inflight.append((runner, batch, pool))
+                        progressed = True
+                    elif not inflight:
+                        raise VirginianError("a request's output buffers do not fit in the arena")
             still = []
-            for runner, batch in inflight:
+            for runner, batch, pool in inflight:
                 if runner.done():
-                    done += self._finish(runner, batch, t0)
-                    free.append(runner)
+                    done += self._finish(runner, batch, t0, reruns)
+                    pool.append(runner)
                     progressed = True
                 else:
-                    still.append((runner, batch))
+                    still.append((runner, batch, pool))
             inflight = still
-            if not progressed and inflight and not waiting:
+            if not progressed:
                 time.sleep(0)
         end = max(r.ready for r in reqs) if reqs else 0.0
         lat = np.array([r.ready - r.arrival for r in reqs]) * 1e3
