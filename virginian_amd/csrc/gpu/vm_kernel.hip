@@ -616,7 +616,7 @@ __device__ __forceinline__ const char *cursor_base(const cursor &C, uint32_t k)
 }
 
 /* which columns load_cols loads */
-enum { VG_LOAD_FILTER = 0, VG_LOAD_WRITE = 1, VG_LOAD_ALL = 2 };
+enum { VG_LOAD_FILTER = 0, VG_LOAD_WRITE = 1, VG_LOAD_PROJECT = 2 };
 
 template <int NS, uint32_t TR>
 __device__ __forceinline__ void load_cols(const vg_prog &P, const cursor &C, uint32_t row0,
@@ -625,7 +625,7 @@ __device__ __forceinline__ void load_cols(const vg_prog &P, const cursor &C, uin
 	const bool aligned = C.aligned != 0;
 	for (uint32_t k = 0; k < ldc(&P.nload); ++k) {
 		const vg_col c = cursor_col(C, k);
-		if (mode == VG_LOAD_WRITE ? !c.wload : mode == VG_LOAD_FILTER && c.phase != VG_PHASE_FILTER)
+		if (mode == VG_LOAD_WRITE ? !c.wload : (c.phase == VG_PHASE_FILTER) != (mode == VG_LOAD_FILTER))
 			continue;
 		const char *base = cursor_base(C, k) + (size_t)row0 * c.width;
 		load_col<NS, TR>(c, base, S, nrows, aligned, regs);
@@ -1222,10 +1222,13 @@ __global__ __launch_bounds__(256) void vg_batch_single_kernel(const vg_batch_arg
 				for (int j = 0; j < 4; ++j)
 					live |= S.row(0) + j < nrows ? 1u << j : 0u;
 			}
+			/* the filter first: the tile's count goes out before any
+			 * projection work, so the tiles after it wait less */
 			rowstate RS;
 			rows_init(RS, live);
-			load_cols<1, T>(P, C, row0, S, nrows, VG_LOAD_ALL, regs);
-			run_uops<1, T>(P, 0, ldc(&P.nuop), ~0ull, regs, S, RS);
+			load_cols<1, T>(P, C, row0, S, nrows, VG_LOAD_FILTER, regs);
+			const uint32_t nf = ldc(&P.nfilter), nu = ldc(&P.nuop);
+			run_uops<1, T>(P, 0, nf, ~0ull, regs, S, RS);
 			const uint32_t mybits = RS.valid & 0xFu;
 			const uint32_t wc = wave_sum((uint32_t)__builtin_popcount(mybits));
 			if (lane == 0)
@@ -1242,7 +1245,13 @@ __global__ __launch_bounds__(256) void vg_batch_single_kernel(const vg_batch_arg
 			if (pre == ~0ull) {
 				if (tid == 0)
 					atomicOr(&ldc(&L.ctrl)->overflow, VG_OVF_WAIT);
-			} else {
+			} else if (wc != 0) {
+				/* the rest of the program, only where rows were selected:
+				 * validity no longer changes after micro-op nf */
+				if (nu > nf) {
+					load_cols<1, T>(P, C, row0, S, nrows, VG_LOAD_PROJECT, regs);
+					run_uops<1, T>(P, nf, nu, ~0ull, regs, S, RS);
+				}
 				uint32_t wave_off = 0;
 				for (uint32_t w = 0; w < wave; ++w)
 					wave_off += s_wc[w];
@@ -1264,9 +1273,9 @@ __global__ __launch_bounds__(256) void vg_batch_single_kernel(const vg_batch_arg
 				if (m < wc && lane == 0)
 					atomicOr(&ldc(&L.ctrl)->overflow, VG_OVF_CAPACITY);
 				write_rows(P, L, C, row0, regs, srcrow, at, m, lane);
-				if (tile + 1 == ldc(&L.num_tiles) && tid == 0)
-					ldc(&L.ctrl)->total = pre + n;
 			}
+			if (pre != ~0ull && tile + 1 == ldc(&L.num_tiles) && tid == 0)
+				ldc(&L.ctrl)->total = pre + n;
 			__syncthreads();        /* registers, srcrow, s_wc, s_pre are reused */
 		}
 		if (tid == 0 && __hip_atomic_fetch_add(&A.q[q].wdone, t1 - t0, __ATOMIC_ACQ_REL,

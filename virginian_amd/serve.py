@@ -27,7 +27,10 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import OP_ResultColumn, VG_QUERY_LIGHT, VG_QUERY_NO_OUTPUT, PersistentBatch, VirginianError
+import ctypes as C
+
+from . import (NP_TYPE, OP_ResultColumn, VG_QUERY_LIGHT, VG_QUERY_NO_OUTPUT, VIRG_SUCCESS, PersistentBatch,
+               VirginianError, lib, make_ops)
 from . import predict as PR
 
 
@@ -66,6 +69,9 @@ class Arena:
                 merged.append((o, s))
         self.free = merged
 
+    def largest(self) -> int:
+        return max((sz for _, sz in self.free), default=0)
+
     def close(self) -> None:
         if self.base:
             self.db.device_free(self.base)
@@ -82,6 +88,42 @@ class Request:
     rerun: bool = False
     cap: int = -1                   # output rows reserved (set at admission)
     result: dict = field(default_factory=dict)
+    wire: tuple = ()                # the op-program as submitted (virg_op array)
+
+    def __post_init__(self):
+        if not self.wire:
+            arr, n, keep = make_ops(self.ops)
+            names = [o[4] if isinstance(o[4], str) else f"c{k}"
+                     for k, o in enumerate([o for o in self.ops if o[0] == OP_ResultColumn])]
+            types = [o[1] for o in self.ops if o[0] == OP_ResultColumn]
+            self.wire = (arr, n, keep, names, types)
+
+
+class _Query:
+    """A served query: the handle of a light vg_query and what fetching
+    its result needs."""
+
+    def __init__(self, db, r: Request, cap: int, outs: list[int]):
+        arr, n, _, self.names, self.types = r.wire
+        err = C.create_string_buffer(256)
+        self.h = lib.vg_query_create(db.ptr, arr, n, cap, VG_QUERY_LIGHT | VG_QUERY_NO_OUTPUT, err, 256)
+        if not self.h:
+            raise VirginianError(err.value.decode() or "vg_query_create failed")
+        ptrs = (C.c_void_p * max(len(outs), 1))(*outs)
+        lib.vg_query_set_outputs(self.h, ptrs, cap)
+        self.last_rows = 0
+
+    def fetch(self) -> dict:
+        outs = [np.empty(self.last_rows, dtype=NP_TYPE[t]) for t in self.types]
+        ptrs = (C.c_void_p * max(len(outs), 1))(*[o.ctypes.data for o in outs])
+        if lib.vg_query_fetch(self.h, ptrs, self.last_rows) != VIRG_SUCCESS:
+            raise VirginianError("vg_query_fetch failed")
+        return dict(zip(self.names, outs))
+
+    def close(self) -> None:
+        if self.h:
+            lib.vg_query_free(self.h)
+            self.h = None
 
 
 # bytes per value of each virg_t (INT, INT64, FLOAT, DOUBLE, CHAR)
@@ -133,57 +175,55 @@ class Server:
             caps.append(PR.capacity(p, st.rows, self.margin))
         return caps
 
-    def _admit(self, waiting: deque) -> list:
-        """Queries for one batch, in arrival order while their buffers fit:
-        prepared, with arena buffers."""
+    def _admit(self, waiting: deque) -> tuple[list, tuple | None]:
+        """Queries for one batch, in arrival order while their buffers fit
+        in one free block of the arena: prepared, with their buffers."""
         cand = [waiting[i] for i in range(min(len(waiting), self.max_batch))]
         todo = [r for r in cand if r.cap < 0]
         for r, c in zip(todo, self._capacities(todo) if todo else []):
             r.cap = c
-        batch = []
+        room = self.arena.largest()
+        take, total = [], 0
         for r in cand:
-            widths = _widths(r.ops)
-            offs = []
-            for w in widths:
-                off = self.arena.alloc(r.cap * w + 256)
-                if off is None:
-                    break
-                offs.append((off, r.cap * w + 256))
-            if len(offs) < len(widths):
-                for off, n in offs:
-                    self.arena.release(off, n)
+            need = sum((max(r.cap, 1) * w + 255) & ~255 for w in _widths(r.ops))
+            if total + need > room:
                 break                       # admission is in arrival order
-            q = self.db.prepare(r.ops, capacity=max(r.cap, 1), flags=VG_QUERY_LIGHT | VG_QUERY_NO_OUTPUT)
-            q.set_outputs([self.arena.base + off for off, _ in offs], max(r.cap, 1))
+            take.append((r, need))
+            total += need
+        if not take:
+            return [], None
+        base = self.arena.alloc(total)
+        batch, off = [], base
+        for r, need in take:
+            outs = []
+            for w in _widths(r.ops):
+                outs.append(self.arena.base + off)
+                off += (max(r.cap, 1) * w + 255) & ~255
+            batch.append((r, _Query(self.db, r, max(r.cap, 1), outs)))
             waiting.popleft()
-            batch.append((r, q, offs))
         self.peak_bytes = max(self.peak_bytes, self.arena.used)
-        return batch
+        return batch, (base, total)
 
     # -- completion ------------------------------------------------------
-    def _release(self, q, offs) -> None:
-        for off, nb in offs:
-            self.arena.release(off, nb)
-        q.close()
-
-    def _finish(self, runner, batch, t0: float, reruns: deque) -> int:
+    def _finish(self, runner, batch, block, t0: float, reruns: deque) -> int:
         """Collect a finished batch; returns the requests completed."""
         rows, _, flags = runner.collect()
         done = 0
-        for (r, q, offs), n, f in zip(batch, rows, flags):
+        for (r, q), n, f in zip(batch, rows, flags):
             if f:
                 # did not fit: again, in two passes, at the exact size counted
-                self._release(q, offs)
                 r.cap = int(n)
                 r.rerun = True
                 reruns.append(r)
-                continue
-            if self.keep:
-                r.result = q.fetch()
-            r.ready = time.perf_counter() - t0
-            r.rows = int(n)
-            self._release(q, offs)
-            done += 1
+            else:
+                q.last_rows = int(n)
+                if self.keep:
+                    r.result = q.fetch()
+                r.ready = time.perf_counter() - t0
+                r.rows = int(n)
+                done += 1
+            q.close()
+        self.arena.release(*block)
         runner.reload([])
         return done
 
@@ -205,23 +245,23 @@ class Server:
             # reruns first (they arrived earlier), then new requests
             for queue, pool in ((reruns, rerun_free), (waiting, free)):
                 if queue and pool:
-                    batch = self._admit(queue)
+                    batch, block = self._admit(queue)
                     if batch:
                         runner = pool.pop()
-                        runner.reload([q for _, q, _ in batch])
+                        runner.reload([q for _, q in batch])
                         runner.launch()
-                        inflight.append((runner, batch, pool))
+                        inflight.append((runner, batch, block, pool))
                         progressed = True
                     elif not inflight:
                         raise VirginianError("a request's output buffers do not fit in the arena")
             still = []
-            for runner, batch, pool in inflight:
+            for runner, batch, block, pool in inflight:
                 if runner.done():
-                    done += self._finish(runner, batch, t0, reruns)
+                    done += self._finish(runner, batch, block, t0, reruns)
                     pool.append(runner)
                     progressed = True
                 else:
-                    still.append((runner, batch, pool))
+                    still.append((runner, batch, block, pool))
             inflight = still
             if not progressed:
                 time.sleep(0)
