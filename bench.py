@@ -356,7 +356,7 @@ def bench_served(a) -> int:
         print(f"# {mode}: {summ['requests']} requests in {summ['seconds']:.3f}s, "
               f"{summ['queries_per_s']:.0f} q/s, p50 {summ['p50_ms']:.2f} ms, p99 {summ['p99_ms']:.2f} ms, "
               f"reruns {summ['reruns']}, peak arena {summ['peak_arena_bytes'] / 2**20:.0f} MiB, "
-              f"{summ['gbs']:.1f} GB/s", file=sys.stderr, flush=True)
+              f"{summ['gbs']:.1f} GB/s; host {summ['host_seconds']}", file=sys.stderr, flush=True)
     net.close()
     same = out["single"]["rows"] == out["two-pass"]["rows"]
     s1, s2 = out["single"], out["two-pass"]

@@ -30,6 +30,7 @@
  * so dispatch is a scalar branch.
  */
 #include <hip/hip_runtime.h>
+#include <unordered_map>
 #include <stdint.h>
 #include "vg_prog.h"
 
@@ -1341,27 +1342,38 @@ static void allow_lds()
 
 /* Workgroups per pass: every CU filled at the pass's occupancy, never
  * more than there are tiles. */
+/* workgroups of a pass kernel (0 count wide, 1 count, 2 write) that fit
+ * on the device with `lds` bytes each, asked once per (kernel, lds) */
+static uint32_t pass_fit(int which, size_t lds)
+{
+	static std::unordered_map<uint64_t, uint32_t> memo;
+	const uint64_t key = ((uint64_t)which << 32) | (uint64_t)lds;
+	auto it = memo.find(key);
+	if (it != memo.end())
+		return it->second;
+	allow_lds();
+	int per = 0;
+	const hipError_t e = which == 0
+		? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, vg_count_kernel<4>, 256, lds)
+		: which == 1
+		? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, vg_count_kernel<1>, 256, lds)
+		: hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, vg_write_kernel, VG_WRITE_THREADS, lds);
+	if (e != hipSuccess || per < 1)
+		per = 1;
+	const uint32_t g = (uint32_t)num_cus() * (uint32_t)per;
+	memo[key] = g;
+	return g;
+}
+
 static uint32_t count_grid(const vg_prog *P, uint32_t num_tiles)
 {
-	allow_lds();
-	int ct = 0;
-	const hipError_t e = count_wide(P)
-		? hipOccupancyMaxActiveBlocksPerMultiprocessor(&ct, vg_count_kernel<4>, 256, count_lds_bytes(P))
-		: hipOccupancyMaxActiveBlocksPerMultiprocessor(&ct, vg_count_kernel<1>, 256, count_lds_bytes(P));
-	if (e != hipSuccess || ct < 1)
-		ct = 1;
-	const uint32_t g = (uint32_t)num_cus() * (uint32_t)ct;
+	const uint32_t g = pass_fit(count_wide(P) ? 0 : 1, count_lds_bytes(P));
 	return g < num_tiles ? g : num_tiles;
 }
 
 static uint32_t write_grid(const vg_prog *P, uint32_t num_tiles)
 {
-	allow_lds();
-	int wr = 0;
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&wr, vg_write_kernel, VG_WRITE_THREADS,
-			write_lds_bytes(P)) != hipSuccess || wr < 1)
-		wr = 1;
-	const uint32_t g = (uint32_t)num_cus() * (uint32_t)wr;
+	const uint32_t g = pass_fit(2, write_lds_bytes(P));
 	return g < num_tiles ? g : num_tiles;
 }
 
@@ -1442,6 +1454,12 @@ static void batch_attrs()
  * fit on the device at once with `lds` bytes of dynamic LDS each. */
 extern "C" uint32_t vg_kernel_batch_grid(int pass, size_t lds)
 {
+	/* the occupancy query is slow next to a batch build: remember answers */
+	static std::unordered_map<uint64_t, uint32_t> memo;
+	const uint64_t key = ((uint64_t)pass << 32) | (uint64_t)lds;
+	auto it = memo.find(key);
+	if (it != memo.end())
+		return it->second;
 	batch_attrs();
 	int per = 0;
 	const hipError_t e = pass == 0
@@ -1451,7 +1469,9 @@ extern "C" uint32_t vg_kernel_batch_grid(int pass, size_t lds)
 		: hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, vg_batch_single_kernel, 256, lds);
 	if (e != hipSuccess || per < 1)
 		per = 1;
-	return (uint32_t)num_cus() * (uint32_t)per;
+	const uint32_t g = (uint32_t)num_cus() * (uint32_t)per;
+	memo[key] = g;
+	return g;
 }
 
 /* Clear the state of every query and of A->nhead item-list heads. */

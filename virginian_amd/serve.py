@@ -155,6 +155,8 @@ class Server:
         self.max_batch = max_batch
         self.keep = keep_results
         self.peak_bytes = 0
+        # host seconds spent per step of the loop, and batches launched
+        self.prof = {"admit": 0.0, "launch": 0.0, "collect": 0.0, "idle": 0.0, "batches": 0}
 
     def close(self) -> None:
         for r in self.runners + [self.rerun_runner]:
@@ -243,31 +245,42 @@ class Server:
                 waiting.append(pending.popleft())
             progressed = False
             # reruns first (they arrived earlier), then new requests
+            prof = self.prof
             for queue, pool in ((reruns, rerun_free), (waiting, free)):
                 if queue and pool:
+                    ta = time.perf_counter()
                     batch, block = self._admit(queue)
+                    tb = time.perf_counter()
+                    prof["admit"] += tb - ta
                     if batch:
                         runner = pool.pop()
                         runner.reload([q for _, q in batch])
                         runner.launch()
                         inflight.append((runner, batch, block, pool))
                         progressed = True
+                        prof["launch"] += time.perf_counter() - tb
+                        prof["batches"] += 1
                     elif not inflight:
                         raise VirginianError("a request's output buffers do not fit in the arena")
             still = []
             for runner, batch, block, pool in inflight:
                 if runner.done():
+                    ta = time.perf_counter()
                     done += self._finish(runner, batch, block, t0, reruns)
+                    prof["collect"] += time.perf_counter() - ta
                     pool.append(runner)
                     progressed = True
                 else:
                     still.append((runner, batch, block, pool))
             inflight = still
             if not progressed:
+                ta = time.perf_counter()
                 time.sleep(0)
+                prof["idle"] += time.perf_counter() - ta
         end = max(r.ready for r in reqs) if reqs else 0.0
         lat = np.array([r.ready - r.arrival for r in reqs]) * 1e3
         return {"requests": len(reqs), "seconds": end, "queries_per_s": len(reqs) / end if end else 0.0,
                 "p50_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
                 "p99_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0,
-                "reruns": sum(r.rerun for r in reqs), "peak_arena_bytes": self.peak_bytes}
+                "reruns": sum(r.rerun for r in reqs), "peak_arena_bytes": self.peak_bytes,
+                "host_seconds": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in self.prof.items()}}
