@@ -1592,6 +1592,10 @@ struct vg_pbatch {
 	vg_item *d_items = nullptr, *d_witems = nullptr;
 	uint32_t *d_icount = nullptr;
 	uint64_t *d_ibase = nullptr;
+	uint64_t *d_stamp = nullptr;            /* finish time of each write item */
+	uint32_t *d_gq = nullptr;               /* query ids, group by group */
+	size_t gq_cap = 0;
+	pinned_buf h_gq;
 	size_t items_cap = 0;
 	uint64_t *d_status = nullptr;           /* single pass: a word per tile */
 	size_t status_cap = 0;
@@ -1624,7 +1628,7 @@ extern "C" void vg_pbatch_free(vg_pbatch *b)
 		(void)hipStreamSynchronize(b->stream);
 	pbatch_drop_graph(b);
 	void *bufs[] = { b->d_progs, b->d_launch, b->d_info, b->d_q, b->d_head, b->d_items, b->d_witems,
-		b->d_icount, b->d_ibase, b->d_status };
+		b->d_icount, b->d_ibase, b->d_status, b->d_stamp, b->d_gq };
 	for (void *p : bufs)
 		if (p)
 			(void)hipFree(p);
@@ -1633,7 +1637,7 @@ extern "C" void vg_pbatch_free(vg_pbatch *b)
 	for (hipStream_t s : b->branch)
 		(void)hipStreamDestroy(s);
 	pinned_buf *pins[] = { &b->h_progs, &b->h_launch, &b->h_info, &b->h_items, &b->h_witems, &b->h_q,
-		&b->h_head };
+		&b->h_head, &b->h_gq };
 	for (pinned_buf *pb : pins)
 		pb->release();
 	if (b->fork)
@@ -1710,7 +1714,7 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		info[i].nitem = (nt + tpi - 1) / tpi;
 		info[i].ioff = ioff;
 		info[i].soff = soff;
-		info[i].pad = 0;
+		info[i].wstart = 0;
 		ioff += info[i].nitem;
 		soff += nt;
 	}
@@ -1745,8 +1749,10 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		goff.push_back(items.size());
 		for (unsigned q : by_size)
 			add(items, q);
-		for (unsigned q : worder)
+		for (unsigned q : worder) {
+			info[q].wstart = (uint32_t)witems.size();
 			add(witems, q);
+		}
 		for (unsigned q : g.qids) {
 			const size_t c = vg_kernel_batch_lds(0, &b->qs[q]->P), w = vg_kernel_batch_lds(1, &b->qs[q]->P);
 			g.clds = c > g.clds ? c : g.clds;
@@ -1763,14 +1769,17 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		if (b->d_witems) (void)hipFree(b->d_witems);
 		if (b->d_icount) (void)hipFree(b->d_icount);
 		if (b->d_ibase) (void)hipFree(b->d_ibase);
+		if (b->d_stamp) (void)hipFree(b->d_stamp);
 		b->d_items = b->d_witems = nullptr;
 		b->d_icount = nullptr;
 		b->d_ibase = nullptr;
+		b->d_stamp = nullptr;
 		b->items_cap = 0;
 		if (herr(hipMalloc((void **)&b->d_items, want * sizeof(vg_item)), "batch items") ||
 			herr(hipMalloc((void **)&b->d_witems, want * sizeof(vg_item)), "batch items") ||
 			herr(hipMalloc((void **)&b->d_icount, want * sizeof(uint32_t)), "batch counts") ||
-			herr(hipMalloc((void **)&b->d_ibase, want * sizeof(uint64_t)), "batch offsets"))
+			herr(hipMalloc((void **)&b->d_ibase, want * sizeof(uint64_t)), "batch offsets") ||
+			herr(hipMalloc((void **)&b->d_stamp, want * sizeof(uint64_t)), "batch stamps"))
 			return qerr(err, errlen, "cannot allocate the batch's item lists");
 		b->items_cap = want;
 	}
@@ -1793,6 +1802,26 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 			return qerr(err, errlen, "cannot create the batch's events");
 		b->join.push_back(e);
 	}
+	if (n > b->gq_cap) {
+		const size_t want = n > 2 * b->gq_cap ? n : 2 * b->gq_cap;
+		if (b->d_gq) (void)hipFree(b->d_gq);
+		b->d_gq = nullptr;
+		b->gq_cap = 0;
+		if (herr(hipMalloc((void **)&b->d_gq, want * sizeof(uint32_t)), "batch groups"))
+			return qerr(err, errlen, "cannot allocate the batch's groups");
+		b->gq_cap = want;
+	}
+	{
+		uint32_t *hg = (uint32_t *)b->h_gq.get(n * sizeof(uint32_t));
+		if (!hg)
+			return qerr(err, errlen, "cannot allocate pinned staging");
+		size_t k = 0;
+		for (pgroup &g : b->groups)
+			for (unsigned q : g.qids)
+				hg[k++] = q;
+		if (herr(hipMemcpyAsync(b->d_gq, hg, n * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream), "groups"))
+			return qerr(err, errlen, "cannot upload the batch's groups");
+	}
 	{
 		void *hi = b->h_items.get(ni * sizeof(vg_item)), *hw = b->h_witems.get(ni * sizeof(vg_item));
 		void *hf = b->h_info.get(n * sizeof(vg_bqinfo));
@@ -1812,6 +1841,7 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		if (!b->cfg.direct && herr(hipStreamSynchronize(b->stream), "batch items"))
 			return qerr(err, errlen, "cannot upload the batch's item lists");
 	}
+	size_t gq_off = 0;
 	for (size_t gi = 0; gi < ng; gi++) {
 		pgroup &g = b->groups[gi];
 		vg_batch_args &A = g.A;
@@ -1826,6 +1856,10 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		A.q = b->d_q;
 		A.icount = b->d_icount;
 		A.ibase = b->d_ibase;
+		A.stamp = b->d_stamp;
+		A.gq = b->d_gq + gq_off;
+		A.ngq = (uint32_t)g.qids.size();
+		gq_off += g.qids.size();
 		A.head = b->d_head + gi;
 		A.nhead = 1;
 		A.spin_limit = VG_SPIN_LIMIT;

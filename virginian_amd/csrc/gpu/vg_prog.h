@@ -148,12 +148,12 @@ typedef struct {
 	void *out[VG_MAX_OUT];
 } vg_launch;
 
-/* ---- batched execution: two persistent launches run many queries ----
+/* ---- batched execution: persistent launches run many queries ----
  * Work items are (query, tile range).  Every query's tiles are cut into
  * items of at most tiles_per_item tiles; count item k and write item k of
  * a query cover the same tiles.  The count launch works through the count
- * list and the write launch through the write list (any order each), every
- * workgroup taking the next item from an atomic counter. */
+ * list and the write launch through the write list (any order each; a
+ * query's write items are consecutive). */
 typedef struct {
 	uint32_t query;
 	uint32_t t0, t1;         /* tiles [t0, t1) of the query's launch */
@@ -164,7 +164,7 @@ typedef struct {
 	uint32_t nitem;          /* items of the query in each pass */
 	uint32_t ioff;           /* its first entry of icount / ibase */
 	uint32_t soff;           /* its first tile status word (single pass) */
-	uint32_t pad;
+	uint32_t wstart;         /* its first item in the write list */
 } vg_bqinfo;
 
 /* Single pass: one status word per tile, published by the workgroup that
@@ -210,6 +210,10 @@ typedef struct {
 	uint32_t spin_limit;     /* single pass: polls of a predecessor before giving up */
 	uint64_t *status;        /* single pass: tile status words of every query */
 	uint64_t nstatus;
+	uint64_t *stamp;         /* device clock when each write item finished */
+	const uint32_t *gq;      /* the queries of this launch's group */
+	uint32_t ngq;
+	uint32_t pad2;
 } vg_batch_args;
 
 #endif

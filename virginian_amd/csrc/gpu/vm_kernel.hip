@@ -994,13 +994,16 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_pro
 }
 
 /* ------------------------------------------------------ batched queries */
-/* Two persistent launches run the count and write items of many queries
- * (each with its own program, tablets and outputs).  In each, a workgroup
- * takes the next item of its list from an atomic counter until the list
- * is empty.  A count item counts its tiles like the count pass (4 waves
- * share a tile); the last count item of a query to finish turns its items'
- * counts into output offsets.  After the count launch, a write item
- * writes its tiles like the write pass, from its item's offset. */
+/* Persistent launches run the items of many queries (each with its own
+ * program, tablets and outputs): a count launch (4 waves share a tile,
+ * as the count pass), a scan launch turning each query's item counts into
+ * output offsets, a write launch (as the write pass) and a finish launch
+ * gathering each query's overflow bits and finish time.  Within a launch
+ * no workgroup waits for another, so items are dealt out statically:
+ * workgroup g takes items g, g + grid, ...  (A single atomic work counter
+ * serialises device-wide, about 75 ns a grab, which cost more than the
+ * balance it bought.)  The single-pass launch further down does wait on
+ * other workgroups and takes its items from a counter in list order. */
 
 /* clear the per-query state and the item counters; stamp the start */
 __global__ __launch_bounds__(256) void vg_batch_reset(const vg_batch_args A)
@@ -1058,77 +1061,88 @@ __device__ __forceinline__ void next_item(const vg_item *items, uint32_t nitems,
 	__syncthreads();                /* s_item is rewritten by the next fetch */
 }
 
+/* Count items, statically: workgroup g takes items g, g + grid, ...  Each
+ * item's count goes to icount; nothing waits on another workgroup. */
 __global__ __launch_bounds__(256) void vg_batch_count_kernel(const vg_batch_args A)
 {
 	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
-	__shared__ vg_item s_item;
-	__shared__ uint32_t s_last;
-	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
-	for (;;) {
-		uint32_t q, t0, t1, k;
-		next_item(A.items, A.nitems, &A.head->next, &s_item, q, t0, t1, k);
-		if (q == ~0u)
-			break;
-		const vg_prog &P = A.progs[q];
-		const vg_launch &L = A.launch[q];
-		const uint32_t nitem = ldc(&A.info[q].nitem), ioff = ldc(&A.info[q].ioff);
-		const uint32_t n = count_tiles<1>(P, L, t0, t1, lds);
-		if (tid == 0)
-			A.icount[ioff + k] = n;
-		/* the item's count is visible device-wide before the query's
-		 * counter moves (every thread's validity bytes and tile counts
-		 * only need to be visible after this launch) */
-		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-		__syncthreads();
-		if (tid == 0)
-			s_last = __hip_atomic_fetch_add(&A.q[q].done, 1u, __ATOMIC_ACQ_REL,
-				__HIP_MEMORY_SCOPE_AGENT) + 1 == nitem;
-		__syncthreads();
-		if (uni(s_last) && wave == 0) {
-			/* last count item of the query: exclusive scan of its items'
-			 * counts into their output offsets, and the query's total */
-			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-			uint64_t carry = 0;
-			for (uint32_t c = 0; c < nitem; c += 64) {
-				const uint32_t v = c + lane < nitem ? A.icount[ioff + c + lane] : 0u;
-				uint64_t incl = v;
-#pragma unroll
-				for (int off = 1; off < 64; off <<= 1) {
-					const uint64_t y = __shfl_up(incl, off, 64);
-					if (lane >= (uint32_t)off)
-						incl += y;
-				}
-				if (c + lane < nitem)
-					A.ibase[ioff + c + lane] = carry + incl - v;
-				carry += __shfl(incl, 63, 64);
-			}
-			if (lane == 0)
-				ldc(&L.ctrl)->total = carry;
-		}
-		__syncthreads();            /* s_last is rewritten by the next item */
+	for (uint32_t i = blockIdx.x; i < A.nitems; i += gridDim.x) {
+		const u32x4 it = ldc((const u32x4 *)(A.items + i));
+		const uint32_t q = it.x, t0 = it.y, t1 = it.z, k = it.w;
+		const uint32_t n = count_tiles<1>(A.progs[q], A.launch[q], t0, t1, lds);
+		if (threadIdx.x == 0)
+			A.icount[ldc(&A.info[q].ioff) + k] = n;
 	}
 }
 
+/* Per query of the launch's group (one wave each): exclusive scan of its
+ * items' counts into output offsets, and its row count. */
+__global__ __launch_bounds__(64) void vg_batch_scan_kernel(const vg_batch_args A)
+{
+	const uint32_t lane = threadIdx.x;
+	if (blockIdx.x >= A.ngq)
+		return;
+	const uint32_t q = ldc(&A.gq[blockIdx.x]);
+	const uint32_t nitem = ldc(&A.info[q].nitem), ioff = ldc(&A.info[q].ioff);
+	uint64_t carry = 0;
+	for (uint32_t c = 0; c < nitem; c += 64) {
+		const uint32_t v = c + lane < nitem ? A.icount[ioff + c + lane] : 0u;
+		uint64_t incl = v;
+#pragma unroll
+		for (int off = 1; off < 64; off <<= 1) {
+			const uint64_t y = __shfl_up(incl, off, 64);
+			if (lane >= (uint32_t)off)
+				incl += y;
+		}
+		if (c + lane < nitem)
+			A.ibase[ioff + c + lane] = carry + incl - v;
+		carry += __shfl(incl, 63, 64);
+	}
+	if (lane == 0) {
+		A.q[q].total = carry;
+		ldc(&A.launch[q].ctrl)->total = carry;
+	}
+}
+
+/* Write items, statically like the count items; each stamps its finish
+ * time (device clock) for the per-query latency. */
 __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_batch_write_kernel(const vg_batch_args A)
 {
 	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
-	__shared__ vg_item s_item;
-	for (;;) {
-		uint32_t q, t0, t1, k;
-		next_item(A.witems, A.nitems, &A.head->wnext, &s_item, q, t0, t1, k);
-		if (q == ~0u)
-			break;
-		const uint32_t nitem = ldc(&A.info[q].nitem), ioff = ldc(&A.info[q].ioff);
-		write_tiles(A.progs[q], A.launch[q], t0, t1, A.ibase[ioff + k], lds);
-		if (threadIdx.x == 0 && __hip_atomic_fetch_add(&A.q[q].wdone, 1u, __ATOMIC_ACQ_REL,
-				__HIP_MEMORY_SCOPE_AGENT) + 1 == nitem)
-			finish_query(A, q);
-		/* Keep this barrier: without it the compiler merges the block
-		 * above with the next item's fetch (both run by thread 0 only)
-		 * across the loop's back edge, and the other threads then reach
-		 * the fetch barrier ahead of the fetch and reread the old item,
-		 * forever. */
+	for (uint32_t i = blockIdx.x; i < A.nitems; i += gridDim.x) {
+		const u32x4 it = ldc((const u32x4 *)(A.witems + i));
+		const uint32_t q = it.x, t0 = it.y, t1 = it.z, k = it.w;
+		write_tiles(A.progs[q], A.launch[q], t0, t1, A.ibase[ldc(&A.info[q].ioff) + k], lds);
+		if (threadIdx.x == 0)
+			A.stamp[i] = wall_clock64();
+		/* Keep this barrier: without one between an item's thread-0
+		 * block and the next item, the compiler may merge the two across
+		 * the loop's back edge. */
 		__syncthreads();
+	}
+}
+
+/* Per query of the group (one wave each): overflow bits from its control
+ * block and its finish time, the latest stamp of its write items (which
+ * are consecutive in the write list from info.wstart). */
+__global__ __launch_bounds__(64) void vg_batch_finish_kernel(const vg_batch_args A)
+{
+	const uint32_t lane = threadIdx.x;
+	if (blockIdx.x >= A.ngq)
+		return;
+	const uint32_t q = ldc(&A.gq[blockIdx.x]);
+	const uint32_t nitem = ldc(&A.info[q].nitem), w0 = ldc(&A.info[q].wstart);
+	uint64_t t = 0;
+	for (uint32_t c = lane; c < nitem; c += 64)
+		t = max(t, A.stamp[w0 + c]);
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		const uint64_t y = __shfl_xor(t, off, 64);
+		t = t > y ? t : y;
+	}
+	if (lane == 0) {
+		A.q[q].flags = ldc(&A.launch[q].ctrl)->overflow;
+		A.q[q].t_done = t;
 	}
 }
 
@@ -1494,7 +1508,13 @@ extern "C" int vg_kernel_batch_passes(const vg_batch_args *A, uint32_t cgrid, si
 	hipLaunchKernelGGL(vg_batch_count_kernel, dim3(cgrid), dim3(256), clds, stream, *A);
 	if (hipGetLastError() != hipSuccess)
 		return -2;
+	hipLaunchKernelGGL(vg_batch_scan_kernel, dim3(A->ngq ? A->ngq : 1), dim3(64), 0, stream, *A);
+	if (hipGetLastError() != hipSuccess)
+		return -2;
 	hipLaunchKernelGGL(vg_batch_write_kernel, dim3(wgrid), dim3(VG_WRITE_THREADS), wlds, stream, *A);
+	if (hipGetLastError() != hipSuccess)
+		return -2;
+	hipLaunchKernelGGL(vg_batch_finish_kernel, dim3(A->ngq ? A->ngq : 1), dim3(64), 0, stream, *A);
 	return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
