@@ -1750,7 +1750,8 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		for (unsigned q : by_size)
 			add(items, q);
 		for (unsigned q : worder) {
-			info[q].wstart = (uint32_t)witems.size();
+			/* within the group's list: stamps are indexed the same way */
+			info[q].wstart = (uint32_t)(witems.size() - goff.back());
 			add(witems, q);
 		}
 		for (unsigned q : g.qids) {
@@ -1856,7 +1857,7 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		A.q = b->d_q;
 		A.icount = b->d_icount;
 		A.ibase = b->d_ibase;
-		A.stamp = b->d_stamp;
+		A.stamp = b->d_stamp + goff[gi];
 		A.gq = b->d_gq + gq_off;
 		A.ngq = (uint32_t)g.qids.size();
 		gq_off += g.qids.size();

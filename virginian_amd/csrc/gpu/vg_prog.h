@@ -164,7 +164,7 @@ typedef struct {
 	uint32_t nitem;          /* items of the query in each pass */
 	uint32_t ioff;           /* its first entry of icount / ibase */
 	uint32_t soff;           /* its first tile status word (single pass) */
-	uint32_t wstart;         /* its first item in the write list */
+	uint32_t wstart;         /* its first item in its group's write list */
 } vg_bqinfo;
 
 /* Single pass: one status word per tile, published by the workgroup that
@@ -210,7 +210,7 @@ typedef struct {
 	uint32_t spin_limit;     /* single pass: polls of a predecessor before giving up */
 	uint64_t *status;        /* single pass: tile status words of every query */
 	uint64_t nstatus;
-	uint64_t *stamp;         /* device clock when each write item finished */
+	uint64_t *stamp;         /* device clock when each of the group's write items finished */
 	const uint32_t *gq;      /* the queries of this launch's group */
 	uint32_t ngq;
 	uint32_t pad2;
