@@ -278,8 +278,9 @@ def bench_served(a) -> int:
     HBM arena; results complete in HBM.  The predictor (3-layer MLP, 32
     hidden, f32) is trained on the GPU on per-tablet counts from two-pass
     runs of other programs, its margin calibrated on held-out programs for
-    at most 2 % reruns.  Reported beside the same stream served in two
-    passes with buffers for every table row (the method without it)."""
+    at most 2 % reruns.  value = the predictor-sized two-pass server;
+    reported beside the predictor-sized single pass and the same stream
+    served with buffers for every table row (the method without it)."""
     import numpy as np
     import virginian_amd as va
     from virginian_amd import predict as PR
@@ -340,8 +341,8 @@ def bench_served(a) -> int:
     stream = [(float(ar), ops) for ar, ops in zip(arrivals, programs(n_req))]
     rows_of = {t: int(stats[t].rows.sum()) for t in tids}
     out = {}
-    for mode in ("single", "two-pass"):
-        srv = S.Server(db, tids, a.arena_gib << 30, mode=mode, predictor=net if mode == "single" else None,
+    for mode in ("sized", "single", "two-pass"):
+        srv = S.Server(db, tids, a.arena_gib << 30, mode=mode, predictor=net if mode != "two-pass" else None,
                        margin=margin, runners=3, max_batch=64, keep_results=False)
         reqs = [S.Request(arrival=ar, ops=ops, rid=i) for i, (ar, ops) in enumerate(stream)]
         try:
@@ -358,10 +359,10 @@ def bench_served(a) -> int:
               f"reruns {summ['reruns']}, peak arena {summ['peak_arena_bytes'] / 2**20:.0f} MiB, "
               f"{summ['gbs']:.1f} GB/s; host {summ['host_seconds']}", file=sys.stderr, flush=True)
     net.close()
-    same = out["single"]["rows"] == out["two-pass"]["rows"]
-    s1, s2 = out["single"], out["two-pass"]
+    same = out["single"]["rows"] == out["two-pass"]["rows"] == out["sized"]["rows"]
+    s1, s2, s0 = out["sized"], out["two-pass"], out["single"]
     line = {
-        "metric": "GB/s scanned by served filter SELECT queries (learned single pass vs two passes)",
+        "metric": "GB/s scanned by served filter SELECT queries (learned sizing vs buffers for every row)",
         "value": round(s1["gbs"], 2), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": round(s1["seconds"] * 1e3, 2), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+i32", "data": "synthetic (seeded Philox generator, seeded programs)",
@@ -371,10 +372,12 @@ def bench_served(a) -> int:
                    "predictor": {"examples": int(len(x)), "train_s": round(t2 - t1, 3), "loss": round(loss, 6),
                                  "heldout_mae": round(mae, 4), "independence_estimate_mae": round(mae_est, 4),
                                  "mean_selectivity_mae": round(mae_const, 4), "margin": margin},
-                   "single_pass": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s1.items()
-                                   if k != "rows"},
-                   "two_pass": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s2.items()
-                                if k != "rows"},
+                   "sized_two_pass": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s1.items()
+                                      if k != "rows"},
+                   "sized_single_pass": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s0.items()
+                                         if k != "rows"},
+                   "two_pass_every_row": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s2.items()
+                                          if k != "rows"},
                    "rerun_rate": round(s1["reruns"] / n_req, 4),
                    "gain_vs_two_pass": {"throughput": round(s1["queries_per_s"] / s2["queries_per_s"], 3),
                                         "p99": round(s2["p99_ms"] / s1["p99_ms"], 3),

@@ -1303,7 +1303,9 @@ __global__ __launch_bounds__(256) void vg_batch_single_kernel(const vg_batch_arg
 /* --------------------------------------------------------------- launch */
 /* the count pass takes a whole tile per wave when the filter's register
  * file is this small */
+#ifndef VG_COUNT_WIDE_SLOTS
 #define VG_COUNT_WIDE_SLOTS 2u
+#endif
 
 static bool count_wide(const vg_prog *P)
 {

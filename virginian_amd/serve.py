@@ -7,8 +7,8 @@ Each step of the loop (one host thread):
      arrival order as long as their output buffers fit in the HBM arena
      (admission by footprint); each is prepared against its table's shared
      tablet descriptors and given buffers sized by the result-size
-     predictor (single pass) or for every row of its table (two passes,
-     the method without a predictor); the admitted queries go to the GPU
+     predictor (modes "single" and "sized") or for every row of its table
+     (mode "two-pass", the method without a predictor); the admitted queries go to the GPU
      as one batch on the runner's stream, while earlier batches are still
      running on theirs, so their work overlaps on the device;
   3. every finished batch is collected: results are copied to the host
@@ -135,8 +135,10 @@ def _widths(ops: list) -> list[int]:
 
 
 class Server:
-    """mode "single": predictor-sized single pass; "two-pass": buffers for
-    every row of the table, count and write launches."""
+    """mode "single": predictor-sized buffers, single pass; "sized":
+    predictor-sized buffers, count and write launches (the write pass flags
+    what does not fit); "two-pass": buffers for every row of the table,
+    count and write launches (the method without a predictor)."""
 
     def __init__(self, db, table_ids: list[int], arena_bytes: int, mode: str = "single", predictor=None,
                  margin: float = 0.05, runners: int = 3, max_batch: int = 64, tiles_per_item: int = 4,
@@ -165,7 +167,7 @@ class Server:
 
     # -- admission -------------------------------------------------------
     def _capacities(self, reqs: list[Request]) -> list[int]:
-        if self.mode != "single":
+        if self.mode == "two-pass":
             return [int(self.stats[PR.table_of(r.ops)].rows.sum()) for r in reqs]
         feats = [PR.program_features(r.ops, self.stats[PR.table_of(r.ops)]) for r in reqs]
         pred = self.net.forward(np.concatenate(feats)) if self.net is not None else None
