@@ -341,7 +341,7 @@ def bench_served(a) -> int:
     stream = [(float(ar), ops) for ar, ops in zip(arrivals, programs(n_req))]
     rows_of = {t: int(stats[t].rows.sum()) for t in tids}
     out = {}
-    for mode in ("sized", "single", "two-pass"):
+    for mode in ("sized", "single", "two-pass", "plain"):
         srv = S.Server(db, tids, a.arena_gib << 30, mode=mode, predictor=net if mode != "two-pass" else None,
                        margin=margin, runners=3, max_batch=64, keep_results=False)
         reqs = [S.Request(arrival=ar, ops=ops, rid=i) for i, (ar, ops) in enumerate(stream)]
@@ -359,7 +359,7 @@ def bench_served(a) -> int:
               f"reruns {summ['reruns']}, peak arena {summ['peak_arena_bytes'] / 2**20:.0f} MiB, "
               f"{summ['gbs']:.1f} GB/s; host {summ['host_seconds']}", file=sys.stderr, flush=True)
     net.close()
-    same = out["single"]["rows"] == out["two-pass"]["rows"] == out["sized"]["rows"]
+    same = out["single"]["rows"] == out["two-pass"]["rows"] == out["sized"]["rows"] == out["plain"]["rows"]
     s1, s2, s0 = out["sized"], out["two-pass"], out["single"]
     line = {
         "metric": "GB/s scanned by served filter SELECT queries (learned sizing vs buffers for every row)",
@@ -378,7 +378,11 @@ def bench_served(a) -> int:
                                          if k != "rows"},
                    "two_pass_every_row": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s2.items()
                                           if k != "rows"},
+                   "plain_unbatched": {k: (round(v, 4) if isinstance(v, float) else v)
+                                       for k, v in out["plain"].items() if k != "rows"},
                    "rerun_rate": round(s1["reruns"] / n_req, 4),
+                   "batched_vs_plain": {"throughput": round(s2["queries_per_s"] / out["plain"]["queries_per_s"], 3),
+                                        "p99": round(out["plain"]["p99_ms"] / s2["p99_ms"], 3)},
                    "gain_vs_two_pass": {"throughput": round(s1["queries_per_s"] / s2["queries_per_s"], 3),
                                         "p99": round(s2["p99_ms"] / s1["p99_ms"], 3),
                                         "peak_arena": round(s2["peak_arena_bytes"] / max(s1["peak_arena_bytes"], 1), 3)},

@@ -44,7 +44,8 @@ def _check(done, want, what):
         assert_same(r.result, w, f"request {r.rid} {what}")
 
 
-@pytest.mark.parametrize("mode,margin", [("single", 0.0), ("single", 0.2), ("sized", 0.0), ("two-pass", 0.0)])
+@pytest.mark.parametrize("mode,margin", [("single", 0.0), ("single", 0.2), ("sized", 0.0), ("two-pass", 0.0),
+                                         ("plain", 0.0)])
 def test_served_results_equal_alone(setup, mode, margin):
     db, tids, reqs, want = setup
     srv = S.Server(db, tids, 256 << 20, mode=mode, predictor=None, margin=margin, runners=3, max_batch=16)

@@ -138,7 +138,8 @@ class Server:
     """mode "single": predictor-sized buffers, single pass; "sized":
     predictor-sized buffers, count and write launches (the write pass flags
     what does not fit); "two-pass": buffers for every row of the table,
-    count and write launches (the method without a predictor)."""
+    count and write launches (the method without a predictor); "plain":
+    no batching, each request run alone as it arrives."""
 
     def __init__(self, db, table_ids: list[int], arena_bytes: int, mode: str = "single", predictor=None,
                  margin: float = 0.05, runners: int = 3, max_batch: int = 64, tiles_per_item: int = 4,
@@ -231,8 +232,46 @@ class Server:
         runner.reload([])
         return done
 
+    def _run_plain(self, reqs: list[Request]) -> None:
+        """The plain way: each request, in arrival order, prepared and run
+        on its own (count and write launches, buffers for every row of its
+        table), waited for, then the next."""
+        t0 = time.perf_counter()
+        for r in sorted(reqs, key=lambda r: r.arrival):
+            while time.perf_counter() - t0 < r.arrival:
+                pass
+            cap = int(self.stats[PR.table_of(r.ops)].rows.sum())
+            need = sum((max(cap, 1) * w + 255) & ~255 for w in _widths(r.ops))
+            base = self.arena.alloc(need)
+            if base is None:
+                raise VirginianError("a request's output buffers do not fit in the arena")
+            outs, off = [], base
+            for w in _widths(r.ops):
+                outs.append(self.arena.base + off)
+                off += (max(cap, 1) * w + 255) & ~255
+            q = _Query(self.db, r, max(cap, 1), outs)
+            n = C.c_ulonglong(0)
+            if lib.vg_query_run(q.h, C.byref(n)) != VIRG_SUCCESS:
+                raise VirginianError("vg_query_run failed")
+            q.last_rows = n.value
+            if self.keep:
+                r.result = q.fetch()
+            r.ready = time.perf_counter() - t0
+            r.rows = n.value
+            q.close()
+            self.arena.release(base, need)
+            self.peak_bytes = max(self.peak_bytes, need)
+
     def run(self, reqs: list[Request]) -> dict:
         """Serve the requests at their arrival times; returns a summary."""
+        if self.mode == "plain":
+            self._run_plain(reqs)
+            end = max(r.ready for r in reqs) if reqs else 0.0
+            lat = np.array([r.ready - r.arrival for r in reqs]) * 1e3
+            return {"requests": len(reqs), "seconds": end, "queries_per_s": len(reqs) / end if end else 0.0,
+                    "p50_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
+                    "p99_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0,
+                    "reruns": 0, "peak_arena_bytes": self.peak_bytes, "host_seconds": {}}
         pending = deque(sorted(reqs, key=lambda r: r.arrival))
         waiting: deque = deque()
         reruns: deque = deque()
