@@ -849,8 +849,8 @@ __device__ __forceinline__ uint64_t write_tiles(const vg_prog &P, const vg_launc
 	constexpr uint32_t T = VG_T;
 	__shared__ uint32_t s_cnt[VG_CHUNK][4];
 	__shared__ uint64_t s_off[VG_CHUNK];
-	uint32_t *regs = lds;
 	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
+	uint32_t *regs = lds;
 	uint16_t *srcrow = (uint16_t *)(lds + ldc(&P.nslot) * T) + wave * 256 * VG_WS;
 	const strips<VG_WS> S{wave * 256u * VG_WS, 256u, lane};
 	uint64_t chunk_base = base;
@@ -970,14 +970,25 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_pro
 	const uint32_t t0 = g * L.wtiles_per_wg;
 	const uint32_t t1 = min(t0 + L.wtiles_per_wg, L.num_tiles);
 
-	if (wave == 0) {
+	{
 		/* offset of this workgroup: the rows selected before tile t0 --
-		 * whole count-pass workgroups, then the tiles of the one t0 is in */
+		 * whole count-pass workgroups, then the tiles of the one t0 is
+		 * in; all 256 threads add, eight independent loads in flight each */
+		__shared__ uint64_t s_part[4];
 		const uint32_t cw = t0 / L.tiles_per_wg;
 		uint64_t base = 0;
-		for (uint32_t k = lane; k < cw; k += 64)
-			base += L.wgcount[k];
-		for (uint32_t t = cw * L.tiles_per_wg + lane; t < t0; t += 64) {
+		for (uint32_t k0 = 0; k0 < cw; k0 += 8 * 256) {
+			uint32_t v[8];
+#pragma unroll
+			for (int j = 0; j < 8; ++j) {
+				const uint32_t k = k0 + j * 256 + tid;
+				v[j] = k < cw ? L.wgcount[k] : 0u;
+			}
+#pragma unroll
+			for (int j = 0; j < 8; ++j)
+				base += v[j];
+		}
+		for (uint32_t t = cw * L.tiles_per_wg + tid; t < t0; t += 256) {
 			const u32x4 v = *(const u32x4 *)(L.tcount + (size_t)t * 4);
 			base += (uint64_t)v.x + v.y + v.z + v.w;
 		}
@@ -985,7 +996,10 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_pro
 		for (int off = 32; off > 0; off >>= 1)
 			base += __shfl_xor(base, off, 64);
 		if (lane == 0)
-			s_base = base;
+			s_part[wave] = base;
+		__syncthreads();
+		if (tid == 0)
+			s_base = s_part[0] + s_part[1] + s_part[2] + s_part[3];
 	}
 	__syncthreads();
 	const uint64_t end = write_tiles(P, L, t0, t1, s_base, lds);
