@@ -26,6 +26,16 @@
 
 enum { RC_NONE = 0, RC_CONST, RC_COLUMN, RC_SLOT };
 
+/* The last micro-op after which a slot may be given to another value.
+ * Built with VG_DECODE_PRIVATE_SLOTS, no slot is ever given twice: every
+ * value keeps a slot of its own, the layout the tests check the reusing
+ * one against. */
+#ifdef VG_DECODE_PRIVATE_SLOTS
+#define HELD_UNTIL(k) (VG_MAX_UOPS + 1)
+#else
+#define HELD_UNTIL(k) (k)
+#endif
+
 typedef struct {
 	int cls;
 	int ndefs;
@@ -447,7 +457,7 @@ int vg_decode(const virg_op *stmt, unsigned num_ops, unsigned pc,
 		R[r].slot = got[0];
 		R[r].slot_hi = need == 2 ? got[1] : got[0];
 		for (int j = 0; j < ng; j++)
-			slot_free_at[got[j]] = R[r].last_use_uop;
+			slot_free_at[got[j]] = HELD_UNTIL(R[r].last_use_uop);
 	}
 
 	/* slots the count pass touches: filter columns and the values the
