@@ -27,11 +27,25 @@ extern "C" {
 #include "virginian.h"
 }
 
+/* device scratch for one batch of B examples */
+struct mlp_work {
+	float *x = nullptr, *t = nullptr, *h1 = nullptr, *h2 = nullptr, *y = nullptr;
+	float *d1 = nullptr, *d2 = nullptr, *d3 = nullptr, *loss = nullptr;
+	unsigned cap = 0;
+};
+
 struct vg_mlp {
 	unsigned nin = 0, nh = 0, np = 0;
 	float skip = 0.0f;           /* output pre-activation += skip * x[0] */
 	float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
 	unsigned long long step = 0;
+	/* inference: a stream that does not wait for other streams' work,
+	 * device scratch and pinned staging kept between calls (grown, never
+	 * shrunk), so a forward pass neither allocates nor waits for the
+	 * query kernels running beside it */
+	hipStream_t s = nullptr;
+	mlp_work inf;
+	float *hx = nullptr, *hy = nullptr;
 };
 
 /* offsets of the parameter blocks */
@@ -154,12 +168,6 @@ static unsigned blocks(size_t n)
 	return (unsigned)((n + 255) / 256);
 }
 
-/* device scratch for one batch of B examples */
-struct mlp_work {
-	float *x = nullptr, *t = nullptr, *h1 = nullptr, *h2 = nullptr, *y = nullptr;
-	float *d1 = nullptr, *d2 = nullptr, *d3 = nullptr, *loss = nullptr;
-	unsigned cap = 0;
-};
 
 static void free_work(mlp_work &w)
 {
@@ -187,15 +195,15 @@ static int alloc_work(mlp_work &w, unsigned B, unsigned nin, unsigned h)
 }
 
 /* forward over w.x (B examples) into w.h1, w.h2, w.y */
-static void forward(const vg_mlp *m, mlp_work &w, unsigned B, const float *x)
+static void forward(const vg_mlp *m, mlp_work &w, unsigned B, const float *x, hipStream_t st = 0)
 {
 	const mlp_layout L = layout(m->nin, m->nh);
 	const float *P = m->d_p;
-	hipLaunchKernelGGL(mlp_layer_fwd, dim3(blocks((size_t)B * m->nh)), dim3(256), 0, 0,
+	hipLaunchKernelGGL(mlp_layer_fwd, dim3(blocks((size_t)B * m->nh)), dim3(256), 0, st,
 		x, m->nin, P + L.w1, P + L.b1, w.h1, m->nh, B, 0, x, m->nin, 0.0f);
-	hipLaunchKernelGGL(mlp_layer_fwd, dim3(blocks((size_t)B * m->nh)), dim3(256), 0, 0,
+	hipLaunchKernelGGL(mlp_layer_fwd, dim3(blocks((size_t)B * m->nh)), dim3(256), 0, st,
 		w.h1, m->nh, P + L.w2, P + L.b2, w.h2, m->nh, B, 0, x, m->nin, 0.0f);
-	hipLaunchKernelGGL(mlp_layer_fwd, dim3(blocks(B)), dim3(256), 0, 0,
+	hipLaunchKernelGGL(mlp_layer_fwd, dim3(blocks(B)), dim3(256), 0, st,
 		w.h2, m->nh, P + L.w3, P + L.b3, w.y, 1u, B, 1, x, m->nin, m->skip);
 }
 
@@ -233,10 +241,25 @@ static float unit(uint64_t &s)
 	return (float)((mix(s) >> 40) * (1.0 / 16777216.0));
 }
 
+static void free_staging(vg_mlp *m)
+{
+	free_work(m->inf);
+	if (m->hx)
+		(void)hipHostFree(m->hx);
+	if (m->hy)
+		(void)hipHostFree(m->hy);
+	m->hx = m->hy = nullptr;
+}
+
 extern "C" void vg_mlp_free(vg_mlp *m)
 {
 	if (!m)
 		return;
+	if (m->s)
+		(void)hipStreamSynchronize(m->s);
+	free_staging(m);
+	if (m->s)
+		(void)hipStreamDestroy(m->s);
 	float *bufs[] = { m->d_p, m->d_g, m->d_m, m->d_v };
 	for (float *p : bufs)
 		if (p)
@@ -259,6 +282,12 @@ extern "C" vg_mlp *vg_mlp_create(unsigned nin, unsigned nh, float skip, unsigned
 	m->skip = skip;
 	const mlp_layout L = layout(nin, nh);
 	m->np = (unsigned)L.n;
+	if (hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking) != hipSuccess) {
+		(void)hipGetLastError();
+		m->s = nullptr;
+		vg_mlp_free(m);
+		return nullptr;
+	}
 	float **bufs[] = { &m->d_p, &m->d_g, &m->d_m, &m->d_v };
 	for (float **b : bufs)
 		if (hipMalloc((void **)b, L.n * sizeof(float)) != hipSuccess ||
@@ -312,17 +341,37 @@ extern "C" int vg_mlp_forward(vg_mlp *m, const float *x, unsigned n, float *y)
 {
 	if (n == 0)
 		return VIRG_SUCCESS;
-	mlp_work w;
-	if (alloc_work(w, n, m->nin, m->nh) == VIRG_FAIL)
-		return VIRG_FAIL;
-	int ok = hipMemcpy(w.x, x, (size_t)n * m->nin * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
-	if (ok) {
-		forward(m, w, n, w.x);
-		ok = hipGetLastError() == hipSuccess &&
-			hipMemcpy(y, w.y, n * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
+	if (n > m->inf.cap) {
+		/* grow (rarely: the first call, or a larger batch than before) */
+		unsigned cap = 4096;
+		while (cap < n)
+			cap *= 2;
+		(void)hipStreamSynchronize(m->s);
+		free_staging(m);
+		if (alloc_work(m->inf, cap, m->nin, m->nh) == VIRG_FAIL ||
+			hipHostMalloc((void **)&m->hx, (size_t)cap * m->nin * sizeof(float), hipHostMallocDefault) != hipSuccess ||
+			hipHostMalloc((void **)&m->hy, (size_t)cap * sizeof(float), hipHostMallocDefault) != hipSuccess) {
+			(void)hipGetLastError();
+			free_staging(m);
+			return VIRG_FAIL;
+		}
 	}
-	free_work(w);
-	return ok ? VIRG_SUCCESS : VIRG_FAIL;
+	mlp_work &w = m->inf;
+	memcpy(m->hx, x, (size_t)n * m->nin * sizeof(float));
+	int ok = hipMemcpyAsync(w.x, m->hx, (size_t)n * m->nin * sizeof(float), hipMemcpyHostToDevice, m->s) ==
+		hipSuccess;
+	if (ok) {
+		forward(m, w, n, w.x, m->s);
+		ok = hipGetLastError() == hipSuccess &&
+			hipMemcpyAsync(m->hy, w.y, n * sizeof(float), hipMemcpyDeviceToHost, m->s) == hipSuccess &&
+			hipStreamSynchronize(m->s) == hipSuccess;
+	}
+	if (!ok) {
+		(void)hipGetLastError();
+		return VIRG_FAIL;
+	}
+	memcpy(y, m->hy, n * sizeof(float));
+	return VIRG_SUCCESS;
 }
 
 /* Loss and gradient (parameter order as above) over n examples. */

@@ -158,6 +158,7 @@ class Server:
         self.max_batch = max_batch
         self.keep = keep_results
         self.peak_bytes = 0
+        self.reserved = 0                   # result bytes reserved, over all admissions
         # host seconds spent per step of the loop, and batches launched
         self.prof = {"admit": 0.0, "launch": 0.0, "collect": 0.0, "idle": 0.0, "batches": 0}
 
@@ -195,6 +196,7 @@ class Server:
                 break                       # admission is in arrival order
             take.append((r, need))
             total += need
+            self.reserved += need
         if not take:
             return [], None
         base = self.arena.alloc(total)
@@ -261,6 +263,7 @@ class Server:
             q.close()
             self.arena.release(base, need)
             self.peak_bytes = max(self.peak_bytes, need)
+            self.reserved += need
 
     def run(self, reqs: list[Request]) -> dict:
         """Serve the requests at their arrival times; returns a summary."""
@@ -271,7 +274,8 @@ class Server:
             return {"requests": len(reqs), "seconds": end, "queries_per_s": len(reqs) / end if end else 0.0,
                     "p50_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
                     "p99_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0,
-                    "reruns": 0, "peak_arena_bytes": self.peak_bytes, "host_seconds": {}}
+                    "reruns": 0, "peak_arena_bytes": self.peak_bytes, "reserved_bytes": self.reserved,
+                    "host_seconds": {}}
         pending = deque(sorted(reqs, key=lambda r: r.arrival))
         waiting: deque = deque()
         reruns: deque = deque()
@@ -324,4 +328,5 @@ class Server:
                 "p50_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
                 "p99_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0,
                 "reruns": sum(r.rerun for r in reqs), "peak_arena_bytes": self.peak_bytes,
+                "reserved_bytes": self.reserved,
                 "host_seconds": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in self.prof.items()}}
