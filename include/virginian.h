@@ -241,7 +241,17 @@ static const size_t virg_testsizes[15] = {
 
 /* ---- slot manager / tablets / tables ---------------------------------- */
 int virg_db_alloc(virginian *v, virg_tablet_meta **meta, int id);
+/* Database files in the reference's format (reference: src/virginian.h:572-576,
+ * src/db/create.c, open.c, close.c, clear.c, write.c).  virg_init leaves an
+ * empty database in memory; virg_db_create(v, file) starts one backed by a
+ * new file, virg_db_open(v, file) opens an existing one (before any tablet
+ * is in memory), virg_db_close writes every tablet in memory and the meta
+ * information.  Tablets must fit a disk slot (VIRG_TABLET_SIZE). */
 int virg_db_create(virginian *v, const char *file);
+int virg_db_open(virginian *v, const char *file);
+int virg_db_close(virginian *v);
+int virg_db_clear(virginian *v, unsigned slot);
+int virg_db_write(virginian *v, unsigned slot);
 int virg_db_load(virginian *v, unsigned tablet_id, virg_tablet_meta **tab);
 int virg_db_loadnext(virginian *v, virg_tablet_meta **tab);
 int virg_db_findslot(virginian *v, unsigned *slot);

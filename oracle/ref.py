@@ -30,6 +30,10 @@ def lib() -> C.CDLL:
         _lib = C.CDLL(str(PATH))
         _lib.ref_open.argtypes = [C.c_char_p, C.c_int, C.c_int]
         _lib.ref_open.restype = C.c_int
+        _lib.ref_open_file.argtypes = [C.c_char_p, C.c_int, C.c_int]
+        _lib.ref_open_file.restype = C.c_int
+        _lib.ref_db_close.argtypes = []
+        _lib.ref_db_close.restype = C.c_int
         _lib.ref_close.argtypes = []
         _lib.ref_table.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_char_p),
                                    C.POINTER(C.c_int)]
@@ -46,16 +50,33 @@ def lib() -> C.CDLL:
 class RefDB:
     """One reference database (virg_init .. virg_close minus CUDA)."""
 
-    def __init__(self, multi: bool = False, threads: int = 4):
-        self.dir = tempfile.TemporaryDirectory(prefix="virgref")
-        path = os.path.join(self.dir.name, "ref.db").encode()
-        if lib().ref_open(path, 1 if multi else 0, threads) != 1:
-            raise RuntimeError("ref_open failed")
+    def __init__(self, multi: bool = False, threads: int = 4, path: str | None = None,
+                 existing: bool = False):
+        """A new database (in a temporary file unless `path` is given), or
+        with existing=True the database file at `path`."""
+        self.dir = None
+        if path is None:
+            self.dir = tempfile.TemporaryDirectory(prefix="virgref")
+            path = os.path.join(self.dir.name, "ref.db")
+        self.path = str(path)
+        opener = lib().ref_open_file if existing else lib().ref_open
+        if opener(self.path.encode(), 1 if multi else 0, threads) != 1:
+            raise RuntimeError(f"reference could not open {self.path}")
         self.rows = 0
 
     def close(self) -> None:
+        """Free the database without writing the file."""
         lib().ref_close()
-        self.dir.cleanup()
+        if self.dir is not None:
+            self.dir.cleanup()
+
+    def write_close(self) -> None:
+        """Write the database file (the reference's virg_db_close) and free."""
+        ok = lib().ref_db_close()
+        if self.dir is not None:
+            self.dir.cleanup()
+        if ok != 1:
+            raise RuntimeError("reference virg_db_close failed")
 
     def table(self, name: str, key: np.ndarray, cols: list[tuple[str, np.ndarray]]) -> int:
         """Create a table and insert every row through virg_table_insert."""

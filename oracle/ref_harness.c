@@ -32,7 +32,7 @@ int virg_db_close(virginian *v);
 
 static virginian *V;
 
-API int ref_open(const char *dbfile, int use_multi, int threads)
+static int ref_init(int use_multi, int threads)
 {
 	int i;
 	V = calloc(1, sizeof(virginian));
@@ -55,8 +55,25 @@ API int ref_open(const char *dbfile, int use_multi, int threads)
 		if (!V->tablet_slots[i])
 			return VIRG_FAIL;
 	}
+	return VIRG_SUCCESS;
+}
+
+/* A new database backed by a new file (the reference's virg_db_create) */
+API int ref_open(const char *dbfile, int use_multi, int threads)
+{
+	if (ref_init(use_multi, threads) != VIRG_SUCCESS)
+		return VIRG_FAIL;
 	unlink(dbfile);
 	return virg_db_create(V, dbfile);
+}
+
+/* An existing database file (the reference's virg_db_open); tablets are
+ * read from it on first use by virg_db_load */
+API int ref_open_file(const char *dbfile, int use_multi, int threads)
+{
+	if (ref_init(use_multi, threads) != VIRG_SUCCESS)
+		return VIRG_FAIL;
+	return virg_db_open(V, dbfile);
 }
 
 API void ref_close(void)
@@ -72,6 +89,19 @@ API void ref_close(void)
 	pthread_mutex_destroy(&V->slot_lock);
 	free(V);
 	V = NULL;
+}
+
+/* Close as src/close.c:20 does: virg_db_close writes every tablet in a slot
+ * and the meta information to the file; then free what ref_init took. */
+API int ref_db_close(void)
+{
+	int ok;
+	if (!V)
+		return VIRG_FAIL;
+	ok = virg_db_close(V);
+	V->db.tablet_info = NULL;   /* freed by virg_db_close */
+	ref_close();
+	return ok;
 }
 
 API int ref_table(const char *name, int key_type, int ncols, const char *const *names,

@@ -1,9 +1,13 @@
 /* A program written against the reference's public API (src/virginian.h),
  * built against include/virginian.h and linked with libvirginian_amd.so.
  * Prints the row counts of the reference's nine condition queries
- * (src/test/sql.cc:15-37); with no HIP device it prints "no-device". */
+ * (src/test/sql.cc:15-37); with no HIP device it prints "no-device".  The
+ * database is backed by the file named on the command line, created afresh
+ * as the reference's test fixture does (src/test/test.cc:15-17) and written
+ * by virg_close. */
 #include <stdio.h>
 #include <stdlib.h>
+#include <unistd.h>
 #include "virginian.h"
 #include "virg_gpu.h"
 
@@ -19,12 +23,15 @@ static const char *queries[9] = {
 	"select col0 from test where col0 < 5 and (col0 = 2 or col0 = 8 or col0 = 1)",
 };
 
-int main(void)
+int main(int argc, char **argv)
 {
+	const char *dbfile = argc > 1 ? argv[1] : "testdb";
 	virginian v;
 	if (virg_init(&v) != VIRG_SUCCESS)
 		return 1;
-	virg_db_create(&v, "unused");
+	unlink(dbfile);
+	if (virg_db_create(&v, dbfile) != VIRG_SUCCESS)
+		return 4;
 	virg_table_create(&v, "test", VIRG_INT);
 	virg_table_addcolumn(&v, 0, "col0", VIRG_INT);
 	virg_table_addcolumn(&v, 0, "col1", VIRG_INT);
@@ -48,8 +55,7 @@ int main(void)
 	int cus = 0;
 	if (vg_device_info(&v, name, sizeof name, &cus) != VIRG_SUCCESS) {
 		printf("no-device\n");
-		virg_close(&v);
-		return 0;
+		return virg_close(&v) == VIRG_SUCCESS ? 0 : 5;
 	}
 	v.use_gpu = 1;
 	for (int k = 0; k < 9; k++) {
@@ -61,6 +67,5 @@ int main(void)
 		printf("q%d %u\n", k, rows);
 		virg_release(&v, r);
 	}
-	virg_close(&v);
-	return 0;
+	return virg_close(&v) == VIRG_SUCCESS ? 0 : 5;
 }

@@ -24,16 +24,21 @@ def _build(tmp_path: Path) -> Path:
 
 
 def test_c_program_builds_and_runs_without_gpu(tmp_path):
-    out = subprocess.run([str(_build(tmp_path))], capture_output=True, text=True, timeout=120)
+    dbfile = tmp_path / "test.db"
+    out = subprocess.run([str(_build(tmp_path)), str(dbfile)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     lines = out.stdout.split("\n")
     assert lines[0] == "rows 10"
     assert lines[1].startswith("ops ")
+    # virg_close wrote the database: the meta block, the entry table, one tablet
+    sizes = va.struct_sizes()           # virg_gpu_getsizes order: [6] virg_tablet_info, [11] virg_db
+    assert dbfile.stat().st_size > sizes[11] + 16 * sizes[6]
 
 
 @pytest.mark.gpu
 def test_c_program_queries(tmp_path):
-    out = subprocess.run([str(_build(tmp_path))], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([str(_build(tmp_path)), str(tmp_path / "test.db")], capture_output=True, text=True,
+                         timeout=120)
     assert out.returncode == 0, out.stderr
     got = [int(l.split()[1]) for l in out.stdout.split("\n") if l.startswith("q")]
     assert got == [3, 2, 1, 1, 1, 1, 1, 3, 2]

@@ -149,6 +149,9 @@ def _load() -> C.CDLL:
     P, V = C.c_void_p, C.POINTER(State)
     sig = {
         "virg_init": (C.c_int, [V]), "virg_close": (C.c_int, [V]),
+        "virg_db_create": (C.c_int, [V, C.c_char_p]), "virg_db_open": (C.c_int, [V, C.c_char_p]),
+        "virg_db_close": (C.c_int, [V]), "virg_db_clear": (C.c_int, [V, C.c_uint]),
+        "virg_db_write": (C.c_int, [V, C.c_uint]),
         "virg_gpu_getsizes": (C.POINTER(C.c_size_t), []),
         "virg_table_create": (C.c_int, [V, C.c_char_p, C.c_int]),
         "virg_table_addcolumn": (C.c_int, [V, C.c_uint, C.c_char_p, C.c_int]),
@@ -264,8 +267,28 @@ class Virginian:
 
     def close(self) -> None:
         if self.ptr is not None:
-            lib.virg_close(self.ptr)
+            ok = lib.virg_close(self.ptr)
             self.ptr = None
+            if ok != VIRG_SUCCESS:
+                raise VirginianError("virg_close failed (the database file was not written)")
+
+    # ---- database files (the reference's format) ----------------------------
+    def create_db(self, path: str) -> None:
+        """Back this (still empty) database with a new file (virg_db_create)."""
+        if lib.virg_db_create(self.ptr, str(path).encode()) != VIRG_SUCCESS:
+            raise VirginianError(f"virg_db_create failed for {path}")
+
+    def open_db(self, path: str) -> None:
+        """Open a database file, before any table is in memory (virg_db_open)."""
+        if lib.virg_db_open(self.ptr, str(path).encode()) != VIRG_SUCCESS:
+            raise VirginianError(f"virg_db_open failed for {path}")
+
+    def close_db(self) -> None:
+        """Write every tablet in memory and the meta information to the file
+        and close it (virg_db_close); the instance then holds an empty
+        database in memory."""
+        if lib.virg_db_close(self.ptr) != VIRG_SUCCESS:
+            raise VirginianError("virg_db_close failed")
 
     def __enter__(self):
         return self

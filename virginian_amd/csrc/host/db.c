@@ -5,9 +5,9 @@
  * 1 when it holds an unlocked tablet and 1+k with k locks
  * (reference: src/db/findslot.c:19-63, src/tablet/lock.c:20-98,
  * src/db/load.c:22-98, src/db/loadnext.c:21-33, src/db/alloc.c:18-42).
- * Differences: tablets never leave memory (no eviction, no database file),
- * slot buffers are sized per tablet, and slots beyond VIRG_MEM_TABLETS come
- * from an extension table.
+ * Differences: tablets never leave memory unless a database file takes
+ * them (no eviction; see dbfile.c), slot buffers are sized per tablet, and
+ * slots beyond VIRG_MEM_TABLETS come from an extension table.
  */
 #include "internal.h"
 #include <stdlib.h>
@@ -168,19 +168,24 @@ int virg_slot_of(virginian *v, unsigned id)
 	return -1;
 }
 
-/* reference: src/db/load.c:22.  Adds a lock to an already resident tablet;
- * with no paging a tablet that is not resident does not exist. */
+/* reference: src/db/load.c:22.  Adds a lock to a resident tablet, or reads
+ * it from the database file into a slot (with one lock). */
 int virg_db_load(virginian *v, unsigned tablet_id, virg_tablet_meta **tab)
 {
 	pthread_mutex_lock(&v->slot_lock);
 	int i = virg_slot_of(v, tablet_id);
+	int ok = VIRG_SUCCESS;
 	if (i >= 0) {
 		if (tab != NULL)
 			tab[0] = *virg_slot_ptr(v, (unsigned)i);
 		(*virg_slot_status(v, (unsigned)i))++;
+	} else if (v->dbfd >= 0) {
+		ok = virg_dbfile_load(v, tablet_id, tab);
+	} else {
+		ok = VIRG_FAIL;
 	}
 	pthread_mutex_unlock(&v->slot_lock);
-	VIRG_CHECK(i < 0, "Could not find tablet id");
+	VIRG_CHECK(ok == VIRG_FAIL, "Could not find tablet id");
 	return VIRG_SUCCESS;
 }
 
@@ -190,29 +195,6 @@ int virg_db_loadnext(virginian *v, virg_tablet_meta **tab)
 	virg_tablet_meta *t = tab[0];
 	VIRG_CHECK(virg_db_load(v, t->next, tab) == VIRG_FAIL, "Could not load next tablet");
 	virg_tablet_unlock(v, t->id);
-	return VIRG_SUCCESS;
-}
-
-/* reference: src/db/create.c:20 (the file argument is accepted and ignored:
- * this database lives in memory) */
-int virg_db_create(virginian *v, const char *file)
-{
-	(void)file;
-	virg_db *db = &v->db;
-	db->num_tablets = 0;
-	db->tablet_id_counter = 0;
-	for (unsigned i = 0; i < VIRG_MAX_TABLES; i++) {
-		db->tables[i][0] = '\0';
-		db->table_status[i] = 0;
-		db->table_tablets[i] = 0;
-		db->write_cursor[i] = 0;
-		db->first_tablet[i] = 0;
-		db->last_tablet[i] = 0;
-	}
-	db->alloced_tablets = 0;
-	db->tablet_info = NULL;
-	db->block_size = sizeof(virg_db);
-	v->dbfd = -1;
 	return VIRG_SUCCESS;
 }
 
@@ -244,17 +226,29 @@ int virg_tablet_unlock(virginian *v, unsigned tablet_id)
 	return VIRG_SUCCESS;
 }
 
-/* reference: src/tablet/remove.c:18.  The slot keeps its buffer for reuse. */
+/* reference: src/tablet/remove.c:18.  The slot keeps its buffer for reuse;
+ * the tablet's entry in the database file, if any, is freed. */
 int virg_tablet_remove(virginian *v, unsigned id)
 {
 	pthread_mutex_lock(&v->slot_lock);
 	int i = virg_slot_of(v, id);
-	if (i >= 0) {
+	int found = i >= 0;
+	if (found) {
+		virg_tablet_meta *t = *virg_slot_ptr(v, (unsigned)i);
+		if (t->info != NULL)
+			t->info->used = 0;
 		*virg_slot_status(v, (unsigned)i) = 0;
 		v->tablet_slots_taken--;
+	} else {
+		virg_db *db = &v->db;
+		for (unsigned k = 0; k < db->alloced_tablets && !found; k++)
+			if (db->tablet_info[k].used == 1 && db->tablet_info[k].id == id) {
+				db->tablet_info[k].used = 0;
+				found = 1;
+			}
 	}
 	pthread_mutex_unlock(&v->slot_lock);
-	VIRG_CHECK(i < 0, "Could not find tablet to remove");
+	VIRG_CHECK(!found, "Could not find tablet to remove");
 	vgd_forget_tablet(v, id);
 	return VIRG_SUCCESS;
 }
@@ -277,11 +271,12 @@ int virg_init(virginian *v)
 	return VIRG_SUCCESS;
 }
 
-/* reference: src/close.c:20 */
+/* reference: src/close.c:20 (an open database file is closed first) */
 int virg_close(virginian *v)
 {
 	virg_state *s = virg_state_of(v);
 	VIRG_CHECK(s == NULL, "virginian state not initialised");
+	int ok = virg_db_close(v);
 	unsigned n = virg_nslots(v);
 	for (unsigned i = 0; i < n; i++) {
 		virg_tablet_meta **p = virg_slot_ptr(v, i);
@@ -298,7 +293,7 @@ int virg_close(virginian *v)
 	free(s);
 	v->gpu_slots = NULL;
 	pthread_mutex_destroy(&v->slot_lock);
-	return VIRG_SUCCESS;
+	return ok;
 }
 
 /* Slot memory allocated since virg_init: pinned, and pageable (no device,

@@ -61,6 +61,8 @@ int virg_findslot_sized(virginian *v, unsigned *slot, size_t bytes);
 int virg_db_alloc_sized(virginian *v, virg_tablet_meta **meta, int id, size_t bytes);
 /* slot holding tablet id, or -1 */
 int virg_slot_of(virginian *v, unsigned id);
+/* the disk half of virg_db_load (dbfile.c); the caller holds slot_lock */
+int virg_dbfile_load(virginian *v, unsigned tablet_id, virg_tablet_meta **tab);
 
 /* host memory for slots: pinned when a device is present */
 void *virg_host_alloc(virginian *v, size_t bytes);
