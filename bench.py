@@ -110,9 +110,11 @@ def bench_outofcore(a) -> int:
     # the system pins about 64 GiB at most: keep some of it for the staging
     # and result buffers; tablets beyond the limit are staged by host threads
     db.pin_limit(a.pin_limit_gib << 30)
-    db.gen_table("test", rows, a.tablet_rows, seed=a.seed, threads=16)
+    # the device table builder (the same bits as the host generator)
+    db.gen_table("test", rows, a.tablet_rows, seed=a.seed, threads=16, device=not a.host_gen)
     pinned, pageable = db.host_memory()
-    print(f"# generated {rows} rows in {time.perf_counter() - t0:.1f}s; tablets: {pinned / 2**30:.1f} GiB "
+    print(f"# generated {rows} rows ({'host' if a.host_gen else 'device'} builder) in "
+          f"{time.perf_counter() - t0:.1f}s; tablets: {pinned / 2**30:.1f} GiB "
           f"pinned, {pageable / 2**30:.1f} GiB pageable", file=sys.stderr, flush=True)
     cap = rows // 8
     for _ in range(a.warmup):
@@ -413,6 +415,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=0, help="table rows (default: the workload's)")
     ap.add_argument("--tablet-rows", type=int, default=1 << 20)
+    ap.add_argument("--host-gen", action="store_true",
+                    help="outofcore: build the table with the host generator instead of the device builder")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")

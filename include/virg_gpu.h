@@ -222,6 +222,16 @@ unsigned vg_query_tablet_rows(vg_query *q, unsigned long long *rows, unsigned n)
  * (VIRG_FAIL and *cus = 0 without one). */
 int vg_device_info(virginian *v, char *name, size_t n, int *cus);
 
+/* Bulk table builder on the device: table `name` with the standard
+ * synthetic schema (int64 key = row index; uniformi, normali5, normali20
+ * int32; uniformf, normalf5, normalf20 f32) and `rows` rows in host tablets
+ * of `tablet_rows` rows, identical bit for bit to what the host generator
+ * builds for the same arguments.  Replaces, for bulk loads, the reference's
+ * row-at-a-time ingest (virg_table_insert, reference src/table/insert.c;
+ * src/tablet/addrows.c:39-73).  Needs a device; tablet_rows > 0. */
+int vg_gen_table(virginian *v, const char *name, unsigned long long rows, unsigned tablet_rows,
+	unsigned seed);
+
 /* Decode the data-parallel section starting at op `pc` against tablet
  * `tab`'s schema into the kernel's micro-program (no device needed).
  * `out` receives a vg_prog (see virginian_amd/csrc/gpu/vg_prog.h). */

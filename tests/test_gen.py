@@ -95,3 +95,15 @@ def test_deterministic_and_position_independent():
         assert np.array_equal(a[k][30_000:].view(np.uint8), b[k].view(np.uint8)), k
     c = gen_columns(50_000, 10)
     assert not np.array_equal(a["uniformf"], c["uniformf"])
+
+
+def test_device_builder_needs_a_device():
+    """vg_gen_table has no host fallback: without a device it fails."""
+    db = va.Virginian()
+    try:
+        if db.device()[1] > 0:
+            pytest.skip("a device is present (the device builder is tested in test_gpu_gen.py)")
+        with pytest.raises(va.VirginianError):
+            db.gen_table("t", 100, 100, seed=1, device=True)
+    finally:
+        db.close()

@@ -194,6 +194,7 @@ def _load() -> C.CDLL:
         "vg_query_info": (C.c_int, [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint), C.POINTER(C.c_uint),
                                     C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
         "vg_device_info": (C.c_int, [V, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]),
+        "vg_gen_table": (C.c_int, [V, C.c_char_p, C.c_ulonglong, C.c_uint, C.c_uint]),
         "vg_stream_select": (C.c_int, [V, C.POINTER(Op), C.c_uint, C.POINTER(P), C.c_ulonglong,
                                        C.c_ulonglong, C.POINTER(StreamStats), C.c_char_p, C.c_size_t]),
         "vg_batch_create": (P, [C.POINTER(P), C.c_uint, C.c_char_p, C.c_size_t]),
@@ -368,9 +369,13 @@ class Virginian:
                 raise VirginianError("virg_table_insert failed")
 
     def gen_table(self, name: str, rows: int, tablet_rows: int = 1 << 20, seed: int = 1,
-                  threads: int = 8) -> int:
-        """Standard 7-column synthetic table (see csrc/host/gen.c)."""
-        if lib.virg_gen_table(self.ptr, name.encode(), rows, tablet_rows, seed, threads) != VIRG_SUCCESS:
+                  threads: int = 8, device: bool = False) -> int:
+        """Standard 7-column synthetic table (see csrc/host/gen.c); with
+        device=True built by the GPU (vg_gen_table), the same bits."""
+        if device:
+            if lib.vg_gen_table(self.ptr, name.encode(), rows, tablet_rows, seed) != VIRG_SUCCESS:
+                raise VirginianError("vg_gen_table failed")
+        elif lib.virg_gen_table(self.ptr, name.encode(), rows, tablet_rows, seed, threads) != VIRG_SUCCESS:
             raise VirginianError("virg_gen_table failed")
         return self.table_id(name)
 

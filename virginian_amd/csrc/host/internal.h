@@ -76,6 +76,10 @@ void virg_host_free(virginian *v, void *p);
  * rows). */
 int virg_table_appendtablet(virginian *v, unsigned table_id, unsigned capacity,
 	unsigned rows, const void *key, const void *const *cols);
+/* (key and cols NULL: the rows are left for the caller to fill in place) */
+
+/* the standard synthetic schema (gen.c), no rows yet */
+int virg_gen_schema(virginian *v, const char *name, unsigned *table_id);
 
 /* device layer entry points used by the host library */
 int vgd_open(virginian *v);            /* 1 if a device is usable */

@@ -116,9 +116,11 @@ int virg_table_appendtablet(virginian *v, unsigned table_id, unsigned capacity,
 	m->variable_block = m->key_block + m->row_stride * capacity;
 	m->size = m->variable_block;
 	if (rows) {
-		memcpy((char *)m + m->key_block, key, m->key_stride * rows);
+		/* with no key or columns given the caller fills them in place */
+		if (key)
+			memcpy((char *)m + m->key_block, key, m->key_stride * rows);
 		memset((char *)m + m->key_pointers_block, 0, m->key_pointer_stride * rows);
-		for (unsigned i = 0; i < m->fixed_columns; i++)
+		for (unsigned i = 0; cols && i < m->fixed_columns; i++)
 			memcpy((char *)m + m->fixed_block + m->fixed_offset[i], cols[i],
 				m->fixed_stride[i] * rows);
 	}
