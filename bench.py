@@ -400,6 +400,52 @@ def bench_served(a) -> int:
     return 0 if same else 3
 
 
+def _execute_ms(va, db, sql: str) -> tuple[float, int]:
+    """One virg_vm_execute (-> virg_vm_gpu): host tablets in, results in host
+    result tablets; wall-clock ms and result rows."""
+    vm = db.compile(sql)
+    try:
+        t0 = time.perf_counter()
+        if va.lib.virg_vm_execute(db.ptr, vm) != va.VIRG_SUCCESS:
+            raise RuntimeError("virg_vm_execute failed")
+        ms = (time.perf_counter() - t0) * 1e3
+        n = C.c_ulonglong(0)
+        va.lib.virg_result_columns(db.ptr, vm, None, C.byref(n))
+        return ms, n.value
+    finally:
+        db.free_vm(vm)
+
+
+def dropin_times(db, per: list, a) -> dict:
+    """The drop-in boundary, virg_vm_execute -> virg_vm_gpu, which hands over
+    host buffers: warm (tablets already in the device tablet cache; the
+    results cross the host link into result tablets), median of 3 per query;
+    and cold for the 10 % query on a fresh instance (every tablet crosses
+    the link first).  Reported beside value, never as value."""
+    import statistics
+    import virginian_amd as va
+    warm = []
+    for p in per:
+        sql = sql_for(p["threshold"])
+        _execute_ms(va, db, sql)
+        runs = [_execute_ms(va, db, sql) for _ in range(3)]
+        ms = statistics.median(r[0] for r in runs)
+        warm.append({"threshold": p["threshold"], "ms": round(ms, 3), "rows": runs[0][1],
+                     "gbs_scanned": round(p["read_bytes"] / ms / 1e6, 1),
+                     "result_bytes": p["write_bytes"]})
+        print(f"# drop-in virg_vm_execute T={p['threshold']}: {ms:.2f} ms with results into host "
+              f"result tablets ({p['read_bytes'] / ms / 1e6:.0f} GB/s scanned)", file=sys.stderr, flush=True)
+    fresh = va.Virginian()
+    try:
+        fresh.gen_table("test", a.rows, a.tablet_rows, seed=a.seed, threads=16)
+        ms, n = _execute_ms(va, fresh, sql_for(10.0))
+    finally:
+        fresh.close()
+    rb = per[1]["read_bytes"]
+    print(f"# drop-in cold (tablets uploaded first) T=10.0: {ms:.1f} ms", file=sys.stderr, flush=True)
+    return {"warm": warm, "cold_t10": {"ms": round(ms, 2), "rows": n, "gbs_scanned": round(rb / ms / 1e6, 1)}}
+
+
 def measured_traffic(workload: str) -> dict | None:
     """HBM traffic per launch pair recorded from counter runs, if present."""
     p = ROOT / "profiles" / f"traffic_{workload}.json"
@@ -481,6 +527,8 @@ def main() -> int:
               f"(count {ms_c:.3f} + write {ms_w:.3f}), {read_b / ms / 1e6:.0f} GB/s scanned, "
               f"{(read_b + write_b) / ms / 1e6:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
 
+    dropin = dropin_times(db, per, a)
+
     # whole job: K steps of the 4-query sweep, one stream per query
     def sync_all():
         for q in queries:
@@ -533,7 +581,8 @@ def main() -> int:
                    "rows": a.rows, "tablet_rows": a.tablet_rows, "queries_per_step": len(queries),
                    "pct_hbm_peak": round(value / HBM_PEAK_GBS * 100, 2),
                    "per_query": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()}
-                                 for p in per]},
+                                 for p in per],
+                   "dropin_virg_vm_execute": dropin},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
