@@ -1540,7 +1540,7 @@ extern "C" int vg_kernel_batch_single(const vg_batch_args *A, uint32_t grid, siz
 struct pinned_buf {
 	void *p = nullptr;
 	size_t cap = 0;
-	void *get(size_t bytes)
+	void *get(size_t bytes, unsigned flags = hipHostMallocDefault)
 	{
 		if (bytes > cap) {
 			if (p)
@@ -1549,7 +1549,7 @@ struct pinned_buf {
 			size_t c = cap ? cap : 4096;
 			while (c < bytes)
 				c *= 2;
-			if (hipHostMalloc(&p, c, hipHostMallocDefault) != hipSuccess) {
+			if (hipHostMalloc(&p, c, flags) != hipSuccess) {
 				(void)hipGetLastError();
 				p = nullptr;
 				cap = 0;
@@ -1609,7 +1609,10 @@ struct vg_pbatch {
 	/* pinned staging: uploads and the state readback are asynchronous on
 	 * the batch's stream (a synchronous copy would wait behind whatever
 	 * holds the compute units) */
-	pinned_buf h_progs, h_launch, h_info, h_items, h_witems, h_q, h_head;
+	pinned_buf h_progs, h_launch, h_info, h_items, h_witems;
+	/* every query's outcome, stored there by the kernels (coherent, so it
+	 * can be read once the stream is idle) */
+	pinned_buf h_res;
 };
 
 static void pbatch_drop_graph(vg_pbatch *b)
@@ -1636,8 +1639,8 @@ extern "C" void vg_pbatch_free(vg_pbatch *b)
 		(void)hipEventDestroy(e);
 	for (hipStream_t s : b->branch)
 		(void)hipStreamDestroy(s);
-	pinned_buf *pins[] = { &b->h_progs, &b->h_launch, &b->h_info, &b->h_items, &b->h_witems, &b->h_q,
-		&b->h_head, &b->h_gq };
+	pinned_buf *pins[] = { &b->h_progs, &b->h_launch, &b->h_info, &b->h_items, &b->h_witems, &b->h_res,
+		&b->h_gq };
 	for (pinned_buf *pb : pins)
 		pb->release();
 	if (b->fork)
@@ -1842,6 +1845,11 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		if (!b->cfg.direct && herr(hipStreamSynchronize(b->stream), "batch items"))
 			return qerr(err, errlen, "cannot upload the batch's item lists");
 	}
+	/* the kernels store each query's outcome through the device's view of
+	 * the pinned results */
+	vg_bresult *d_res = nullptr;
+	if (b->h_res.p && herr(hipHostGetDevicePointer((void **)&d_res, b->h_res.p, 0), "batch results"))
+		return qerr(err, errlen, "cannot map the batch's results");
 	size_t gq_off = 0;
 	for (size_t gi = 0; gi < ng; gi++) {
 		pgroup &g = b->groups[gi];
@@ -1864,6 +1872,7 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		A.head = b->d_head + gi;
 		A.nhead = 1;
 		A.spin_limit = VG_SPIN_LIMIT;
+		A.hres = d_res;
 		A.status = single ? b->d_status : nullptr;
 		A.nstatus = 0;
 		const uint32_t cmax = vg_kernel_batch_grid(0, g.clds), wmax = vg_kernel_batch_grid(single ? 2 : 1, g.wlds);
@@ -1947,7 +1956,7 @@ static int pbatch_set_queries(vg_pbatch *b, vg_query *const *qs, unsigned n, cha
 	}
 	if (n) {
 		void *hp = b->h_progs.get(n * sizeof(vg_prog)), *hl = b->h_launch.get(n * sizeof(vg_launch));
-		if (!hp || !hl || !b->h_q.get(n * sizeof(vg_bquery)) || !b->h_head.get(sizeof(vg_bhead)))
+		if (!hp || !hl || !b->h_res.get(n * sizeof(vg_bresult), hipHostMallocCoherent))
 			return qerr(err, errlen, "cannot allocate pinned staging");
 		memcpy(hp, progs.data(), n * sizeof(vg_prog));
 		memcpy(hl, launches.data(), n * sizeof(vg_launch));
@@ -2056,21 +2065,21 @@ extern "C" int vg_pbatch_launch(vg_pbatch *b)
 static int pbatch_state(vg_pbatch *b, unsigned long long *rows, double *latency_ms, unsigned *flags)
 {
 	const size_t n = b->qs.size();
-	vg_bquery *hq = (vg_bquery *)b->h_q.get(n * sizeof(vg_bquery));
-	vg_bhead *hh = (vg_bhead *)b->h_head.get(sizeof(vg_bhead));
-	if (!hq || !hh)
-		return VIRG_FAIL;
-	HCHK(hipMemcpyAsync(hq, b->d_q, n * sizeof(vg_bquery), hipMemcpyDeviceToHost, b->stream), "batch state");
-	HCHK(hipMemcpyAsync(hh, b->d_head, sizeof(vg_bhead), hipMemcpyDeviceToHost, b->stream), "batch state");
+	/* idle after a run that finished: nothing to copy, the kernels stored
+	 * every query's outcome in h_res */
 	HCHK(hipStreamSynchronize(b->stream), "batch");
+	const vg_bresult *hr = (const vg_bresult *)b->h_res.p;
+	if (n && !hr)
+		return VIRG_FAIL;
 	for (size_t i = 0; i < n; i++) {
+		const vg_bresult r = hr[i];
 		if (latency_ms)
-			latency_ms[i] = hq[i].t_done > hh->t_start ? (double)(hq[i].t_done - hh->t_start) / b->clock_khz : 0.0;
+			latency_ms[i] = r.t_done > r.t_start ? (double)(r.t_done - r.t_start) / b->clock_khz : 0.0;
 		if (rows)
-			rows[i] = hq[i].total;
+			rows[i] = r.total;
 		if (flags)
-			flags[i] = hq[i].flags;
-		b->qs[i]->last_rows = hq[i].flags ? 0 : hq[i].total;
+			flags[i] = r.flags;
+		b->qs[i]->last_rows = r.flags ? 0 : r.total;
 	}
 	return VIRG_SUCCESS;
 }

@@ -188,6 +188,18 @@ typedef struct {
 	uint64_t total;          /* rows selected, when finished */
 } vg_bquery;
 
+/* A query's outcome of a run, stored by the kernel that finishes the
+ * query into pinned host memory (the reset clears it): once the run's
+ * stream is idle the host reads it where it lies, with no copy that would
+ * queue behind kernels still holding the compute units. */
+typedef struct {
+	uint64_t total;          /* rows selected */
+	uint64_t t_done;         /* device clock when its last item finished */
+	uint64_t t_start;        /* device clock when the run started */
+	uint32_t flags;          /* overflow bits */
+	uint32_t pad;
+} vg_bresult;
+
 typedef struct {
 	uint32_t next;           /* next count item to hand out */
 	uint32_t wnext;          /* next write item to hand out */
@@ -214,6 +226,7 @@ typedef struct {
 	const uint32_t *gq;      /* the queries of this launch's group */
 	uint32_t ngq;
 	uint32_t pad2;
+	vg_bresult *hres;        /* per query, in pinned host memory */
 } vg_batch_args;
 
 #endif

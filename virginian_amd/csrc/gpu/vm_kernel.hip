@@ -1030,6 +1030,9 @@ __global__ __launch_bounds__(256) void vg_batch_reset(const vg_batch_args A)
 		vg_ctrl *c = A.launch[i].ctrl;
 		c->total = 0;
 		c->overflow = 0;
+		vg_bresult r;
+		memset(&r, 0, sizeof r);
+		A.hres[i] = r;
 	}
 	if (i < A.nhead) {
 		A.head[i].next = 0;
@@ -1046,9 +1049,16 @@ __global__ __launch_bounds__(256) void vg_batch_reset(const vg_batch_args A)
 __device__ __forceinline__ void finish_query(const vg_batch_args &A, uint32_t q)
 {
 	vg_ctrl *c = ldc(&A.launch[q].ctrl);
-	A.q[q].total = __hip_atomic_load(&c->total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	A.q[q].flags = __hip_atomic_load(&c->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	A.q[q].t_done = wall_clock64();
+	vg_bresult r;
+	r.total = __hip_atomic_load(&c->total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	r.flags = __hip_atomic_load(&c->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	r.t_done = wall_clock64();
+	r.t_start = ldc(&A.head)->t_start;
+	r.pad = 0;
+	A.q[q].total = r.total;
+	A.q[q].flags = r.flags;
+	A.q[q].t_done = r.t_done;
+	A.hres[q] = r;
 }
 
 /* the next item of a list, in every thread (query ~0u: none left) */
@@ -1155,8 +1165,15 @@ __global__ __launch_bounds__(64) void vg_batch_finish_kernel(const vg_batch_args
 		t = t > y ? t : y;
 	}
 	if (lane == 0) {
-		A.q[q].flags = ldc(&A.launch[q].ctrl)->overflow;
+		vg_bresult r;
+		r.total = A.q[q].total;
+		r.flags = ldc(&A.launch[q].ctrl)->overflow;
+		r.t_done = t;
+		r.t_start = ldc(&A.head)->t_start;
+		r.pad = 0;
+		A.q[q].flags = r.flags;
 		A.q[q].t_done = t;
+		A.hres[q] = r;
 	}
 }
 

@@ -89,8 +89,12 @@ class Request:
     cap: int = -1                   # output rows reserved (set at admission)
     result: dict = field(default_factory=dict)
     wire: tuple = ()                # the op-program as submitted (virg_op array)
+    widths: tuple = ()              # bytes per value of each result column
+    need: int = -1                  # arena bytes for cap rows (set with cap)
 
     def __post_init__(self):
+        if not self.widths:
+            self.widths = tuple(_WIDTH[o[1]] for o in self.ops if o[0] == OP_ResultColumn)
         if not self.wire:
             arr, n, keep = make_ops(self.ops)
             names = [o[4] if isinstance(o[4], str) else f"c{k}"
@@ -188,10 +192,13 @@ class Server:
         todo = [r for r in cand if r.cap < 0]
         for r, c in zip(todo, self._capacities(todo) if todo else []):
             r.cap = c
+            r.need = -1
         room = self.arena.largest()
         take, total = [], 0
         for r in cand:
-            need = sum((max(r.cap, 1) * w + 255) & ~255 for w in _widths(r.ops))
+            if r.need < 0:
+                r.need = sum((max(r.cap, 1) * w + 255) & ~255 for w in r.widths)
+            need = r.need
             if total + need > room:
                 break                       # admission is in arrival order
             take.append((r, need))
@@ -203,7 +210,7 @@ class Server:
         batch, off = [], base
         for r, need in take:
             outs = []
-            for w in _widths(r.ops):
+            for w in r.widths:
                 outs.append(self.arena.base + off)
                 off += (max(r.cap, 1) * w + 255) & ~255
             batch.append((r, _Query(self.db, r, max(r.cap, 1), outs)))
@@ -220,6 +227,7 @@ class Server:
             if f:
                 # did not fit: again, in two passes, at the exact size counted
                 r.cap = int(n)
+                r.need = -1
                 r.rerun = True
                 reruns.append(r)
             else:
@@ -283,16 +291,22 @@ class Server:
         rerun_free = [self.rerun_runner]
         inflight = []
         done = 0
+        # admission is tried again only when something changed: an arrival,
+        # a finished batch (arena and a runner freed) or a batch launched
+        # (the waiting line moved); otherwise it would fail the same way
+        changed = True
         t0 = time.perf_counter()
         while done < len(reqs):
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
+                changed = True
             progressed = False
             # reruns first (they arrived earlier), then new requests
             prof = self.prof
+            admit_now, changed = changed, False
             for queue, pool in ((reruns, rerun_free), (waiting, free)):
-                if queue and pool:
+                if admit_now and queue and pool:
                     ta = time.perf_counter()
                     batch, block = self._admit(queue)
                     tb = time.perf_counter()
@@ -302,7 +316,7 @@ class Server:
                         runner.reload([q for _, q in batch])
                         runner.launch()
                         inflight.append((runner, batch, block, pool))
-                        progressed = True
+                        progressed = changed = True
                         prof["launch"] += time.perf_counter() - tb
                         prof["batches"] += 1
                     elif not inflight:
@@ -314,7 +328,7 @@ class Server:
                     done += self._finish(runner, batch, block, t0, reruns)
                     prof["collect"] += time.perf_counter() - ta
                     pool.append(runner)
-                    progressed = True
+                    progressed = changed = True
                 else:
                     still.append((runner, batch, block, pool))
             inflight = still
