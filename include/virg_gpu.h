@@ -193,6 +193,36 @@ void vg_host_free(void *p);
  * buffer of `bytes` over `reps` copies each. */
 int vg_link_probe(virginian *v, size_t bytes, int reps, double *h2d_gbs, double *d2h_gbs);
 
+/* Features of a (program, tablet) pair for the result-size predictor (no
+ * counterpart in the reference), one row of VG_FEATURES floats per tablet:
+ *   [0..2] logit(e_u) / VG_FEAT_SKIP for u = 1/2, 1/4, 3/4, where e_u is the
+ *          fraction of the tablet's rows the filter keeps if its compares
+ *          were independent, `column op constant` compares passing at the
+ *          column histogram's estimate and any other compare at u
+ *          (logit clipped to p in [1e-4, 1 - 1e-4]);
+ *   [3]    e_{1/2};
+ *   [4..10] compares judged by histograms / 4, not judged / 4, keep
+ *          branches / 4, drop branches / 4, Invalid ops, logic ops,
+ *          program length / 40;
+ *   [11]   tablet rows / 2^20;
+ *   [12..] the tablet's statistics as given (tabfeat).
+ * Statistics per table: edges[c][0..BINS] of each fixed column's range,
+ * per tablet hist[t][c][bin] (fractions), is_int[c], and tabfeat[t][...]
+ * (VG_FEAT_COLS * (VG_FEAT_BINS + 2) values). */
+#define VG_FEAT_COLS 6
+#define VG_FEAT_BINS 16
+#define VG_FEAT_SKIP 4.0
+#define VG_FEATURES (12 + VG_FEAT_COLS * (VG_FEAT_BINS + 2))
+typedef struct {
+	unsigned ntab;
+	const double *rows;
+	const double *edges;
+	const double *hist;
+	const int *is_int;
+	const double *tabfeat;
+} vg_table_stats;
+int vg_program_features(const virg_op *ops, unsigned num_ops, const vg_table_stats *st, float *out);
+
 /* Result-size predictor (no counterpart in the reference): a 3-layer
  * perceptron nin -> nh -> nh -> 1 (ReLU, ReLU, sigmoid; f32) predicting
  * the fraction of a tablet's rows a program selects, trained on the GPU

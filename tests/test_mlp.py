@@ -6,9 +6,11 @@ from __future__ import annotations
 import numpy as np
 
 import virginian_amd as va
+from oracle import features as FE
 from oracle import mlp as M
-from tests.helpers import CONFIG1_SQL
+from tests.helpers import CONFIG1_SQL, QUERY_SUITE
 from virginian_amd import predict as PR
+from virginian_amd import workload as W
 
 
 def _fd(p, x, t, nin, nh, eps=1e-6):
@@ -70,10 +72,10 @@ def test_features_and_estimates(db):
     st = PR.TableStats(db, tid)
     assert len(st.rows) == 5 and st.rows.sum() == 300_000
     # uniformi < 50: half the rows on every tablet
-    est = st.passing(0, va.OP_Lt, 50.0)
+    est = FE.passing(st, 0, va.OP_Lt, 50.0)
     assert np.all(np.abs(est - 0.5) < 0.02)
     # uniformf > 90 (uniform on [0, 100))
-    est = st.passing(3, va.OP_Gt, 90.0)
+    est = FE.passing(st, 3, va.OP_Gt, 90.0)
     assert np.all(np.abs(est - 0.1) < 0.02)
     vm = db.compile(CONFIG1_SQL)
     try:
@@ -87,6 +89,35 @@ def test_features_and_estimates(db):
     assert f[0, 4] == 2 / 4 and f[0, 5] == 0
     # uniformi < 50 and normali5 > 0: 0.5 * P(N(0,5) rounds above 0) = 0.23
     assert np.all(np.abs(f[:, 3] - 0.23) < 0.03), f[:, 3]
+    # under independence the AND keeps the product of its parts, per tablet
+    both = FE.passing(st, 0, va.OP_Lt, 50.0) * FE.passing(st, 1, va.OP_Gt, 0.0)
+    assert np.allclose(f[:, 3], both, rtol=1e-6, atol=1e-7), (f[:, 3], both)
+
+
+def test_features_match_the_restatement(db):
+    """vg_program_features (C) against oracle/features.py on seeded random
+    programs over a table of several tablets, and on the published queries."""
+    db.gen_table("test", 200_000, 1 << 15, seed=4)
+    tid = db.table_id("test")
+    st = PR.TableStats(db, tid)
+    rng = np.random.default_rng(17)
+    progs = [W.random_program(rng, max_ops=40, selectivity=float(rng.uniform(0.0001, 0.8)) if k % 3 else None,
+                              table=tid) for k in range(300)]
+    # constant first: the compare is read mirrored (50 < uniformi is uniformi > 50)
+    mirrored = ["select id from test where 50 < uniformi", "select id from test where 50.0 > uniformf",
+                "select id from test where 10 <= normali20 and 3 = uniformi",
+                "select id from test where -2.5 >= normalf5 or 7 != uniformi"]
+    for sql in QUERY_SUITE + [CONFIG1_SQL] + mirrored:
+        vm = db.compile(sql)
+        try:
+            progs.append(va.program(vm))
+        finally:
+            db.free_vm(vm)
+    for k, ops in enumerate(progs):
+        got = PR.program_features(ops, st)
+        want = FE.program_features(ops, st)
+        assert got.shape == want.shape == (len(st.rows), PR.NFEATURES)
+        np.testing.assert_allclose(got, want, rtol=2e-6, atol=1e-6, err_msg=f"program {k}")
 
 
 def test_capacity_and_calibration():

@@ -30,7 +30,7 @@ OUT = PKG / "lib"
 OBJ = PKG / "lib" / "obj"
 LIBNAME = "libvirginian_amd.so"
 
-HOST_C = ["db.c", "dbfile.c", "tablet.c", "table.c", "vm.c", "reader.c", "sql.c", "gen.c"]
+HOST_C = ["db.c", "dbfile.c", "tablet.c", "table.c", "vm.c", "reader.c", "sql.c", "gen.c", "features.c"]
 GPU_C = ["decode.c"]
 GPU_HIP = ["vm_kernel.hip", "device.hip", "mlp.hip", "gen.hip"]
 

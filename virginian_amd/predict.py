@@ -29,14 +29,10 @@ import math
 
 import numpy as np
 
-from . import (NP_TYPE, OP_And, OP_Column, OP_Eq, OP_Float, OP_Ge, OP_Gt, OP_Integer, OP_Invalid, OP_Le,
-               OP_Lt, OP_Neq, OP_Not, OP_Or, OP_Parallel, OP_Result, OP_Rowid, OP_Table, VIRG_SUCCESS,
-               VirginianError, lib)
+from . import NP_TYPE, OP_Table, VIRG_SUCCESS, VirginianError, lib, make_ops
 
 NCOLS = 6
 NBINS = 16
-CMP_OPS = [OP_Lt, OP_Le, OP_Gt, OP_Ge, OP_Eq, OP_Neq]
-MIRROR = {OP_Lt: OP_Gt, OP_Le: OP_Ge, OP_Gt: OP_Lt, OP_Ge: OP_Le, OP_Eq: OP_Eq, OP_Neq: OP_Neq}
 NFEATURES = 4 + 7 + 1 + NCOLS * (2 + NBINS)
 # feature 0 is logit(estimate) / SKIP; the network adds SKIP * feature 0 to
 # its output's pre-activation and so learns a correction to the estimate
@@ -82,116 +78,47 @@ class TableStats:
             ((mins - self.lo) / span)[:, :, None], ((maxs - self.lo) / span)[:, :, None], self.hist],
             axis=2).reshape(len(tabs), -1)
 
-    def cdf(self, c: int, v: float) -> np.ndarray:
-        """Fraction of each tablet's rows with column c < v (histogram,
-        linear inside a bin)."""
-        e = self.edges[c]
-        if v <= e[0]:
-            return np.zeros(len(self.rows))
-        if v >= e[-1]:
-            return np.ones(len(self.rows))
-        k = min(int(np.searchsorted(e, v, side="right")) - 1, NBINS - 1)
-        part = (v - e[k]) / (e[k + 1] - e[k])
-        return self.hist[:, c, :k].sum(axis=1) + part * self.hist[:, c, k]
+    def c_stats(self) -> "VgTableStats":
+        """The statistics as vg_program_features takes them (arrays kept
+        alive by this object)."""
+        if getattr(self, "_c", None) is None:
+            self._arrs = [np.ascontiguousarray(self.rows, dtype=np.float64),
+                          np.ascontiguousarray(np.stack(self.edges), dtype=np.float64),
+                          np.ascontiguousarray(self.hist, dtype=np.float64),
+                          np.ascontiguousarray([int(x) for x in self.is_int] + [0] * (NCOLS - len(self.is_int)),
+                                               dtype=np.int32),
+                          np.ascontiguousarray(self.tablet_features, dtype=np.float64)]
+            d = C.POINTER(C.c_double)
+            r, e, h, i, t = self._arrs
+            self._c = VgTableStats(len(self.rows), r.ctypes.data_as(d), e.ctypes.data_as(d), h.ctypes.data_as(d),
+                                   i.ctypes.data_as(C.POINTER(C.c_int)), t.ctypes.data_as(d))
+        return self._c
 
-    def passing(self, c: int, op: int, v: float) -> np.ndarray:
-        """Estimated fraction of each tablet's rows with (column c op v)."""
-        if self.is_int[c]:
-            lt, le = self.cdf(c, math.ceil(v)), self.cdf(c, math.floor(v) + 1)
-            eq = le - self.cdf(c, math.floor(v)) if float(v).is_integer() else np.zeros(len(self.rows))
-        else:
-            lt = le = self.cdf(c, v)
-            eq = np.zeros(len(self.rows))
-        return {OP_Lt: lt, OP_Le: le, OP_Gt: 1 - le, OP_Ge: 1 - lt, OP_Eq: eq, OP_Neq: 1 - eq}[op]
+
+class VgTableStats(C.Structure):
+    """vg_table_stats (include/virg_gpu.h)."""
+    _fields_ = [("ntab", C.c_uint), ("rows", C.POINTER(C.c_double)), ("edges", C.POINTER(C.c_double)),
+                ("hist", C.POINTER(C.c_double)), ("is_int", C.POINTER(C.c_int)),
+                ("tabfeat", C.POINTER(C.c_double))]
 
 
 # ------------------------------------------------------- program features
-def _value(op) -> float:
-    return float(op[4]) if op[0] == OP_Float else float(op[2])
-
-
-def _walk(ops: list[tuple], stats: TableStats, unknown: np.ndarray):
-    """One pass over the filter: for each value in `unknown` (the pass rate
-    assumed for compares the histograms cannot judge), the fraction of each
-    tablet's rows reaching Result valid if the compares were independent,
-    each `column op constant` passing at its histogram estimate; a taken
-    compare jumps to p3, keeping validity when p4 is set and dropping the
-    row otherwise; Invalid drops the rows reaching it.  Also the program's
-    shape counts."""
-    ntab = len(stats.rows)
-    u = np.asarray(unknown, dtype=np.float64)[:, None] * np.ones((1, ntab))
-    start = next((i + 1 for i, o in enumerate(ops) if o[0] == OP_Parallel), 0)
-    res = next((i for i, o in enumerate(ops) if o[0] == OP_Result and i >= start), len(ops))
-    arrive = {start: np.ones_like(u)}
-    regdef: dict[int, tuple] = {}
-    known = nunknown = keep = drop = ninvalid = nlogic = 0
-    for pc in range(start, res):
-        o = ops[pc]
-        m = arrive.pop(pc, None)
-        k = o[0]
-        nxt = m
-        if k == OP_Column:
-            regdef[o[1]] = ("col", o[2])
-        elif k in (OP_Integer, OP_Float):
-            regdef[o[1]] = ("const", _value(o))
-        elif k == OP_Invalid:
-            ninvalid += 1
-            nxt = None
-        elif k in CMP_OPS or k in (OP_And, OP_Or, OP_Not):
-            p = u
-            if k in CMP_OPS:
-                a, b = regdef.get(o[1], ("expr",)), regdef.get(o[2], ("expr",))
-                op = k
-                if a[0] == "const" and b[0] == "col":
-                    a, b, op = b, a, MIRROR[k]
-                if a[0] == "col" and a[1] < NCOLS and b[0] == "const":
-                    p = np.clip(stats.passing(a[1], op, b[1]), 0.0, 1.0)[None, :]
-                    known += 1
-                else:
-                    nunknown += 1
-                if o[4] or 0:
-                    keep += 1
-                else:
-                    drop += 1
-            else:
-                nlogic += 1
-            if m is not None:
-                if (o[4] or 0) and o[3] > pc:
-                    arrive[o[3]] = arrive.get(o[3], 0) + m * p
-                nxt = m * (1 - p)
-        elif len(o) > 1 and k != OP_Table:
-            regdef[o[1]] = ("expr",)
-        if nxt is not None:
-            arrive[pc + 1] = arrive.get(pc + 1, 0) + nxt
-    est = arrive.get(res)
-    est = np.zeros_like(u) if est is None else np.clip(est * np.ones_like(u), 0.0, 1.0)
-    return est, (known, nunknown, keep, drop, ninvalid, nlogic)
-
-
-def structural_estimate(ops: list[tuple], stats: TableStats, unknown: float = 0.5) -> np.ndarray:
-    """Per tablet: the fraction of rows the filter keeps if its compares
-    were independent (see _walk), compares the histograms cannot judge
-    passing at `unknown`."""
-    return _walk(ops, stats, np.array([unknown]))[0][0]
+def program_features_ops(arr, n: int, stats: TableStats) -> np.ndarray:
+    """Feature rows (one per tablet) of an op array (virg_op * n) over the
+    table `stats` describes: vg_program_features in the host library."""
+    _bind()
+    out = np.empty((len(stats.rows), NFEATURES), dtype=np.float32)
+    if lib.vg_program_features(arr, n, C.byref(stats.c_stats()),
+                               out.ctypes.data_as(C.POINTER(C.c_float))) != VIRG_SUCCESS:
+        raise VirginianError("vg_program_features failed")
+    return out
 
 
 def program_features(ops: list[tuple], stats: TableStats) -> np.ndarray:
     """Feature rows (one per tablet) of a program over the table `stats`
-    describes."""
-    ntab = len(stats.rows)
-    ests, (known, unknown, keep, drop, ninvalid, nlogic) = _walk(ops, stats, np.array([0.5, 0.25, 0.75]))
-    est, lo, hi = ests
-    shape = np.array([known / 4, unknown / 4, keep / 4, drop / 4, ninvalid, nlogic, len(ops) / 40.0])
-    return np.concatenate([_logit(est)[:, None], _logit(lo)[:, None], _logit(hi)[:, None], est[:, None],
-                           np.tile(shape, (ntab, 1)), (stats.rows / float(1 << 20))[:, None],
-                           stats.tablet_features], axis=1).astype(np.float32)
-
-
-def _logit(p: np.ndarray) -> np.ndarray:
-    """log(p / (1 - p)) / SKIP, clipped: the network's output is a
-    sigmoid, so an estimate enters as (a scaled) logit."""
-    q = np.clip(p, 1e-4, 1 - 1e-4)
-    return np.log(q / (1 - q)) / SKIP
+    describes (see include/virg_gpu.h, vg_program_features)."""
+    arr, n, _keep = make_ops(ops)
+    return program_features_ops(arr, n, stats)
 
 
 # ------------------------------------------------------------------- MLP
@@ -278,6 +205,7 @@ def _bind() -> None:
         "vg_mlp_train": (C.c_int, [P, F, F, C.c_uint, C.c_uint, C.c_uint, C.c_float, C.c_ulonglong,
                                    C.POINTER(C.c_float)]),
         "vg_query_tablet_rows": (C.c_uint, [P, C.POINTER(C.c_ulonglong), C.c_uint]),
+        "vg_program_features": (C.c_int, [P, C.c_uint, P, F]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

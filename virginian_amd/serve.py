@@ -175,7 +175,7 @@ class Server:
     def _capacities(self, reqs: list[Request]) -> list[int]:
         if self.mode == "two-pass":
             return [int(self.stats[PR.table_of(r.ops)].rows.sum()) for r in reqs]
-        feats = [PR.program_features(r.ops, self.stats[PR.table_of(r.ops)]) for r in reqs]
+        feats = [PR.program_features_ops(r.wire[0], r.wire[1], self.stats[PR.table_of(r.ops)]) for r in reqs]
         pred = self.net.forward(np.concatenate(feats)) if self.net is not None else None
         caps, k = [], 0
         for r, f in zip(reqs, feats):
