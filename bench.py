@@ -345,8 +345,11 @@ def bench_served(a) -> int:
     rows_of = {t: int(stats[t].rows.sum()) for t in tids}
     out = {}
     for mode in ("sized", "single", "two-pass", "plain"):
+        # single-pass batches run one at a time: their tiles wait on each
+        # other's published counts, and three such kernels side by side were
+        # measured 20x slower than one
         srv = S.Server(db, tids, a.arena_gib << 30, mode=mode, predictor=net if mode != "two-pass" else None,
-                       margin=margin, runners=3, max_batch=64, keep_results=False)
+                       margin=margin, runners=1 if mode == "single" else 3, max_batch=64, keep_results=False)
         reqs = [S.Request(arrival=ar, ops=ops, rid=i) for i, (ar, ops) in enumerate(stream)]
         try:
             summ = srv.run(reqs)

@@ -239,6 +239,9 @@ unsigned vg_mlp_num_params(const vg_mlp *m);
 int vg_mlp_get_params(const vg_mlp *m, float *out);
 int vg_mlp_set_params(vg_mlp *m, const float *in);
 int vg_mlp_forward(vg_mlp *m, const float *x, unsigned n, float *y);
+/* The same on the host, from a copy of the parameters taken whenever they
+ * change: no device call, for callers that must not wait for the device. */
+int vg_mlp_forward_host(const vg_mlp *m, const float *x, unsigned n, float *y);
 int vg_mlp_gradient(vg_mlp *m, const float *x, const float *t, unsigned n, float *loss, float *grad);
 int vg_mlp_train(vg_mlp *m, const float *x, const float *t, unsigned n, unsigned epochs, unsigned batch,
 	float lr, unsigned long long seed, float *loss);

@@ -32,6 +32,10 @@ def test_forward_and_gradient_match_oracle(gpu_db):
         y = net.forward(x)
         y_ref, _ = M.forward(p, x.astype(np.float64), nin, nh, skip)
         assert np.allclose(y, y_ref, rtol=1e-4, atol=1e-6), (nin, np.abs(y - y_ref).max())
+        # the host forward pass (same parameters, copied when set) agrees too
+        yh = net.forward_host(x)
+        assert np.allclose(yh, y_ref, rtol=1e-4, atol=1e-6), (nin, np.abs(yh - y_ref).max())
+        assert np.allclose(yh, y, rtol=1e-5, atol=1e-6), (nin, np.abs(yh - y).max())
         loss, g = net.gradient(x, t)
         loss_ref, g_ref = M.gradient(p, x.astype(np.float64), t.astype(np.float64), nin, nh, skip)
         assert np.isclose(loss, loss_ref, rtol=1e-4)
@@ -50,6 +54,8 @@ def test_training_fits_a_known_function(gpu_db):
     assert after < before / 10 and after < 2e-3, (before, after)
     # the loss the GPU reports is the oracle's loss of the trained weights
     assert np.isclose(after, M.loss(net.params().astype(np.float64), x, t, 6, 32), rtol=1e-3)
+    # the host copy of the parameters follows training
+    assert np.allclose(net.forward_host(x), net.forward(x), rtol=1e-5, atol=1e-6)
     net.close()
 
 

@@ -46,7 +46,17 @@ struct vg_mlp {
 	hipStream_t s = nullptr;
 	mlp_work inf;
 	float *hx = nullptr, *hy = nullptr;
+	/* the parameters in host memory, refreshed whenever they change on the
+	 * device, for vg_mlp_forward_host */
+	std::vector<float> hp;
 };
+
+/* copy the parameters to the host copy (after training or setting them) */
+static int refresh_host_params(vg_mlp *m)
+{
+	m->hp.resize(m->np);
+	return hipMemcpy(m->hp.data(), m->d_p, m->np * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
+}
 
 /* offsets of the parameter blocks */
 struct mlp_layout {
@@ -312,6 +322,7 @@ extern "C" vg_mlp *vg_mlp_create(unsigned nin, unsigned nh, float skip, unsigned
 		vg_mlp_free(m);
 		return nullptr;
 	}
+	m->hp = p;
 	return m;
 }
 
@@ -332,8 +343,8 @@ extern "C" int vg_mlp_set_params(vg_mlp *m, const float *in)
 	if (hipMemset(m->d_m, 0, m->np * sizeof(float)) != hipSuccess ||
 		hipMemset(m->d_v, 0, m->np * sizeof(float)) != hipSuccess)
 		return VIRG_FAIL;
-	return hipMemcpy(m->d_p, in, m->np * sizeof(float), hipMemcpyHostToDevice) == hipSuccess
-		? VIRG_SUCCESS : VIRG_FAIL;
+	return hipMemcpy(m->d_p, in, m->np * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+		refresh_host_params(m) ? VIRG_SUCCESS : VIRG_FAIL;
 }
 
 /* y[i] = network(x[i]) for n examples (host arrays, x row-major n x nin) */
@@ -371,6 +382,42 @@ extern "C" int vg_mlp_forward(vg_mlp *m, const float *x, unsigned n, float *y)
 		return VIRG_FAIL;
 	}
 	memcpy(y, m->hy, n * sizeof(float));
+	return VIRG_SUCCESS;
+}
+
+/* The same network evaluated on the host from the parameters' host copy,
+ * in the GPU layers' order of operations (bias, skip, then the inputs in
+ * order; f32): for callers that must not wait for the device, such as a
+ * serving loop whose batches hold every compute unit. */
+extern "C" int vg_mlp_forward_host(const vg_mlp *m, const float *x, unsigned n, float *y)
+{
+	const mlp_layout L = layout(m->nin, m->nh);
+	if (m->hp.size() != L.n)
+		return VIRG_FAIL;
+	const float *P = m->hp.data();
+	const unsigned nin = m->nin, h = m->nh;
+	std::vector<float> h1(h), h2(h);
+	for (unsigned b = 0; b < n; b++) {
+		const float *xb = x + (size_t)b * nin;
+		for (unsigned j = 0; j < h; j++) {
+			float z = P[L.b1 + j];
+			for (unsigned i = 0; i < nin; i++)
+				z += xb[i] * P[L.w1 + (size_t)i * h + j];
+			h1[j] = z > 0.0f ? z : 0.0f;
+		}
+		for (unsigned j = 0; j < h; j++) {
+			float z = P[L.b2 + j];
+			for (unsigned i = 0; i < h; i++)
+				z += h1[i] * P[L.w2 + (size_t)i * h + j];
+			h2[j] = z > 0.0f ? z : 0.0f;
+		}
+		float z = P[L.b3];
+		if (m->skip != 0.0f)
+			z += m->skip * xb[0];
+		for (unsigned i = 0; i < h; i++)
+			z += h2[i] * P[L.w3 + i];
+		y[b] = 1.0f / (1.0f + expf(-z));
+	}
 	return VIRG_SUCCESS;
 }
 
@@ -445,5 +492,6 @@ extern "C" int vg_mlp_train(vg_mlp *m, const float *x, const float *t, unsigned 
 			hipMemcpy(loss, w.loss, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
 	}
 	free_work(w);
+	ok = ok && refresh_host_params(m);
 	return ok ? VIRG_SUCCESS : VIRG_FAIL;
 }

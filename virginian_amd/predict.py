@@ -151,6 +151,16 @@ class MLP:
             raise VirginianError("vg_mlp_forward failed")
         return y
 
+    def forward_host(self, x: np.ndarray) -> np.ndarray:
+        """The same network on the host (vg_mlp_forward_host): no device
+        call, so it does not wait for kernels that hold the compute units."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty(len(x), dtype=np.float32)
+        if len(x) and lib.vg_mlp_forward_host(self.h, x.ctypes.data_as(C.POINTER(C.c_float)), len(x),
+                                              y.ctypes.data_as(C.POINTER(C.c_float))) != VIRG_SUCCESS:
+            raise VirginianError("vg_mlp_forward_host failed")
+        return y
+
     def gradient(self, x: np.ndarray, t: np.ndarray) -> tuple[float, np.ndarray]:
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
@@ -201,6 +211,7 @@ def _bind() -> None:
         "vg_mlp_get_params": (C.c_int, [P, F]),
         "vg_mlp_set_params": (C.c_int, [P, F]),
         "vg_mlp_forward": (C.c_int, [P, F, C.c_uint, F]),
+        "vg_mlp_forward_host": (C.c_int, [P, F, C.c_uint, F]),
         "vg_mlp_gradient": (C.c_int, [P, F, F, C.c_uint, C.POINTER(C.c_float), F]),
         "vg_mlp_train": (C.c_int, [P, F, F, C.c_uint, C.c_uint, C.c_uint, C.c_float, C.c_ulonglong,
                                    C.POINTER(C.c_float)]),

@@ -176,7 +176,9 @@ class Server:
         if self.mode == "two-pass":
             return [int(self.stats[PR.table_of(r.ops)].rows.sum()) for r in reqs]
         feats = [PR.program_features_ops(r.wire[0], r.wire[1], self.stats[PR.table_of(r.ops)]) for r in reqs]
-        pred = self.net.forward(np.concatenate(feats)) if self.net is not None else None
+        # on the host: the batches in flight hold every compute unit, and a
+        # device forward pass would wait for them
+        pred = self.net.forward_host(np.concatenate(feats)) if self.net is not None else None
         caps, k = [], 0
         for r, f in zip(reqs, feats):
             st = self.stats[PR.table_of(r.ops)]
