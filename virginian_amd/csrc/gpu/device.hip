@@ -856,6 +856,9 @@ extern "C" const size_t *virg_gpu_getsizes(void)
 }
 
 /* reference: src/vm/vm_gpu.cu:107 */
+/* largest result tablet the device path allocates (rows) */
+#define VG_RESULT_TABLET_ROWS (1u << 26)
+
 extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 	virg_tablet_meta **res, unsigned num_tablets)
 {
@@ -961,7 +964,13 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 			break;
 		VIRG_CHECK(r->possible_rows == 0, "result tablet has no capacity");
 		virg_tablet_meta *old = r;
-		VIRG_CHECK(virg_vm_allocresult(v, vm, res, old) == VIRG_FAIL, "result allocation failed");
+		/* the rest in one tablet (few large copies instead of one per 8 MB),
+		 * unless a database file must be able to hold every tablet */
+		const uint64_t left = nres - done;
+		const unsigned want = (unsigned)(left < VG_RESULT_TABLET_ROWS ? left : VG_RESULT_TABLET_ROWS);
+		VIRG_CHECK((v->dbfd < 0 && want > old->possible_rows
+				? virg_vm_allocresult_rows(v, vm, res, old, want)
+				: virg_vm_allocresult(v, vm, res, old)) == VIRG_FAIL, "result allocation failed");
 		virg_tablet_unlock(v, old->id);
 	}
 	HCHK(hipEventRecord(ev[3], d->stream), "event");

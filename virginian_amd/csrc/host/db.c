@@ -110,9 +110,14 @@ int virg_findslot_sized(virginian *v, unsigned *slot_, size_t bytes)
 {
 	unsigned n = virg_nslots(v);
 	unsigned slot;
+	/* a free slot whose buffer is big enough, else the first free one */
 	for (slot = 0; slot < n; slot++)
-		if (*virg_slot_status(v, slot) == 0)
+		if (*virg_slot_status(v, slot) == 0 && *virg_slot_ptr(v, slot) && *virg_slot_bytes(v, slot) >= bytes)
 			break;
+	if (slot == n)
+		for (slot = 0; slot < n; slot++)
+			if (*virg_slot_status(v, slot) == 0)
+				break;
 	if (slot == n) {
 		VIRG_CHECK(ext_grow(v) == VIRG_FAIL, "All tablet slots in use");
 		slot = n;

@@ -78,6 +78,10 @@ int virg_table_appendtablet(virginian *v, unsigned table_id, unsigned capacity,
 	unsigned rows, const void *key, const void *const *cols);
 /* (key and cols NULL: the rows are left for the caller to fill in place) */
 
+/* a result tablet laid out for `rows` rows (vm.c) */
+int virg_vm_allocresult_rows(virginian *v, virg_vm *vm, virg_tablet_meta **meta,
+	virg_tablet_meta *template_, unsigned rows);
+
 /* the standard synthetic schema (gen.c), no rows yet */
 int virg_gen_schema(virginian *v, const char *name, unsigned *table_id);
 

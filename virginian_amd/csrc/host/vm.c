@@ -87,6 +87,48 @@ int virg_vm_allocresult(virginian *v, virg_vm *vm, virg_tablet_meta **meta,
 	return VIRG_SUCCESS;
 }
 
+/* A result tablet like virg_vm_allocresult's from `template_`, laid out for
+ * `rows` rows instead of the template's capacity (the reference layout,
+ * tablet/addcolumn.c:36-37, applied to `rows`): the device path gives the
+ * rest of a large result one tablet instead of many 8 MB ones. */
+int virg_vm_allocresult_rows(virginian *v, virg_vm *vm, virg_tablet_meta **meta,
+	virg_tablet_meta *template_, unsigned rows)
+{
+	virg_tablet_meta t = *template_;
+	t.possible_rows = rows;
+	t.key_block = sizeof(virg_tablet_meta);
+	t.key_pointers_block = t.key_block + t.key_stride * rows;
+	t.fixed_block = t.key_pointers_block + t.key_pointer_stride * rows;
+	for (unsigned c = 0; c < t.fixed_columns; c++)
+		t.fixed_offset[c] = c == 0 ? 0 : t.fixed_offset[c - 1] + t.fixed_stride[c - 1] * rows;
+	t.variable_block = t.key_block + t.row_stride * rows;
+	t.size = t.variable_block;
+	const size_t bytes = sizeof(virg_tablet_meta) + t.size;
+	unsigned id = v->db.tablet_id_counter++;
+	virg_tablet_meta *tab;
+	VIRG_CHECK(virg_db_alloc_sized(v, &tab, (int)id, bytes < VIRG_TABLET_SIZE ? VIRG_TABLET_SIZE : bytes) == VIRG_FAIL,
+		"Could not allocate result tablet");
+	*tab = t;
+	template_->last_tablet = 0;
+	template_->next = id;
+	tab->id = id;
+	tab->rows = 0;
+	tab->next = 0;
+	tab->last_tablet = 1;
+	tab->info = NULL;
+	virg_result_node *node = (virg_result_node *)malloc(sizeof(virg_result_node));
+	VIRG_CHECK(node == NULL, "Out of memory");
+	node->id = id;
+	node->next = NULL;
+	if (vm->head_result == NULL)
+		vm->head_result = node;
+	else
+		vm->tail_result->next = node;
+	vm->tail_result = node;
+	meta[0] = tab;
+	return VIRG_SUCCESS;
+}
+
 /* reference: src/vm/freeresults.c:17 */
 int virg_vm_freeresults(virginian *v, virg_vm *vm)
 {
