@@ -146,7 +146,7 @@ class Server:
     no batching, each request run alone as it arrives."""
 
     def __init__(self, db, table_ids: list[int], arena_bytes: int, mode: str = "single", predictor=None,
-                 margin: float = 0.05, runners: int = 3, max_batch: int = 64, tiles_per_item: int = 4,
+                 margin: float = 0.05, runners: int = 3, max_batch: int = 64, tiles_per_item: int = 8,
                  keep_results: bool = True):
         self.db = db
         self.mode = mode
