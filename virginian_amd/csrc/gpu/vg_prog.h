@@ -183,7 +183,7 @@ typedef struct {
 	uint32_t done;           /* count items finished */
 	uint32_t flags;          /* the query's overflow bits, when finished */
 	uint32_t wdone;          /* write (or single-pass) items finished */
-	uint32_t pad;
+	uint32_t next;           /* single pass: the query's next tile item */
 	uint64_t t_done;         /* device clock when its last item finished */
 	uint64_t total;          /* rows selected, when finished */
 } vg_bquery;
@@ -201,8 +201,6 @@ typedef struct {
 } vg_bresult;
 
 typedef struct {
-	uint32_t next;           /* next count item to hand out */
-	uint32_t wnext;          /* next write item to hand out */
 	uint64_t t_start;        /* device clock when the run started */
 } vg_bhead;
 

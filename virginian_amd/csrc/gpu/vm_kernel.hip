@@ -1034,11 +1034,8 @@ __global__ __launch_bounds__(256) void vg_batch_reset(const vg_batch_args A)
 		memset(&r, 0, sizeof r);
 		A.hres[i] = r;
 	}
-	if (i < A.nhead) {
-		A.head[i].next = 0;
-		A.head[i].wnext = 0;
+	if (i < A.nhead)
 		A.head[i].t_start = wall_clock64();
-	}
 	for (uint64_t j = i; j < A.nstatus; j += (uint64_t)gridDim.x * 256)
 		A.status[j] = 0;
 }
@@ -1061,16 +1058,29 @@ __device__ __forceinline__ void finish_query(const vg_batch_args &A, uint32_t q)
 	A.hres[q] = r;
 }
 
-/* the next item of a list, in every thread (query ~0u: none left) */
-__device__ __forceinline__ void next_item(const vg_item *items, uint32_t nitems, uint32_t *counter,
-	vg_item *s_item, uint32_t &q, uint32_t &t0, uint32_t &t1, uint32_t &k)
+/* The single pass's next item: each query's tiles are taken in order
+ * through the query's own counter, so workgroups on different queries do
+ * not contend on one device-wide counter.  A workgroup stays with a query
+ * (`cur`, an index into the group's queries; thread 0's copy is the one
+ * used) until its tiles are all taken, then moves on, and gets query ~0u
+ * when a whole round over the group's queries found none.  Within a query
+ * tiles are still taken in order by running workgroups, which the
+ * look-back needs. */
+__device__ __forceinline__ void next_query_item(const vg_batch_args &A, uint32_t &cur, vg_item *s_item,
+	uint32_t &q, uint32_t &t0, uint32_t &t1, uint32_t &k)
 {
 	if (threadIdx.x == 0) {
-		const uint32_t i = atomicAdd(counter, 1u);
 		uint32_t iq = ~0u, i0 = 0, i1 = 0, ik = 0;
-		if (i < nitems) {
-			const u32x4 v = *(const u32x4 *)(items + i);
-			iq = v.x; i0 = v.y; i1 = v.z; ik = v.w;
+		const uint32_t ngq = A.ngq;
+		for (uint32_t tries = 0; tries < ngq; ++tries) {
+			const uint32_t qq = ldc(&A.gq[cur]);
+			const uint32_t i = atomicAdd(&A.q[qq].next, 1u);
+			if (i < ldc(&A.info[qq].nitem)) {
+				const u32x4 v = *(const u32x4 *)(A.witems + ldc(&A.info[qq].wstart) + i);
+				iq = v.x; i0 = v.y; i1 = v.z; ik = v.w;
+				break;
+			}
+			cur = cur + 1 == ngq ? 0 : cur + 1;
 		}
 		s_item->query = iq;
 		s_item->t0 = i0;
@@ -1244,9 +1254,10 @@ __global__ __launch_bounds__(256) void vg_batch_single_kernel(const vg_batch_arg
 	__shared__ uint64_t s_pre;
 	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
 	const strips<1> S{wave * 256u, 256u, lane};
+	uint32_t cur = A.ngq ? blockIdx.x % A.ngq : 0;
 	for (;;) {
 		uint32_t q, t0, t1, k;
-		next_item(A.witems, A.nitems, &A.head->wnext, &s_item, q, t0, t1, k);
+		next_query_item(A, cur, &s_item, q, t0, t1, k);
 		if (q == ~0u)
 			break;
 		const vg_prog &P = A.progs[q];
