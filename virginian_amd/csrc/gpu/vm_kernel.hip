@@ -741,7 +741,9 @@ __global__ __launch_bounds__(256) void vg_count_kernel(const vg_prog P, const vg
  * result column.  LDS: nslot * T words of registers, then 256*WS row
  * numbers per wave. */
 #define VG_CHUNK 64
+#ifndef VG_WS
 #define VG_WS 1                               /* strips per write-pass wave */
+#endif
 #define VG_WRITE_THREADS (256 / VG_WS)
 
 /* rows selected in this wave's strips of staged tile i */
@@ -973,22 +975,22 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_pro
 	{
 		/* offset of this workgroup: the rows selected before tile t0 --
 		 * whole count-pass workgroups, then the tiles of the one t0 is
-		 * in; all 256 threads add, eight independent loads in flight each */
-		__shared__ uint64_t s_part[4];
+		 * in; all threads add, eight independent loads in flight each */
+		__shared__ uint64_t s_part[VG_WRITE_THREADS / 64];
 		const uint32_t cw = t0 / L.tiles_per_wg;
 		uint64_t base = 0;
-		for (uint32_t k0 = 0; k0 < cw; k0 += 8 * 256) {
+		for (uint32_t k0 = 0; k0 < cw; k0 += 8 * VG_WRITE_THREADS) {
 			uint32_t v[8];
 #pragma unroll
 			for (int j = 0; j < 8; ++j) {
-				const uint32_t k = k0 + j * 256 + tid;
+				const uint32_t k = k0 + j * VG_WRITE_THREADS + tid;
 				v[j] = k < cw ? L.wgcount[k] : 0u;
 			}
 #pragma unroll
 			for (int j = 0; j < 8; ++j)
 				base += v[j];
 		}
-		for (uint32_t t = cw * L.tiles_per_wg + tid; t < t0; t += 256) {
+		for (uint32_t t = cw * L.tiles_per_wg + tid; t < t0; t += VG_WRITE_THREADS) {
 			const u32x4 v = *(const u32x4 *)(L.tcount + (size_t)t * 4);
 			base += (uint64_t)v.x + v.y + v.z + v.w;
 		}
@@ -998,8 +1000,13 @@ __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_pro
 		if (lane == 0)
 			s_part[wave] = base;
 		__syncthreads();
-		if (tid == 0)
-			s_base = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+		if (tid == 0) {
+			uint64_t sum = 0;
+#pragma unroll
+			for (uint32_t w = 0; w < VG_WRITE_THREADS / 64; ++w)
+				sum += s_part[w];
+			s_base = sum;
+		}
 	}
 	__syncthreads();
 	const uint64_t end = write_tiles(P, L, t0, t1, s_base, lds);
