@@ -103,6 +103,37 @@ def typed_table(db, rows, seed):
     return cols
 
 
+def cast_programs() -> list[list[tuple]]:
+    """Programs over typed_table() whose Casts the reference runs the same
+    way as the clean definition: the reference's Cast takes its type and
+    register from the last op that fetched operands (virginia.c:682-862),
+    so each Cast here follows an Integer with the same p1 and p2
+    (Integer r<type> = <register>, a harmless write to a low register);
+    only conversions that change the type and do not widen the value
+    (the reference converts in place in row order, so a 4-to-8-byte
+    widening overwrites rows before it reads them, and a cast to the same
+    type leaves its rows behind, which the reference's own assertion at
+    virginia.c:376 rejects); no double or float is cast to char (out of
+    range there is undefined in C).  The second program first drops the
+    rows with i > 300, which then skip the Casts."""
+    O = va
+
+    def cast(to, reg, col):
+        return [(O.OP_Column, reg, col, 0, None), (O.OP_Integer, to, reg, 0, None),
+                (O.OP_Cast, to, reg, 0, None)]
+
+    # typed_table columns: 0 c (char), 1 d (double), 2 l (int64), 3 i (int), 4 f (float)
+    first = (cast(O.VIRG_INT, 10, 1) + cast(O.VIRG_INT, 11, 4) + cast(O.VIRG_FLOAT, 12, 3)
+             + cast(O.VIRG_DOUBLE, 13, 2) + cast(O.VIRG_INT64, 14, 1) + cast(O.VIRG_FLOAT, 15, 2))
+    second = ([(O.OP_Column, 9, 3, 0, None), (O.OP_Integer, 8, 300, 0, None), (O.OP_Gt, 9, 8, "R", 0)]
+              + cast(O.VIRG_INT, 10, 2) + cast(O.VIRG_FLOAT, 11, 1) + cast(O.VIRG_CHAR, 12, 3)
+              + cast(O.VIRG_CHAR, 13, 2) + cast(O.VIRG_INT64, 14, 1) + cast(O.VIRG_INT, 15, 4))
+    return [op_program([O.VIRG_INT, O.VIRG_INT, O.VIRG_FLOAT, O.VIRG_DOUBLE, O.VIRG_INT64,
+                        O.VIRG_FLOAT], first, 10),
+            op_program([O.VIRG_INT, O.VIRG_FLOAT, O.VIRG_CHAR, O.VIRG_CHAR, O.VIRG_INT64,
+                        O.VIRG_INT], second, 10)]
+
+
 def typed_program() -> list[tuple]:
     """A program over typed_table() exercising char, double and int64
     arithmetic, Cast, And/Or/Not/Neq branches that keep validity, a

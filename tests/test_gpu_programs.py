@@ -9,7 +9,7 @@ import pytest
 
 import virginian_amd as va
 from oracle import oracle
-from tests.helpers import assert_same, typed_program, typed_table
+from tests.helpers import assert_same, cast_programs, typed_program, typed_table
 from virginian_amd import workload as W
 
 pytestmark = pytest.mark.gpu
@@ -51,6 +51,18 @@ def test_types_and_ops(gpu_db):
     assert len(want["c0"]) > 1000
     got = _gpu(gpu_db, ops)
     assert_same(got, want, "typed program")
+
+
+def test_casts(gpu_db):
+    """The Cast programs whose oracle results equal the reference VM's own
+    (tests/test_ref_parity.py): same-width and narrowing conversions
+    between every numeric type, rows dropped before the Casts."""
+    typed_table(gpu_db, 70_000, 21)
+    for k, ops in enumerate(cast_programs()):
+        want, ill = oracle.select(gpu_db, ops)
+        assert ill == 0
+        got = _gpu(gpu_db, ops)
+        assert_same(got, want, f"cast program {k}")
 
 
 def test_capacity_overflow_is_an_error(gpu_db):

@@ -126,3 +126,30 @@ def test_multi_core_reference_same_multiset(tables):
         r.__init__()
         r.table("test", c["id"], [(n, c[n]) for n in ("uniformi", "normali5", "normali20",
                                                        "uniformf", "normalf5", "normalf20")])
+
+
+def test_cast_reference():
+    """Cast, which no query generates, against the reference VM itself on
+    the programs where the reference's Cast is well defined
+    (tests/helpers.py cast_programs: the op before each Cast carries the
+    same type and register, and no value widens or keeps its type)."""
+    from tests.helpers import cast_programs, typed_table
+    rows = 40_000
+    db = va.Virginian()
+    r = ref.RefDB()
+    try:
+        c, d, l, i, f = typed_table(db, rows, 21)
+        r.table("test", np.arange(rows, dtype=np.int64) * 3,
+                [("c", c), ("d", d), ("l", l), ("i", i), ("f", f)])
+        for k, prog in enumerate(cast_programs()):
+            arr, n, _ = va.make_ops(prog)
+            names = [o[4] for o in prog if o[0] == va.OP_ResultColumn]
+            types = [o[1] for o in prog if o[0] == va.OP_ResultColumn]
+            got = r.run((arr, n), names, types, rows)
+            want, ill = oracle.select(db, prog)
+            assert ill == 0
+            assert len(want["c0"]) > (rows // 4 if k else rows - 1)
+            assert_same(want, got, f"cast program {k}")
+    finally:
+        r.close()
+        db.close()
