@@ -201,7 +201,7 @@ int vg_link_probe(virginian *v, size_t bytes, int reps, double *h2d_gbs, double 
  *          column histogram's estimate and any other compare at u
  *          (logit clipped to p in [1e-4, 1 - 1e-4]);
  *   [3]    e_{1/2};
- *   [4..10] compares judged by histograms / 4, not judged / 4, keep
+ *   [4..10] compares estimated by histograms / 4, not estimated / 4, keep
  *          branches / 4, drop branches / 4, Invalid ops, logic ops,
  *          program length / 40;
  *   [11]   tablet rows / 2^20;

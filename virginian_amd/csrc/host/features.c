@@ -5,7 +5,7 @@
  * `column op constant` compare passing at its histogram estimate on the
  * tablet and any other compare at u, for u = 1/2, 1/4, 3/4 (as logits
  * scaled by 1/4, and the u = 1/2 estimate itself); the program's shape
- * (compares judged by histograms and not, keep and drop branches, Invalid
+ * (compares estimated by histograms and not, keep and drop branches, Invalid
  * and logic ops, length); the tablet's rows; and the tablet's own column
  * statistics (min, max, 16-bin histogram per column, given).
  *
@@ -169,7 +169,7 @@ int vg_program_features(const virg_op *ops, unsigned num_ops, const vg_table_sta
 			ninvalid++;
 			next = 0;
 		} else if (is_cmp(k) || k == OP_And || k == OP_Or || k == OP_Not) {
-			int judged = 0;
+			int estimated = 0;
 			if (is_cmp(k)) {
 				const int r2 = o->p2 >= 0 && o->p2 < VIRG_REGS ? o->p2 : -1;
 				int ak = r1 >= 0 ? rkind[r1] : RD_EXPR, bk = r2 >= 0 ? rkind[r2] : RD_EXPR;
@@ -182,7 +182,7 @@ int vg_program_features(const virg_op *ops, unsigned num_ops, const vg_table_sta
 				}
 				if (ak == RD_COL && av < VG_FEAT_COLS && bk == RD_CONST) {
 					passing(st, (unsigned)av, op, bv, p, tmp);
-					judged = 1;
+					estimated = 1;
 					known++;
 				} else {
 					unknown++;
@@ -199,7 +199,7 @@ int vg_program_features(const virg_op *ops, unsigned num_ops, const vg_table_sta
 				double *tp = jumps ? arrive + (size_t)o->p3 * row : NULL;
 				for (unsigned u = 0; u < NU; u++)
 					for (unsigned t = 0; t < nt; t++) {
-						const double pr = judged ? p[t] : U[u];
+						const double pr = estimated ? p[t] : U[u];
 						const double x = mp[u * nt + t];
 						if (tp)
 							tp[u * nt + t] += x * pr;

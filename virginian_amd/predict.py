@@ -13,8 +13,8 @@ Features of a (program, tablet) pair:
   - the filter evaluated under independence (structural_estimate): each
     `column op constant` compare passing at its histogram estimate on the
     tablet, and any other compare at 1/2 -- and at 1/4 and 3/4, which
-    brackets what the compares the histograms cannot judge may do;
-  - program shape: compares the histograms judge and those they cannot,
+    brackets what the compares the histograms cannot estimate may do;
+  - program shape: compares the histograms estimate and those they cannot,
     keep and drop branches, Invalid and logic ops, program length;
   - the tablet: rows, and per column its min, max and a 16-bin histogram
     over the table's range.
