@@ -40,9 +40,19 @@ typedef struct vg_query vg_query;
  *                       shared with other queries, the query uses the
  *                       device's stream, its control block comes from a
  *                       pool and the two-pass arrays are made on first use;
- *   VG_QUERY_NO_OUTPUT  no output memory; give it with vg_query_set_outputs. */
+ *   VG_QUERY_NO_OUTPUT  no output memory; give it with vg_query_set_outputs;
+ *   VG_QUERY_INTERP     run on the two-pass interpreter even when the
+ *                       program could have a specialised kernel.
+ * Unless VG_QUERY_INTERP or the policy (vg_jit_policy) says otherwise, the
+ * program's data-parallel section is compiled at prepare time into a
+ * single-pass kernel of its own (hiprtc, cached by program structure, so
+ * programs that differ only in constants share it): every column is read
+ * from HBM once and the result is written in row order in one launch.  A
+ * program it cannot take runs on the interpreter; the results are the
+ * same bit for bit. */
 #define VG_QUERY_LIGHT     1
 #define VG_QUERY_NO_OUTPUT 2
+#define VG_QUERY_INTERP    4
 vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned num_ops,
 	unsigned long long capacity, int flags, char *err, size_t errlen);
 /* Results go to caller device memory from now on: out[j] has room for
@@ -54,6 +64,17 @@ unsigned vg_query_out_widths(vg_query *q, unsigned *widths);
 void *vg_device_alloc(virginian *v, size_t bytes);
 void vg_device_free(virginian *v, void *p);
 void vg_query_free(vg_query *q);
+
+/* Set when prepared queries get a specialised kernel (policy >= 0) and
+ * return the previous policy: 0 never, 1 (the default, or the value of the
+ * environment variable VG_JIT) queries that are neither light nor flagged
+ * VG_QUERY_INTERP, 2 light queries too. */
+int vg_jit_policy(int policy);
+/* What runs q: *fused = 1 for its specialised single-pass kernel (rows per
+ * lane, tiles per chunk, workgroups, compile milliseconds), 0 for the
+ * interpreter's count and write passes. */
+int vg_query_kernel(vg_query *q, int *fused, int *rows_per_lane, unsigned *tiles_per_chunk,
+	unsigned *grid, double *compile_ms);
 
 /* Launch the section once on the query's stream (asynchronous). */
 int vg_query_launch(vg_query *q);
@@ -71,7 +92,9 @@ int vg_query_time(vg_query *q, int warmup, int iters, double *ms_per_launch,
 	unsigned long long *rows);
 /* Mean milliseconds of the count pass and of the write pass over `iters`
  * launches after `warmup` untimed ones, each launch bracketed by HIP
- * events on the query's stream and waited for before the next. */
+ * events on the query's stream and waited for before the next.  A query
+ * on its specialised kernel has one pass: *ms_count is 0 and *ms_write the
+ * whole launch. */
 int vg_query_time_passes(vg_query *q, int warmup, int iters, double *ms_count,
 	double *ms_write, unsigned long long *rows);
 /* Rows scanned, tiles, referenced-column bytes per row, result bytes per

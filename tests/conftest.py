@@ -5,6 +5,10 @@ from pathlib import Path
 import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
+# Prepared queries run on the two-pass interpreter unless a test asks for
+# the specialised kernels (tests/test_gpu_jit.py sets the policy itself):
+# compiling one per random program would dominate the suite's time.
+os.environ.setdefault("VG_JIT", "0")
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 

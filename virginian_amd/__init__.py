@@ -18,7 +18,7 @@ _LIB_PATH = Path(os.environ.get("VIRGINIAN_AMD_LIB") or
                  Path(__file__).resolve().parent / "lib" / "libvirginian_amd.so")
 
 VIRG_FAIL, VIRG_SUCCESS = 0, 1
-VG_QUERY_LIGHT, VG_QUERY_NO_OUTPUT = 1, 2
+VG_QUERY_LIGHT, VG_QUERY_NO_OUTPUT, VG_QUERY_INTERP = 1, 2, 4
 VIRG_MAX_COLUMNS = 16
 VIRG_MAX_COLUMN_NAME = 16
 VIRG_MAX_TABLES = 16
@@ -227,6 +227,13 @@ def _load() -> C.CDLL:
         "vg_link_probe": (C.c_int, [V, C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "vg_decode_program": (C.c_int, [C.POINTER(Op), C.c_uint, C.c_uint, C.POINTER(TabletMeta),
                                         P, C.c_size_t, C.c_char_p, C.c_size_t]),
+        "vg_jit_policy": (C.c_int, [C.c_int]),
+        "vg_query_kernel": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint),
+                                      C.POINTER(C.c_uint), C.POINTER(C.c_double)]),
+        "vg_jit_source": (C.c_long, [C.POINTER(Op), C.c_uint, C.c_uint, C.POINTER(TabletMeta), C.c_char_p,
+                                     C.c_size_t, C.c_char_p, C.c_size_t]),
+        "vg_jit_compile_check": (C.c_int, [C.POINTER(Op), C.c_uint, C.c_uint, C.POINTER(TabletMeta),
+                                           C.POINTER(C.c_double), C.c_char_p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -248,6 +255,13 @@ class _Lib:
 
 
 lib = _Lib()
+
+
+def jit_policy(policy: int = -1) -> int:
+    """vg_jit_policy: when prepared queries get a specialised single-pass
+    kernel (0 never, 1 non-light queries, 2 all); returns the previous
+    policy, and only reads it when policy < 0."""
+    return lib.vg_jit_policy(policy)
 
 
 def struct_sizes() -> list[int]:
@@ -627,6 +641,14 @@ class DeviceQuery:
             raise VirginianError("vg_query_time failed")
         self.last_rows = n.value
         return ms.value, n.value
+
+    def kernel(self) -> dict:
+        """What runs this query (vg_query_kernel): the specialised single-pass
+        kernel (fused) or the interpreter's two passes."""
+        f, r, sc, g, ms = C.c_int(), C.c_int(), C.c_uint(), C.c_uint(), C.c_double()
+        lib.vg_query_kernel(self.h, C.byref(f), C.byref(r), C.byref(sc), C.byref(g), C.byref(ms))
+        return {"fused": bool(f.value), "rows_per_lane": r.value, "tiles_per_chunk": sc.value,
+                "grid": g.value, "compile_ms": ms.value}
 
     def time_passes(self, warmup: int, iters: int) -> tuple[float, float, int]:
         """(count-pass ms, write-pass ms, rows) from HIP events on the query's stream."""

@@ -32,7 +32,7 @@ LIBNAME = "libvirginian_amd.so"
 
 HOST_C = ["db.c", "dbfile.c", "tablet.c", "table.c", "vm.c", "reader.c", "sql.c", "gen.c", "features.c"]
 GPU_C = ["decode.c"]
-GPU_HIP = ["vm_kernel.hip", "device.hip", "mlp.hip", "gen.hip"]
+GPU_HIP = ["vm_kernel.hip", "device.hip", "mlp.hip", "gen.hip", "jit.hip"]
 
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", "-Wextra",
           "-Wno-unused-parameter", "-pthread", f"-I{INCLUDE}", f"-I{CSRC / 'host'}",
@@ -83,7 +83,7 @@ def build_product(jobs: int = 8) -> Path:
     lib = OUT / LIBNAME
     tmp = OUT / (LIBNAME + ".tmp")
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
-          *map(str, objs), "-lpthread", "-lm"])
+          *map(str, objs), "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib", "-lpthread", "-lm"])
     os.replace(tmp, lib)
     return lib
 

@@ -30,6 +30,7 @@
 extern "C" {
 #include "../host/internal.h"
 #include "vg_prog.h"
+#include "vgx.h"
 #include "virg_gpu.h"
 }
 
@@ -1076,6 +1077,12 @@ struct vg_query {
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	uint64_t last_rows = 0;
 	uint64_t read_bytes_per_row = 0;
+	/* the program's specialised single-pass kernel (jit.hip), if it has one */
+	vgx_kernel *jk = nullptr;
+	vgx_tile *d_jtiles = nullptr;
+	uint64_t *d_jstate = nullptr;      /* chunk counter, epoch, one status word per chunk */
+	uint32_t jgrid = 0;
+	vgx_launch_args ja;
 };
 
 /* timing events, made on first use (light queries have none) */
@@ -1099,6 +1106,114 @@ static int qerr(char *err, size_t n, const char *fmt, ...)
 	return VIRG_FAIL;
 }
 
+/* When prepared queries get a specialised kernel: 0 never (the two-pass
+ * interpreter only), 1 queries prepared with vg_query_create other than
+ * light ones and those flagged VG_QUERY_INTERP, 2 also light queries and
+ * the drop-in path.  The environment variable VG_JIT sets the initial
+ * value (default 1). */
+static int g_jit_policy = -1;
+
+static int jit_policy()
+{
+	if (g_jit_policy < 0) {
+		const char *e = getenv("VG_JIT");
+		g_jit_policy = e && *e ? atoi(e) : 1;
+	}
+	return g_jit_policy;
+}
+
+extern "C" int vg_jit_policy(int policy)
+{
+	const int old = jit_policy();
+	if (policy >= 0)
+		g_jit_policy = policy;
+	return old;
+}
+
+/* Tile table of a plan for tiles of TR rows: (tablet, first row) of each. */
+static std::vector<vgx_tile> jit_tiles(const plan &p, uint32_t TR)
+{
+	std::vector<vgx_tile> t;
+	for (size_t i = 0; i < p.desc.size(); i++)
+		for (uint32_t r = 0; r < p.desc[i].rows; r += TR)
+			t.push_back(vgx_tile{(uint32_t)i, r});
+	return t;
+}
+
+/* Immediates of a decoded program as the specialised kernel reads them. */
+static void jit_immediates(const vg_prog &P, vgx_launch_args &a)
+{
+	for (uint32_t k = 0; k < VG_MAX_UOPS; k++) {
+		a.imm[k][0] = k < P.nuop ? P.uop[k].ia : 0;
+		a.imm[k][1] = k < P.nuop ? P.uop[k].ib : 0;
+	}
+	for (uint32_t j = 0; j < VG_MAX_OUT; j++)
+		a.oimm[j] = j < P.nout ? P.out[j].imm : 0;
+}
+
+/* Give q its specialised kernel, tile table and launch state.  Returns 0,
+ * or -1 (with a message) when the program has no specialised form; the
+ * query then runs on the interpreter. */
+static int jit_setup(vg_query *q, char *err, size_t errlen)
+{
+	const plan &p = *q->p;
+	bool aligned = true;
+	for (const vg_tabdesc &d : p.desc)
+		aligned = aligned && d.aligned;
+	vgx_kernel *k = vgx_get(&q->P, aligned ? 1 : 0, err, errlen);
+	if (!k)
+		return -1;
+	const vgx_shape *sh = vgx_kernel_shape(k);
+	const std::vector<vgx_tile> tiles = jit_tiles(p, sh->TR);
+	if (tiles.empty())
+		return -1;
+	const uint32_t nchunks = (uint32_t)((tiles.size() + sh->S - 1) / sh->S);
+	const size_t state_bytes = (2 + (size_t)nchunks) * sizeof(uint64_t);
+	if (hipMalloc((void **)&q->d_jtiles, tiles.size() * sizeof(vgx_tile)) != hipSuccess ||
+		hipMalloc((void **)&q->d_jstate, state_bytes) != hipSuccess ||
+		hipMemcpyAsync(q->d_jtiles, tiles.data(), tiles.size() * sizeof(vgx_tile), hipMemcpyHostToDevice,
+			q->stream) != hipSuccess ||
+		hipMemsetAsync(q->d_jstate, 0, state_bytes, q->stream) != hipSuccess ||
+		hipStreamSynchronize(q->stream) != hipSuccess) {
+		(void)hipGetLastError();
+		qerr(err, errlen, "cannot set up the specialised kernel's launch state");
+		return -1;
+	}
+	vgx_launch_args &a = q->ja;
+	memset(&a, 0, sizeof a);
+	a.tabs = p.d_desc;
+	a.tiles = q->d_jtiles;
+	a.num_tiles = (uint32_t)tiles.size();
+	a.nchunks = nchunks;
+	a.state = q->d_jstate;
+	a.status = q->d_jstate + 2;
+	a.ctrl = q->lr.ctrl;
+	a.capacity = q->capacity;
+	a.spin_limit = 1u << 20;
+	for (unsigned j = 0; j < q->P.nout; j++)
+		a.out[j] = q->out[j];
+	jit_immediates(q->P, a);
+	q->jgrid = vgx_kernel_grid(k, nchunks, (unsigned)q->d->prop.multiProcessorCount);
+	q->jk = k;
+	return 0;
+}
+
+/* One run of q on `stream`: its specialised kernel, or the interpreter's
+ * count and write passes. */
+static int query_launch(vg_query *q, hipStream_t stream, hipEvent_t mid = nullptr)
+{
+	if (q->jk) {
+		if (mid)
+			HCHK(hipEventRecord(mid, stream), "event");
+		if (vgx_launch(q->jk, &q->ja, q->jgrid, stream) != 0) {
+			herr(hipGetLastError(), "specialised kernel launch");
+			return VIRG_FAIL;
+		}
+		return VIRG_SUCCESS;
+	}
+	return launch(q->P, *q->p, q->lr, q->out, q->capacity, stream, mid);
+}
+
 extern "C" void vg_query_free(vg_query *q)
 {
 	if (!q)
@@ -1107,6 +1222,10 @@ extern "C" void vg_query_free(vg_query *q)
 	if (q->stream && !q->pooled_ctrl)
 		(void)hipStreamSynchronize(q->stream);
 	free_plan(q->own);
+	if (q->d_jtiles)
+		(void)hipFree(q->d_jtiles);
+	if (q->d_jstate)
+		(void)hipFree(q->d_jstate);
 	if (q->pooled_ctrl && q->lr.ctrl) {
 		q->d->ctrl_free.push_back(q->lr.ctrl);
 		q->lr.ctrl = nullptr;
@@ -1245,21 +1364,44 @@ extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned 
 		vg_query_free(q);
 		return nullptr;
 	}
+	const int pol = jit_policy();
+	if (q->p->num_tiles && !(flags & VG_QUERY_INTERP) && (pol >= 2 || (pol == 1 && !light))) {
+		char jerr[512];
+		if (jit_setup(q, jerr, sizeof jerr) != 0) {
+			static bool said = false;
+			if (!said)
+				fprintf(stderr, "::: Virginian: no specialised kernel (%s); the interpreter runs it\n", jerr);
+			said = true;
+		}
+	}
 	return q;
 }
 
 /* Launch once, asynchronously, on the query's stream. */
 extern "C" int vg_query_launch(vg_query *q)
 {
-	if (q->pooled_ctrl)
+	if (q->pooled_ctrl && !q->jk)
 		take_pooled_lr(q->d, q->lr, q->p->num_tiles, vg_kernel_grid(&q->P, q->p->num_tiles));
-	return launch(q->P, *q->p, q->lr, q->out, q->capacity, q->stream);
+	return query_launch(q, q->stream);
 }
 
 /* Wait for the last launch; returns the selected row count. */
 extern "C" int vg_query_sync(vg_query *q, unsigned long long *rows)
 {
 	uint64_t n = 0;
+	if (q->jk && q->p->num_tiles) {
+		/* a look-back that gave up waiting leaves an unusable result: run
+		 * the interpreter's two passes instead */
+		vg_ctrl c;
+		HCHK(hipMemcpyAsync(&c, q->lr.ctrl, sizeof c, hipMemcpyDeviceToHost, q->stream), "control block");
+		HCHK(hipStreamSynchronize(q->stream), "kernel");
+		if (c.overflow & VG_OVF_WAIT) {
+			fprintf(stderr, "::: Virginian: single-pass wait gave up; rerunning in two passes\n");
+			HCHK(hipMemsetAsync(q->lr.ctrl, 0, sizeof(vg_ctrl), q->stream), "control block");
+			if (launch(q->P, *q->p, q->lr, q->out, q->capacity, q->stream) == VIRG_FAIL)
+				return VIRG_FAIL;
+		}
+	}
 	if (q->p->num_tiles && read_ctrl(q->lr, q->stream, &n) == VIRG_FAIL)
 		return VIRG_FAIL;
 	q->last_rows = n;
@@ -1325,7 +1467,7 @@ extern "C" int vg_query_time_passes(vg_query *q, int warmup, int iters, double *
 	int rc = VIRG_SUCCESS;
 	for (int i = 0; i < warmup + iters && rc == VIRG_SUCCESS; i++) {
 		if (hipEventRecord(q->e0, q->stream) != hipSuccess ||
-			launch(q->P, *q->p, q->lr, q->out, q->capacity, q->stream, mid) == VIRG_FAIL ||
+			query_launch(q, q->stream, mid) == VIRG_FAIL ||
 			hipEventRecord(q->e1, q->stream) != hipSuccess ||
 			vg_query_sync(q, rows) == VIRG_FAIL) {
 			rc = VIRG_FAIL;
@@ -1361,6 +1503,20 @@ extern "C" int vg_query_info(vg_query *q, unsigned long long *rows, unsigned *ti
 		w += (unsigned)q->width[j];
 	write_bytes_per_row[0] = w;
 	nout[0] = q->P.nout;
+	return VIRG_SUCCESS;
+}
+
+/* What runs q: *fused = 1 for a specialised single-pass kernel (then its
+ * rows per lane, tiles per chunk, grid and compile time), 0 for the
+ * interpreter's two passes. */
+extern "C" int vg_query_kernel(vg_query *q, int *fused, int *rows_per_lane, unsigned *tiles_per_chunk,
+	unsigned *grid, double *compile_ms)
+{
+	*fused = q->jk != nullptr;
+	*rows_per_lane = q->jk ? vgx_kernel_shape(q->jk)->R : 0;
+	*tiles_per_chunk = q->jk ? vgx_kernel_shape(q->jk)->S : 0;
+	*grid = q->jk ? q->jgrid : 0;
+	*compile_ms = q->jk ? vgx_kernel_compile_ms(q->jk) : 0.0;
 	return VIRG_SUCCESS;
 }
 
@@ -1461,7 +1617,7 @@ extern "C" vg_batch *vg_batch_create(vg_query *const *qs, unsigned n, char *err,
 	for (unsigned i = 0; i < n && ok; i++) {
 		vg_query *q = qs[i];
 		ok = hipStreamWaitEvent(q->stream, b->fork, 0) == hipSuccess &&
-			launch(q->P, *q->p, q->lr, q->out, q->capacity, q->stream) == VIRG_SUCCESS &&
+			query_launch(q, q->stream) == VIRG_SUCCESS &&
 			hipEventRecord(b->joins[i], q->stream) == hipSuccess &&
 			hipStreamWaitEvent(b->stream, b->joins[i], 0) == hipSuccess;
 	}
@@ -2217,7 +2373,7 @@ extern "C" int vg_plain_time(vg_query *const *qs, unsigned n, int warmup, int it
 			hipStreamSynchronize(qs[i]->stream) == hipSuccess;
 	auto round = [&](bool mark) {
 		for (unsigned i = 0; i < n && ok; i++) {
-			ok = launch(qs[i]->P, *qs[i]->p, qs[i]->lr, qs[i]->out, qs[i]->capacity, s) == VIRG_SUCCESS;
+			ok = query_launch(qs[i], s) == VIRG_SUCCESS;
 			if (mark)
 				ok = ok && hipEventRecord(ev[i + 2], s) == hipSuccess;
 		}
@@ -2262,9 +2418,12 @@ extern "C" int vg_query_set_outputs(vg_query *q, void *const *out, unsigned long
 				(void)hipFree(q->out[j]);
 			}
 	q->own_out = false;
-	for (unsigned j = 0; j < q->P.nout; j++)
+	for (unsigned j = 0; j < q->P.nout; j++) {
 		q->out[j] = out[j];
+		q->ja.out[j] = out[j];
+	}
 	q->capacity = capacity;
+	q->ja.capacity = capacity;
 	return VIRG_SUCCESS;
 }
 

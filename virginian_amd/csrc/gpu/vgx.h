@@ -1,0 +1,67 @@
+/*
+ * Prepare-time specialised kernels (jit.hip): one single-pass kernel per
+ * decoded program structure, compiled with hiprtc and cached.
+ */
+#ifndef VGX_H
+#define VGX_H
+
+#include <stdint.h>
+#include "vg_prog.h"
+
+/* geometry of a specialised kernel */
+typedef struct {
+	int R;                     /* rows per lane per tile (4, 8 or 16) */
+	uint32_t S;                /* tiles per chunk (one grab, one look-back) */
+	uint32_t TR;               /* rows per tile = 256 * R */
+	uint32_t fbytes_row;       /* bytes per row of the filter columns staged in LDS */
+	uint32_t ftile_bytes;      /* LDS bytes of one tile's staged filter columns */
+	uint32_t stage_off;        /* LDS offset of the output staging area */
+	uint32_t stage_wave_bytes; /* output staging bytes per wave */
+	uint32_t lds_bytes;        /* dynamic LDS per workgroup */
+	int aligned;               /* every tablet's column bases are 16-byte aligned:
+	                            * result columns load 16 bytes per lane */
+} vgx_shape;
+
+/* a tile of a specialised launch: tablet (index into the descriptors) and
+ * its first row in that tablet */
+typedef struct {
+	uint32_t tab;
+	uint32_t row0;
+} vgx_tile;
+
+/* kernel arguments; the kernel's own struct has this layout */
+typedef struct {
+	const vg_tabdesc *tabs;
+	const vgx_tile *tiles;
+	uint32_t num_tiles;
+	uint32_t nchunks;
+	uint64_t *state;           /* [0] chunk counter, [1] launch epoch */
+	uint64_t *status;          /* one word per chunk */
+	vg_ctrl *ctrl;
+	uint64_t capacity;
+	uint32_t spin_limit;
+	uint32_t pad;
+	void *out[VG_MAX_OUT];
+	uint64_t imm[VG_MAX_UOPS][2];
+	uint64_t oimm[VG_MAX_OUT];
+} vgx_launch_args;
+
+typedef struct vgx_kernel vgx_kernel;
+
+/* shape for a program (R_override, S_override: 0 = chosen from the program) */
+int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S_override, vgx_shape *sh, char *err,
+	size_t errlen);
+/* kernel source (malloc'd) */
+int vgx_source(const vg_prog *P, const vgx_shape *sh, char **src);
+/* compiled kernel for P (from the cache, or compiled now); NULL + message */
+vgx_kernel *vgx_get(const vg_prog *P, int aligned, char *err, size_t errlen);
+const vgx_shape *vgx_kernel_shape(const vgx_kernel *k);
+double vgx_kernel_compile_ms(const vgx_kernel *k);
+/* workgroups for a launch of nchunks chunks on `cus` compute units */
+unsigned vgx_kernel_grid(const vgx_kernel *k, unsigned nchunks, unsigned cus);
+
+#ifdef __HIP_PLATFORM_AMD__
+int vgx_launch(const vgx_kernel *k, const vgx_launch_args *a, unsigned grid, hipStream_t stream);
+#endif
+
+#endif
