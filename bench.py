@@ -505,6 +505,77 @@ def dropin_times(db, per: list, a) -> dict:
     return {"warm": warm, "cold_t10": {"ms": round(ms, 2), "rows": n, "gbs_scanned": round(rb / ms / 1e6, 1)}}
 
 
+PMC_RUNS = 3
+
+
+def pmc_run(va, queries) -> int:
+    """Counter runs (rocprofv3 --pmc): each query's kernel PMC_RUNS times
+    in threshold order, one at a time, so the dispatches can be told apart
+    by their order."""
+    for q in queries:
+        for _ in range(PMC_RUNS):
+            q.run()
+    print(json.dumps({"pmc_run": [q.last_rows for q in queries], "runs_per_query": PMC_RUNS}))
+    return 0
+
+
+def summarize_pmc(outdir: str, dest: str) -> int:
+    """Per-launch HBM bytes from two counter runs of --pmc-run (FETCH_SIZE
+    and WRITE_SIZE, in KiB per dispatch) and of --copy-probe 256 (the same
+    counters on device copies of known size, which calibrate them), written
+    as the traffic summary bench.py reads."""
+    import csv
+    from pathlib import Path as P
+
+    def rows(counter, name):
+        f = next(P(outdir).glob(f"pmc_{counter}/**/{name}_counter_collection.csv"))
+        out = []
+        for r in csv.DictReader(open(f)):
+            if r["Counter_Name"] == counter:
+                out.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+        return sorted(out)
+
+    cal = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        copies = [v for _, k, v in rows(c, "copy") if "copy" in k.lower()]
+        cal[c] = sorted(copies)[len(copies) // 2] / (256 * 1024)
+    per = {c: [v * 1024 / cal[c] for _, k, v in rows(c, "config2") if k.startswith("vgx_main")]
+           for c in ("FETCH_SIZE", "WRITE_SIZE")}
+    n = len(THRESHOLDS)
+    assert all(len(v) == n * (1 + PMC_RUNS) for v in per.values()), {k: len(v) for k, v in per.items()}
+    queries = []
+    for i, t in enumerate(THRESHOLDS):
+        # the first dispatch of each query is bench.py's q.run() before the counter runs
+        idx = [n + i * PMC_RUNS + j for j in range(PMC_RUNS)]
+        rd = sum(per["FETCH_SIZE"][k] for k in idx) / PMC_RUNS
+        wr = sum(per["WRITE_SIZE"][k] for k in idx) / PMC_RUNS
+        queries.append({"threshold": t, "read_bytes": round(rd), "write_bytes": round(wr),
+                        "bytes_per_launch": round(rd + wr)})
+    doc = {"calibration": {k: round(v, 5) for k, v in cal.items()},
+           "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch) / counter KiB per true KiB "
+                     "measured on a 256 MiB device copy",
+           "kernel": "vgx_main (the query's specialised single-pass kernel)",
+           "queries": queries,
+           "bytes_per_launch_mean": round(sum(q["bytes_per_launch"] for q in queries) / n),
+           "commands": [
+               "cd /tmp && export TMPDIR=/tmp",
+               "rocprofv3 --pmc FETCH_SIZE --output-format csv -d <out>/pmc_FETCH_SIZE -o config2 -- "
+               "python3 -u <repo>/bench.py --pmc-run",
+               "rocprofv3 --pmc FETCH_SIZE --output-format csv -d <out>/pmc_FETCH_SIZE -o copy -- "
+               "python3 -u <repo>/bench.py --copy-probe 256",
+               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d <out>/pmc_WRITE_SIZE -o config2 -- "
+               "python3 -u <repo>/bench.py --pmc-run",
+               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d <out>/pmc_WRITE_SIZE -o copy -- "
+               "python3 -u <repo>/bench.py --copy-probe 256",
+               "python3 <repo>/bench.py --summarize-pmc <out> profiles/traffic_config2_64M.json"],
+           "attribution": f"dispatches of vgx_main in order: one run per query, then {PMC_RUNS} per query "
+                          "in threshold order; counter KiB x 1024 / calibration = bytes",
+           "hardware": "MI355X (gfx950, 256 CUs)"}
+    P(dest).write_text(json.dumps(doc, indent=1) + "\n")
+    print(json.dumps(doc, indent=1))
+    return 0
+
+
 def measured_traffic(workload: str) -> dict | None:
     """HBM traffic per launch pair recorded from counter runs, if present."""
     p = ROOT / "profiles" / f"traffic_{workload}.json"
@@ -538,7 +609,13 @@ def main() -> int:
     ap.add_argument("--train-programs", type=int, default=600, help="served: predictor training runs")
     ap.add_argument("--queries", type=int, default=256, help="batched: queries per batch")
     ap.add_argument("--batch-rows", type=int, default=8 << 20)
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="counter runs: each resident query's kernel a few times, one at a time")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("OUTDIR", "DEST"),
+                    help="traffic summary from --pmc-run counter runs (no GPU needed)")
     a = ap.parse_args()
+    if a.summarize_pmc:
+        return summarize_pmc(*a.summarize_pmc)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the executor on one GPU; figures for several GPUs are "
               "not produced (tablets shard across GPUs by design, but no multi-GPU path is "
@@ -573,21 +650,32 @@ def main() -> int:
     queries = [db.prepare(sql_for(t)) for t in THRESHOLDS]
     for q in queries:
         q.run()
+    kinfo = queries[0].kernel()
+    if a.pmc_run:
+        return pmc_run(va, queries)
 
-    # per query: launch-pair time back to back, and each pass on its own
+    # per query: kernel time back to back (HIP events on the query's
+    # stream), beside the interpreter's count + write passes for the same
+    # query
     per = []
     for t, q in zip(THRESHOLDS, queries):
         ms, n = q.time(a.warmup, a.steps)
-        ms_c, ms_w, _ = q.time_passes(a.warmup, a.steps)
+        qi = db.prepare(sql_for(t), flags=va.VG_QUERY_INTERP)
+        qi.run()
+        ms_i, _ = qi.time(a.warmup, a.steps)
+        ms_c, ms_w, _ = qi.time_passes(a.warmup, a.steps)
+        qi.close()
         read_b = q.rows * q.read_bytes_per_row
         write_b = n * q.write_bytes_per_row
         per.append({"threshold": t, "selected": n, "selectivity": n / q.rows, "kernel_ms": ms,
-                    "count_pass_ms": ms_c, "write_pass_ms": ms_w,
                     "read_bytes": read_b, "write_bytes": write_b,
-                    "gbs_scanned": read_b / ms / 1e6, "gbs_total": (read_b + write_b) / ms / 1e6})
-        print(f"# T={t}: {n} rows ({n / q.rows:.4%}) {ms:.3f} ms/launch pair "
-              f"(count {ms_c:.3f} + write {ms_w:.3f}), {read_b / ms / 1e6:.0f} GB/s scanned, "
-              f"{(read_b + write_b) / ms / 1e6:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
+                    "gbs_scanned": read_b / ms / 1e6, "gbs_total": (read_b + write_b) / ms / 1e6,
+                    "interpreter_ms": ms_i, "interpreter_count_pass_ms": ms_c,
+                    "interpreter_write_pass_ms": ms_w})
+        print(f"# T={t}: {n} rows ({n / q.rows:.4%}) {ms:.3f} ms/launch "
+              f"({'specialised single pass' if q.kernel()['fused'] else 'interpreter'}), "
+              f"{read_b / ms / 1e6:.0f} GB/s scanned, {(read_b + write_b) / ms / 1e6:.0f} GB/s algorithmic; "
+              f"interpreter {ms_i:.3f} ms (count {ms_c:.3f} + write {ms_w:.3f})", file=sys.stderr, flush=True)
 
     dropin = dropin_times(db, per, a)
 
@@ -618,8 +706,9 @@ def main() -> int:
     tr = measured_traffic("config2_64M")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": tr["bytes_per_launch_pair_mean"] if tr else None,
-                "kernel": "vg_count_kernel + vg_write_kernel (one query)",
+                "traffic": tr.get("bytes_per_launch_mean") if tr else None,
+                "kernel": ("vgx_main: the query's specialised single-pass kernel" if kinfo["fused"]
+                           else "vg_count_kernel + vg_write_kernel (one query)"),
                 "algorithmic_bytes_per_launch": round(mean_bytes)}
 
     cpu = None
@@ -642,6 +731,7 @@ def main() -> int:
                                "tablets, selectivity sweep 0.1/10/50/90 %",
                    "rows": a.rows, "tablet_rows": a.tablet_rows, "queries_per_step": len(queries),
                    "pct_hbm_peak": round(value / HBM_PEAK_GBS * 100, 2),
+                   "kernel": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in kinfo.items()},
                    "per_query": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()}
                                  for p in per],
                    "dropin_virg_vm_execute": dropin},

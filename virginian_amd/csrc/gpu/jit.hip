@@ -162,7 +162,12 @@ __device__ u64 lookback(u64 *st, u32 c, u32 n, u32 lane, u64 epoch, u32 spin, bo
 				*gave_up = true;
 				break;
 			}
-			__builtin_amdgcn_s_sleep(2);
+			/* back off: polls are device-scope loads that compete with
+			 * the streaming loads of every other wave */
+			if (polls < 4)
+				__builtin_amdgcn_s_sleep(2);
+			else
+				__builtin_amdgcn_s_sleep(12);
 			continue;
 		}
 		sum += wsum64(((upto >> lane) & 1ull) ? (v & ST_VAL) : 0ull);
@@ -527,7 +532,7 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 	  "\t\ts_epoch = (u32)e;\n\t\ts_chunk = grab(A, e);\n\t}\n\t__syncthreads();\n");
 	o("\tconst u64 epoch = s_epoch;\n\tu32 ovf = 0;\n");
 	o("\tfor (;;) {\n\t\tconst u32 c = uni(s_chunk);\n\t\tif (c >= A.nchunks)\n\t\t\tbreak;\n");
-	o("\t\tu32 nxt = 0;\n\t\tif (tid == 0)\n\t\t\tnxt = grab(A, epoch);\n");
+	o("\t\tu32 nxt = 0;\n");
 	o("\t\tconst u32 t0 = c * S_;\n\t\tconst u32 nt = uni(min((u32)S_, A.num_tiles - t0));\n");
 
 	/* tile lookup */
@@ -622,6 +627,10 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 	o("\t\t/* phase 2: result columns, tile by tile */\n");
 	o("\t\tu64 pre = 0;\n\t\tu32 tdone = 0;\n");
 	o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
+	/* the next chunk is taken only as this one's last tile starts: a chunk
+	 * taken is a chunk worked on, so later chunks never wait for one that
+	 * sits behind a whole chunk of its workgroup's work */
+	o("\t\t\tif (s + 1 == nt && tid == 0)\n\t\t\t\tnxt = grab(A, epoch);\n");
 	o("\t\t\tconst u32 sel = (u32)(vb >> (s * R)) & RMASK;\n");
 	o("%s", decl.s.c_str());
 	/* HBM loads of project / emit columns, only for 4-row groups with a
@@ -821,6 +830,21 @@ static vgx_kernel *compile(const std::string &src, const vgx_shape &sh, char *er
 	std::vector<char> code(cs);
 	hiprtcGetCode(prog, code.data());
 	hiprtcDestroyProgram(&prog);
+	if (const char *dir = getenv("VG_JIT_DUMP")) {
+		/* source and code object of every kernel compiled, for inspection */
+		char path[512];
+		const size_t h = std::hash<std::string>()(src);
+		snprintf(path, sizeof path, "%s/vgx_%016zx.hip", dir, h);
+		if (FILE *f = fopen(path, "w")) {
+			fwrite(src.data(), 1, src.size(), f);
+			fclose(f);
+		}
+		snprintf(path, sizeof path, "%s/vgx_%016zx.co", dir, h);
+		if (FILE *f = fopen(path, "wb")) {
+			fwrite(code.data(), 1, code.size(), f);
+			fclose(f);
+		}
+	}
 	vgx_kernel *k = new vgx_kernel();
 	k->sh = sh;
 	if (hipModuleLoadData(&k->mod, code.data()) != hipSuccess ||
