@@ -294,6 +294,26 @@ int vg_gen_table(virginian *v, const char *name, unsigned long long rows, unsign
 int vg_decode_program(const virg_op *ops, unsigned num_ops, unsigned pc,
 	const virg_tablet_meta *tab, void *out, size_t out_bytes, char *err, size_t errlen);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops and harnesses that
+ * forbid hipMalloc).  Runs the data-parallel section of a generated program
+ * (ops as virg_sql or virg_vm_addop make them) over `rows` rows already in
+ * device memory -- cols[c] is column c's row 0, col_types[c] its virg_t,
+ * key the key column (read only by Rowid; may be NULL otherwise) -- into
+ * the caller's device outputs out[j] (room for `capacity` rows of result
+ * column j), on `stream`, asynchronously, with the interpreter's count and
+ * write passes (the results equal vg_query_run's bit for bit).  result is a
+ * 32-byte device block: bytes 0-7 the rows selected, bytes 8-11 nonzero
+ * when they exceeded capacity.  The call allocates nothing: its scratch is
+ * a fixed array of about 4.4 MB per GPU that the library holds until the
+ * process ends, so rows beyond 16,777,216 are refused.  Calls that overlap
+ * on one GPU share that array and race.  HIP errors on the stream are the
+ * caller's to check.  Declared here with no stream type: stream is a
+ * hipStream_t. */
+int vg_select_noalloc(const virg_op *ops, unsigned num_ops, const void *const *cols,
+	const int *col_types, unsigned ncols, const void *key, int key_type, unsigned long long rows,
+	void *const *out, unsigned long long capacity, void *result, void *stream, char *err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -228,6 +228,9 @@ def _load() -> C.CDLL:
         "vg_decode_program": (C.c_int, [C.POINTER(Op), C.c_uint, C.c_uint, C.POINTER(TabletMeta),
                                         P, C.c_size_t, C.c_char_p, C.c_size_t]),
         "vg_jit_policy": (C.c_int, [C.c_int]),
+        "vg_select_noalloc": (C.c_int, [C.POINTER(Op), C.c_uint, C.POINTER(P), C.POINTER(C.c_int), C.c_uint, P,
+                                        C.c_int, C.c_ulonglong, C.POINTER(P), C.c_ulonglong, P, P, C.c_char_p,
+                                        C.c_size_t]),
         "vg_query_kernel": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint),
                                       C.POINTER(C.c_uint), C.POINTER(C.c_double)]),
         "vg_jit_source": (C.c_long, [C.POINTER(Op), C.c_uint, C.c_uint, C.POINTER(TabletMeta), C.c_char_p,

@@ -22,6 +22,7 @@
 #include <stdarg.h>
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <chrono>
 #include <thread>
 #include <unordered_map>
@@ -114,6 +115,45 @@ static vgd_device *dev_of(virginian *v)
 	return s ? s->dev : nullptr;
 }
 
+/* ---------------------------------------------- allocation-free launches */
+/* Scratch of vg_select_noalloc: the launch's tablet descriptor and tile
+ * table, validity bytes and per-wave counts of every tile, per-workgroup
+ * counts.  One array per GPU the library runs on, held until the process
+ * ends.  Host code never uses the array's name as an address: the device
+ * address of each GPU's copy comes from hipGetSymbolAddress, once per
+ * device with that device current (noalloc_scratch). */
+#define VG_NOALLOC_MAX_ROWS  (16u << 20)
+#define VG_NOALLOC_MAX_TILES (VG_NOALLOC_MAX_ROWS / 1024u)
+#define VG_NA_DESC    0u
+#define VG_NA_TILES   256u
+#define VG_NA_VBITS   (VG_NA_TILES + VG_NOALLOC_MAX_TILES * 4u)
+#define VG_NA_TCOUNT  (VG_NA_VBITS + VG_NOALLOC_MAX_TILES * 256u)
+#define VG_NA_WGCOUNT (VG_NA_TCOUNT + VG_NOALLOC_MAX_TILES * 16u)
+#define VG_NA_BYTES   (VG_NA_WGCOUNT + VG_NOALLOC_MAX_TILES * 4u)
+static __device__ __align__(256) uint8_t scratch[VG_NA_BYTES];
+
+#define VG_NA_MAX_DEVICES 64
+static std::mutex g_na_mu;
+static uint8_t *g_na_ptr[VG_NA_MAX_DEVICES];
+
+/* Device address of the current device's scratch (looked up once per
+ * device); NULL if the lookup fails. */
+static uint8_t *noalloc_scratch(int dev)
+{
+	if (dev < 0 || dev >= VG_NA_MAX_DEVICES)
+		return nullptr;
+	std::lock_guard<std::mutex> g(g_na_mu);
+	if (!g_na_ptr[dev]) {
+		void *p = nullptr;
+		if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+			(void)hipGetLastError();
+			return nullptr;
+		}
+		g_na_ptr[dev] = (uint8_t *)p;
+	}
+	return g_na_ptr[dev];
+}
+
 extern "C" int vgd_open(virginian *v)
 {
 	virg_state *s = virg_state_of(v);
@@ -134,6 +174,7 @@ extern "C" int vgd_open(virginian *v)
 		return 0;
 	}
 	s->dev = d;
+	(void)noalloc_scratch(d->device);     /* set up while allocation is allowed */
 	return 1;
 }
 
@@ -2480,4 +2521,110 @@ extern "C" unsigned vg_query_tablet_rows(vg_query *q, unsigned long long *rows, 
 		rows[i] = s;
 	}
 	return (unsigned)p.desc.size();
+}
+
+/* the launch's one-tablet descriptor and its all-zero tile table, written
+ * into the scratch by a kernel (no memset or copy nodes, so the call can be
+ * captured in a graph) */
+__global__ __launch_bounds__(256) void vg_noalloc_setup(vg_tabdesc *desc, uint32_t *tiles, uint32_t ntiles,
+	const vg_tabdesc d)
+{
+	const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+	if (i == 0)
+		*desc = d;
+	if (i < ntiles)
+		tiles[i] = 0;
+}
+
+/* reference: none (a variant of vg_query_run for callers that cannot
+ * allocate).  Runs the data-parallel section of ops over `rows` rows whose
+ * columns are already in device memory (cols[c]: column c's row 0; key:
+ * the key column, needed only by Rowid) into the caller's outputs, with
+ * the interpreter's count and write passes, on `stream`, asynchronously.
+ * result (device, 32 bytes, a vg_ctrl): [0] rows selected, [1] low word
+ * nonzero when they exceeded `capacity`.  Scratch comes from the library's
+ * per-GPU array; calls that overlap on one GPU race on it. */
+extern "C" int vg_select_noalloc(const virg_op *ops, unsigned num_ops, const void *const *cols,
+	const int *col_types, unsigned ncols, const void *key, int key_type, unsigned long long rows,
+	void *const *out, unsigned long long capacity, void *result, void *stream_, char *err, size_t errlen)
+{
+	hipStream_t stream = (hipStream_t)stream_;
+	if (rows > VG_NOALLOC_MAX_ROWS)
+		return qerr(err, errlen, "%llu rows: the allocation-free path takes at most %u rows", rows,
+			VG_NOALLOC_MAX_ROWS);
+	if (ncols > VIRG_MAX_COLUMNS || ncols > VG_MAX_COLS - 1)
+		return qerr(err, errlen, "%u columns: at most %d", ncols, VIRG_MAX_COLUMNS);
+	if (!result)
+		return qerr(err, errlen, "no result block");
+	int par = -1;
+	for (unsigned i = 0; i < num_ops && par < 0; i++)
+		if (ops[i].op == OP_Parallel)
+			par = (int)i;
+	if (par < 0)
+		return qerr(err, errlen, "program has no Parallel op");
+	virg_tablet_meta tab;
+	memset(&tab, 0, sizeof tab);
+	tab.fixed_columns = ncols;
+	tab.key_type = (virg_t)key_type;
+	for (unsigned c = 0; c < ncols; c++) {
+		tab.fixed_type[c] = (virg_t)col_types[c];
+		tab.fixed_stride[c] = virg_sizeof((virg_t)col_types[c]);
+	}
+	vg_prog P;
+	if (vg_decode(ops, num_ops, (unsigned)par + 1, &tab, nullptr, &P, err, errlen) == VIRG_FAIL)
+		return VIRG_FAIL;
+	if (vg_kernel_check(&P) != 0)
+		return qerr(err, errlen, "op-program needs more register slots than the LDS holds");
+	for (unsigned j = 0; j < P.nout; j++)
+		if (!out[j])
+			return qerr(err, errlen, "result column %u has no output", j);
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess) {
+		(void)hipGetLastError();
+		return qerr(err, errlen, "no current device");
+	}
+	uint8_t *sp = noalloc_scratch(dev);
+	if (!sp)
+		return qerr(err, errlen, "scratch of device %d not found", dev);
+	vg_tabdesc d;
+	memset(&d, 0, sizeof d);
+	bool aligned = true;
+	for (unsigned c = 0; c < ncols; c++) {
+		d.col[c] = (uint64_t)cols[c];
+		aligned = aligned && (d.col[c] % 16 == 0);
+	}
+	d.col[VG_KEY_SRC] = (uint64_t)key;
+	aligned = aligned && ((uint64_t)key % 16 == 0);
+	d.rows = (uint32_t)rows;
+	d.aligned = aligned ? 1u : 0u;
+	const uint32_t T = vg_kernel_tile_rows();
+	const uint32_t ntiles = (uint32_t)((rows + T - 1) / T);
+	vg_ctrl *ctrl = (vg_ctrl *)result;
+	if (ntiles == 0) {
+		/* nothing to scan: the result block still says 0 rows */
+		hipLaunchKernelGGL(vg_noalloc_setup, dim3(1), dim3(256), 0, stream, (vg_tabdesc *)(sp + VG_NA_DESC),
+			(uint32_t *)(sp + VG_NA_TILES), 0u, d);
+		return herr(hipGetLastError(), "setup") ? VIRG_FAIL : VIRG_SUCCESS;
+	}
+	hipLaunchKernelGGL(vg_noalloc_setup, dim3((ntiles + 255) / 256), dim3(256), 0, stream,
+		(vg_tabdesc *)(sp + VG_NA_DESC), (uint32_t *)(sp + VG_NA_TILES), ntiles, d);
+	if (herr(hipGetLastError(), "setup"))
+		return VIRG_FAIL;
+	vg_launch L;
+	memset(&L, 0, sizeof L);
+	L.tabs = (const vg_tabdesc *)(sp + VG_NA_DESC);
+	L.tile_tab = (const uint32_t *)(sp + VG_NA_TILES);
+	L.num_tiles = ntiles;
+	L.capacity = capacity;
+	L.ctrl = ctrl;
+	L.vbits = sp + VG_NA_VBITS;
+	L.tcount = (uint32_t *)(sp + VG_NA_TCOUNT);
+	L.wgcount = (uint32_t *)(sp + VG_NA_WGCOUNT);
+	for (unsigned j = 0; j < P.nout; j++)
+		L.out[j] = out[j];
+	if (vg_kernel_grid(&P, ntiles) > VG_NOALLOC_MAX_TILES)
+		return qerr(err, errlen, "count grid larger than the scratch holds");
+	if (vg_kernel_launch(&P, &L, stream, nullptr) != 0)
+		return qerr(err, errlen, "kernel launch failed");
+	return VIRG_SUCCESS;
 }
