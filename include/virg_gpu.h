@@ -56,13 +56,17 @@ typedef struct vg_query vg_query;
 vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned num_ops,
 	unsigned long long capacity, int flags, char *err, size_t errlen);
 /* Results go to caller device memory from now on: out[j] has room for
- * `capacity` rows of result column j. */
+ * `capacity` rows of result column j.  Refused (VIRG_FAIL) while a batch
+ * (vg_batch, vg_pbatch) holds the query: a batch keeps a copy of each
+ * query's launch arguments, outputs included. */
 int vg_query_set_outputs(vg_query *q, void *const *out, unsigned long long capacity);
 /* Bytes per row of each result column; returns the number of columns. */
 unsigned vg_query_out_widths(vg_query *q, unsigned *widths);
 /* Device memory for caller-managed outputs (a serving arena). */
 void *vg_device_alloc(virginian *v, size_t bytes);
 void vg_device_free(virginian *v, void *p);
+/* Refused (with a message, the query left as it was) while a batch holds
+ * the query: free the batch, or reload it without the query, first. */
 void vg_query_free(vg_query *q);
 
 /* Set when prepared queries get a specialised kernel (policy >= 0) and
@@ -103,11 +107,13 @@ int vg_query_info(vg_query *q, unsigned long long *rows, unsigned *tiles,
 	unsigned *read_bytes_per_row, unsigned *write_bytes_per_row, unsigned *nout);
 
 /* A batch of prepared queries (distinct, on one device) replayed as one
- * HIP graph: each query's launch pair on its own forked stream, so small
- * queries run side by side and the host submits one graph per batch.
- * Results stay with each query (vg_query_fetch after vg_batch_sync).
- * The queries must not be launched by other means while the batch is in
- * flight. */
+ * HIP graph: each query's launch (its specialised kernel, or the count and
+ * write passes) on its own forked stream, so small queries run side by
+ * side and the host submits one graph per batch.  Results stay with each
+ * query (vg_query_fetch after vg_batch_sync).  The queries must not be
+ * launched by other means while the batch is in flight, and the batch
+ * holds them (vg_query_free and vg_query_set_outputs refuse them) until
+ * it is freed. */
 typedef struct vg_batch vg_batch;
 vg_batch *vg_batch_create(vg_query *const *qs, unsigned n, char *err, size_t errlen);
 void vg_batch_free(vg_batch *b);
@@ -118,11 +124,15 @@ int vg_batch_sync(vg_batch *b, unsigned long long *rows);
 int vg_batch_time(vg_batch *b, int warmup, int iters, double *ms);
 
 /* A batch of prepared queries (distinct, on one device) run by persistent
- * launches: a count launch works through a device-resident list of
- * (query, tile range) items, every workgroup taking the next item from an
- * atomic counter, and the last item of each query to finish turns its
- * items' counts into output offsets; a write launch then works through the
- * write items the same way.  Queries are grouped by the occupancy their
+ * launches over device-resident lists of (query, tile range) items: a
+ * count launch whose workgroups take the items statically (workgroup g
+ * items g, g + grid, ...), a scan launch turning each query's item counts
+ * into output offsets, a write launch dealing out the write items the same
+ * way, and a finish launch recording each query's outcome.  The single
+ * pass (config.single) instead takes each query's tiles in order through
+ * that query's own counter and finds each tile's output offset by
+ * look-back.  The batch holds its queries (vg_query_free and
+ * vg_query_set_outputs refuse them) until it is freed or reloaded.  Queries are grouped by the occupancy their
  * register files allow, each group with its own pair of launches on its
  * own branch; the reset launch and every group's pair are captured as one
  * HIP graph, so a run is one graph launch whatever the number of queries.

@@ -44,6 +44,9 @@ def lib() -> C.CDLL:
         _lib.ref_run.argtypes = [C.POINTER(va.Op), C.c_uint, C.POINTER(C.c_void_p), C.c_uint,
                                  C.c_ulonglong]
         _lib.ref_run.restype = C.c_longlong
+        _lib.ref_run_reader.argtypes = [C.POINTER(va.Op), C.c_uint, C.c_void_p, C.c_uint, C.c_ulonglong,
+                                        C.POINTER(C.c_ulonglong)]
+        _lib.ref_run_reader.restype = C.c_longlong
     return _lib
 
 
@@ -108,3 +111,16 @@ class RefDB:
         if rows < 0:
             raise RuntimeError("ref_run failed")
         return {nm: o[:rows] for nm, o in zip(names, outs)}
+
+    def run_reader(self, ops, row_bytes: int, cap: int) -> tuple[np.ndarray, int]:
+        """Run an op program, then read its result with the reference's own
+        virg_reader_row: the reader's buffer (row_bytes of it) after every
+        call up to and including the first VIRG_FAIL, and the number of
+        calls that returned VIRG_SUCCESS."""
+        arr, n = ops
+        out = np.zeros((cap, row_bytes), dtype=np.uint8)
+        ok = C.c_ulonglong(0)
+        calls = lib().ref_run_reader(arr, n, out.ctypes.data, row_bytes, cap, C.byref(ok))
+        if calls < 0:
+            raise RuntimeError("ref_run_reader failed")
+        return out[:calls], ok.value

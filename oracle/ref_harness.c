@@ -139,15 +139,13 @@ API int ref_insert(unsigned table, const void *key, int key_width, int ncols,
 	return VIRG_SUCCESS;
 }
 
-/* Run a program; copy the logical result (the result-tablet chain in
- * order) into out[j] up to cap rows.  Returns the row count or -1. */
-API long long ref_run(const virg_op *ops, unsigned num_ops, void **out, unsigned nout,
-	unsigned long long cap)
+/* Run a program through the serial ops of execute.c and the reference's
+ * virg_vm_cpu; returns the vm with its result-tablet chain, or NULL. */
+static virg_vm *ref_exec(const virg_op *ops, unsigned num_ops)
 {
 	virg_vm *vm = virg_vm_init();
 	virg_tablet_meta *tab = NULL, *res;
 	unsigned i;
-	long long total = 0;
 	for (i = 0; i < num_ops; i++) {
 		virg_var p4 = ops[i].p4;
 		/* virg_vm_cleanup frees ResultColumn names (vm/cleanup.c:17-22) */
@@ -155,11 +153,11 @@ API long long ref_run(const virg_op *ops, unsigned num_ops, void **out, unsigned
 			const char *nm = ops[i].p4.s ? ops[i].p4.s : "";
 			p4.s = malloc(strlen(nm) + 1);
 			if (!p4.s)
-				return -1;
+				return NULL;
 			memcpy(p4.s, nm, strlen(nm) + 1);
 		}
 		if (virg_vm_addop(vm, ops[i].op, ops[i].p1, ops[i].p2, ops[i].p3, p4) == VIRG_FAIL)
-			return -1;
+			return NULL;
 	}
 	/* execute.c:38-44 */
 	vm->num_tables = 0;
@@ -186,9 +184,21 @@ API long long ref_run(const virg_op *ops, unsigned num_ops, void **out, unsigned
 			virg_tablet_unlock(V, res->id);
 			break;
 		} else {
-			return -1;
+			return NULL;
 		}
 	}
+	return vm;
+}
+
+/* Run a program; copy the logical result (the result-tablet chain in
+ * order) into out[j] up to cap rows.  Returns the row count or -1. */
+API long long ref_run(const virg_op *ops, unsigned num_ops, void **out, unsigned nout,
+	unsigned long long cap)
+{
+	virg_vm *vm = ref_exec(ops, num_ops);
+	long long total = 0;
+	if (!vm)
+		return -1;
 	/* the logical result: every result tablet of the chain, in order */
 	virg_result_node *node = vm->head_result;
 	while (node) {
@@ -209,4 +219,35 @@ API long long ref_run(const virg_op *ops, unsigned num_ops, void **out, unsigned
 	virg_vm_freeresults(V, vm);
 	virg_vm_cleanup(V, vm);
 	return total;
+}
+
+/* Run a program as ref_run does, then read its result with the reference's
+ * reader (src/reader/init.c, row.c): rows + k * row_bytes receives
+ * r.buffer after the k-th call of virg_reader_row, for every call up to
+ * and including the first that returns VIRG_FAIL (at most cap calls);
+ * *nsuccess = the calls that returned VIRG_SUCCESS.  Returns the calls
+ * made, or -1. */
+API long long ref_run_reader(const virg_op *ops, unsigned num_ops, void *rows, unsigned row_bytes,
+	unsigned long long cap, unsigned long long *nsuccess)
+{
+	virg_vm *vm = ref_exec(ops, num_ops);
+	virg_reader r;
+	unsigned long long calls = 0;
+	if (!vm || row_bytes > VIRG_ROW_BUFFER)
+		return -1;
+	*nsuccess = 0;
+	if (virg_reader_init(V, &r, vm) != VIRG_SUCCESS)
+		return -1;
+	while (calls < cap) {
+		const int rc = virg_reader_row(V, &r);
+		memcpy((char *)rows + calls * row_bytes, r.buffer, row_bytes);
+		calls++;
+		if (rc == VIRG_FAIL)
+			break;
+		(*nsuccess)++;
+	}
+	virg_reader_free(V, &r);
+	virg_vm_freeresults(V, vm);
+	virg_vm_cleanup(V, vm);
+	return (long long)calls;
 }

@@ -4,6 +4,7 @@ result it gives alone (and the oracle's, bit for bit and in row order),
 on every replay, with eager launches of the same queries in between."""
 from __future__ import annotations
 
+import ctypes as C
 import time
 
 import numpy as np
@@ -202,3 +203,30 @@ def test_single_pass_capacity(gpu_db):
         b.close()
         for q in qs:
             q.close()
+
+
+def test_batch_holds_its_queries(gpu_db):
+    """While a batch holds a query's launch arguments, the query's outputs
+    cannot change and the query cannot be freed (nothing is launched until
+    the batch has let go)."""
+    gpu_db.gen_table("test", 5000, 1024, seed=1)
+    q = gpu_db.prepare("select id from test where uniformi < 50")
+    q2 = gpu_db.prepare("select id from test where uniformi < 20")
+    buf = gpu_db.device_alloc(5000 * 8)
+    ptrs = (C.c_void_p * 1)(buf)
+    for make in (lambda: va.GraphBatch([q, q2]), lambda: va.PersistentBatch([q, q2])):
+        b = make()
+        assert va.lib.vg_query_set_outputs(q.h, ptrs, 5000) == va.VIRG_FAIL
+        va.lib.vg_query_free(q.h)          # refused: q stays usable
+        b.close()
+    pb = va.PersistentBatch([q, q2], direct=1, db=gpu_db)
+    pb.reload([q2])                        # lets go of q
+    assert va.lib.vg_query_set_outputs(q.h, ptrs, 5000) == va.VIRG_SUCCESS
+    assert va.lib.vg_query_set_outputs(q2.h, ptrs, 5000) == va.VIRG_FAIL
+    pb.close()
+    n = q.run()
+    want, _ = oracle.select(gpu_db, "select id from test where uniformi < 50")
+    assert n == len(want["id"])
+    q.close()
+    q2.close()
+    gpu_db.device_free(buf)

@@ -3,15 +3,22 @@ format: tablets are read from the file by virg_db_load on first use, then
 run through the drop-in boundary (virg_vm_execute -> virg_vm_gpu) and as a
 resident query (vg_query_*); results equal the oracle's over the database
 before it was written.  A file written by the reference's own code
-(oracle/_ref) is queried the same way where that library was built."""
+(tests/golden/ref_db_20000.db.gz, made in the development container by
+tests/golden/make_ref_db.py; the reference never runs here) is queried the
+same way."""
 from __future__ import annotations
+
+import gzip
+from pathlib import Path
 
 import pytest
 
 import virginian_amd as va
-from oracle import oracle, ref
+from oracle import oracle
 from tests.helpers import CONFIG1_SQL, assert_same, config2_sql
 from tests.test_oracle import gen_columns
+
+REF_DB = Path(__file__).resolve().parent / "golden" / "ref_db_20000.db.gz"
 
 pytestmark = pytest.mark.gpu
 
@@ -58,15 +65,19 @@ def test_gpu_queries_over_our_file(tmp_path):
         db.close()
 
 
-@pytest.mark.skipif(not ref.available(), reason="reference VM not built (oracle/build_ref.sh)")
 def test_gpu_queries_over_reference_file(tmp_path):
-    c = gen_columns(ROWS, SEED)
+    """The committed file the reference wrote: 20,000 rows of the seeded
+    generator (seed 7), which the oracle also reads from the file."""
     path = tmp_path / "ref.db"
-    r = ref.RefDB(path=path)
-    r.table("test", c["id"], [(n, c[n]) for n in COLS])
-    r.write_close()
+    path.write_bytes(gzip.decompress(REF_DB.read_bytes()))
     db = _open(path)
     try:
+        tabs = db.tablets(db.table_id("test"))
+        assert sum(t.contents.rows for t in tabs) == 20_000
+        c = gen_columns(20_000, SEED)
+        got = db.query("select id, uniformi, normalf20 from test where uniformi < 1000")
+        assert (got["id"] == c["id"]).all() and (got["uniformi"] == c["uniformi"]).all()
+        assert (got["normalf20"].view("u4") == c["normalf20"].view("u4")).all()
         want = [oracle.select(db, q)[0] for q in SQLS]
         _check_gpu(db, want)
     finally:

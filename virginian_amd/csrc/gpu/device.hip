@@ -945,7 +945,17 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 		return VIRG_SUCCESS;
 	}
 
-	hipEvent_t ev[4];
+	/* timing events, destroyed on every way out */
+	struct events {
+		hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+		~events()
+		{
+			for (hipEvent_t x : e)
+				if (x)
+					(void)hipEventDestroy(x);
+		}
+	} evs;
+	hipEvent_t *ev = evs.e;
 	for (int i = 0; i < 4; i++)
 		HCHK(hipEventCreate(&ev[i]), "event");
 	HCHK(hipEventRecord(ev[0], d->stream), "event");
@@ -1023,8 +1033,6 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 	vm->timing1 = ms[0] / 1000.0f;
 	vm->timing2 = ms[1] / 1000.0f;
 	vm->timing3 = ms[2] / 1000.0f;
-	for (int i = 0; i < 4; i++)
-		(void)hipEventDestroy(ev[i]);
 	return VIRG_SUCCESS;
 }
 
@@ -1124,7 +1132,23 @@ struct vg_query {
 	uint64_t *d_jstate = nullptr;      /* chunk counter, epoch, one status word per chunk */
 	uint32_t jgrid = 0;
 	vgx_launch_args ja;
+	/* batches (vg_batch, vg_pbatch) holding this query's launch arguments:
+	 * while any does, its outputs may not change and it may not be freed */
+	unsigned batch_refs = 0;
 };
+
+static void refs_take(const std::vector<vg_query *> &qs)
+{
+	for (vg_query *q : qs)
+		q->batch_refs++;
+}
+
+static void refs_drop(const std::vector<vg_query *> &qs)
+{
+	for (vg_query *q : qs)
+		if (q->batch_refs)
+			q->batch_refs--;
+}
 
 /* timing events, made on first use (light queries have none) */
 static int ensure_events(vg_query *q)
@@ -1259,6 +1283,11 @@ extern "C" void vg_query_free(vg_query *q)
 {
 	if (!q)
 		return;
+	if (q->batch_refs) {
+		fprintf(stderr, "::: Virginian: vg_query_free refused: %u batch(es) still hold the query "
+			"(free or reload them first)\n", q->batch_refs);
+		return;
+	}
 	/* a light query is freed after its results were collected */
 	if (q->stream && !q->pooled_ctrl)
 		(void)hipStreamSynchronize(q->stream);
@@ -1594,6 +1623,7 @@ extern "C" int vg_decode_program(const virg_op *ops, unsigned num_ops, unsigned 
  * launch resources are sized when the batch is built. */
 struct vg_batch {
 	std::vector<vg_query *> qs;
+	bool refs = false;          /* the queries count this batch in batch_refs */
 	hipStream_t stream = nullptr;
 	hipEvent_t fork = nullptr;
 	std::vector<hipEvent_t> joins;
@@ -1607,6 +1637,8 @@ extern "C" void vg_batch_free(vg_batch *b)
 		return;
 	if (b->stream)
 		(void)hipStreamSynchronize(b->stream);
+	if (b->refs)
+		refs_drop(b->qs);
 	if (b->exec) (void)hipGraphExecDestroy(b->exec);
 	if (b->graph) (void)hipGraphDestroy(b->graph);
 	for (hipEvent_t e : b->joins)
@@ -1634,6 +1666,8 @@ extern "C" vg_batch *vg_batch_create(vg_query *const *qs, unsigned n, char *err,
 				return nullptr;
 			}
 	}
+	refs_take(b->qs);
+	b->refs = true;
 #define BCHK(call, what) do { if (herr((call), (what))) { qerr(err, errlen, "%s failed", what); \
 	vg_batch_free(b); return nullptr; } } while (0)
 	if (n)
@@ -1835,6 +1869,7 @@ extern "C" void vg_pbatch_free(vg_pbatch *b)
 		return;
 	if (b->stream)
 		(void)hipStreamSynchronize(b->stream);
+	refs_drop(b->qs);
 	pbatch_drop_graph(b);
 	void *bufs[] = { b->d_progs, b->d_launch, b->d_info, b->d_q, b->d_head, b->d_items, b->d_witems,
 		b->d_icount, b->d_ibase, b->d_status, b->d_stamp, b->d_gq };
@@ -2122,7 +2157,9 @@ static int pbatch_set_queries(vg_pbatch *b, vg_query *const *qs, unsigned n, cha
 			if (qs[j] == qs[i])
 				return qerr(err, errlen, "query %u appears twice in the batch", i);
 	}
+	refs_drop(b->qs);
 	b->qs.assign(qs, qs + n);
+	refs_take(b->qs);
 	std::vector<vg_prog> progs(n);
 	std::vector<vg_launch> launches(n);
 	for (unsigned i = 0; i < n; i++) {
@@ -2452,6 +2489,11 @@ extern "C" int vg_plain_time(vg_query *const *qs, unsigned n, int warmup, int it
  * room for `capacity` rows of result column j (its width in bytes). */
 extern "C" int vg_query_set_outputs(vg_query *q, void *const *out, unsigned long long capacity)
 {
+	if (q->batch_refs) {
+		fprintf(stderr, "::: Virginian: vg_query_set_outputs refused: %u batch(es) hold the query's "
+			"launch arguments (free or reload them first)\n", q->batch_refs);
+		return VIRG_FAIL;
+	}
 	if (q->own_out)
 		for (unsigned j = 0; j < VG_MAX_OUT; j++)
 			if (q->out[j]) {

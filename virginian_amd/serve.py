@@ -130,6 +130,9 @@ class _Query:
             self.h = None
 
 
+# vg_ctrl overflow bits (csrc/gpu/vg_prog.h): a single-pass wait gave up
+VG_OVF_WAIT = 2
+
 # bytes per value of each virg_t (INT, INT64, FLOAT, DOUBLE, CHAR)
 _WIDTH = [4, 8, 4, 8, 1]
 
@@ -227,8 +230,10 @@ class Server:
         done = 0
         for (r, q), n, f in zip(batch, rows, flags):
             if f:
-                # did not fit: again, in two passes, at the exact size counted
-                r.cap = int(n)
+                # did not fit: again, in two passes, at the exact size
+                # counted -- unless a single-pass wait gave up, which leaves
+                # no usable count: then at the table's size
+                r.cap = int(n) if not (f & VG_OVF_WAIT) else int(self.stats[PR.table_of(r.ops)].rows.sum())
                 r.need = -1
                 r.rerun = True
                 reruns.append(r)
@@ -239,9 +244,11 @@ class Server:
                 r.ready = time.perf_counter() - t0
                 r.rows = int(n)
                 done += 1
+        # the runner lets go of the queries before they are freed
+        runner.reload([])
+        for _, q in batch:
             q.close()
         self.arena.release(*block)
-        runner.reload([])
         return done
 
     def _run_plain(self, reqs: list[Request]) -> None:
