@@ -70,21 +70,45 @@ def copy_probe(mib: int, reps: int = 5) -> None:
     hip.hipFree(b)
 
 
-def cpu_baseline(db, sql: str, seconds: float) -> dict:
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: this process's cores, at most the
+    16 a one-GPU host gives a job."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(n, 16))
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(db, sql: str, seconds: float, table: str | None = None) -> dict:
     """The oracle's multi-threaded CPU VM (the reference's multi-core
-    scheduling: threads take whole tablets) over the table, repeated until
-    about `seconds` of wall time have run."""
+    scheduling: threads take whole tablets, src/vm/vm_cpu.c:58-91) over the
+    table, results materialised as the GPU's are, repeated until about
+    `seconds` of wall time have run."""
     from oracle import oracle
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 16))
+    threads = cpu_threads()
     reps = 0
     t0 = time.perf_counter()
     while True:
-        oracle.select(db, sql, threads=threads, ordered=False, count_only=True)
+        oracle.select(db, sql, table=table, threads=threads, ordered=False)
         reps += 1
         if time.perf_counter() - t0 >= seconds:
             break
     return {"reps": reps, "seconds": time.perf_counter() - t0, "threads": threads}
+
+
+def cpu_calibration() -> dict | None:
+    """The restatement against the reference's own CPU VM, timed on the same
+    inputs in the development container (tests/golden/make_cpu_calibration.py)."""
+    p = ROOT / "tests" / "golden" / "cpu_calibration.json"
+    return json.loads(p.read_text()) if p.exists() else None
 
 
 CONFIG1_SQL = "select id, uniformi, normalf5 from test where uniformi < 50 and normali5 > 0"
@@ -109,11 +133,12 @@ def bench_config1(a) -> int:
     q = db.prepare(CONFIG1_SQL)
     q.run()
     pair = [q.time(1, 1)[0] for _ in range(20)]
+    check = verify_columns(q.fetch(), golden("config1") if a.seed == 1 else None)
     dropin = [_execute_ms(va, db, CONFIG1_SQL)[0] for _ in range(21)][1:]
     cpu = []
     for _ in range(20):
         t0 = time.perf_counter()
-        oracle.select(db, CONFIG1_SQL, count_only=True)
+        oracle.select(db, CONFIG1_SQL)
         cpu.append((time.perf_counter() - t0) * 1e3)
     read_b = q.rows * q.read_bytes_per_row
 
@@ -128,6 +153,7 @@ def bench_config1(a) -> int:
         "value": round(statistics.median(pair), 4), "unit": "ms", "n_gpus": 1, "steps": 20, "warmup": 1,
         "ms_per_step": round(statistics.median(pair), 4), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "i32+f32", "data": "synthetic (seeded Philox generator, 7-column schema)",
+        "verified": check["verified"],
         "config": {"workload": "plain filter SELECT (23 % selected) over one 1M-row tablet resident in HBM",
                    "rows": q.rows, "selected": q.last_rows, "read_bytes": read_b,
                    "launch_pair": stats(pair), "dropin_virg_vm_execute": stats(dropin)},
@@ -178,8 +204,9 @@ def bench_outofcore(a) -> int:
     link = secs = 0.0
     n = 0
     per = []
+    last = None
     for _ in range(a.steps):
-        _, st = db.stream_select(CONFIG3_SQL, capacity=cap, batch_rows=a.batch_rows)
+        last, st = db.stream_select(CONFIG3_SQL, capacity=cap, batch_rows=a.batch_rows)
         link += st.h2d_bytes + st.d2h_bytes
         secs += st.seconds
         n = st.rows_out
@@ -188,13 +215,18 @@ def bench_outofcore(a) -> int:
               f"{st.h2d_bytes / st.seconds / 1e9:.1f} GB/s columns in, {st.batches} batches",
               file=sys.stderr, flush=True)
     value = link / secs / 1e9
+    # the last step's rows against the oracle's digest (outside the timed steps)
+    check = verify_columns(last, golden("config3") if (rows, a.tablet_rows, a.seed) == (2_000_000_000, 1 << 20, 1)
+                           else None) if last is not None else {"verified": None}
+    del last
+    print(f"# results against the oracle's digest: {check['verified']}", file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu:
         from oracle import oracle
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        threads = cpu_threads()
         sample_tabs = 200
         t1 = time.perf_counter()
-        got, _ = oracle.select(db, CONFIG3_SQL, threads=threads, ordered=False, count_only=True,
+        got, _ = oracle.select(db, CONFIG3_SQL, threads=threads, ordered=False,
                                max_tablets=sample_tabs)
         dt = time.perf_counter() - t1
         srows = min(rows, sample_tabs * a.tablet_rows)
@@ -205,6 +237,7 @@ def bench_outofcore(a) -> int:
         "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(secs / a.steps * 1e3, 1), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+i32", "data": "synthetic (seeded Philox generator)",
+        "verified": check["verified"],
         "config": {"workload": "out-of-core filter SELECT over a 2e9-row 7-column table in pinned host "
                                "memory (1M-row tablets), referenced columns streamed through HBM",
                    "rows": rows, "tablet_rows": a.tablet_rows, "selected": n,
@@ -264,7 +297,7 @@ def bench_batched(a) -> int:
             continue
         progs.append(p)
         qtab.append(k)
-    qs = [db.prepare(p) for p in progs]
+    qs = [db.prepare(p, flags=va.VG_QUERY_INTERP) for p in progs]
     scanned = sum(q.rows for q in qs)
     print(f"# {len(qs)} queries over {len(sizes)} tables ({sum(sizes)} rows), {scanned} rows scanned "
           f"per round; ops per program {min(map(len, progs))}-{max(map(len, progs))}",
@@ -576,6 +609,25 @@ def summarize_pmc(outdir: str, dest: str) -> int:
     return 0
 
 
+GOLDEN = ROOT / "tests" / "golden" / "workload_digests.json"
+
+
+def golden(key: str) -> dict | None:
+    """The oracle's digests of a workload (tests/golden/make_workload_digests.py)."""
+    if not GOLDEN.exists():
+        return None
+    return json.loads(GOLDEN.read_text()).get(key)
+
+
+def verify_columns(cols: dict, want: dict | None) -> dict:
+    """Compare a result (column arrays in order) with its recorded digest."""
+    from virginian_amd.verify import digest
+    got = {"selected": len(next(iter(cols.values()))) if cols else 0, "digest": digest(list(cols.values()))}
+    if want is None:
+        return {**got, "verified": None}
+    return {**got, "verified": got["selected"] == want["selected"] and got["digest"] == want["digest"]}
+
+
 def measured_traffic(workload: str) -> dict | None:
     """HBM traffic per launch pair recorded from counter runs, if present."""
     p = ROOT / "profiles" / f"traffic_{workload}.json"
@@ -594,7 +646,7 @@ def main() -> int:
     ap.add_argument("--host-gen", action="store_true",
                     help="outofcore: build the table with the host generator instead of the device builder")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pin-limit-gib", type=int, default=52,
                     help="out-of-core: pin at most this much tablet memory")
@@ -700,6 +752,18 @@ def main() -> int:
     scanned = sum(p["read_bytes"] for p in per) * a.steps
     value = scanned / wall / 1e9
 
+    # the rows the timed launches produced, against the oracle's digests
+    # (outside the timed region)
+    g = golden("config2") if (a.rows, a.tablet_rows, a.seed) == (64 << 20, 1 << 20, 1) else None
+    checks = []
+    for t, q in zip(THRESHOLDS, queries):
+        q.sync()
+        want = next((x for x in g["queries"] if x["threshold"] == t), None) if g else None
+        checks.append({"threshold": t, **verify_columns(q.fetch(), want)})
+    verified = all(c["verified"] for c in checks) if g else None
+    print(f"# results against the oracle's digests: {'verified' if verified else verified}",
+          file=sys.stderr, flush=True)
+
     mean_bytes = sum(p["read_bytes"] + p["write_bytes"] for p in per) / len(per)
     mean_ms = sum(p["kernel_ms"] for p in per) / len(per)
     achieved = mean_bytes / mean_ms / 1e6
@@ -713,20 +777,36 @@ def main() -> int:
 
     cpu = None
     if not a.no_cpu:
+        # the T=10 query over the same rows, in the GPU's 1M-row tablets and
+        # in the reference's 8 MB tablets
         q10 = queries[1]
-        r = cpu_baseline(db, sql_for(10.0), a.cpu_seconds)
         rb = q10.rows * q10.read_bytes_per_row
-        cpu = {"value": round(rb * r["reps"] / r["seconds"] / 1e9, 3), "unit": "GB/s",
-               "cores": r["threads"], "kind": "port",
-               "sample": f"the T=10 query over the same {q10.rows}-row resident table, "
-                         f"{r['reps']} full passes in {r['seconds']:.1f}s "
-                         f"({q10.rows * r['reps'] / r['seconds']:.3e} rows/s)"}
+        r = cpu_baseline(db, sql_for(10.0), a.cpu_seconds)
+        db.gen_table("test8mb", a.rows, 0, seed=a.seed, threads=cpu_threads())
+        r8 = cpu_baseline(db, sql_for(10.0), a.cpu_seconds, table="test8mb")
+        cal = cpu_calibration()
+        ratio = cal["queries"]["config2_T10"]["multi"]["restatement_over_reference"] if cal else None
+        value_cpu = rb * r["reps"] / r["seconds"] / 1e9
+        cpu = {"value": round(value_cpu, 3), "unit": "GB/s", "cores": r["threads"], "kind": "port",
+               "sample": f"the T=10 query over the same {q10.rows}-row table in 1M-row tablets, results "
+                         f"materialised, {r['threads']} threads ({len(os.sched_getaffinity(0))} cores visible, "
+                         f"{cpu_model()}), {r['reps']} full passes in {r['seconds']:.1f}s "
+                         f"({q10.rows * r['reps'] / r['seconds']:.3e} rows/s)",
+               "tablets_8mb": {"value": round(rb * r8["reps"] / r8["seconds"] / 1e9, 3), "unit": "GB/s",
+                               "sample": f"the same query over the same rows in the reference's 8 MB tablets, "
+                                         f"{r8['reps']} passes in {r8['seconds']:.1f}s"},
+               "calibration": None if cal is None else {
+                   "restatement_over_reference_time": ratio,
+                   "reference_vm_estimate_gbs": round(value_cpu * ratio, 3),
+                   "measured": f"{cal['rows']} rows, {cal['threads']} threads, {cal['cpu']} "
+                               "(tests/golden/cpu_calibration.json)"}}
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "GB/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+i32",
         "data": "synthetic (seeded Philox generator, 7-column benchmark schema)",
+        "verified": verified,
         "config": {"workload": "filter+math SELECT, 64M-row 7-column table resident in HBM in 1M-row "
                                "tablets, selectivity sweep 0.1/10/50/90 %",
                    "rows": a.rows, "tablet_rows": a.tablet_rows, "queries_per_step": len(queries),
@@ -734,7 +814,9 @@ def main() -> int:
                    "kernel": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in kinfo.items()},
                    "per_query": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()}
                                  for p in per],
-                   "dropin_virg_vm_execute": dropin},
+                   "dropin_virg_vm_execute": dropin,
+                   "verification": {"fixture": "tests/golden/workload_digests.json (config2)",
+                                    "queries": checks}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

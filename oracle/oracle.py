@@ -60,10 +60,11 @@ def _ops(db: va.Virginian, prog):
 
 def select(db: va.Virginian, prog, table: str | None = None, threads: int = 0,
            ordered: bool = True, count_only: bool = False,
-           max_tablets: int | None = None) -> tuple[dict, int]:
+           max_tablets: int | None = None, tablets: tuple[int, int] | None = None) -> tuple[dict, int]:
     """Run the program's data-parallel section on the CPU oracle over the
     table named by its Table op (or `table`).  Returns (columns, illformed).
-    threads = 0 runs the single-threaded reference-order VM."""
+    threads = 0 runs the single-threaded reference-order VM; tablets =
+    (i, j) runs it over the table's tablets i..j-1 only."""
     arr, n, names, types, _keep = _ops(db, prog)
     tid = None
     par = None
@@ -79,6 +80,8 @@ def select(db: va.Virginian, prog, table: str | None = None, threads: int = 0,
     tabs = db.tablets(tid)
     if max_tablets is not None:
         tabs = tabs[:max_tablets]
+    if tablets is not None:
+        tabs = tabs[tablets[0]:tablets[1]]
     tarr = (C.POINTER(va.TabletMeta) * len(tabs))(*tabs)
     cap = sum(t.contents.rows for t in tabs)
     outs = [] if count_only else [np.empty(cap, dtype=va.NP_TYPE[t]) for t in types]

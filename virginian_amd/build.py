@@ -39,7 +39,7 @@ CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", "-Wextra",
           f"-I{CSRC / 'gpu'}"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC / 'host'}", f"-I{CSRC / 'gpu'}"]
+            "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC / 'host'}", f"-I{CSRC / 'gpu'}", f"-I{OBJ}"]
 
 
 def _hipcc() -> str:
@@ -59,9 +59,22 @@ def _run(cmd: list[str]) -> None:
         print(out)
 
 
+def embed_headers() -> None:
+    """lib/obj/vgx_headers.inc: vg_prog.h and vgx_abi.h as one C string
+    literal, the text jit.hip puts in front of every kernel it compiles at
+    run time, so those kernels see the structs the host library uses."""
+    text = "".join((CSRC / "gpu" / h).read_text() for h in ("vg_prog.h", "vgx_abi.h"))
+    assert ")VGXH\"" not in text
+    inc = OBJ / "vgx_headers.inc"
+    body = 'R"VGXH(' + text + ')VGXH"\n'
+    if not inc.exists() or inc.read_text() != body:
+        inc.write_text(body)
+
+
 def build_product(jobs: int = 8) -> Path:
     OUT.mkdir(parents=True, exist_ok=True)
     OBJ.mkdir(parents=True, exist_ok=True)
+    embed_headers()
     hipcc = _hipcc()
     steps = []
     objs = []

@@ -52,20 +52,21 @@ extern "C" {
 }
 
 /* ------------------------------------------------------------ prelude */
+/* vg_prog.h and vgx_abi.h, as the build embedded them */
+static const char *k_headers =
+#include "vgx_headers.inc"
+	;
+
 static const char *k_prelude = R"VGX(
 typedef unsigned char u8; typedef signed char i8; typedef unsigned short u16;
 typedef unsigned int u32; typedef int i32; typedef unsigned long long u64; typedef long long i64;
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));
 typedef u32 u32x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) void lds_void;
-struct vgx_tab { u64 col[17]; u32 rows; u32 first_tile; u32 aligned; u32 pad; };
-struct vgx_tile { u32 tab; u32 row0; };
-struct vgx_ctrl { u64 total; u32 overflow; u32 pad[5]; };
-struct vgx_args {
-	const vgx_tab *tabs; const vgx_tile *tiles; u32 num_tiles; u32 nchunks;
-	u64 *state; u64 *status; vgx_ctrl *ctrl; u64 capacity; u32 spin_limit; u32 pad;
-	void *out[16]; u64 imm[48][2]; u64 oimm[16];
-};
+typedef vg_tabdesc vgx_tab;
+typedef vg_ctrl vgx_ctrl;
+typedef vgx_launch_args vgx_args;
+#define OOB 0x80000000u   /* a buffer offset past every resource: loads nothing, reads 0 */
 #define ST_AGG (1ull << 62)
 #define ST_PRE (2ull << 62)
 #define ST_VBITS 38
@@ -459,13 +460,17 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 }
 
 /* Generate the kernel source of program P at shape sh. */
-extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
+/* Program P at shape sh as namespace `ns`: its constants and chunk(), the
+ * work of one chunk (both phases, the look-back, the writes), inlined into
+ * the kernels that call it.  chunk() calls grab_next() in thread 0 as the
+ * chunk's last tile starts, for the caller's next chunk. */
+static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const char *ns)
 {
-	out_buf o;
-	o.s = k_prelude;
 	const int R = sh->R, NQ = R / 4;
-	o("#define R %d\n#define NQ %d\n#define TR %u\n#define S_ %u\n#define AL %d\n#define OOB 0x80000000u\n", R, NQ, sh->TR, sh->S, sh->aligned);
-	o("#define RMASK %s\n", R == 32 ? "0xFFFFFFFFu" : R == 16 ? "0xFFFFu" : R == 8 ? "0xFFu" : "0xFu");
+	o("\nnamespace %s {\n", ns);
+	o("constexpr int R = %d, NQ = %d;\nconstexpr u32 TR = %uu, S_ = %uu;\nconstexpr bool AL = %s;\n", R, NQ, sh->TR,
+	  sh->S, sh->aligned ? "true" : "false");
+	o("constexpr u32 RMASK = %s;\n", R == 32 ? "0xFFFFFFFFu" : R == 16 ? "0xFFFFu" : R == 8 ? "0xFFu" : "0xFu");
 
 	/* column variables: loaded columns live in their slots; emit-only
 	 * columns get variables of their own (e<c>, e<c>h) */
@@ -521,18 +526,11 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 			decl("\t\t\tu32 e%u[R], e%uh[R];\n#pragma unroll\n\t\t\tfor (int r = 0; r < R; ++r) { e%u[r] = 0u; e%uh[r] = 0u; }\n",
 				c, c, c, c);
 
-	o("\nextern \"C\" __global__ __launch_bounds__(256) void vgx_main(const vgx_args A)\n{\n");
-	o("\textern __shared__ __attribute__((aligned(16))) u8 lds[];\n");
-	o("\t__shared__ u32 s_cnt[S_][4];\n\t__shared__ u64 s_pre;\n\t__shared__ u32 s_chunk;\n\t__shared__ u32 s_epoch;\n");
+	o("template <typename G>\n__device__ __attribute__((always_inline)) inline void chunk(const vgx_args &A, const u32 c, "
+	  "const u64 epoch, u32 (*s_cnt)[4], u64 *s_pre, u8 *lds, u32 &ovf, G grab_next)\n{\n");
 	o("\tconst u32 tid = threadIdx.x, lane = tid & 63u, wave = uni(tid >> 6);\n");
 	o("\tu8 *const ostage = lds + %u + wave * %u;\n", sh->stage_off, sh->stage_wave_bytes);
-	o("\tif (tid == 0) {\n"
-	  "\t\tconst u64 e = __hip_atomic_load(&A.state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
-	  "\t\twait_vm();   /* the epoch is in hand before the first grab */\n"
-	  "\t\ts_epoch = (u32)e;\n\t\ts_chunk = grab(A, e);\n\t}\n\t__syncthreads();\n");
-	o("\tconst u64 epoch = s_epoch;\n\tu32 ovf = 0;\n");
-	o("\tfor (;;) {\n\t\tconst u32 c = uni(s_chunk);\n\t\tif (c >= A.nchunks)\n\t\t\tbreak;\n");
-	o("\t\tu32 nxt = 0;\n");
+	o("\t{\n");
 	o("\t\tconst u32 t0 = c * S_;\n\t\tconst u32 nt = uni(min((u32)S_, A.num_tiles - t0));\n");
 
 	/* tile lookup */
@@ -630,7 +628,7 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 	/* the next chunk is taken only as this one's last tile starts: a chunk
 	 * taken is a chunk worked on, so later chunks never wait for one that
 	 * sits behind a whole chunk of its workgroup's work */
-	o("\t\t\tif (s + 1 == nt && tid == 0)\n\t\t\t\tnxt = grab(A, epoch);\n");
+	o("\t\t\tif (s + 1 == nt && tid == 0)\n\t\t\t\tgrab_next();\n");
 	o("\t\t\tconst u32 sel = (u32)(vb >> (s * R)) & RMASK;\n");
 	o("%s", decl.s.c_str());
 	/* HBM loads of project / emit columns, only for 4-row groups with a
@@ -658,25 +656,25 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 		  "\t\t\t\t\tconst u32 g = (sel >> (4 * q)) & 0xFu;\n"
 		  "\t\t\t\t\tconst u32 rr = wave * (NQ * 256u) + q * 256u + lane * 4u;\n");
 		if (cl.width == 4) {
-			o("#if AL\n\t\t\t\t\tconst u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, g ? rr * 4u : OOB, 0, 0);\n"
+			o("\t\t\t\t\tif constexpr (AL) {\n\t\t\t\t\tconst u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, g ? rr * 4u : OOB, 0, 0);\n"
 			  "\t\t\t\t\t%s[4 * q] = v.x; %s[4 * q + 1] = v.y; %s[4 * q + 2] = v.z; %s[4 * q + 3] = v.w;\n"
-			  "#else\n#pragma unroll\n\t\t\t\t\tfor (int j = 0; j < 4; ++j)\n"
-			  "\t\t\t\t\t\t%s[4 * q + j] = __builtin_amdgcn_raw_buffer_load_b32(rs, ((g >> j) & 1u) ? (rr + j) * 4u : OOB, 0, 0);\n#endif\n",
+			  "\t\t\t\t\t} else {\n#pragma unroll\n\t\t\t\t\tfor (int j = 0; j < 4; ++j)\n"
+			  "\t\t\t\t\t\t%s[4 * q + j] = __builtin_amdgcn_raw_buffer_load_b32(rs, ((g >> j) & 1u) ? (rr + j) * 4u : OOB, 0, 0);\n\t\t\t\t\t}\n",
 			  lo, lo, lo, lo, lo);
 		} else if (cl.width == 8) {
-			o("#if AL\n\t\t\t\t\tconst u32x4 v0 = __builtin_amdgcn_raw_buffer_load_b128(rs, g ? rr * 8u : OOB, 0, 0);\n"
+			o("\t\t\t\t\tif constexpr (AL) {\n\t\t\t\t\tconst u32x4 v0 = __builtin_amdgcn_raw_buffer_load_b128(rs, g ? rr * 8u : OOB, 0, 0);\n"
 			  "\t\t\t\t\tconst u32x4 v1 = __builtin_amdgcn_raw_buffer_load_b128(rs, g ? rr * 8u + 16u : OOB, 0, 0);\n"
 			  "\t\t\t\t\t%s[4 * q] = v0.x; %s[4 * q] = v0.y; %s[4 * q + 1] = v0.z; %s[4 * q + 1] = v0.w;\n"
 			  "\t\t\t\t\t%s[4 * q + 2] = v1.x; %s[4 * q + 2] = v1.y; %s[4 * q + 3] = v1.z; %s[4 * q + 3] = v1.w;\n"
-			  "#else\n#pragma unroll\n\t\t\t\t\tfor (int j = 0; j < 4; ++j) {\n"
+			  "\t\t\t\t\t} else {\n#pragma unroll\n\t\t\t\t\tfor (int j = 0; j < 4; ++j) {\n"
 			  "\t\t\t\t\t\tconst u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, ((g >> j) & 1u) ? (rr + j) * 8u : OOB, 0, 0);\n"
-			  "\t\t\t\t\t\t%s[4 * q + j] = v.x; %s[4 * q + j] = v.y;\n\t\t\t\t\t}\n#endif\n",
+			  "\t\t\t\t\t\t%s[4 * q + j] = v.x; %s[4 * q + j] = v.y;\n\t\t\t\t\t}\n\t\t\t\t\t}\n",
 			  lo, hi, lo, hi, lo, hi, lo, hi, lo, hi);
 		} else {
-			o("#if AL\n\t\t\t\t\tconst u32 v = __builtin_amdgcn_raw_buffer_load_b32(rs, g ? rr : OOB, 0, 0);\n"
+			o("\t\t\t\t\tif constexpr (AL) {\n\t\t\t\t\tconst u32 v = __builtin_amdgcn_raw_buffer_load_b32(rs, g ? rr : OOB, 0, 0);\n"
 			  "#pragma unroll\n\t\t\t\t\tfor (int j = 0; j < 4; ++j) %s[4 * q + j] = (u32)(i32)(i8)(v >> (8 * j));\n"
-			  "#else\n#pragma unroll\n\t\t\t\t\tfor (int j = 0; j < 4; ++j)\n"
-			  "\t\t\t\t\t\t%s[4 * q + j] = (u32)(i32)(i8)__builtin_amdgcn_raw_buffer_load_b8(rs, ((g >> j) & 1u) ? rr + j : OOB, 0, 0);\n#endif\n",
+			  "\t\t\t\t\t} else {\n#pragma unroll\n\t\t\t\t\tfor (int j = 0; j < 4; ++j)\n"
+			  "\t\t\t\t\t\t%s[4 * q + j] = (u32)(i32)(i8)__builtin_amdgcn_raw_buffer_load_b8(rs, ((g >> j) & 1u) ? rr + j : OOB, 0, 0);\n\t\t\t\t\t}\n",
 			  lo, lo);
 		}
 		o("\t\t\t\t}\n\t\t\t}\n");
@@ -688,8 +686,8 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 	/* the chunk's output offset: wave 0 looks back while the loads fly */
 	o("\t\t\tif (s == 0) {\n\t\t\t\tif (wave == 0) {\n\t\t\t\t\tbool gave = false;\n"
 	  "\t\t\t\t\tconst u64 p = lookback(A.status, c, n, lane, epoch, A.spin_limit, &gave);\n"
-	  "\t\t\t\t\tif (gave) ovf |= 2u;\n\t\t\t\t\tif (lane == 0) s_pre = p;\n\t\t\t\t}\n"
-	  "\t\t\t\t__syncthreads();\n\t\t\t\tpre = s_pre;\n"
+	  "\t\t\t\t\tif (gave) ovf |= 2u;\n\t\t\t\t\tif (lane == 0) *s_pre = p;\n\t\t\t\t}\n"
+	  "\t\t\t\t__syncthreads();\n\t\t\t\tpre = *s_pre;\n"
 	  "\t\t\t\tif (c == A.nchunks - 1 && tid == 0) A.ctrl->total = pre + n;\n\t\t\t}\n");
 	o("\t\t\tu32 valid = sel, active = sel, w0 = 0, w1 = 0, w2 = 0, w3 = 0;\n");
 	if (P->wfull)
@@ -759,6 +757,36 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 	}
 	o("\t\t\t\t}\n\t\t\t\tat += qn;\n\t\t\t}\n");
 	o("\t\t\ttdone += s_cnt[s][0] + s_cnt[s][1] + s_cnt[s][2] + s_cnt[s][3];\n\t\t}\n");
+	o("\t}\n}\n} /* namespace %s */\n", ns);
+}
+
+static void gen_prelude(out_buf &o)
+{
+	o.s = "typedef unsigned char uint8_t; typedef unsigned short uint16_t; typedef unsigned int uint32_t;\n"
+	      "typedef unsigned long long uint64_t; typedef int int32_t; typedef long long int64_t;\n";
+	o.s += k_headers;
+	o.s += k_prelude;
+}
+
+/* Generate the kernel source of program P at shape sh. */
+extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
+{
+	out_buf o;
+	gen_prelude(o);
+	gen_program(o, P, sh, "p0");
+	o("\nextern \"C\" __global__ __launch_bounds__(256) void vgx_main(const vgx_args A)\n{\n");
+	o("\textern __shared__ __attribute__((aligned(16))) u8 lds[];\n");
+	o("\t__shared__ u32 s_cnt[%u][4];\n\t__shared__ u64 s_pre;\n\t__shared__ u32 s_chunk;\n\t__shared__ u32 s_epoch;\n",
+	  sh->S);
+	o("\tconst u32 tid = threadIdx.x;\n");
+	o("\tif (tid == 0) {\n"
+	  "\t\tconst u64 e = __hip_atomic_load(&A.state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+	  "\t\twait_vm();   /* the epoch is in hand before the first grab */\n"
+	  "\t\ts_epoch = (u32)e;\n\t\ts_chunk = grab(A, e);\n\t}\n\t__syncthreads();\n");
+	o("\tconst u64 epoch = s_epoch;\n\tu32 ovf = 0;\n");
+	o("\tfor (;;) {\n\t\tconst u32 c = uni(s_chunk);\n\t\tif (c >= A.nchunks)\n\t\t\tbreak;\n");
+	o("\t\tu32 nxt = 0;\n");
+	o("\t\tp0::chunk(A, c, epoch, s_cnt, &s_pre, lds, ovf, [&]() { nxt = grab(A, epoch); });\n");
 	o("\t\t__syncthreads();\n\t\tif (tid == 0)\n\t\t\ts_chunk = nxt;\n\t\t__syncthreads();\n\t}\n");
 	o("\tif (ovf)\n\t\tatomicOr(&A.ctrl->overflow, ovf);\n}\n");
 

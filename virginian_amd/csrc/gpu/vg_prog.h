@@ -20,7 +20,9 @@
 #ifndef VG_PROG_H
 #define VG_PROG_H
 
+#ifndef __HIPCC_RTC__
 #include <stdint.h>
+#endif
 
 #define VG_MAX_UOPS      48
 #define VG_MAX_COLS      17      /* distinct column sources: 16 fixed + key */

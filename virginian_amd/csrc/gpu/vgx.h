@@ -7,6 +7,7 @@
 
 #include <stdint.h>
 #include "vg_prog.h"
+#include "vgx_abi.h"
 
 /* geometry of a specialised kernel */
 typedef struct {
@@ -21,30 +22,6 @@ typedef struct {
 	int aligned;               /* every tablet's column bases are 16-byte aligned:
 	                            * result columns load 16 bytes per lane */
 } vgx_shape;
-
-/* a tile of a specialised launch: tablet (index into the descriptors) and
- * its first row in that tablet */
-typedef struct {
-	uint32_t tab;
-	uint32_t row0;
-} vgx_tile;
-
-/* kernel arguments; the kernel's own struct has this layout */
-typedef struct {
-	const vg_tabdesc *tabs;
-	const vgx_tile *tiles;
-	uint32_t num_tiles;
-	uint32_t nchunks;
-	uint64_t *state;           /* [0] chunk counter, [1] launch epoch */
-	uint64_t *status;          /* one word per chunk */
-	vg_ctrl *ctrl;
-	uint64_t capacity;
-	uint32_t spin_limit;
-	uint32_t pad;
-	void *out[VG_MAX_OUT];
-	uint64_t imm[VG_MAX_UOPS][2];
-	uint64_t oimm[VG_MAX_OUT];
-} vgx_launch_args;
 
 typedef struct vgx_kernel vgx_kernel;
 
