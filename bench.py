@@ -270,11 +270,13 @@ def _pct(xs: list[float], p: float) -> float:
 def bench_batched(a) -> int:
     """Batched queries: 256 seeded random op-programs (4-40 ops, a
     uniformf < 100*s conjunct with s ~ U[0.01 %, 80 %]) over 16 resident
-    tables of log-uniform 1e3-1e7 rows, run by the tuned persistent batch
-    (one graph: reset, count and write launches over device item lists)
-    and, for comparison, by plain per-query launches on one stream.
-    value = GB/s scanned by the batch (the config-2 byte formula summed
-    over the queries); queries/s and p50/p99 latency for both ways."""
+    tables of log-uniform 1e3-1e7 rows, run by the specialised batch (the
+    programs compiled into one module at build time, one persistent launch
+    taking (query, chunk) items from a device counter) and, for
+    comparison, by the interpreter's tuned persistent batch and by plain
+    per-query launches on one stream.  value = GB/s scanned by the
+    specialised batch (the config-2 byte formula summed over the queries);
+    queries/s and p50/p99 latency for all three."""
     import numpy as np
     import virginian_amd as va
     from virginian_amd import workload as W
@@ -302,6 +304,23 @@ def bench_batched(a) -> int:
     print(f"# {len(qs)} queries over {len(sizes)} tables ({sum(sizes)} rows), {scanned} rows scanned "
           f"per round; ops per program {min(map(len, progs))}-{max(map(len, progs))}",
           file=sys.stderr, flush=True)
+    if a.pmc_run:
+        # counter runs: every query's plain launches once, then one run of
+        # the interpreter's batch (default configuration) and one of the
+        # specialised batch, told apart by kernel name
+        for q in qs:
+            q.run()
+        b = va.PersistentBatch(qs)
+        rows_i, _ = b.run()
+        b.close()
+        print(f"# compiling the batch module ({len(qs)} programs)", file=sys.stderr, flush=True)
+        jb = va.PersistentBatch(qs, jit=1)
+        rows_j, _, flags = jb.results()
+        jb.close()
+        nbytes = sum(q.rows * q.read_bytes_per_row + r * q.write_bytes_per_row for q, r in zip(qs, rows_j))
+        print(json.dumps({"pmc_run": "batched", "queries": len(qs), "rows_scanned": scanned,
+                          "bytes_algorithmic": nbytes, "same_rows": rows_i == rows_j and not any(flags)}))
+        return 0
     # plain: each query's launch pair, one after another
     plain_ms, plain_lat = va.plain_time(qs, a.warmup, a.steps)
     plain_rows = [q.sync() for q in qs]
@@ -323,6 +342,25 @@ def bench_batched(a) -> int:
     print(f"# batch: {batch_ms:.3f} ms per run, p50 {_pct(batch_lat, 50):.3f} p99 {_pct(batch_lat, 99):.3f} ms, "
           f"results identical to plain: {same}", file=sys.stderr, flush=True)
     b.close()
+    # specialised: the batch's programs compiled into one module, one
+    # persistent launch taking (query, chunk) items in order
+    t0 = time.perf_counter()
+    print(f"# compiling the batch module ({len(qs)} programs)", file=sys.stderr, flush=True)
+    jb = va.PersistentBatch(qs, jit=1)
+    compile_s = time.perf_counter() - t0
+    jit_ms = jb.time(a.warmup, a.steps)
+    jlats = []
+    for _ in range(a.steps):
+        jrows, jlat, jflags = jb.results()
+        jlats.append(jlat)
+    jit_lat = [float(np.median([l[i] for l in jlats])) for i in range(len(qs))]
+    jit_same = jrows == plain_rows and not any(jflags) and [_digest(q) for q in qs] == plain_dig
+    print(f"# specialised batch: built in {compile_s:.1f}s, {jit_ms:.3f} ms per run, p50 {_pct(jit_lat, 50):.3f} "
+          f"p99 {_pct(jit_lat, 99):.3f} ms, results identical to plain: {jit_same}", file=sys.stderr, flush=True)
+    jb.close()
+    same = same and jit_same
+    interp_ms, interp_lat = batch_ms, batch_lat
+    batch_ms, batch_lat = jit_ms, jit_lat
     nbytes = sum(q.rows * q.read_bytes_per_row + r * q.write_bytes_per_row for q, r in zip(qs, rows))
     value = nbytes / (batch_ms * 1e-3) / 1e9
     line = {
@@ -334,9 +372,14 @@ def bench_batched(a) -> int:
                                "tables of log-uniform 1e3-1e7 rows, selectivity U[0.01 %, 80 %]",
                    "queries": len(qs), "table_rows": sizes, "rows_scanned_per_round": scanned,
                    "bytes_per_round": nbytes, "pct_hbm_peak": round(value / HBM_PEAK_GBS * 100, 2),
-                   "tuned": cfg,
-                   "batched": {"ms": round(batch_ms, 4), "queries_per_s": round(len(qs) / batch_ms * 1e3, 1),
+                   "batched": {"kind": "one persistent launch of the batch's programs compiled together",
+                               "build_s": round(compile_s, 1),
+                               "ms": round(batch_ms, 4), "queries_per_s": round(len(qs) / batch_ms * 1e3, 1),
                                "p50_ms": round(_pct(batch_lat, 50), 4), "p99_ms": round(_pct(batch_lat, 99), 4)},
+                   "interpreter_batch": {"tuned": cfg, "ms": round(interp_ms, 4),
+                                         "queries_per_s": round(len(qs) / interp_ms * 1e3, 1),
+                                         "p50_ms": round(_pct(interp_lat, 50), 4),
+                                         "p99_ms": round(_pct(interp_lat, 99), 4)},
                    "plain": {"ms": round(plain_ms, 4), "queries_per_s": round(len(qs) / plain_ms * 1e3, 1),
                              "p50_ms": round(_pct(plain_lat, 50), 4), "p99_ms": round(_pct(plain_lat, 99), 4)},
                    "speedup": round(plain_ms / batch_ms, 3), "identical_results": same},
@@ -552,6 +595,42 @@ def pmc_run(va, queries) -> int:
     return 0
 
 
+def summarize_counters(outdir: str, dest: str) -> int:
+    """Per-kernel means of every counter in the rocprofv3 --pmc runs under
+    outdir (one directory per pass), for the batched workload's counter
+    runs (--workload batched --pmc-run): dispatches, and per dispatch and
+    per run the counter values (FETCH_SIZE and WRITE_SIZE in KiB)."""
+    import csv
+    from collections import defaultdict
+    from pathlib import Path as P
+    acc = defaultdict(lambda: defaultdict(list))
+    for f in sorted(P(outdir).glob("**/*counter_collection.csv")):
+        for r in csv.DictReader(open(f)):
+            k = r["Kernel_Name"].split("(")[0]
+            acc[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    kernels = {}
+    for k, cs in sorted(acc.items()):
+        kernels[k] = {c: {"dispatches": len(v), "sum": round(sum(v)), "mean": round(sum(v) / len(v), 1)}
+                      for c, v in sorted(cs.items())}
+    doc = {"method": "rocprofv3 --pmc, one pass per counter group; per kernel: dispatches, sum over the run "
+                     "and mean per dispatch (FETCH_SIZE / WRITE_SIZE in KiB, uncalibrated)",
+           "commands": [
+               "cd /tmp && export TMPDIR=/tmp",
+               "rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_SMEM SQ_INSTS_LDS SQ_WAVES SQ_WAVE_CYCLES "
+               "SQ_BUSY_CYCLES --output-format csv -d <out>/sq -o batched -- python3 -u <repo>/bench.py "
+               "--workload batched --pmc-run",
+               "rocprofv3 --pmc FETCH_SIZE --output-format csv -d <out>/fetch -o batched -- python3 -u "
+               "<repo>/bench.py --workload batched --pmc-run",
+               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d <out>/write -o batched -- python3 -u "
+               "<repo>/bench.py --workload batched --pmc-run",
+               f"python3 <repo>/bench.py --summarize-counters <out> {dest}"],
+           "kernels": kernels,
+           "hardware": "MI355X (gfx950, 256 CUs)"}
+    P(dest).write_text(json.dumps(doc, indent=1) + "\n")
+    print(json.dumps(doc, indent=1))
+    return 0
+
+
 def summarize_pmc(outdir: str, dest: str) -> int:
     """Per-launch HBM bytes from two counter runs of --pmc-run (FETCH_SIZE
     and WRITE_SIZE, in KiB per dispatch) and of --copy-probe 256 (the same
@@ -662,12 +741,17 @@ def main() -> int:
     ap.add_argument("--queries", type=int, default=256, help="batched: queries per batch")
     ap.add_argument("--batch-rows", type=int, default=8 << 20)
     ap.add_argument("--pmc-run", action="store_true",
-                    help="counter runs: each resident query's kernel a few times, one at a time")
+                    help="counter runs: each resident query's kernel a few times, one at a time; "
+                         "batched: plain launches, the interpreter's batch and the specialised batch once")
+    ap.add_argument("--summarize-counters", nargs=2, metavar=("OUTDIR", "DEST"),
+                    help="per-kernel means of the counters in the batched workload's counter runs")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("OUTDIR", "DEST"),
                     help="traffic summary from --pmc-run counter runs (no GPU needed)")
     a = ap.parse_args()
     if a.summarize_pmc:
         return summarize_pmc(*a.summarize_pmc)
+    if a.summarize_counters:
+        return summarize_counters(*a.summarize_counters)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the executor on one GPU; figures for several GPUs are "
               "not produced (tablets shard across GPUs by design, but no multi-GPU path is "

@@ -153,6 +153,13 @@ typedef struct {
 	                            * by decoupled look-back over the query's earlier tiles
 	                            * and writes its rows there; a query whose result does
 	                            * not fit reports it (vg_pbatch_results) */
+	unsigned jit;              /* 1: one persistent launch of specialised code: the
+	                            * batch's programs compiled into one module at build
+	                            * time (hiprtc), workgroups taking (query, chunk) items
+	                            * in order from a device counter, each chunk read once
+	                            * (filter columns staged in LDS), its output offset by
+	                            * look-back over the query's earlier chunks; items,
+	                            * order, grid and grouping settings do not apply */
 } vg_pbatch_config;
 typedef struct vg_pbatch vg_pbatch;
 vg_pbatch *vg_pbatch_create(vg_query *const *qs, unsigned n, const vg_pbatch_config *cfg,

@@ -230,3 +230,93 @@ def test_batch_holds_its_queries(gpu_db):
     q.close()
     q2.close()
     gpu_db.device_free(buf)
+
+
+def test_jit_batch(mix):
+    """One persistent launch of the batch's programs compiled together:
+    every query's result in row order, bit for bit the oracle's, on every
+    graph replay, with eager launches of the same queries in between; the
+    batch keeps its own control blocks, so the eager runs do not disturb
+    its outcome."""
+    qs, want = mix
+    t0 = time.perf_counter()
+    b = va.PersistentBatch(qs, jit=1)
+    print(f"jit batch built in {time.perf_counter() - t0:.1f}s", flush=True)
+    try:
+        assert b.config()["jit"] == 1
+        for replay in range(3):
+            print(f"jit batch replay {replay}", flush=True)
+            t0 = time.perf_counter()
+            rows, lat, flags = b.results()
+            wall_ms = (time.perf_counter() - t0) * 1e3
+            assert not any(flags), flags
+            _check(qs, want, rows, f"jit replay {replay}")
+            assert all(0 < x <= wall_ms for q, x in zip(qs, lat) if q.tiles), (wall_ms, lat)
+            for i in range(replay, len(qs), 11):
+                qs[i].run()
+                assert_same(qs[i].fetch(), want[i], f"query {i} eager")
+        ms = b.time(1, 3)
+        assert ms > 0
+        rows, _, flags = b.results()
+        assert not any(flags)
+        _check(qs, want, rows, "jit after timing")
+    finally:
+        b.close()
+
+
+def test_jit_batch_direct_and_reload(mix):
+    """Direct (uncaptured) runs, a reload to a reordered subset (a module
+    of fewer programs), and a switch back to the interpreter's batch."""
+    qs, want = mix
+    b = va.PersistentBatch(qs[:20], jit=1, direct=1)
+    try:
+        rows, _, flags = b.results()
+        assert not any(flags)
+        _check(qs[:20], want[:20], rows, "jit direct")
+        sub = [qs[i] for i in (7, 3, 41, 0, 63, 12)]
+        sw = [want[i] for i in (7, 3, 41, 0, 63, 12)]
+        b.reload(sub)
+        for replay in range(2):
+            rows, _, flags = b.results()
+            assert not any(flags)
+            _check(sub, sw, rows, f"jit reload replay {replay}")
+        b.configure(tiles_per_item=4)
+        rows, _, flags = b.results()
+        assert not any(flags)
+        _check(sub, sw, rows, "interpreter after jit")
+        b.configure(jit=1)
+        rows, _, flags = b.results()
+        _check(sub, sw, rows, "jit again")
+    finally:
+        b.close()
+
+
+def test_jit_batch_capacity(gpu_db):
+    """As the single pass: a result that does not fit is reported with its
+    exact row count and never written past the capacity; one that fits
+    exactly is whole; an empty result is 0 rows."""
+    gpu_db.gen_table("test", 300_001, 1 << 16, seed=7)
+    sqls = ["select id, uniformf from test where uniformi < 30",
+            "select normalf5 from test where uniformf < 0.5",
+            "select id from test where normali20 > 100"]
+    exact = []
+    for sql in sqls:
+        w, _ = oracle.select(gpu_db, sql)
+        exact.append((w, len(next(iter(w.values())))))
+    assert exact[2][1] == 0
+    qs = [gpu_db.prepare(sqls[0], capacity=exact[0][1]),
+          gpu_db.prepare(sqls[1], capacity=max(exact[1][1] - 1, 1)),
+          gpu_db.prepare(sqls[0], capacity=1000),
+          gpu_db.prepare(sqls[2], capacity=1)]
+    b = va.PersistentBatch(qs, jit=1)
+    try:
+        for replay in range(2):
+            rows, _, flags = b.results()
+            assert [f & 1 for f in flags] == [0, 1, 1, 0]
+            assert rows[:3] == [exact[0][1], exact[1][1], exact[0][1]]
+            assert rows[3] == 0
+            assert_same(qs[0].fetch(), exact[0][0], f"exact fit replay {replay}")
+    finally:
+        b.close()
+        for q in qs:
+            q.close()

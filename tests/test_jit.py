@@ -88,3 +88,33 @@ def test_random_programs_compile(db):
     for _ in range(6):
         prog = W.random_program(rng, max_ops=40, selectivity=float(rng.uniform(0.01, 0.8)))
         compiles(db, prog, tab)
+
+
+def batch_module(db, progs, tab, compile_it=True):
+    """vg_jitb_check over progs: (distinct programs, fn per program)."""
+    n = len(progs)
+    arrs = [_ops(db, p) for p in progs]
+    OpP = C.POINTER(va.Op)
+    ops = (OpP * n)(*[C.cast(a, OpP) for a, _ in arrs])
+    nums = (C.c_uint * n)(*[k for _, k in arrs])
+    pcs = (C.c_uint * n)(*[_pc(a, k) for a, k in arrs])
+    tabs = (C.POINTER(va.TabletMeta) * n)(*[tab] * n)
+    al = (C.c_int * n)(*[1] * n)
+    fn = (C.c_uint32 * n)()
+    ms = C.c_double(0)
+    err = C.create_string_buffer(4096)
+    k = va.lib.vg_jitb_check(ops, nums, pcs, tabs, al, n, fn, 1 if compile_it else 0, C.byref(ms), err, 4096)
+    assert k > 0, err.value.decode()
+    return k, list(fn)
+
+
+def test_batch_module_dedups_and_compiles(db):
+    """A batch module holds each distinct program once: the selectivity
+    sweep shares one function, config 1 and the query suite add theirs."""
+    tid = db.gen_table("test", 5000, 4096, seed=1)
+    tab = db.tablets(tid)[0]
+    progs = [config2_sql(t) for t in (0.1, 10.0, 50.0, 90.0)] + [CONFIG1_SQL] + QUERY_SUITE[:3]
+    k, fn = batch_module(db, progs, tab)
+    assert fn[:4] == [0, 0, 0, 0]
+    assert fn[4] == 1
+    assert k == len(set(fn)) and k >= 3

@@ -134,7 +134,8 @@ class StreamStats(C.Structure):
 class PBatchConfig(C.Structure):
     """vg_pbatch_config (include/virg_gpu.h)."""
     _fields_ = [("tiles_per_item", C.c_uint), ("grid_div", C.c_uint), ("split", C.c_uint),
-                ("order", C.c_uint), ("direct", C.c_uint), ("db", C.c_void_p), ("single", C.c_uint)]
+                ("order", C.c_uint), ("direct", C.c_uint), ("db", C.c_void_p), ("single", C.c_uint),
+                ("jit", C.c_uint)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "db"}
@@ -237,6 +238,10 @@ def _load() -> C.CDLL:
                                      C.c_size_t, C.c_char_p, C.c_size_t]),
         "vg_jit_compile_check": (C.c_int, [C.POINTER(Op), C.c_uint, C.c_uint, C.POINTER(TabletMeta),
                                            C.POINTER(C.c_double), C.c_char_p, C.c_size_t]),
+        "vg_jitb_check": (C.c_uint, [C.POINTER(C.POINTER(Op)), C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                     C.POINTER(C.POINTER(TabletMeta)), C.POINTER(C.c_int), C.c_uint,
+                                     C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_double), C.c_char_p,
+                                     C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -718,22 +723,24 @@ def _handles(queries) -> C.Array:
 
 
 class PersistentBatch:
-    """Prepared queries run by two persistent launches over device item
-    lists, captured as one HIP graph (vg_pbatch_* in include/virg_gpu.h)."""
+    """Prepared queries run by persistent launches over device item lists,
+    captured as one HIP graph (vg_pbatch_* in include/virg_gpu.h): the
+    interpreter's two passes, its single pass (single=1), or one launch of
+    the batch's programs compiled together (jit=1)."""
 
     def __init__(self, queries: list[DeviceQuery], tiles_per_item: int = 0, order: int = 0,
-                 grid_div: int = 0, split: int = 0, single: int = 0, direct: int = 0, db=None):
+                 grid_div: int = 0, split: int = 0, single: int = 0, direct: int = 0, db=None, jit: int = 0):
         self.queries = list(queries)
         cfg = PBatchConfig(tiles_per_item, grid_div, split, order, direct,
-                           C.cast(db.ptr, C.c_void_p) if db else None, single)
+                           C.cast(db.ptr, C.c_void_p) if db else None, single, jit)
         err = C.create_string_buffer(256)
         self.h = lib.vg_pbatch_create(_handles(self.queries), len(self.queries), C.byref(cfg), err, 256)
         if not self.h:
             raise VirginianError(err.value.decode() or "vg_pbatch_create failed")
 
     def configure(self, tiles_per_item: int = 0, order: int = 0, grid_div: int = 0, split: int = 0,
-                  single: int = 0, direct: int = 0) -> None:
-        cfg = PBatchConfig(tiles_per_item, grid_div, split, order, direct, None, single)
+                  single: int = 0, direct: int = 0, jit: int = 0) -> None:
+        cfg = PBatchConfig(tiles_per_item, grid_div, split, order, direct, None, single, jit)
         err = C.create_string_buffer(256)
         if lib.vg_pbatch_configure(self.h, C.byref(cfg), err, 256) != VIRG_SUCCESS:
             raise VirginianError(err.value.decode() or "vg_pbatch_configure failed")

@@ -1853,7 +1853,34 @@ struct vg_pbatch {
 	/* every query's outcome, stored there by the kernels (coherent, so it
 	 * can be read once the stream is idle) */
 	pinned_buf h_res;
+	/* config.jit: the batch's module and the device state of its launch */
+	vgx_kernel *jk = nullptr;
+	vgxb_args JA;
+	unsigned jgrid = 0;
+	vgx_launch_args *d_qa = nullptr;
+	uint32_t *d_fn = nullptr;
+	vgxb_item *d_jitems = nullptr;
+	vgx_tile *d_jtiles = nullptr;
+	uint64_t *d_jstate = nullptr;          /* counter, epoch, start; then status words */
+	uint32_t *d_jdone = nullptr;
+	vg_ctrl *d_jctrl = nullptr;            /* per query, the batch's own (eager runs keep theirs) */
 };
+
+static void pbatch_free_jit(vg_pbatch *b)
+{
+	void *bufs[] = { b->d_qa, b->d_fn, b->d_jitems, b->d_jtiles, b->d_jstate, b->d_jdone, b->d_jctrl };
+	for (void *p : bufs)
+		if (p)
+			(void)hipFree(p);
+	b->d_qa = nullptr;
+	b->d_fn = nullptr;
+	b->d_jitems = nullptr;
+	b->d_jtiles = nullptr;
+	b->d_jstate = nullptr;
+	b->d_jdone = nullptr;
+	b->d_jctrl = nullptr;
+	b->jk = nullptr;
+}
 
 static void pbatch_drop_graph(vg_pbatch *b)
 {
@@ -1871,6 +1898,7 @@ extern "C" void vg_pbatch_free(vg_pbatch *b)
 		(void)hipStreamSynchronize(b->stream);
 	refs_drop(b->qs);
 	pbatch_drop_graph(b);
+	pbatch_free_jit(b);
 	void *bufs[] = { b->d_progs, b->d_launch, b->d_info, b->d_q, b->d_head, b->d_items, b->d_witems,
 		b->d_icount, b->d_ibase, b->d_status, b->d_stamp, b->d_gq };
 	for (void *p : bufs)
@@ -1911,6 +1939,8 @@ static void fill_launch(const vg_query *q, vg_launch &L)
  * group on its own branch when there are several. */
 static int pbatch_record(vg_pbatch *b)
 {
+	if (b->jk)
+		return vgxb_launch(b->jk, &b->JA, b->jgrid, b->stream) == 0;
 	const size_t ng = b->groups.size();
 	const bool single = b->cfg.single != 0;
 	int rc = vg_kernel_batch_reset(&b->R, b->stream) == 0 && hipEventRecord(b->fork, b->stream) == hipSuccess;
@@ -1928,11 +1958,137 @@ static int pbatch_record(vg_pbatch *b)
 	return rc;
 }
 
+/* Capture the batch's launches as its graph (not for direct batches). */
+static int pbatch_capture(vg_pbatch *b, char *err, size_t errlen)
+{
+	pbatch_drop_graph(b);
+	if (b->cfg.direct)
+		return VIRG_SUCCESS;
+	if (herr(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture"))
+		return qerr(err, errlen, "graph capture failed");
+	const int rc = pbatch_record(b);
+	const hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
+	if (!rc || ec != hipSuccess) {
+		(void)hipGetLastError();
+		return qerr(err, errlen, "graph capture failed");
+	}
+	if (herr(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate"))
+		return qerr(err, errlen, "graph instantiate failed");
+	return VIRG_SUCCESS;
+}
+
+/* config.jit: compile the batch's programs into one module, lay out each
+ * query's launch (tile table, status words) and the work items (every
+ * query's chunks in order, the smallest query first: the many small
+ * queries of a mixed batch complete early, the large ones spread over the
+ * whole device at the end), upload them. */
+static int pbatch_build_jit(vg_pbatch *b, char *err, size_t errlen)
+{
+	pbatch_free_jit(b);
+	const unsigned n = (unsigned)b->qs.size();
+	if (n == 0)
+		return pbatch_capture(b, err, errlen);
+	std::vector<const vg_prog *> progs(n);
+	std::vector<int> aligned(n);
+	for (unsigned i = 0; i < n; i++) {
+		progs[i] = &b->qs[i]->P;
+		bool al = true;
+		for (const vg_tabdesc &d : b->qs[i]->p->desc)
+			al = al && d.aligned;
+		aligned[i] = al ? 1 : 0;
+	}
+	std::vector<uint32_t> fn(n);
+	vgx_kernel *k = vgxb_get(progs.data(), aligned.data(), n, fn.data(), err, errlen);
+	if (!k)
+		return VIRG_FAIL;
+	std::vector<vgx_launch_args> qa(n);
+	std::vector<vgx_tile> tiles;
+	std::vector<uint32_t> nchunks(n);
+	size_t status_words = 0;
+	std::vector<size_t> toff(n), soff(n);
+	for (unsigned i = 0; i < n; i++) {
+		vg_query *q = b->qs[i];
+		vgx_shape sh;
+		if (vgx_shape_of(&q->P, aligned[i], 0, 0, &sh, err, errlen) != 0)
+			return VIRG_FAIL;
+		const std::vector<vgx_tile> t = jit_tiles(*q->p, sh.TR);
+		toff[i] = tiles.size();
+		tiles.insert(tiles.end(), t.begin(), t.end());
+		nchunks[i] = (uint32_t)((t.size() + sh.S - 1) / sh.S);
+		soff[i] = status_words;
+		status_words += nchunks[i];
+	}
+	std::vector<unsigned> order(n);
+	for (unsigned i = 0; i < n; i++)
+		order[i] = i;
+	std::stable_sort(order.begin(), order.end(), [&](unsigned x, unsigned y) { return nchunks[x] < nchunks[y]; });
+	std::vector<vgxb_item> items;
+	for (unsigned i : order)
+		for (uint32_t c = 0; c < nchunks[i]; c++)
+			items.push_back(vgxb_item{i, c});
+	const size_t state_bytes = (3 + status_words) * sizeof(uint64_t);
+	if (herr(hipMalloc((void **)&b->d_qa, n * sizeof(vgx_launch_args)), "batch") ||
+		herr(hipMalloc((void **)&b->d_fn, n * sizeof(uint32_t)), "batch") ||
+		herr(hipMalloc((void **)&b->d_jitems, (items.size() ? items.size() : 1) * sizeof(vgxb_item)), "batch") ||
+		herr(hipMalloc((void **)&b->d_jtiles, (tiles.size() ? tiles.size() : 1) * sizeof(vgx_tile)), "batch") ||
+		herr(hipMalloc((void **)&b->d_jstate, state_bytes), "batch") ||
+		herr(hipMalloc((void **)&b->d_jdone, n * sizeof(uint32_t)), "batch") ||
+		herr(hipMalloc((void **)&b->d_jctrl, n * sizeof(vg_ctrl)), "batch"))
+		return qerr(err, errlen, "cannot allocate the batch");
+	for (unsigned i = 0; i < n; i++) {
+		vg_query *q = b->qs[i];
+		vgx_launch_args &a = qa[i];
+		memset(&a, 0, sizeof a);
+		a.tabs = q->p->d_desc;
+		a.tiles = b->d_jtiles + toff[i];
+		a.num_tiles = (uint32_t)(i + 1 < n ? toff[i + 1] - toff[i] : tiles.size() - toff[i]);
+		a.nchunks = nchunks[i];
+		a.status = b->d_jstate + 3 + soff[i];
+		a.ctrl = b->d_jctrl + i;
+		a.capacity = q->capacity;
+		a.spin_limit = 1u << 20;
+		for (unsigned j = 0; j < q->P.nout; j++)
+			a.out[j] = q->out[j];
+		jit_immediates(q->P, a);
+	}
+	if (herr(hipMemcpy(b->d_qa, qa.data(), n * sizeof(vgx_launch_args), hipMemcpyHostToDevice), "batch") ||
+		herr(hipMemcpy(b->d_fn, fn.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice), "batch") ||
+		(items.size() && herr(hipMemcpy(b->d_jitems, items.data(), items.size() * sizeof(vgxb_item),
+			hipMemcpyHostToDevice), "batch")) ||
+		(tiles.size() && herr(hipMemcpy(b->d_jtiles, tiles.data(), tiles.size() * sizeof(vgx_tile),
+			hipMemcpyHostToDevice), "batch")) ||
+		herr(hipMemset(b->d_jstate, 0, state_bytes), "batch") ||
+		herr(hipMemset(b->d_jctrl, 0, n * sizeof(vg_ctrl)), "batch"))
+		return qerr(err, errlen, "cannot upload the batch");
+	if (b->h_res.p)
+		memset(b->h_res.p, 0, n * sizeof(vg_bresult));
+	vg_bresult *d_res = nullptr;
+	if (b->h_res.p && herr(hipHostGetDevicePointer((void **)&d_res, b->h_res.p, 0), "batch results"))
+		return qerr(err, errlen, "cannot map the batch's results");
+	vgxb_args &A = b->JA;
+	memset(&A, 0, sizeof A);
+	A.qa = b->d_qa;
+	A.fn = b->d_fn;
+	A.items = b->d_jitems;
+	A.nitems = (uint32_t)items.size();
+	A.nquery = n;
+	A.state = b->d_jstate;
+	A.done = b->d_jdone;
+	A.hres = d_res;
+	b->jgrid = vgxb_grid(k, A.nitems, (unsigned)b->d->prop.multiProcessorCount);
+	b->jk = k;
+	b->groups.clear();
+	return pbatch_capture(b, err, errlen);
+}
+
 /* Group the queries (b->cfg.split 0: by the write launch's occupancy,
  * 1: all together), build the item lists, upload them and capture the
  * graph. */
 static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 {
+	if (b->cfg.jit && !b->cfg.single)
+		return pbatch_build_jit(b, err, errlen);
+	pbatch_free_jit(b);
 	const unsigned n = (unsigned)b->qs.size();
 	/* a single-pass item is one tile: a tile waits for the counts of the
 	 * tiles before it, which must not sit behind other tiles of one item */
@@ -2130,20 +2286,7 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 	b->R.nhead = (uint32_t)ng;
 	b->R.status = single ? b->d_status : nullptr;
 	b->R.nstatus = single ? soff : 0;
-	pbatch_drop_graph(b);
-	if (b->cfg.direct)
-		return VIRG_SUCCESS;
-	if (herr(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture"))
-		return qerr(err, errlen, "graph capture failed");
-	const int rc = pbatch_record(b);
-	const hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
-	if (!rc || ec != hipSuccess) {
-		(void)hipGetLastError();
-		return qerr(err, errlen, "graph capture failed");
-	}
-	if (herr(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate"))
-		return qerr(err, errlen, "graph instantiate failed");
-	return VIRG_SUCCESS;
+	return pbatch_capture(b, err, errlen);
 }
 
 /* Take a new set of queries: check them, make sure each has what the

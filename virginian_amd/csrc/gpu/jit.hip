@@ -63,9 +63,11 @@ typedef unsigned int u32; typedef int i32; typedef unsigned long long u64; typed
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));
 typedef u32 u32x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) void lds_void;
+#define las __attribute__((address_space(3)))
 typedef vg_tabdesc vgx_tab;
 typedef vg_ctrl vgx_ctrl;
 typedef vgx_launch_args vgx_args;
+typedef const __attribute__((address_space(4))) vgx_args cvgx_args;
 #define OOB 0x80000000u   /* a buffer offset past every resource: loads nothing, reads 0 */
 #define ST_AGG (1ull << 62)
 #define ST_PRE (2ull << 62)
@@ -413,12 +415,27 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 	int R = P->nslot <= 10 ? 16 : P->nslot <= 20 ? 8 : 4;
 	if (R_override == 4 || R_override == 8 || R_override == 16)
 		R = R_override;
-	/* filter columns staged in LDS: bytes per row */
+	/* output staging: per wave, 256 rows x the widest group of outputs
+	 * written together (groups of at most 16 bytes a row) */
+	uint32_t stage_w = 0, gw = 0;
+	for (uint32_t j = 0; j < P->nout; j++) {
+		const uint32_t w = P->out[j].width;
+		if (gw + w > 16u)
+			gw = 0;
+		gw += w;
+		if (gw > stage_w)
+			stage_w = gw;
+	}
+	const uint32_t stage_wave = 256u * (stage_w ? stage_w : 4u);
+	/* filter columns staged in LDS: bytes per row; with the staging and
+	 * the kernel's own LDS (under 1 KB) a workgroup takes at most 79 KB,
+	 * so two fit a CU's 160 KB */
 	uint32_t fbytes = 0;
 	for (uint32_t c = 0; c < P->ncol; c++)
 		if (P->col[c].phase == VG_PHASE_FILTER)
 			fbytes += P->col[c].width == 1 ? 1u : P->col[c].width;
-	const uint32_t budget = 64u * 1024u;
+	const uint32_t two_per_cu = 79u * 1024u - 4u * stage_wave;
+	const uint32_t budget = two_per_cu < 64u * 1024u ? two_per_cu : 64u * 1024u;
 	while (R > 4 && fbytes * 256u * (uint32_t)R > budget)
 		R /= 2;
 	const uint32_t TR = 256u * (uint32_t)R;
@@ -437,34 +454,25 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 		snprintf(err, errlen, "chunk of %u tiles of %u rows per lane is too large", S, (unsigned)R);
 		return -1;
 	}
-	/* output staging: per wave, 256 rows x the widest group of outputs
-	 * written together (groups of at most 16 bytes a row) */
-	uint32_t stage_w = 0, gw = 0;
-	for (uint32_t j = 0; j < P->nout; j++) {
-		const uint32_t w = P->out[j].width;
-		if (gw + w > 16u)
-			gw = 0;
-		gw += w;
-		if (gw > stage_w)
-			stage_w = gw;
-	}
 	sh->R = R;
 	sh->S = S;
 	sh->TR = TR;
 	sh->fbytes_row = fbytes;
 	sh->ftile_bytes = ftile;
 	sh->stage_off = ftile * S;
-	sh->stage_wave_bytes = 256u * (stage_w ? stage_w : 4u);
+	sh->stage_wave_bytes = stage_wave;
 	sh->lds_bytes = sh->stage_off + 4u * sh->stage_wave_bytes;
 	return 0;
 }
 
-/* Generate the kernel source of program P at shape sh. */
 /* Program P at shape sh as namespace `ns`: its constants and chunk(), the
- * work of one chunk (both phases, the look-back, the writes), inlined into
- * the kernels that call it.  chunk() calls grab_next() in thread 0 as the
- * chunk's last tile starts, for the caller's next chunk. */
-static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const char *ns)
+ * work of one chunk (both phases, the look-back, the writes), returning
+ * its VG_OVF_* bits.  chunk() calls grab_next() in thread 0 as the chunk's
+ * last tile starts, for the caller's next chunk.  It is inlined into the
+ * kernel that calls it, or, `outlined`, a function of its own (a module of
+ * many programs then compiles in time linear in their number); its LDS
+ * arguments then carry their address space. */
+static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const char *ns, bool outlined = false)
 {
 	const int R = sh->R, NQ = R / 4;
 	o("\nnamespace %s {\n", ns);
@@ -526,8 +534,15 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 			decl("\t\t\tu32 e%u[R], e%uh[R];\n#pragma unroll\n\t\t\tfor (int r = 0; r < R; ++r) { e%u[r] = 0u; e%uh[r] = 0u; }\n",
 				c, c, c, c);
 
-	o("template <typename G>\n__device__ __attribute__((always_inline)) inline void chunk(const vgx_args &A, const u32 c, "
-	  "const u64 epoch, u32 (*s_cnt)[4], u64 *s_pre, u8 *lds, u32 &ovf, G grab_next)\n{\n");
+	if (outlined)
+		o("template <typename AT, typename G>\n__device__ __attribute__((noinline)) u32 chunk(AT &A, const u32 c, "
+		  "const u64 epoch, las u32 (*s_cnt3)[4], las u64 *s_pre3, las u8 *lds3, G grab_next)\n{\n"
+		  "\tu32 (*const s_cnt)[4] = (u32 (*)[4])s_cnt3;\n\tu64 *const s_pre = (u64 *)s_pre3;\n"
+		  "\tu8 *const lds = (u8 *)lds3;\n");
+	else
+		o("template <typename AT, typename G>\n__device__ __attribute__((always_inline)) inline u32 chunk(AT &A, "
+		  "const u32 c, const u64 epoch, u32 (*s_cnt)[4], u64 *s_pre, u8 *lds, G grab_next)\n{\n");
+	o("\tu32 ovf = 0;\n");
 	o("\tconst u32 tid = threadIdx.x, lane = tid & 63u, wave = uni(tid >> 6);\n");
 	o("\tu8 *const ostage = lds + %u + wave * %u;\n", sh->stage_off, sh->stage_wave_bytes);
 	o("\t{\n");
@@ -757,7 +772,7 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 	}
 	o("\t\t\t\t}\n\t\t\t\tat += qn;\n\t\t\t}\n");
 	o("\t\t\ttdone += s_cnt[s][0] + s_cnt[s][1] + s_cnt[s][2] + s_cnt[s][3];\n\t\t}\n");
-	o("\t}\n}\n} /* namespace %s */\n", ns);
+	o("\t}\n\treturn ovf;\n}\n} /* namespace %s */\n", ns);
 }
 
 static void gen_prelude(out_buf &o)
@@ -786,7 +801,7 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 	o("\tconst u64 epoch = s_epoch;\n\tu32 ovf = 0;\n");
 	o("\tfor (;;) {\n\t\tconst u32 c = uni(s_chunk);\n\t\tif (c >= A.nchunks)\n\t\t\tbreak;\n");
 	o("\t\tu32 nxt = 0;\n");
-	o("\t\tp0::chunk(A, c, epoch, s_cnt, &s_pre, lds, ovf, [&]() { nxt = grab(A, epoch); });\n");
+	o("\t\tovf |= p0::chunk(A, c, epoch, s_cnt, &s_pre, lds, [&]() { nxt = grab(A, epoch); });\n");
 	o("\t\t__syncthreads();\n\t\tif (tid == 0)\n\t\t\ts_chunk = nxt;\n\t\t__syncthreads();\n\t}\n");
 	o("\tif (ovf)\n\t\tatomicOr(&A.ctrl->overflow, ovf);\n}\n");
 
@@ -802,6 +817,7 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 struct vgx_kernel {
 	hipModule_t mod = nullptr;
 	hipFunction_t fn = nullptr;
+	hipFunction_t fn2 = nullptr;   /* a batch module's reset kernel */
 	vgx_shape sh;
 	int per_cu = 0;            /* workgroups per CU at this kernel's resources */
 	double compile_ms = 0;
@@ -824,8 +840,30 @@ static const char *arch_name()
 	return g_arch.c_str();
 }
 
+/* With VG_JIT_DUMP=<dir>: the source and code object of every module
+ * compiled, for inspection. */
+static void dump(const std::string &src, const std::vector<char> &code)
+{
+	const char *dir = getenv("VG_JIT_DUMP");
+	if (!dir)
+		return;
+	char path[512];
+	const size_t h = std::hash<std::string>()(src);
+	snprintf(path, sizeof path, "%s/vgx_%016zx.hip", dir, h);
+	if (FILE *f = fopen(path, "w")) {
+		fwrite(src.data(), 1, src.size(), f);
+		fclose(f);
+	}
+	snprintf(path, sizeof path, "%s/vgx_%016zx.co", dir, h);
+	if (FILE *f = fopen(path, "wb")) {
+		fwrite(code.data(), 1, code.size(), f);
+		fclose(f);
+	}
+}
+
 /* Compile `src` for the current device; NULL with a message on failure. */
-static vgx_kernel *compile(const std::string &src, const vgx_shape &sh, char *err, size_t errlen)
+static vgx_kernel *compile(const std::string &src, const vgx_shape &sh, char *err, size_t errlen,
+	const char *fname = "vgx_main", const char *fname2 = nullptr)
 {
 	double t0 = 0;
 	{
@@ -858,25 +896,12 @@ static vgx_kernel *compile(const std::string &src, const vgx_shape &sh, char *er
 	std::vector<char> code(cs);
 	hiprtcGetCode(prog, code.data());
 	hiprtcDestroyProgram(&prog);
-	if (const char *dir = getenv("VG_JIT_DUMP")) {
-		/* source and code object of every kernel compiled, for inspection */
-		char path[512];
-		const size_t h = std::hash<std::string>()(src);
-		snprintf(path, sizeof path, "%s/vgx_%016zx.hip", dir, h);
-		if (FILE *f = fopen(path, "w")) {
-			fwrite(src.data(), 1, src.size(), f);
-			fclose(f);
-		}
-		snprintf(path, sizeof path, "%s/vgx_%016zx.co", dir, h);
-		if (FILE *f = fopen(path, "wb")) {
-			fwrite(code.data(), 1, code.size(), f);
-			fclose(f);
-		}
-	}
+	dump(src, code);
 	vgx_kernel *k = new vgx_kernel();
 	k->sh = sh;
 	if (hipModuleLoadData(&k->mod, code.data()) != hipSuccess ||
-		hipModuleGetFunction(&k->fn, k->mod, "vgx_main") != hipSuccess) {
+		hipModuleGetFunction(&k->fn, k->mod, fname) != hipSuccess ||
+		(fname2 && hipModuleGetFunction(&k->fn2, k->mod, fname2) != hipSuccess)) {
 		(void)hipGetLastError();
 		snprintf(err, errlen, "cannot load the compiled kernel");
 		if (k->mod)
@@ -954,6 +979,189 @@ extern "C" int vgx_launch(const vgx_kernel *k, const vgx_launch_args *a, unsigne
 	return e == hipSuccess ? 0 : -2;
 }
 
+/* ------------------------------------------------------------- batches */
+/* Source of a batch module: every distinct program of the batch once (as
+ * namespace p<k>), a reset kernel and the persistent kernel.  fn[i] is
+ * query i's program; lds: the dynamic LDS the kernel takes. */
+static std::string vgxb_text(const vg_prog *const *P, const int *aligned, unsigned n, uint32_t *fn,
+	uint32_t *lds, char *err, size_t errlen)
+{
+	out_buf o;
+	gen_prelude(o);
+	std::unordered_map<std::string, uint32_t> seen;
+	uint32_t nk = 0, maxlds = 0, maxs = 1;
+	for (unsigned i = 0; i < n; i++) {
+		vgx_shape sh;
+		if (vgx_shape_of(P[i], aligned[i], 0, 0, &sh, err, errlen) != 0)
+			return std::string();
+		out_buf body;
+		gen_program(body, P[i], &sh, "pK", true);
+		auto it = seen.find(body.s);
+		if (it != seen.end()) {
+			fn[i] = it->second;
+			continue;
+		}
+		char ns[32];
+		snprintf(ns, sizeof ns, "p%u", nk);
+		gen_program(o, P[i], &sh, ns, true);
+		seen[body.s] = nk;
+		fn[i] = nk++;
+		maxlds = sh.lds_bytes > maxlds ? sh.lds_bytes : maxlds;
+		maxs = sh.S > maxs ? sh.S : maxs;
+	}
+	*lds = maxlds;
+	o(R"VGXB(
+typedef const __attribute__((address_space(4))) vgxb_args cvgxb_args;
+
+/* the next work item; the grab completing the launch's nitems + grid
+ * grabs sets the counter back and moves the epoch on */
+__device__ __forceinline__ u32 grab_item(u64 *state, u32 nitems, u64 epoch)
+{
+	const u64 g = atomicAdd(&state[0], 1ull);
+	if (g == (u64)nitems + gridDim.x - 1) {
+		__hip_atomic_store(&state[0], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		__hip_atomic_store(&state[1], (epoch + 1) & ST_EMASK, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+	return g < nitems ? (u32)g : 0xFFFFFFFFu;
+}
+
+/* thread 0's call for the next item, which it leaves in LDS: plain values
+ * (the chunk functions are not inlined, and a closure over the kernel's
+ * locals would put them in scratch) */
+struct next_item {
+	u64 *state;
+	u32 nitems;
+	u64 epoch;
+	u32 *slot;
+	__device__ void operator()() const { *slot = grab_item(state, nitems, epoch); }
+};
+
+/* each query's control block, chunk count and outcome cleared; the start stamped */
+extern "C" __global__ __launch_bounds__(256) void vgxb_reset(const vgxb_args B)
+{
+	const u32 i = blockIdx.x * 256u + threadIdx.x;
+	if (i < B.nquery) {
+		vg_ctrl *c = B.qa[i].ctrl;
+		c->total = 0;
+		c->overflow = 0;
+		B.done[i] = 0;
+		vg_bresult r;
+		r.total = 0; r.t_done = 0; r.t_start = 0; r.flags = 0; r.pad = 0;
+		B.hres[i] = r;
+	}
+	if (i == 0)
+		B.state[2] = wall_clock64();
+}
+
+)VGXB");
+	/* Occupancy bound for the chunk functions' registers (they are not
+	 * compiled under the kernel's own occupancy target): three workgroups
+	 * per CU, or what the LDS allows when that is fewer. */
+	const uint32_t by_lds = 163840u / (maxlds + 1024u);
+	const uint32_t waves = by_lds < 1 ? 1 : by_lds > 3 ? 3 : by_lds;
+	o("extern \"C\" __global__ __launch_bounds__(256, %u) void vgxb_main(const vgxb_args B)\n{\n"
+	  "\textern __shared__ __attribute__((aligned(16))) u8 lds[];\n", waves);
+	o("\t__shared__ u32 s_cnt[%u][4];\n", maxs);
+	o(R"VGXB(	__shared__ u64 s_pre;
+	__shared__ u32 s_item, s_next, s_epoch, s_ovf;
+	const u32 tid = threadIdx.x;
+	if (tid == 0) {
+		const u64 e = __hip_atomic_load(&B.state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		wait_vm();   /* the epoch is in hand before the first grab */
+		s_epoch = (u32)e;
+		s_ovf = 0;
+		s_item = grab_item(B.state, B.nitems, e);
+	}
+	__syncthreads();
+	const u64 epoch = s_epoch;
+	for (;;) {
+		const u32 i = uni(s_item);
+		if (i >= B.nitems)
+			break;
+		const vgxb_item it = ldc(B.items + i);
+		const u32 q = uni(it.q), c = uni(it.c);
+		cvgx_args &A = *(cvgx_args *)(B.qa + q);
+		const next_item next = {B.state, B.nitems, epoch, &s_next};
+		u32 ovf = 0;
+		switch (ldc(B.fn + q)) {
+)VGXB");
+	for (uint32_t k = 0; k < nk; k++)
+		o("\t\tcase %u: ovf = p%u::chunk(A, c, epoch, (las u32 (*)[4])s_cnt, (las u64 *)&s_pre, (las u8 *)lds, next); "
+		  "break;\n", k, k);
+	o(R"VGXB(		}
+		if (ovf)
+			atomicOr(&s_ovf, ovf);
+		__syncthreads();
+		if (tid == 0) {
+			const u32 f = s_ovf;
+			s_ovf = 0;
+			if (f)
+				atomicOr(&A.ctrl->overflow, f);
+			/* the chunk that wrote the query's total, and any that set flags,
+			 * publish before counting themselves done; the last one done
+			 * reads the query's outcome after them */
+			const bool pub = c + 1 == A.nchunks || f != 0;
+			const u32 d = pub ? __hip_atomic_fetch_add(&B.done[q], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT)
+			                  : __hip_atomic_fetch_add(&B.done[q], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			if (d + 1 == A.nchunks) {
+				__atomic_thread_fence(__ATOMIC_ACQUIRE);
+				vg_bresult r;
+				r.total = __hip_atomic_load(&A.ctrl->total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				r.flags = __hip_atomic_load(&A.ctrl->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				r.t_done = wall_clock64();
+				r.t_start = B.state[2];
+				r.pad = 0;
+				B.hres[q] = r;
+			}
+			s_item = s_next;
+		}
+		__syncthreads();
+	}
+}
+)VGXB");
+	return o.s;
+}
+
+extern "C" vgx_kernel *vgxb_get(const vg_prog *const *P, const int *aligned, unsigned n, uint32_t *fn,
+	char *err, size_t errlen)
+{
+	uint32_t lds = 0;
+	const std::string src = vgxb_text(P, aligned, n, fn, &lds, err, errlen);
+	if (src.empty())
+		return nullptr;
+	std::lock_guard<std::mutex> g(g_mu);
+	auto it = g_cache.find(src);
+	if (it != g_cache.end())
+		return it->second;
+	vgx_shape sh;
+	memset(&sh, 0, sizeof sh);
+	sh.lds_bytes = lds;
+	vgx_kernel *k = compile(src, sh, err, errlen, "vgxb_main", "vgxb_reset");
+	if (k)
+		g_cache[src] = k;
+	return k;
+}
+
+extern "C" unsigned vgxb_grid(const vgx_kernel *k, unsigned nitems, unsigned cus)
+{
+	return vgx_kernel_grid(k, nitems, cus);
+}
+
+/* the reset, then the persistent launch, on `stream` */
+extern "C" int vgxb_launch(const vgx_kernel *k, const vgxb_args *a, unsigned grid, hipStream_t stream)
+{
+	size_t sz = sizeof *a;
+	void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, (void *)a, HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz,
+		HIP_LAUNCH_PARAM_END};
+	const unsigned rgrid = (a->nquery + 255) / 256;
+	if (hipModuleLaunchKernel(k->fn2, rgrid ? rgrid : 1, 1, 1, 256, 1, 1, 0, stream, nullptr, cfg) != hipSuccess)
+		return -2;
+	if (grid == 0)
+		return 0;
+	return hipModuleLaunchKernel(k->fn, grid, 1, 1, 256, 1, 1, k->sh.lds_bytes, stream, nullptr, cfg) == hipSuccess
+		? 0 : -2;
+}
+
 /* ------------------------------------------------- inspection (no device) */
 extern "C" int vg_decode(const virg_op *stmt, unsigned num_ops, unsigned pc,
 	const virg_tablet_meta *tab, const virg_tablet_meta *res, vg_prog *P, char *err, size_t errlen);
@@ -982,6 +1190,42 @@ extern "C" long vg_jit_source(const virg_op *ops, unsigned num_ops, unsigned pc,
 	return (long)len;
 }
 
+/* hiprtc compile of src for gfx950, not loaded: 1, or 0 with the log */
+static int rtc_check(const char *src, double *ms, char *err, size_t errlen)
+{
+	struct timespec t0, t1;
+	clock_gettime(CLOCK_MONOTONIC, &t0);
+	hiprtcProgram prog;
+	int ok = 0;
+	if (hiprtcCreateProgram(&prog, src, "vgx.hip", 0, nullptr, nullptr) == HIPRTC_SUCCESS) {
+		const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+			"-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+		const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+		if (rc == HIPRTC_SUCCESS) {
+			ok = 1;
+			size_t cs = 0;
+			if (getenv("VG_JIT_DUMP") && hiprtcGetCodeSize(prog, &cs) == HIPRTC_SUCCESS) {
+				std::vector<char> code(cs);
+				if (hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS)
+					dump(src, code);
+			}
+		} else {
+			size_t ls = 0;
+			hiprtcGetProgramLogSize(prog, &ls);
+			std::string log(ls + 1, '\0');
+			hiprtcGetProgramLog(prog, &log[0]);
+			snprintf(err, errlen, "%s: %.900s", hiprtcGetErrorString(rc), log.c_str());
+		}
+		hiprtcDestroyProgram(&prog);
+	} else {
+		snprintf(err, errlen, "hiprtc: cannot create program");
+	}
+	clock_gettime(CLOCK_MONOTONIC, &t1);
+	if (ms)
+		*ms = (t1.tv_sec - t0.tv_sec) * 1e3 + (t1.tv_nsec - t0.tv_nsec) / 1e6;
+	return ok;
+}
+
 /* Compile a program's specialised kernel for gfx950 with hiprtc without
  * loading it (no device needed): 1 on success, 0 with the compiler's
  * message.  *ms: compile time. */
@@ -997,30 +1241,35 @@ extern "C" int vg_jit_compile_check(const virg_op *ops, unsigned num_ops, unsign
 	char *src = nullptr;
 	if (vgx_source(&P, &sh, &src) != 0)
 		return 0;
-	struct timespec t0, t1;
-	clock_gettime(CLOCK_MONOTONIC, &t0);
-	hiprtcProgram prog;
-	int ok = 0;
-	if (hiprtcCreateProgram(&prog, src, "vgx.hip", 0, nullptr, nullptr) == HIPRTC_SUCCESS) {
-		const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
-			"-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-		const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
-		if (rc == HIPRTC_SUCCESS) {
-			ok = 1;
-		} else {
-			size_t ls = 0;
-			hiprtcGetProgramLogSize(prog, &ls);
-			std::string log(ls + 1, '\0');
-			hiprtcGetProgramLog(prog, &log[0]);
-			snprintf(err, errlen, "%s: %.900s", hiprtcGetErrorString(rc), log.c_str());
-		}
-		hiprtcDestroyProgram(&prog);
-	} else {
-		snprintf(err, errlen, "hiprtc: cannot create program");
-	}
-	clock_gettime(CLOCK_MONOTONIC, &t1);
-	if (ms)
-		*ms = (t1.tv_sec - t0.tv_sec) * 1e3 + (t1.tv_nsec - t0.tv_nsec) / 1e6;
+	const int ok = rtc_check(src, ms, err, errlen);
 	free(src);
 	return ok;
+}
+
+/* The batch module of n programs (program i: ops[i], its Parallel section
+ * at pc[i], over tablets like tab[i]; aligned[i] selects the aligned
+ * variant): fn[i] receives the program's index in the module; the module
+ * is compiled with hiprtc, not loaded, when `compile` is set.  Returns the
+ * number of distinct programs, or 0 with a message; *ms: compile time. */
+extern "C" unsigned vg_jitb_check(const virg_op *const *ops, const unsigned *num_ops, const unsigned *pc,
+	const virg_tablet_meta *const *tab, const int *aligned, unsigned n, uint32_t *fn, int compile,
+	double *ms, char *err, size_t errlen)
+{
+	std::vector<vg_prog> P(n);
+	std::vector<const vg_prog *> pp(n);
+	for (unsigned i = 0; i < n; i++) {
+		if (vg_decode(ops[i], num_ops[i], pc[i], tab[i], nullptr, &P[i], err, errlen) != VIRG_SUCCESS)
+			return 0;
+		pp[i] = &P[i];
+	}
+	uint32_t lds = 0;
+	const std::string src = vgxb_text(pp.data(), aligned, n, fn, &lds, err, errlen);
+	if (src.empty())
+		return 0;
+	if (compile && !rtc_check(src.c_str(), ms, err, errlen))
+		return 0;
+	uint32_t k = 0;
+	for (unsigned i = 0; i < n; i++)
+		k = fn[i] + 1 > k ? fn[i] + 1 : k;
+	return k;
 }

@@ -37,8 +37,15 @@ double vgx_kernel_compile_ms(const vgx_kernel *k);
 /* workgroups for a launch of nchunks chunks on `cus` compute units */
 unsigned vgx_kernel_grid(const vgx_kernel *k, unsigned nchunks, unsigned cus);
 
+/* a batch module for programs P[0..n) (aligned[i]: query i's tablets are
+ * 16-byte aligned); fn[i] = query i's program index in it */
+vgx_kernel *vgxb_get(const vg_prog *const *P, const int *aligned, unsigned n, uint32_t *fn, char *err,
+	size_t errlen);
+unsigned vgxb_grid(const vgx_kernel *k, unsigned nitems, unsigned cus);
+
 #ifdef __HIP_PLATFORM_AMD__
 int vgx_launch(const vgx_kernel *k, const vgx_launch_args *a, unsigned grid, hipStream_t stream);
+int vgxb_launch(const vgx_kernel *k, const vgxb_args *a, unsigned grid, hipStream_t stream);
 #endif
 
 #endif

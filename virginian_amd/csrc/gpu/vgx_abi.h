@@ -36,4 +36,24 @@ typedef struct {
 	uint64_t oimm[VG_MAX_OUT];
 } vgx_launch_args;
 
+/* A batch of queries run by one persistent specialised launch: work items
+ * are (query, chunk) in a list where every query's chunks are in order;
+ * workgroups take them from a counter, so a chunk only ever waits for
+ * chunks taken before it. */
+typedef struct {
+	uint32_t q;                  /* query */
+	uint32_t c;                  /* chunk of the query */
+} vgxb_item;
+
+typedef struct {
+	const vgx_launch_args *qa;   /* per query: its launch (status words, control block, outputs) */
+	const uint32_t *fn;          /* per query: its program's index in the module */
+	const vgxb_item *items;
+	uint32_t nitems;
+	uint32_t nquery;
+	uint64_t *state;             /* [0] item counter, [1] launch epoch, [2] device clock at the start */
+	uint32_t *done;              /* per query: chunks finished in this run */
+	vg_bresult *hres;            /* per query: its outcome, in pinned host memory */
+} vgxb_args;
+
 #endif
