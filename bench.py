@@ -294,6 +294,12 @@ def bench_batched(a) -> int:
     progs, qtab = [], []
     while len(progs) < a.queries:
         k = int(rng.integers(0, len(tids)))
+        if a.programs and len(progs) >= a.programs:
+            # the structure of an earlier program over another table
+            base = progs[len(progs) % a.programs]
+            progs.append([(o[0], tids[k], *o[2:]) if o[0] == va.OP_Table else o for o in base])
+            qtab.append(k)
+            continue
         p = W.random_program(rng, max_ops=40, selectivity=float(rng.uniform(0.0001, 0.8)), table=tids[k])
         if len(p) < 4:
             continue
@@ -369,7 +375,8 @@ def bench_batched(a) -> int:
         "ms_per_step": round(batch_ms, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+i32", "data": "synthetic (seeded Philox generator, seeded programs)",
         "config": {"workload": f"batched: {len(qs)} random op-programs (4-40 ops) over {len(sizes)} resident "
-                               "tables of log-uniform 1e3-1e7 rows, selectivity U[0.01 %, 80 %]",
+                               "tables of log-uniform 1e3-1e7 rows, selectivity U[0.01 %, 80 %]"
+                               + (f", {a.programs} distinct structures" if a.programs else ""),
                    "queries": len(qs), "table_rows": sizes, "rows_scanned_per_round": scanned,
                    "bytes_per_round": nbytes, "pct_hbm_peak": round(value / HBM_PEAK_GBS * 100, 2),
                    "batched": {"kind": "one persistent launch of the batch's programs compiled together",
@@ -739,6 +746,8 @@ def main() -> int:
                     help="served: rerun rate the predictor's margin is calibrated for")
     ap.add_argument("--train-programs", type=int, default=600, help="served: predictor training runs")
     ap.add_argument("--queries", type=int, default=256, help="batched: queries per batch")
+    ap.add_argument("--programs", type=int, default=0,
+                    help="batched: distinct program structures (0: every query its own)")
     ap.add_argument("--batch-rows", type=int, default=8 << 20)
     ap.add_argument("--pmc-run", action="store_true",
                     help="counter runs: each resident query's kernel a few times, one at a time; "

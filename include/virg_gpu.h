@@ -75,10 +75,11 @@ void vg_query_free(vg_query *q);
  * VG_QUERY_INTERP, 2 light queries too. */
 int vg_jit_policy(int policy);
 /* What runs q: *fused = 1 for its specialised single-pass kernel (rows per
- * lane, tiles per chunk, workgroups, compile milliseconds), 0 for the
- * interpreter's count and write passes. */
+ * lane, tiles per chunk, workgroups, compile milliseconds; *mode bit 0: the
+ * variant for 16-byte aligned columns, bit 1: filter columns staged in
+ * LDS), 0 for the interpreter's count and write passes. */
 int vg_query_kernel(vg_query *q, int *fused, int *rows_per_lane, unsigned *tiles_per_chunk,
-	unsigned *grid, double *compile_ms);
+	unsigned *grid, double *compile_ms, int *mode);
 
 /* Launch the section once on the query's stream (asynchronous). */
 int vg_query_launch(vg_query *q);

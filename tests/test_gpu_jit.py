@@ -177,3 +177,24 @@ def test_random_programs(gpu_db, jit, tablet_rows):
         assert ill == 0
         got, _ = _run(gpu_db, prog)
         assert_same(got, want, f"program {i}: {prog}")
+
+
+@pytest.mark.parametrize("stage", ["1", "0"], ids=["staged", "registers"])
+def test_mask_logic_regression(gpu_db, jit, monkeypatch, stage):
+    """Filters that drop the rows failing two OR-ed compares, with the
+    filter columns staged in LDS or loaded into registers: compiled with
+    v_bitop3 the mask logic once got a wrong truth table (0 rows instead
+    of 509 for P187; 2210 instead of 3244 for the seeded program)."""
+    from tests.test_kernel_cpu import P187
+    monkeypatch.setenv("VG_JIT_STAGE", stage)
+    gpu_db.gen_table("test", 4096, 0, seed=11)
+    rng = np.random.default_rng(177)
+    for i in range(18):
+        sel = float(rng.uniform(0.0001, 0.8)) if rng.random() < 0.7 else None
+        prog = W.random_program(rng, max_ops=40, selectivity=sel)
+    for prog in (P187, prog):
+        want, ill = oracle.select(gpu_db, prog)
+        assert ill == 0
+        got, k = _run(gpu_db, prog)
+        assert k["staged"] == (stage == "1")
+        assert_same(got, want, f"{prog}")

@@ -233,7 +233,7 @@ def _load() -> C.CDLL:
                                         C.c_int, C.c_ulonglong, C.POINTER(P), C.c_ulonglong, P, P, C.c_char_p,
                                         C.c_size_t]),
         "vg_query_kernel": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_uint),
-                                      C.POINTER(C.c_uint), C.POINTER(C.c_double)]),
+                                      C.POINTER(C.c_uint), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "vg_jit_source": (C.c_long, [C.POINTER(Op), C.c_uint, C.c_uint, C.POINTER(TabletMeta), C.c_char_p,
                                      C.c_size_t, C.c_char_p, C.c_size_t]),
         "vg_jit_compile_check": (C.c_int, [C.POINTER(Op), C.c_uint, C.c_uint, C.POINTER(TabletMeta),
@@ -653,10 +653,10 @@ class DeviceQuery:
     def kernel(self) -> dict:
         """What runs this query (vg_query_kernel): the specialised single-pass
         kernel (fused) or the interpreter's two passes."""
-        f, r, sc, g, ms = C.c_int(), C.c_int(), C.c_uint(), C.c_uint(), C.c_double()
-        lib.vg_query_kernel(self.h, C.byref(f), C.byref(r), C.byref(sc), C.byref(g), C.byref(ms))
+        f, r, sc, g, ms, m = C.c_int(), C.c_int(), C.c_uint(), C.c_uint(), C.c_double(), C.c_int()
+        lib.vg_query_kernel(self.h, C.byref(f), C.byref(r), C.byref(sc), C.byref(g), C.byref(ms), C.byref(m))
         return {"fused": bool(f.value), "rows_per_lane": r.value, "tiles_per_chunk": sc.value,
-                "grid": g.value, "compile_ms": ms.value}
+                "grid": g.value, "compile_ms": ms.value, "aligned": bool(m.value & 1), "staged": bool(m.value & 2)}
 
     def time_passes(self, warmup: int, iters: int) -> tuple[float, float, int]:
         """(count-pass ms, write-pass ms, rows) from HIP events on the query's stream."""

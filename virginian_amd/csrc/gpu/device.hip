@@ -1205,17 +1205,6 @@ static std::vector<vgx_tile> jit_tiles(const plan &p, uint32_t TR)
 	return t;
 }
 
-/* Immediates of a decoded program as the specialised kernel reads them. */
-static void jit_immediates(const vg_prog &P, vgx_launch_args &a)
-{
-	for (uint32_t k = 0; k < VG_MAX_UOPS; k++) {
-		a.imm[k][0] = k < P.nuop ? P.uop[k].ia : 0;
-		a.imm[k][1] = k < P.nuop ? P.uop[k].ib : 0;
-	}
-	for (uint32_t j = 0; j < VG_MAX_OUT; j++)
-		a.oimm[j] = j < P.nout ? P.out[j].imm : 0;
-}
-
 /* Give q its specialised kernel, tile table and launch state.  Returns 0,
  * or -1 (with a message) when the program has no specialised form; the
  * query then runs on the interpreter. */
@@ -1257,7 +1246,7 @@ static int jit_setup(vg_query *q, char *err, size_t errlen)
 	a.spin_limit = 1u << 20;
 	for (unsigned j = 0; j < q->P.nout; j++)
 		a.out[j] = q->out[j];
-	jit_immediates(q->P, a);
+	vgx_immediates(&q->P, &a);
 	q->jgrid = vgx_kernel_grid(k, nchunks, (unsigned)q->d->prop.multiProcessorCount);
 	q->jk = k;
 	return 0;
@@ -1580,9 +1569,11 @@ extern "C" int vg_query_info(vg_query *q, unsigned long long *rows, unsigned *ti
  * rows per lane, tiles per chunk, grid and compile time), 0 for the
  * interpreter's two passes. */
 extern "C" int vg_query_kernel(vg_query *q, int *fused, int *rows_per_lane, unsigned *tiles_per_chunk,
-	unsigned *grid, double *compile_ms)
+	unsigned *grid, double *compile_ms, int *mode)
 {
 	*fused = q->jk != nullptr;
+	/* bit 0: the aligned variant; bit 1: filter columns staged in LDS */
+	*mode = q->jk ? (vgx_kernel_shape(q->jk)->aligned ? 1 : 0) | (vgx_kernel_shape(q->jk)->staged ? 2 : 0) : 0;
 	*rows_per_lane = q->jk ? vgx_kernel_shape(q->jk)->R : 0;
 	*tiles_per_chunk = q->jk ? vgx_kernel_shape(q->jk)->S : 0;
 	*grid = q->jk ? q->jgrid : 0;
@@ -2049,7 +2040,7 @@ static int pbatch_build_jit(vg_pbatch *b, char *err, size_t errlen)
 		a.spin_limit = 1u << 20;
 		for (unsigned j = 0; j < q->P.nout; j++)
 			a.out[j] = q->out[j];
-		jit_immediates(q->P, a);
+		vgx_immediates(&q->P, &a);
 	}
 	if (herr(hipMemcpy(b->d_qa, qa.data(), n * sizeof(vgx_launch_args), hipMemcpyHostToDevice), "batch") ||
 		herr(hipMemcpy(b->d_fn, fn.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice), "batch") ||

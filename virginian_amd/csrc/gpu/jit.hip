@@ -436,18 +436,28 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 			fbytes += P->col[c].width == 1 ? 1u : P->col[c].width;
 	const uint32_t two_per_cu = 79u * 1024u - 4u * stage_wave;
 	const uint32_t budget = two_per_cu < 64u * 1024u ? two_per_cu : 64u * 1024u;
+	/* a chunk is 16K rows (one grab and one look-back each); staging its
+	 * filter columns in LDS saves reading them twice, when they fit at
+	 * that size; otherwise they are loaded into registers tile by tile */
+	const int R0 = R;
 	while (R > 4 && fbytes * 256u * (uint32_t)R > budget)
 		R /= 2;
-	const uint32_t TR = 256u * (uint32_t)R;
+	uint32_t TR = 256u * (uint32_t)R;
 	uint32_t ftile = fbytes * TR;
-	if (ftile > 96u * 1024u) {
-		snprintf(err, errlen, "filter columns need %u bytes of LDS per tile", ftile);
-		return -1;
-	}
 	uint32_t S = 16384u / TR;
 	if (ftile)
 		while (S > 1 && ftile * S > budget)
 			S--;
+	const char *mode = getenv("VG_JIT_STAGE");
+	int staged = S * TR >= 16384u && ftile <= budget;
+	if (mode && (mode[0] == '0' || mode[0] == '1'))
+		staged = mode[0] == '1' && ftile <= 96u * 1024u;
+	if (!staged) {
+		R = R0;
+		TR = 256u * (uint32_t)R;
+		ftile = 0;
+		S = 16384u / TR;
+	}
 	if (S_override >= 1 && S_override <= 16)
 		S = (uint32_t)S_override;
 	if (S * (uint32_t)R > 64u) {
@@ -459,6 +469,7 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 	sh->TR = TR;
 	sh->fbytes_row = fbytes;
 	sh->ftile_bytes = ftile;
+	sh->staged = staged;
 	sh->stage_off = ftile * S;
 	sh->stage_wave_bytes = stage_wave;
 	sh->lds_bytes = sh->stage_off + 4u * sh->stage_wave_bytes;
@@ -557,6 +568,7 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 		"\t\t\tconst bool fast = nrows == TR && ldc(&tb->aligned) != 0;\n";
 
 	/* phase 1: stage filter columns into LDS */
+	if (sh->staged) {
 	o("\t\t/* phase 1: the chunk's filter columns, HBM -> LDS */\n");
 	o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
 	o("\t\t\tu8 *const st = lds + s * %uu;\n", sh->ftile_bytes);
@@ -590,6 +602,7 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 		o("\t\t\t\t}\n\t\t\t}\n");
 	}
 	o("\t\t}\n\t\twait_vm();\n");
+	}
 
 	/* read filter column c of tile s from LDS into its slots (this lane's rows) */
 	auto lds_read = [&](out_buf &b, uint32_t c, const char *ind) {
@@ -613,47 +626,11 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 		b("%s\t}\n%s}\n", ind, ind);
 	};
 
-	/* phase 1 evaluation */
-	o("\t\tu64 vb = 0;\n");
-	o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
-	o("\t\t\t(void)fast;\n");
-	o("\t\t\tu32 live = RMASK;\n\t\t\tif (nrows < TR) {\n\t\t\t\tlive = 0;\n"
-	  "#pragma unroll\n\t\t\t\tfor (int q = 0; q < NQ; ++q)\n#pragma unroll\n\t\t\t\t\tfor (int j = 0; j < 4; ++j)\n"
-	  "\t\t\t\t\t\tlive |= (wave * (NQ * 256u) + q * 256u + lane * 4u + j < nrows) ? 1u << (4 * q + j) : 0u;\n\t\t\t}\n");
-	o("%s", decl.s.c_str());
-	for (uint32_t c = 0; c < P->ncol; c++)
-		if (is_filter[c])
-			lds_read(o, c, "\t\t\t");
-	o("\t\t\tu32 valid = live, active = live, w0 = 0, w1 = 0, w2 = 0, w3 = 0;\n");
-	gen_uops(o, P, P->nfilter, ~0ull);
-	o("\t\t\t(void)active; (void)w0; (void)w1; (void)w2; (void)w3;\n");
-	o("\t\t\tvb |= (u64)valid << (s * R);\n");
-	o("\t\t\tconst u32 cnt = wsum((u32)__builtin_popcount(valid));\n");
-	o("\t\t\tif (lane == 0)\n\t\t\t\ts_cnt[s][wave] = cnt;\n\t\t}\n");
-	o("\t\t__syncthreads();\n");
-	o("\t\tu32 n = 0;\n\t\tfor (u32 s = 0; s < nt; ++s)\n\t\t\tn += s_cnt[s][0] + s_cnt[s][1] + s_cnt[s][2] + s_cnt[s][3];\n");
-	o("\t\tn = uni(n);\n");
-	o("\t\tif (tid == 0)\n\t\t\t__hip_atomic_store(&A.status[c], (c == 0 ? ST_PRE : ST_AGG) | (epoch << ST_VBITS) | n,\n"
-	  "\t\t\t\t__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n");
-
-	/* phase 2 */
-	o("\t\t/* phase 2: result columns, tile by tile */\n");
-	o("\t\tu64 pre = 0;\n\t\tu32 tdone = 0;\n");
-	o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
-	/* the next chunk is taken only as this one's last tile starts: a chunk
-	 * taken is a chunk worked on, so later chunks never wait for one that
-	 * sits behind a whole chunk of its workgroup's work */
-	o("\t\t\tif (s + 1 == nt && tid == 0)\n\t\t\t\tgrab_next();\n");
-	o("\t\t\tconst u32 sel = (u32)(vb >> (s * R)) & RMASK;\n");
-	o("%s", decl.s.c_str());
-	/* HBM loads of project / emit columns, only for 4-row groups with a
-	 * selected row: buffer loads whose offset is out of range for the other
+	/* HBM loads of column c for the 4-row groups with a row in `sel`: buffer loads whose offset is out of range for the other
 	 * groups (they read nothing and return 0), so no lane branches and all
 	 * loads in flight together; tiles past a tablet's last row are cut off
 	 * by the resource's size */
-	for (uint32_t c = 0; c < P->ncol; c++) {
-		if (is_filter[c] || !need2[c])
-			continue;
+	auto col_load = [&](uint32_t c) {
 		const vg_col &cl = P->col[c];
 		const bool emit = cl.phase == VG_PHASE_EMIT;
 		char lo[32], hi[32];
@@ -693,10 +670,58 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 			  lo, lo);
 		}
 		o("\t\t\t\t}\n\t\t\t}\n");
+	};
+
+	/* phase 1 evaluation */
+	o("\t\tu64 vb = 0;\n");
+	o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
+	o("\t\t\t(void)fast;\n");
+	o("\t\t\tu32 live = RMASK;\n\t\t\tif (nrows < TR) {\n\t\t\t\tlive = 0;\n"
+	  "#pragma unroll\n\t\t\t\tfor (int q = 0; q < NQ; ++q)\n#pragma unroll\n\t\t\t\t\tfor (int j = 0; j < 4; ++j)\n"
+	  "\t\t\t\t\t\tlive |= (wave * (NQ * 256u) + q * 256u + lane * 4u + j < nrows) ? 1u << (4 * q + j) : 0u;\n\t\t\t}\n");
+	o("%s", decl.s.c_str());
+	if (sh->staged) {
+		for (uint32_t c = 0; c < P->ncol; c++)
+			if (is_filter[c])
+				lds_read(o, c, "\t\t\t");
+	} else {
+		/* filter columns straight into their slots, every live row */
+		o("\t\t\t{\n\t\t\tconst u32 sel = live;\n");
+		for (uint32_t c = 0; c < P->ncol; c++)
+			if (is_filter[c])
+				col_load(c);
+		o("\t\t\t}\n");
 	}
+	o("\t\t\tu32 valid = live, active = live, w0 = 0, w1 = 0, w2 = 0, w3 = 0;\n");
+	gen_uops(o, P, P->nfilter, ~0ull);
+	o("\t\t\t(void)active; (void)w0; (void)w1; (void)w2; (void)w3;\n");
+	o("\t\t\tvb |= (u64)valid << (s * R);\n");
+	o("\t\t\tconst u32 cnt = wsum((u32)__builtin_popcount(valid));\n");
+	o("\t\t\tif (lane == 0)\n\t\t\t\ts_cnt[s][wave] = cnt;\n\t\t}\n");
+	o("\t\t__syncthreads();\n");
+	o("\t\tu32 n = 0;\n\t\tfor (u32 s = 0; s < nt; ++s)\n\t\t\tn += s_cnt[s][0] + s_cnt[s][1] + s_cnt[s][2] + s_cnt[s][3];\n");
+	o("\t\tn = uni(n);\n");
+	o("\t\tif (tid == 0)\n\t\t\t__hip_atomic_store(&A.status[c], (c == 0 ? ST_PRE : ST_AGG) | (epoch << ST_VBITS) | n,\n"
+	  "\t\t\t\t__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n");
+
+	/* phase 2 */
+	o("\t\t/* phase 2: result columns, tile by tile */\n");
+	o("\t\tu64 pre = 0;\n\t\tu32 tdone = 0;\n");
+	o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
+	/* the next chunk is taken only as this one's last tile starts: a chunk
+	 * taken is a chunk worked on, so later chunks never wait for one that
+	 * sits behind a whole chunk of its workgroup's work */
+	o("\t\t\tif (s + 1 == nt && tid == 0)\n\t\t\t\tgrab_next();\n");
+	o("\t\t\tconst u32 sel = (u32)(vb >> (s * R)) & RMASK;\n");
+	o("%s", decl.s.c_str());
+	/* result-phase columns: project / emit columns, and filter columns
+	 * that were not staged */
+	for (uint32_t c = 0; c < P->ncol; c++)
+		if (need2[c] && !(is_filter[c] && sh->staged))
+			col_load(c);
 	/* filter columns again, from LDS */
 	for (uint32_t c = 0; c < P->ncol; c++)
-		if (is_filter[c] && need2[c])
+		if (is_filter[c] && need2[c] && sh->staged)
 			lds_read(o, c, "\t\t\t");
 	/* the chunk's output offset: wave 0 looks back while the loads fly */
 	o("\t\t\tif (s == 0) {\n\t\t\t\tif (wave == 0) {\n\t\t\t\t\tbool gave = false;\n"
@@ -878,9 +903,13 @@ static vgx_kernel *compile(const std::string &src, const vgx_shape &sh, char *er
 	}
 	std::string archopt = std::string("--offload-arch=") + arch_name();
 	/* the options the library's own kernels are built with: IEEE f32
-	 * arithmetic operation by operation (no contraction), no packed fp32 */
+	 * arithmetic operation by operation (no contraction), no packed fp32
+	 * and no v_bitop3 (this compiler has folded three-input mask logic
+	 * into v_bitop3 with a wrong truth table: a filter that drops the
+	 * rows failing two OR-ed compares dropped those passing the first) */
 	const char *opts[] = {archopt.c_str(), "-O3", "-std=c++17", "-ffp-contract=off",
-		"-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+		"-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+		"-Xclang", "-target-feature", "-Xclang", "-bitop3-insts"};
 	hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
 	if (rc != HIPRTC_SUCCESS) {
 		size_t ls = 0;
@@ -1190,6 +1219,49 @@ extern "C" long vg_jit_source(const virg_op *ops, unsigned num_ops, unsigned pc,
 	return (long)len;
 }
 
+/* Immediates of a decoded program as the specialised kernel reads them. */
+extern "C" void vgx_immediates(const vg_prog *P, vgx_launch_args *a)
+{
+	for (uint32_t k = 0; k < VG_MAX_UOPS; k++) {
+		a->imm[k][0] = k < P->nuop ? P->uop[k].ia : 0;
+		a->imm[k][1] = k < P->nuop ? P->uop[k].ib : 0;
+	}
+	for (uint32_t j = 0; j < VG_MAX_OUT; j++)
+		a->oimm[j] = j < P->nout ? P->out[j].imm : 0;
+}
+
+/* Everything a launch of a program's specialised kernel takes besides its
+ * tablets (no device needed): the kernel's shape for tablets like `tab`
+ * (aligned: the 16-byte aligned variant), the immediates (a->imm, a->oimm),
+ * the result columns' widths (out_width[VG_MAX_OUT]) and the source (into
+ * buf, n bytes).  Returns the source length, or -1 with a message. */
+extern "C" long vg_jit_program(const virg_op *ops, unsigned num_ops, unsigned pc, const virg_tablet_meta *tab,
+	int aligned, vgx_shape *sh, vgx_launch_args *a, unsigned *nout, unsigned *out_width, char *buf, size_t n,
+	char *err, size_t errlen)
+{
+	vg_prog P;
+	if (vg_decode(ops, num_ops, pc, tab, nullptr, &P, err, errlen) != VIRG_SUCCESS)
+		return -1;
+	if (vgx_shape_of(&P, aligned, env_int("VG_JIT_R", 0), env_int("VG_JIT_S", 0), sh, err, errlen) != 0)
+		return -1;
+	memset(a, 0, sizeof *a);
+	vgx_immediates(&P, a);
+	*nout = P.nout;
+	for (unsigned j = 0; j < VG_MAX_OUT; j++)
+		out_width[j] = j < P.nout ? P.out[j].width : 0;
+	char *src = nullptr;
+	if (vgx_source(&P, sh, &src) != 0)
+		return -1;
+	const size_t len = strlen(src);
+	if (buf && n) {
+		const size_t m = len < n - 1 ? len : n - 1;
+		memcpy(buf, src, m);
+		buf[m] = '\0';
+	}
+	free(src);
+	return (long)len;
+}
+
 /* hiprtc compile of src for gfx950, not loaded: 1, or 0 with the log */
 static int rtc_check(const char *src, double *ms, char *err, size_t errlen)
 {
@@ -1199,7 +1271,8 @@ static int rtc_check(const char *src, double *ms, char *err, size_t errlen)
 	int ok = 0;
 	if (hiprtcCreateProgram(&prog, src, "vgx.hip", 0, nullptr, nullptr) == HIPRTC_SUCCESS) {
 		const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
-			"-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+			"-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+			"-Xclang", "-target-feature", "-Xclang", "-bitop3-insts"};
 		const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
 		if (rc == HIPRTC_SUCCESS) {
 			ok = 1;

@@ -14,13 +14,18 @@ typedef struct {
 	int R;                     /* rows per lane per tile (4, 8 or 16) */
 	uint32_t S;                /* tiles per chunk (one grab, one look-back) */
 	uint32_t TR;               /* rows per tile = 256 * R */
-	uint32_t fbytes_row;       /* bytes per row of the filter columns staged in LDS */
+	uint32_t fbytes_row;       /* bytes per row of the filter columns */
 	uint32_t ftile_bytes;      /* LDS bytes of one tile's staged filter columns */
 	uint32_t stage_off;        /* LDS offset of the output staging area */
 	uint32_t stage_wave_bytes; /* output staging bytes per wave */
 	uint32_t lds_bytes;        /* dynamic LDS per workgroup */
 	int aligned;               /* every tablet's column bases are 16-byte aligned:
 	                            * result columns load 16 bytes per lane */
+	int staged;                /* 1: a chunk's filter columns are staged in LDS in
+	                            * the filter phase and read again from there;
+	                            * 0: loaded into registers tile by tile, and those
+	                            * the result phase needs loaded again like the
+	                            * result columns (for the rows selected) */
 } vgx_shape;
 
 typedef struct vgx_kernel vgx_kernel;
@@ -28,6 +33,8 @@ typedef struct vgx_kernel vgx_kernel;
 /* shape for a program (R_override, S_override: 0 = chosen from the program) */
 int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S_override, vgx_shape *sh, char *err,
 	size_t errlen);
+/* the immediates of P as its kernel reads them (a->imm, a->oimm) */
+void vgx_immediates(const vg_prog *P, vgx_launch_args *a);
 /* kernel source (malloc'd) */
 int vgx_source(const vg_prog *P, const vgx_shape *sh, char **src);
 /* compiled kernel for P (from the cache, or compiled now); NULL + message */
