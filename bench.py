@@ -624,7 +624,7 @@ def summarize_counters(outdir: str, dest: str) -> int:
            "commands": [
                "cd /tmp && export TMPDIR=/tmp",
                "rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_SMEM SQ_INSTS_LDS SQ_WAVES SQ_WAVE_CYCLES "
-               "SQ_BUSY_CYCLES --output-format csv -d <out>/sq -o batched -- python3 -u <repo>/bench.py "
+               "SQ_BUSY_CYCLES SQ_WAIT_ANY --output-format csv -d <out>/sq -o batched -- python3 -u <repo>/bench.py "
                "--workload batched --pmc-run",
                "rocprofv3 --pmc FETCH_SIZE --output-format csv -d <out>/fetch -o batched -- python3 -u "
                "<repo>/bench.py --workload batched --pmc-run",
