@@ -226,6 +226,23 @@ int vg_host_memory(virginian *v, unsigned long long *pinned, unsigned long long 
  * copies into a pinned staging buffer. */
 int vg_host_pin_limit(virginian *v, unsigned long long bytes);
 
+/* Slot pager.  With a database file open, at most `tablets` tablets stay
+ * in host memory (default VIRG_MEM_TABLETS = 64, as the reference's slot
+ * table; 0 = no limit): loading or creating one more writes an unlocked
+ * tablet to the file, chosen round-robin, and reuses its slot; a tablet not
+ * in memory is read back from the file when it is next loaded (reference:
+ * src/db/findslot.c:36-57, src/db/load.c:44-93).  Tablets larger than a
+ * disk slot (VIRG_TABLET_SIZE) are never written out.  Pointers from
+ * virg_table_tablets stay valid only while their tablets stay in memory;
+ * the device paths copy what they need while a tablet is locked.
+ * Returns the previous limit. */
+unsigned vg_tablet_limit(virginian *v, unsigned tablets);
+/* Tablets in host memory, tablets written out to make room and tablets
+ * read back from the file since virg_init; *resident_id (if not NULL) = 1
+ * when tablet `id` is in host memory. */
+int vg_pager_stats(virginian *v, unsigned *resident, unsigned long long *evictions,
+	unsigned long long *file_loads, unsigned id, int *resident_id);
+
 /* Pinned host memory (results copy into it at link speed). */
 void *vg_host_alloc(size_t bytes);
 void vg_host_free(void *p);

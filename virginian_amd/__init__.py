@@ -153,6 +153,9 @@ def _load() -> C.CDLL:
         "virg_db_create": (C.c_int, [V, C.c_char_p]), "virg_db_open": (C.c_int, [V, C.c_char_p]),
         "virg_db_close": (C.c_int, [V]), "virg_db_clear": (C.c_int, [V, C.c_uint]),
         "virg_db_write": (C.c_int, [V, C.c_uint]),
+        "virg_db_load": (C.c_int, [V, C.c_uint, C.POINTER(C.POINTER(TabletMeta))]),
+        "virg_tablet_lock": (C.c_int, [V, C.c_uint]),
+        "virg_tablet_unlock": (C.c_int, [V, C.c_uint]),
         "virg_gpu_getsizes": (C.POINTER(C.c_size_t), []),
         "virg_table_create": (C.c_int, [V, C.c_char_p, C.c_int]),
         "virg_table_addcolumn": (C.c_int, [V, C.c_uint, C.c_char_p, C.c_int]),
@@ -205,6 +208,9 @@ def _load() -> C.CDLL:
         "vg_batch_time": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "vg_host_memory": (C.c_int, [V, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
         "vg_host_pin_limit": (C.c_int, [V, C.c_ulonglong]),
+        "vg_tablet_limit": (C.c_uint, [V, C.c_uint]),
+        "vg_pager_stats": (C.c_int, [V, C.POINTER(C.c_uint), C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong),
+                                     C.c_uint, C.POINTER(C.c_int)]),
         "vg_pbatch_create": (P, [C.POINTER(P), C.c_uint, C.POINTER(PBatchConfig), C.c_char_p, C.c_size_t]),
         "vg_pbatch_free": (None, [P]),
         "vg_pbatch_configure": (C.c_int, [P, C.POINTER(PBatchConfig), C.c_char_p, C.c_size_t]),
@@ -305,6 +311,17 @@ class Virginian:
         """Open a database file, before any table is in memory (virg_db_open)."""
         if lib.virg_db_open(self.ptr, str(path).encode()) != VIRG_SUCCESS:
             raise VirginianError(f"virg_db_open failed for {path}")
+
+    def tablet_limit(self, tablets: int) -> int:
+        """Tablets kept in host memory while a database file is open
+        (vg_tablet_limit; 0 = no limit); returns the previous limit."""
+        return lib.vg_tablet_limit(self.ptr, tablets)
+
+    def pager_stats(self, tablet_id: int = 0) -> dict:
+        """Resident tablets, evictions and reloads (vg_pager_stats)."""
+        r, e, f, ri = C.c_uint(), C.c_ulonglong(), C.c_ulonglong(), C.c_int()
+        lib.vg_pager_stats(self.ptr, C.byref(r), C.byref(e), C.byref(f), tablet_id, C.byref(ri))
+        return {"resident": r.value, "evictions": e.value, "file_loads": f.value, "tablet_resident": bool(ri.value)}
 
     def close_db(self) -> None:
         """Write every tablet in memory and the meta information to the file

@@ -284,6 +284,26 @@ static void *resident_tablet(vgd_device *d, const virg_tablet_meta *t, hipStream
 	return r.dptr;
 }
 
+/* Extra locks taken on tablets a walk has moved past, released when the
+ * walk's user is done (the slot pager leaves locked tablets in place). */
+struct held_locks {
+	virginian *v;
+	std::vector<unsigned> ids;
+	explicit held_locks(virginian *v_) : v(v_) {}
+	int take(unsigned id)
+	{
+		if (virg_tablet_lock(v, id) == VIRG_FAIL)
+			return VIRG_FAIL;
+		ids.push_back(id);
+		return VIRG_SUCCESS;
+	}
+	~held_locks()
+	{
+		for (unsigned id : ids)
+			virg_tablet_unlock(v, id);
+	}
+};
+
 static int ensure_lr(launch_res &lr, uint32_t num_tiles, uint32_t grid, hipStream_t stream)
 {
 	if (!lr.ctrl) {
@@ -815,12 +835,25 @@ extern "C" int vg_stream_select(virginian *v, const virg_op *ops, unsigned num_o
 		virg_tablet_unlock(v, t->id);
 		return VIRG_FAIL;
 	}
+	/* the tablets' host memory is read while streaming: with a database
+	 * file open each keeps a lock until then (the slot pager) */
 	std::vector<const virg_tablet_meta *> tabs;
+	held_locks held(v);
 	while (1) {
 		tabs.push_back(t);
+		if (v->dbfd >= 0 && held.take(t->id) == VIRG_FAIL) {
+			virg_tablet_unlock(v, t->id);
+			snprintf(err, errlen, "could not lock a tablet");
+			return VIRG_FAIL;
+		}
 		if (t->last_tablet)
 			break;
-		virg_db_loadnext(v, &t);
+		if (virg_db_loadnext(v, &t) == VIRG_FAIL) {
+			virg_tablet_unlock(v, t->id);
+			snprintf(err, errlen, "could not load the next tablet: every tablet in host memory is "
+				"locked (a streamed table keeps each locked; raise vg_tablet_limit)");
+			return VIRG_FAIL;
+		}
 	}
 	virg_tablet_unlock(v, t->id);
 	array_sink sink(out, capacity);
@@ -918,11 +951,27 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 	VIRG_CHECK(vg_kernel_check(&P) != 0, "op-program needs more register slots than the LDS holds");
 
 	/* collect the tablets: tab[0] arrives locked; like the reference the
-	 * walk unlocks each tablet as it moves on and leaves the last locked */
+	 * walk unlocks each tablet as it moves on and leaves the last locked.
+	 * With a database file open an unlocked tablet may be written out and
+	 * its slot reused (the slot pager), so the resident path makes each
+	 * tablet resident in HBM while it is locked and keeps a copy of its
+	 * meta, and the streaming path, which reads host memory later, keeps a
+	 * lock on every tablet until it is done. */
+	const bool paged = v->dbfd >= 0;
 	std::vector<const virg_tablet_meta *> tabs;
+	std::vector<virg_tablet_meta> metas;
+	held_locks held(v);
 	unsigned proced = 0;
 	while (1) {
-		tabs.push_back(tab[0]);
+		if (paged && !v->use_stream) {
+			VIRG_CHECK(!resident_tablet(d, tab[0], d->stream) || hipStreamSynchronize(d->stream) != hipSuccess,
+				"could not make a tablet resident");
+			metas.push_back(*tab[0]);
+		} else {
+			tabs.push_back(tab[0]);
+			if (paged)
+				VIRG_CHECK(held.take(tab[0]->id) == VIRG_FAIL, "could not lock a tablet");
+		}
 		proced++;
 		if (tab[0]->last_tablet || (num_tablets != 0 && proced >= num_tablets))
 			break;
@@ -930,6 +979,8 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 		if (tab[0]->rows == 0)
 			break;
 	}
+	for (const virg_tablet_meta &m : metas)
+		tabs.push_back(&m);
 
 	if (v->use_stream) {
 		/* the reference's streaming mode: the tablets stay in host memory
@@ -1113,6 +1164,7 @@ struct vg_query {
 	vgd_device *d = nullptr;
 	vg_prog P;
 	std::vector<const virg_tablet_meta *> tabs;
+	std::vector<virg_tablet_meta> tabmeta;   /* copies of the metas, when the tablets may be paged out */
 	plan own;
 	plan *p = &own;          /* own, or the device's shared plan of the table */
 	bool own_stream = true;  /* else the device's stream */
@@ -1355,13 +1407,34 @@ extern "C" vg_query *vg_query_create(virginian *v, const virg_op *ops, unsigned 
 		delete q;
 		return nullptr;
 	}
+	/* with a database file open an unlocked tablet may be written out and
+	 * its slot reused (the slot pager): each tablet is made resident in HBM
+	 * while it is locked and the query keeps a copy of its meta */
+	const bool paged = v->dbfd >= 0;
 	while (1) {
-		q->tabs.push_back(t);
+		if (paged) {
+			if (!resident_tablet(d, t, d->stream) || hipStreamSynchronize(d->stream) != hipSuccess) {
+				virg_tablet_unlock(v, t->id);
+				qerr(err, errlen, "cannot make the table resident");
+				delete q;
+				return nullptr;
+			}
+			q->tabmeta.push_back(*t);
+		} else {
+			q->tabs.push_back(t);
+		}
 		if (t->last_tablet)
 			break;
-		virg_db_loadnext(v, &t);
+		if (virg_db_loadnext(v, &t) == VIRG_FAIL) {
+			virg_tablet_unlock(v, t->id);
+			qerr(err, errlen, "cannot load the table's next tablet");
+			delete q;
+			return nullptr;
+		}
 	}
 	virg_tablet_unlock(v, t->id);
+	for (const virg_tablet_meta &m : q->tabmeta)
+		q->tabs.push_back(&m);
 	if (vg_kernel_check(&q->P) != 0) {
 		qerr(err, errlen, "op-program needs more register slots than the LDS holds");
 		delete q;

@@ -5,9 +5,12 @@
  * 1 when it holds an unlocked tablet and 1+k with k locks
  * (reference: src/db/findslot.c:19-63, src/tablet/lock.c:20-98,
  * src/db/load.c:22-98, src/db/loadnext.c:21-33, src/db/alloc.c:18-42).
- * Differences: tablets never leave memory unless a database file takes
- * them (no eviction; see dbfile.c), slot buffers are sized per tablet, and
- * slots beyond VIRG_MEM_TABLETS come from an extension table.
+ * Differences: eviction happens only with a database file open (without
+ * one a tablet has nowhere to go), and then once more than resident_limit
+ * tablets (VIRG_MEM_TABLETS unless set with vg_tablet_limit) are in host
+ * memory, not when a fixed slot table is full; tablets larger than a disk
+ * slot are never chosen; slot buffers are sized per tablet, and slots beyond
+ * VIRG_MEM_TABLETS come from an extension table.
  */
 #include "internal.h"
 #include <stdlib.h>
@@ -106,8 +109,36 @@ static int ext_grow(virginian *v)
 	return VIRG_SUCCESS;
 }
 
+/* Make room: the first unlocked tablet from the round-robin position on
+ * (reference: src/db/findslot.c:36-57) is written to the database file and
+ * its slot freed.  Caller holds v->slot_lock. */
+static int evict_one(virginian *v)
+{
+	virg_state *s = virg_state_of(v);
+	const unsigned n = virg_nslots(v);
+	for (unsigned k = 0; k < n; k++) {
+		const unsigned slot = (v->tablet_slot_counter + k) % n;
+		if (*virg_slot_status(v, slot) != 1)
+			continue;
+		const virg_tablet_meta *t = *virg_slot_ptr(v, slot);
+		if (t->size > VIRG_TABLET_SIZE)
+			continue;               /* no disk slot holds it */
+		v->tablet_slot_counter = (slot + 1) % n;
+		VIRG_CHECK(virg_db_write(v, slot) == VIRG_FAIL, "Could not write a tablet out to make room");
+		*virg_slot_status(v, slot) = 0;
+		v->tablet_slots_taken--;
+		s->evictions++;
+		return VIRG_SUCCESS;
+	}
+	VIRG_CHECK(1, "All tablets locked");
+	return VIRG_FAIL;
+}
+
 int virg_findslot_sized(virginian *v, unsigned *slot_, size_t bytes)
 {
+	virg_state *s = virg_state_of(v);
+	if (v->dbfd >= 0 && s->resident_limit && v->tablet_slots_taken >= s->resident_limit)
+		VIRG_CHECK(evict_one(v) == VIRG_FAIL, "No tablet slot");
 	unsigned n = virg_nslots(v);
 	unsigned slot;
 	/* a free slot whose buffer is big enough, else the first free one */
@@ -270,6 +301,7 @@ int virg_init(virginian *v)
 	virg_state *s = (virg_state *)calloc(1, sizeof(virg_state));
 	VIRG_CHECK(s == NULL, "Out of memory");
 	s->magic = VIRG_STATE_MAGIC;
+	s->resident_limit = VIRG_MEM_TABLETS;
 	v->gpu_slots = s;
 	virg_db_create(v, NULL);
 	s->pinned = vgd_open(v);
@@ -320,5 +352,41 @@ int vg_host_pin_limit(virginian *v, unsigned long long bytes)
 	virg_state *s = virg_state_of(v);
 	VIRG_CHECK(s == NULL, "database not open");
 	s->pin_limit = bytes;
+	return VIRG_SUCCESS;
+}
+
+/* At most `tablets` tablets in host memory while a database file is open
+ * (0: no limit); beyond it unlocked tablets are written to the file and
+ * read back when next loaded.  Returns the previous limit. */
+unsigned vg_tablet_limit(virginian *v, unsigned tablets)
+{
+	virg_state *s = virg_state_of(v);
+	if (s == NULL)
+		return 0;
+	pthread_mutex_lock(&v->slot_lock);
+	const unsigned old = s->resident_limit;
+	s->resident_limit = tablets;
+	pthread_mutex_unlock(&v->slot_lock);
+	return old;
+}
+
+/* Tablets in host memory, tablets written out to make room and tablets
+ * read back from the database file since virg_init; *resident_id (when not
+ * NULL) = 1 if tablet `id` is in host memory. */
+int vg_pager_stats(virginian *v, unsigned *resident, unsigned long long *evictions,
+	unsigned long long *file_loads, unsigned id, int *resident_id)
+{
+	virg_state *s = virg_state_of(v);
+	VIRG_CHECK(s == NULL, "database not open");
+	pthread_mutex_lock(&v->slot_lock);
+	if (resident)
+		*resident = v->tablet_slots_taken;
+	if (evictions)
+		*evictions = s->evictions;
+	if (file_loads)
+		*file_loads = s->file_loads;
+	if (resident_id)
+		*resident_id = virg_slot_of(v, id) >= 0;
+	pthread_mutex_unlock(&v->slot_lock);
 	return VIRG_SUCCESS;
 }

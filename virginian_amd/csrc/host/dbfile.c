@@ -17,9 +17,11 @@
  * other slot's index drops by one and block_size grows by one slot, which
  * leaves each tablet where it was on disk.
  *
- * Differences from the reference: tablets stay in memory once created or
- * loaded (nothing is evicted to make room), and are written when their slot
- * is cleared or the database closes, whatever their lock count; a tablet
+ * Differences from the reference: once more tablets than the resident
+ * limit are in memory, the slot pager (db.c) writes an unlocked one out to
+ * make room, as the reference does when its 64 slots are full; tablets are
+ * also written when their slot is cleared or the database closes,
+ * whatever their lock count; a tablet
  * larger than a disk slot (this library's tablets may hold more rows than
  * the reference's 8 MB) cannot be stored in the format and is refused with
  * a message; a database file cannot be opened once tablets are in memory.
@@ -300,6 +302,7 @@ int virg_dbfile_load(virginian *v, unsigned tablet_id, virg_tablet_meta **tab)
 	}
 	t->info = &db->tablet_info[i];
 	*virg_slot_id(v, slot) = tablet_id;
+	virg_state_of(v)->file_loads++;
 	if (tab != NULL)
 		tab[0] = t;
 	return VIRG_SUCCESS;

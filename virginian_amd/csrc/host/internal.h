@@ -41,6 +41,12 @@ typedef struct {
 	unsigned long long pinned_bytes;    /* slot memory allocated pinned ... */
 	unsigned long long pageable_bytes;  /* ... and pageable (pinning failed) */
 	unsigned long long pin_limit;       /* pin no slot memory beyond this (0: no limit) */
+	/* slot pager: with a database file open, at most resident_limit tablets
+	 * stay in host memory (0: no limit); beyond it an unlocked one is
+	 * written to the file and its slot reused */
+	unsigned       resident_limit;
+	unsigned long long evictions;       /* tablets written out to make room */
+	unsigned long long file_loads;      /* tablets read back from the file */
 } virg_state;
 
 #define VIRG_STATE_MAGIC 0x56474d49u
