@@ -454,10 +454,14 @@ def bench_served(a) -> int:
         return out
 
     t0 = time.perf_counter()
-    x, t = PR.collect(db, programs(a.train_programs), stats)
+    # a fifth of the training programs validate (early stopping)
+    nval = max(1, a.train_programs // 5)
+    x, t = PR.collect(db, programs(a.train_programs - nval), stats)
+    xv, tv = PR.collect(db, programs(nval), stats)
     t1 = time.perf_counter()
     net = PR.MLP(seed=a.seed)
-    loss = net.train(x, t)
+    fit = net.fit(x, t, xv, tv)
+    loss = fit["validation_mae"]
     t2 = time.perf_counter()
     held = programs(200)
     hx, ht = PR.collect(db, held, stats)
@@ -474,8 +478,9 @@ def bench_served(a) -> int:
     mae = float(np.mean(np.abs(pred - ht)))
     mae_est = float(np.mean(np.abs(hx[:, 3] - ht)))
     mae_const = float(np.mean(np.abs(float(np.mean(t)) - ht)))
-    print(f"# predictor: {len(x)} examples collected in {t1 - t0:.1f}s, trained in {t2 - t1:.2f}s "
-          f"(loss {loss:.5f}); held-out MAE {mae:.4f} (independence estimate {mae_est:.4f}, "
+    print(f"# predictor: {len(x)} + {len(xv)} examples collected in {t1 - t0:.1f}s, trained in {t2 - t1:.2f}s "
+          f"({fit['best_epoch']} of {fit['epochs']} epochs, validation MAE {loss:.4f} from "
+          f"{fit['validation_mae_start']:.4f}); held-out MAE {mae:.4f} (independence estimate {mae_est:.4f}, "
           f"mean selectivity {mae_const:.4f}); margin {margin:.3f}", file=sys.stderr, flush=True)
 
     n_req = int(a.rate * a.duration)

@@ -263,14 +263,19 @@ int vg_link_probe(virginian *v, size_t bytes, int reps, double *h2d_gbs, double 
  *          branches / 4, drop branches / 4, Invalid ops, logic ops,
  *          program length / 40;
  *   [11]   tablet rows / 2^20;
- *   [12..] the tablet's statistics as given (tabfeat).
+ *   [12..15] logit(e) / VG_FEAT_SKIP with the compares the histograms
+ *          cannot estimate passing by kind: ordered (<, <=, >, >=) at 1/2,
+ *          = at 0.05, != at 0.95; ordered at 1/4, the others at 1/2;
+ *          ordered at 3/4, the others at 1/2; ordered at 1/2, = at 0,
+ *          != at 1;
+ *   [16..] the tablet's statistics as given (tabfeat).
  * Statistics per table: edges[c][0..BINS] of each fixed column's range,
  * per tablet hist[t][c][bin] (fractions), is_int[c], and tabfeat[t][...]
  * (VG_FEAT_COLS * (VG_FEAT_BINS + 2) values). */
 #define VG_FEAT_COLS 6
 #define VG_FEAT_BINS 16
 #define VG_FEAT_SKIP 4.0
-#define VG_FEATURES (12 + VG_FEAT_COLS * (VG_FEAT_BINS + 2))
+#define VG_FEATURES (16 + VG_FEAT_COLS * (VG_FEAT_BINS + 2))
 typedef struct {
 	unsigned ntab;
 	const double *rows;
@@ -301,6 +306,9 @@ int vg_mlp_forward(vg_mlp *m, const float *x, unsigned n, float *y);
  * change: no device call, for callers that must not wait for the device. */
 int vg_mlp_forward_host(const vg_mlp *m, const float *x, unsigned n, float *y);
 int vg_mlp_gradient(vg_mlp *m, const float *x, const float *t, unsigned n, float *loss, float *grad);
+/* The loss vg_mlp_gradient and vg_mlp_train use: 0 mean squared error
+ * (the default), 1 mean absolute error. */
+int vg_mlp_set_loss(vg_mlp *m, int kind);
 int vg_mlp_train(vg_mlp *m, const float *x, const float *t, unsigned n, unsigned epochs, unsigned batch,
 	float lr, unsigned long long seed, float *loss);
 

@@ -10,7 +10,8 @@ the row otherwise; a row that does not take it falls through; Invalid
 drops the rows reaching it; jumps are forward only.  Each `column op
 constant` compare passes at the column histogram's estimate on the tablet
 (linear inside a bin; integer columns by their integer steps), any other
-compare at u, for u = 1/2, 1/4, 3/4.
+compare at a rate given by its kind (ordered, =, !=), in seven variants
+(VARIANTS).
 
 Pinned by: closed forms on uniform columns (uniformi < 50 keeps half of
 each tablet, uniformf > 90 a tenth), a two-compare AND whose estimate is
@@ -29,6 +30,9 @@ CMP_OPS = [va.OP_Lt, va.OP_Le, va.OP_Gt, va.OP_Ge, va.OP_Eq, va.OP_Neq]
 MIRROR = {va.OP_Lt: va.OP_Gt, va.OP_Le: va.OP_Ge, va.OP_Gt: va.OP_Lt, va.OP_Ge: va.OP_Le,
           va.OP_Eq: va.OP_Eq, va.OP_Neq: va.OP_Neq}
 NCOLS, NBINS, SKIP = 6, 16, 4.0
+# passing rate of an unknown (ordered or logic, =, !=) compare, per variant
+VARIANTS = [(0.5, 0.5, 0.5), (0.25, 0.25, 0.25), (0.75, 0.75, 0.75), (0.5, 0.05, 0.95), (0.25, 0.5, 0.5),
+            (0.75, 0.5, 0.5), (0.5, 0.0, 1.0)]
 
 
 def cdf(st, c: int, v: float) -> np.ndarray:
@@ -58,11 +62,13 @@ def _value(op) -> float:
     return float(np.float32(op[4])) if op[0] == va.OP_Float else float(op[2])
 
 
-def walk(ops: list[tuple], st, unknown: np.ndarray):
-    """Per value of `unknown`, the fraction of each tablet's rows reaching
-    Result valid; and the program's shape counts."""
+def walk(ops: list[tuple], st, unknown):
+    """Per variant of `unknown` (rows of (ordered, =, !=) passing rates),
+    the fraction of each tablet's rows reaching Result valid; and the
+    program's shape counts."""
     ntab = len(st.rows)
-    u = np.asarray(unknown, dtype=np.float64)[:, None] * np.ones((1, ntab))
+    rates = np.asarray(unknown, dtype=np.float64)
+    u = rates[:, :1] * np.ones((1, ntab))
     start = next((i + 1 for i, o in enumerate(ops) if o[0] == va.OP_Parallel), 0)
     res = next((i for i, o in enumerate(ops) if o[0] == va.OP_Result and i >= start), len(ops))
     arrive = {start: np.ones_like(u)}
@@ -81,7 +87,8 @@ def walk(ops: list[tuple], st, unknown: np.ndarray):
             ninvalid += 1
             nxt = None
         elif k in CMP_OPS or k in (va.OP_And, va.OP_Or, va.OP_Not):
-            p = u
+            kind = 1 if k == va.OP_Eq else 2 if k == va.OP_Neq else 0
+            p = rates[:, kind:kind + 1] * np.ones((1, ntab))
             if k in CMP_OPS:
                 a, b = regdef.get(o[1], ("expr",)), regdef.get(o[2], ("expr",))
                 op = k
@@ -119,9 +126,10 @@ def _logit(p: np.ndarray) -> np.ndarray:
 def program_features(ops: list[tuple], st) -> np.ndarray:
     """Feature rows (one per tablet), the layout vg_program_features writes."""
     ntab = len(st.rows)
-    ests, (known, unknown, keep, drop, ninvalid, nlogic) = walk(ops, st, np.array([0.5, 0.25, 0.75]))
-    est, lo, hi = ests
+    ests, (known, unknown, keep, drop, ninvalid, nlogic) = walk(ops, st, VARIANTS)
+    est, lo, hi = ests[0], ests[1], ests[2]
     shape = np.array([known / 4, unknown / 4, keep / 4, drop / 4, ninvalid, nlogic, len(ops) / 40.0])
     return np.concatenate([_logit(est)[:, None], _logit(lo)[:, None], _logit(hi)[:, None], est[:, None],
                            np.tile(shape, (ntab, 1)), (st.rows / float(1 << 20))[:, None],
+                           np.stack([_logit(ests[v]) for v in range(3, len(VARIANTS))], axis=1),
                            st.tablet_features], axis=1).astype(np.float32)

@@ -5,7 +5,7 @@ network with its own HIP kernels, virginian_amd/csrc/gpu/mlp.hip).
 The network has no counterpart in the reference, so there is no reference
 file to follow.  Its definition is the textbook one:
     h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), y = sigmoid(h2 W3 + b3)
-    loss = mean((y - t)^2)
+    loss = mean((y - t)^2), or mean(|y - t|)
 with parameters packed flat as W1[nin][nh] b1[nh] W2[nh][nh] b2[nh] W3[nh] b3,
 and optionally `skip` times the first input added to the output's
 pre-activation (the network then learns a correction to an estimate given
@@ -48,17 +48,22 @@ def forward(p: np.ndarray, x: np.ndarray, nin: int, nh: int, skip: float = 0.0):
     return y, (h1, h2)
 
 
-def loss(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int, skip: float = 0.0) -> float:
+def loss(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int, skip: float = 0.0,
+         absolute: bool = False) -> float:
     y, _ = forward(p, x, nin, nh, skip)
-    return float(np.mean((y - t) ** 2))
+    return float(np.mean(np.abs(y - t))) if absolute else float(np.mean((y - t) ** 2))
 
 
-def gradient(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int, skip: float = 0.0):
-    """(loss, gradient packed like p) by backpropagation."""
+def gradient(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int, skip: float = 0.0,
+             absolute: bool = False):
+    """(loss, gradient packed like p) by backpropagation; the loss is the
+    mean squared error, or with absolute=True the mean absolute error
+    (derivative sign(y - t), 0 where y = t)."""
     w = unpack(p, nin, nh)
     y, (h1, h2) = forward(p, x, nin, nh, skip)
     B = x.shape[0]
-    d3 = (2.0 * (y - t) / B * y * (1.0 - y))[:, None]     # dL/dz3
+    dl = np.sign(y - t) if absolute else 2.0 * (y - t)
+    d3 = (dl / B * y * (1.0 - y))[:, None]                 # dL/dz3
     gW3 = h2.T @ d3
     gb3 = d3.sum(axis=0)
     d2 = (d3 @ w["W3"].T) * (h2 > 0)
@@ -68,4 +73,4 @@ def gradient(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int, ski
     gW1 = x.T @ d1
     gb1 = d1.sum(axis=0)
     g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
-    return float(np.mean((y - t) ** 2)), g
+    return (float(np.mean(np.abs(y - t))) if absolute else float(np.mean((y - t) ** 2))), g

@@ -17,10 +17,12 @@ from virginian_amd import workload as W
 pytestmark = pytest.mark.gpu
 
 
-def test_forward_and_gradient_match_oracle(gpu_db):
+@pytest.mark.parametrize("loss", ["squared", "absolute"])
+def test_forward_and_gradient_match_oracle(gpu_db, loss):
+    absolute = loss == "absolute"
     for nin, nh, B, seed, skip in [(7, 5, 9, 1, 0.0), (PR.NFEATURES, 32, 300, 2, PR.SKIP), (40, 32, 1, 3, 0.0),
                                    (12, 32, 64, 4, 2.5)]:
-        net = PR.MLP(nin, nh, seed=seed, skip=skip)
+        net = PR.MLP(nin, nh, seed=seed, skip=skip, loss=loss)
         rng = np.random.default_rng(seed)
         # biases away from zero so that ReLUs are on and off
         p = net.params().astype(np.float64)
@@ -36,9 +38,9 @@ def test_forward_and_gradient_match_oracle(gpu_db):
         yh = net.forward_host(x)
         assert np.allclose(yh, y_ref, rtol=1e-4, atol=1e-6), (nin, np.abs(yh - y_ref).max())
         assert np.allclose(yh, y, rtol=1e-5, atol=1e-6), (nin, np.abs(yh - y).max())
-        loss, g = net.gradient(x, t)
-        loss_ref, g_ref = M.gradient(p, x.astype(np.float64), t.astype(np.float64), nin, nh, skip)
-        assert np.isclose(loss, loss_ref, rtol=1e-4)
+        lv, g = net.gradient(x, t)
+        loss_ref, g_ref = M.gradient(p, x.astype(np.float64), t.astype(np.float64), nin, nh, skip, absolute)
+        assert np.isclose(lv, loss_ref, rtol=1e-4)
         scale = np.abs(g_ref).max()
         assert np.allclose(g, g_ref, rtol=1e-4, atol=1e-5 * scale), (nin, np.abs(g - g_ref).max(), scale)
         net.close()
@@ -48,7 +50,7 @@ def test_training_fits_a_known_function(gpu_db):
     rng = np.random.default_rng(5)
     x = rng.uniform(-1, 1, (2000, 6)).astype(np.float32)
     t = (1 / (1 + np.exp(-(2 * x[:, 0] - x[:, 1] * x[:, 2])))).astype(np.float32)
-    net = PR.MLP(6, 32, seed=3, skip=0.0)
+    net = PR.MLP(6, 32, seed=3, skip=0.0, loss="squared")
     before = M.loss(net.params().astype(np.float64), x, t, 6, 32)
     after = net.train(x, t, epochs=40, batch=128, lr=3e-3)
     assert after < before / 10 and after < 2e-3, (before, after)
@@ -78,10 +80,12 @@ def learned(gpu_db):
             out.append(W.random_program(rng, max_ops=40, selectivity=sel, table=int(rng.choice(tids))))
         return out
 
-    train, held = progs(240), progs(60)
+    train, val, held = progs(400), progs(100), progs(150)
     x, t = PR.collect(gpu_db, train, stats)
+    xv, tv = PR.collect(gpu_db, val, stats)
     net = PR.MLP(seed=7)
-    net.train(x, t)
+    fit = net.fit(x, t, xv, tv)
+    print("fit", fit, flush=True)
     yield gpu_db, net, stats, train, held, (x, t)
     net.close()
 
@@ -99,6 +103,8 @@ def test_predictor_gains_over_sizing_without_it(learned):
     mae_est = float(np.mean(np.abs(hx[:, 3] - ht)))
     print(f"  (the independence estimate alone: {mae_est:.4f})")
     assert mae_net < 0.5 * mae_const
+    # the network gains over the estimate it corrects, on programs it never saw
+    assert mae_net < mae_est, (mae_net, mae_est)
     pred_net, pred_const, true_rows, rows = [], [], [], []
     for ops in held:
         st = stats[PR.table_of(ops)]

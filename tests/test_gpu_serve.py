@@ -95,9 +95,10 @@ def test_served_with_trained_predictor(setup):
     rng = np.random.default_rng(9)
     train = [W.random_program(rng, max_ops=40, selectivity=float(rng.uniform(0.0001, 0.8)),
                               table=int(rng.choice(tids))) for _ in range(120)]
-    x, t = PR.collect(db, train, stats)
+    x, t = PR.collect(db, train[:-16], stats)
+    xv, tv = PR.collect(db, train[-16:], stats)
     net = PR.MLP(seed=2)
-    net.train(x, t)
+    net.fit(x, t, xv, tv)
     srv = S.Server(db, tids, 256 << 20, mode="single", predictor=net, margin=0.05, runners=3, max_batch=16)
     try:
         done = _fresh(reqs)

@@ -39,6 +39,26 @@ def test_oracle_gradient_matches_finite_differences():
         assert np.allclose(g, fd, rtol=1e-5, atol=1e-8), (seed, np.abs(g - fd).max())
 
 
+def test_oracle_absolute_error_gradient_matches_finite_differences():
+    """Mean absolute error: away from y = t the loss is smooth and the
+    gradient is backpropagated sign(y - t)."""
+    for seed in range(4):
+        p, x, t, nin, nh = _case(seed + 20)
+        y, _ = M.forward(p, x, nin, nh)
+        assert np.abs(y - t).min() > 1e-4
+        _, g = M.gradient(p, x, t, nin, nh, 0.0, True)
+        fd = np.zeros_like(p)
+        for i in range(len(p)):
+            a, b = p.copy(), p.copy()
+            a[i] += 1e-6
+            b[i] -= 1e-6
+            fd[i] = (M.loss(a, x, t, nin, nh, 0.0, True) - M.loss(b, x, t, nin, nh, 0.0, True)) / 2e-6
+        assert np.allclose(g, fd, rtol=1e-5, atol=1e-8), (seed, np.abs(g - fd).max())
+        # and it is not the squared error's gradient
+        _, g2 = M.gradient(p, x, t, nin, nh)
+        assert not np.allclose(g, g2, rtol=1e-3)
+
+
 def test_finite_difference_check_catches_a_dropped_relu_mask():
     p, x, t, nin, nh = _case(11)
     w = M.unpack(p, nin, nh)

@@ -37,6 +37,7 @@ struct mlp_work {
 struct vg_mlp {
 	unsigned nin = 0, nh = 0, np = 0;
 	float skip = 0.0f;           /* output pre-activation += skip * x[0] */
+	int loss = 0;                /* 0: mean squared error, 1: mean absolute error */
 	float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
 	unsigned long long step = 0;
 	/* inference: a stream that does not wait for other streams' work,
@@ -94,14 +95,16 @@ __global__ void mlp_layer_fwd(const float *in, unsigned nin, const float *W, con
 	out[t] = act == 0 ? (z > 0.0f ? z : 0.0f) : 1.0f / (1.0f + expf(-z));
 }
 
-/* output delta: d(loss)/dz3 = 2 (y - t) / B * y (1 - y) */
-__global__ void mlp_out_delta(const float *y, const float *target, unsigned B, float *dz)
+/* output delta: d(loss)/dz3 = 2 (y - t) / B * y (1 - y) for the squared
+ * error, sign(y - t) / B * y (1 - y) for the absolute error */
+__global__ void mlp_out_delta(const float *y, const float *target, unsigned B, float *dz, int absolute)
 {
 	const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
 	if (b >= B)
 		return;
-	const float yy = y[b];
-	dz[b] = 2.0f * (yy - target[b]) / (float)B * yy * (1.0f - yy);
+	const float yy = y[b], r = yy - target[b];
+	const float dl = absolute ? (r > 0.0f ? 1.0f : r < 0.0f ? -1.0f : 0.0f) : 2.0f * r;
+	dz[b] = dl / (float)B * yy * (1.0f - yy);
 }
 
 /* hidden delta: dz[b][i] = [h[b][i] > 0] * sum_j dnext[b][j] * Wn[i][j] */
@@ -138,14 +141,14 @@ __global__ void mlp_wgrad(const float *in, unsigned nin, const float *dz, unsign
 	}
 }
 
-/* mean squared error into *loss (one workgroup of 256) */
-__global__ void mlp_loss(const float *y, const float *target, unsigned B, float *loss)
+/* mean squared (or absolute) error into *loss (one workgroup of 256) */
+__global__ void mlp_loss(const float *y, const float *target, unsigned B, float *loss, int absolute)
 {
 	__shared__ float part[256];
 	float s = 0.0f;
 	for (unsigned b = threadIdx.x; b < B; b += 256) {
 		const float d = y[b] - target[b];
-		s += d * d;
+		s += absolute ? fabsf(d) : d * d;
 	}
 	part[threadIdx.x] = s;
 	__syncthreads();
@@ -224,7 +227,7 @@ static void backward(vg_mlp *m, mlp_work &w, unsigned B, const float *x, const f
 	const float *P = m->d_p;
 	float *G = m->d_g;
 	const unsigned h = m->nh;
-	hipLaunchKernelGGL(mlp_out_delta, dim3(blocks(B)), dim3(256), 0, 0, w.y, t, B, w.d3);
+	hipLaunchKernelGGL(mlp_out_delta, dim3(blocks(B)), dim3(256), 0, 0, w.y, t, B, w.d3, m->loss);
 	hipLaunchKernelGGL(mlp_wgrad, dim3(blocks((size_t)(h + 1) * 1)), dim3(256), 0, 0,
 		w.h2, h, w.d3, 1u, B, G + L.w3, G + L.b3);
 	hipLaunchKernelGGL(mlp_hidden_delta, dim3(blocks((size_t)B * h)), dim3(256), 0, 0,
@@ -275,6 +278,16 @@ extern "C" void vg_mlp_free(vg_mlp *m)
 		if (p)
 			(void)hipFree(p);
 	delete m;
+}
+
+/* The loss training and vg_mlp_gradient use: 0 mean squared error (the
+ * default), 1 mean absolute error. */
+extern "C" int vg_mlp_set_loss(vg_mlp *m, int kind)
+{
+	if (kind != 0 && kind != 1)
+		return VIRG_FAIL;
+	m->loss = kind;
+	return VIRG_SUCCESS;
 }
 
 /* A network for `nin` features and `nh` hidden units per layer, weights
@@ -434,7 +447,7 @@ extern "C" int vg_mlp_gradient(vg_mlp *m, const float *x, const float *t, unsign
 	if (ok) {
 		forward(m, w, n, w.x);
 		backward(m, w, n, w.x, w.t);
-		hipLaunchKernelGGL(mlp_loss, dim3(1), dim3(256), 0, 0, w.y, w.t, n, w.loss);
+		hipLaunchKernelGGL(mlp_loss, dim3(1), dim3(256), 0, 0, w.y, w.t, n, w.loss, m->loss);
 		ok = hipGetLastError() == hipSuccess &&
 			hipMemcpy(loss, w.loss, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess &&
 			hipMemcpy(grad, m->d_g, m->np * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
@@ -487,7 +500,7 @@ extern "C" int vg_mlp_train(vg_mlp *m, const float *x, const float *t, unsigned 
 		}
 	if (ok) {
 		forward(m, w, n, w.x);
-		hipLaunchKernelGGL(mlp_loss, dim3(1), dim3(256), 0, 0, w.y, w.t, n, w.loss);
+		hipLaunchKernelGGL(mlp_loss, dim3(1), dim3(256), 0, 0, w.y, w.t, n, w.loss, m->loss);
 		ok = hipGetLastError() == hipSuccess &&
 			hipMemcpy(loss, w.loss, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
 	}

@@ -3,8 +3,10 @@
  *
  * For each tablet: the filter evaluated under independence, each
  * `column op constant` compare passing at its histogram estimate on the
- * tablet and any other compare at u, for u = 1/2, 1/4, 3/4 (as logits
- * scaled by 1/4, and the u = 1/2 estimate itself); the program's shape
+ * tablet and any other compare at a value given by its kind (ordered, =,
+ * !=) in each of seven variants: all at 1/2, 1/4 or 3/4, and the four
+ * of include/virg_gpu.h (as logits scaled by 1/4, and the all-1/2
+ * estimate itself); the program's shape
  * (compares estimated by histograms and not, keep and drop branches, Invalid
  * and logic ops, length); the tablet's rows; and the tablet's own column
  * statistics (min, max, 16-bin histogram per column, given).
@@ -21,8 +23,12 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define NU 3                       /* values of u */
-static const double U[NU] = { 0.5, 0.25, 0.75 };
+#define NU 7                       /* variants of the unknown compares' passing rates */
+/* per variant, the rate of an ordered (or logic), an = and a != compare */
+static const double U[NU][3] = {
+	{ 0.5, 0.5, 0.5 }, { 0.25, 0.25, 0.25 }, { 0.75, 0.75, 0.75 },
+	{ 0.5, 0.05, 0.95 }, { 0.25, 0.5, 0.5 }, { 0.75, 0.5, 0.5 }, { 0.5, 0.0, 1.0 },
+};
 
 enum { RD_EXPR = 0, RD_COL, RD_CONST };
 
@@ -194,12 +200,13 @@ int vg_program_features(const virg_op *ops, unsigned num_ops, const vg_table_sta
 			} else {
 				nlogic++;
 			}
+			const int kind = k == OP_Eq ? 1 : k == OP_Neq ? 2 : 0;
 			if (m) {
 				const int jumps = o->p4.i && o->p3 > (int)pc && o->p3 <= (int)num_ops;
 				double *tp = jumps ? arrive + (size_t)o->p3 * row : NULL;
 				for (unsigned u = 0; u < NU; u++)
 					for (unsigned t = 0; t < nt; t++) {
-						const double pr = estimated ? p[t] : U[u];
+						const double pr = estimated ? p[t] : U[u][kind];
 						const double x = mp[u * nt + t];
 						if (tp)
 							tp[u * nt + t] += x * pr;
@@ -235,8 +242,10 @@ int vg_program_features(const virg_op *ops, unsigned num_ops, const vg_table_sta
 		for (int j = 0; j < 7; j++)
 			f[4 + j] = shape[j];
 		f[11] = (float)(st->rows[t] / (double)(1 << 20));
+		for (unsigned u = 3; u < NU; u++)
+			f[12 + u - 3] = scaled_logit(e[u]);
 		for (int j = 0; j < VG_FEAT_COLS * (VG_FEAT_BINS + 2); j++)
-			f[12 + j] = (float)st->tabfeat[(size_t)t * VG_FEAT_COLS * (VG_FEAT_BINS + 2) + j];
+			f[16 + j] = (float)st->tabfeat[(size_t)t * VG_FEAT_COLS * (VG_FEAT_BINS + 2) + j];
 	}
 	free(arrive);
 	free(have);

@@ -10,10 +10,12 @@ only means smaller buffers (more queries admitted into HBM) and fewer
 reruns.
 
 Features of a (program, tablet) pair:
-  - the filter evaluated under independence (structural_estimate): each
-    `column op constant` compare passing at its histogram estimate on the
-    tablet, and any other compare at 1/2 -- and at 1/4 and 3/4, which
-    brackets what the compares the histograms cannot estimate may do;
+  - the filter evaluated under independence: each `column op constant`
+    compare passing at its histogram estimate on the tablet, and any other
+    compare at 1/2 (the estimate the network corrects) -- and in six more
+    variants with those compares at 1/4 or 3/4, or by kind (= rarely, !=
+    almost always, ordered ones at 1/4, 1/2 or 3/4), which bracket what
+    the compares the histograms cannot estimate may do;
   - program shape: compares the histograms estimate and those they cannot,
     keep and drop branches, Invalid and logic ops, program length;
   - the tablet: rows, and per column its min, max and a 16-bin histogram
@@ -33,7 +35,7 @@ from . import NP_TYPE, OP_Table, VIRG_SUCCESS, VirginianError, lib, make_ops
 
 NCOLS = 6
 NBINS = 16
-NFEATURES = 4 + 7 + 1 + NCOLS * (2 + NBINS)
+NFEATURES = 4 + 7 + 1 + 4 + NCOLS * (2 + NBINS)
 # feature 0 is logit(estimate) / SKIP; the network adds SKIP * feature 0 to
 # its output's pre-activation and so learns a correction to the estimate
 SKIP = 4.0
@@ -125,12 +127,19 @@ def program_features(ops: list[tuple], stats: TableStats) -> np.ndarray:
 class MLP:
     """The predictor network on the GPU (vg_mlp_* in include/virg_gpu.h)."""
 
-    def __init__(self, nin: int = NFEATURES, nh: int = 32, seed: int = 1, skip: float = SKIP):
+    def __init__(self, nin: int = NFEATURES, nh: int = 32, seed: int = 1, skip: float = SKIP,
+                 loss: str = "absolute"):
+        """loss: "absolute" (mean absolute error, what sizing is judged by;
+        the squared error pulls predictions toward the mean of skewed
+        selectivities) or "squared"."""
         _bind()
         self.nin, self.nh, self.skip = nin, nh, skip
+        self.absolute = loss == "absolute"
         self.h = lib.vg_mlp_create(nin, nh, skip, seed)
         if not self.h:
             raise VirginianError("vg_mlp_create failed (no device?)")
+        if lib.vg_mlp_set_loss(self.h, 1 if self.absolute else 0) != VIRG_SUCCESS:
+            raise VirginianError("vg_mlp_set_loss failed")
         self.nparams = lib.vg_mlp_num_params(self.h)
 
     def params(self) -> np.ndarray:
@@ -172,8 +181,13 @@ class MLP:
             raise VirginianError("vg_mlp_gradient failed")
         return loss.value, g
 
-    def train(self, x: np.ndarray, t: np.ndarray, epochs: int = 10, batch: int = 256, lr: float = 2e-3,
-              seed: int = 1) -> float:
+    def train(self, x: np.ndarray, t: np.ndarray, epochs: int = 0, batch: int = 256, lr: float = 2e-3,
+              seed: int = 1, steps: int = 600) -> float:
+        """Adam on minibatches; epochs = 0: as many passes as `steps`
+        updates take (the example sets are small: a few thousand tablets
+        of a few hundred programs)."""
+        if epochs <= 0:
+            epochs = max(1, math.ceil(steps / max(1, math.ceil(len(x) / batch))))
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
         loss = C.c_float(0)
@@ -182,6 +196,29 @@ class MLP:
                             C.byref(loss)) != VIRG_SUCCESS:
             raise VirginianError("vg_mlp_train failed")
         return loss.value
+
+    def fit(self, x: np.ndarray, t: np.ndarray, xv: np.ndarray, tv: np.ndarray, max_epochs: int = 300,
+            patience: int = 60, batch: int = 256, lr: float = 2e-3, seed: int = 1) -> dict:
+        """Train epoch by epoch (vg_mlp_train, Adam state carried over) and
+        keep the parameters with the least mean absolute error on the
+        validation examples (xv, tv: tablets of programs kept out of x);
+        stop after `patience` epochs without improvement (the first tens
+        of epochs, while the correction grows from zero, may not improve
+        it).  A few hundred programs are easy to memorise: without this
+        the held-out error rises above the estimate the network corrects."""
+        best = float(np.mean(np.abs(self.forward_host(xv) - tv)))
+        start = best
+        best_p, best_epoch, epoch = self.params(), 0, 0
+        for epoch in range(1, max_epochs + 1):
+            self.train(x, t, epochs=1, batch=batch, lr=lr, seed=seed + epoch)
+            mae = float(np.mean(np.abs(self.forward_host(xv) - tv)))
+            if mae < best:
+                best, best_p, best_epoch = mae, self.params(), epoch
+            elif epoch - best_epoch >= patience:
+                break
+        self.set_params(best_p)
+        return {"validation_mae": best, "validation_mae_start": start, "best_epoch": best_epoch,
+                "epochs": epoch}
 
     def close(self) -> None:
         if self.h:
@@ -213,6 +250,7 @@ def _bind() -> None:
         "vg_mlp_forward": (C.c_int, [P, F, C.c_uint, F]),
         "vg_mlp_forward_host": (C.c_int, [P, F, C.c_uint, F]),
         "vg_mlp_gradient": (C.c_int, [P, F, F, C.c_uint, C.POINTER(C.c_float), F]),
+        "vg_mlp_set_loss": (C.c_int, [P, C.c_int]),
         "vg_mlp_train": (C.c_int, [P, F, F, C.c_uint, C.c_uint, C.c_uint, C.c_float, C.c_ulonglong,
                                    C.POINTER(C.c_float)]),
         "vg_query_tablet_rows": (C.c_uint, [P, C.POINTER(C.c_ulonglong), C.c_uint]),
