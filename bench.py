@@ -475,25 +475,37 @@ def bench_served(a) -> int:
         rowsl.append(st.rows)
         k += n
     margin = PR.calibrate(preds, true_rows, rowsl, a.rerun_target)
+    # the method without the network: the independence estimate (feature 3)
+    # with its own margin, calibrated on the same held-out programs
+    est, k = [], 0
+    for ops in held:
+        n = len(stats[PR.table_of(ops)].rows)
+        est.append(hx[k:k + n, 3])
+        k += n
+    margin_est = PR.calibrate(est, true_rows, rowsl, a.rerun_target)
     mae = float(np.mean(np.abs(pred - ht)))
     mae_est = float(np.mean(np.abs(hx[:, 3] - ht)))
     mae_const = float(np.mean(np.abs(float(np.mean(t)) - ht)))
     print(f"# predictor: {len(x)} + {len(xv)} examples collected in {t1 - t0:.1f}s, trained in {t2 - t1:.2f}s "
           f"({fit['best_epoch']} of {fit['epochs']} epochs, validation MAE {loss:.4f} from "
           f"{fit['validation_mae_start']:.4f}); held-out MAE {mae:.4f} (independence estimate {mae_est:.4f}, "
-          f"mean selectivity {mae_const:.4f}); margin {margin:.3f}", file=sys.stderr, flush=True)
+          f"mean selectivity {mae_const:.4f}); margin {margin:.3f} (estimate alone {margin_est:.3f})",
+          file=sys.stderr, flush=True)
 
     n_req = int(a.rate * a.duration)
     arrivals = np.cumsum(rng.exponential(1.0 / a.rate, n_req))
     stream = [(float(ar), ops) for ar, ops in zip(arrivals, programs(n_req))]
     rows_of = {t: int(stats[t].rows.sum()) for t in tids}
     out = {}
-    for mode in ("sized", "single", "two-pass", "plain"):
+    for mode in a.modes.split(","):
         # single-pass batches run one at a time: their tiles wait on each
         # other's published counts, and three such kernels side by side were
         # measured 20x slower than one
-        srv = S.Server(db, tids, a.arena_gib << 30, mode=mode, predictor=net if mode != "two-pass" else None,
-                       margin=margin, runners=1 if mode == "single" else 3, max_batch=64, keep_results=False)
+        srv = S.Server(db, tids, a.arena_gib << 30, mode=mode,
+                       predictor=net if mode in ("sized", "single") else None,
+                       margin=margin_est if mode == "estimate" else margin,
+                       runners=1 if mode == "single" else a.runners, max_batch=a.max_batch, keep_results=False,
+                       split=a.split, grid_div=a.grid_div)
         reqs = [S.Request(arrival=ar, ops=ops, rid=i) for i, (ar, ops) in enumerate(stream)]
         try:
             summ = srv.run(reqs)
@@ -509,35 +521,51 @@ def bench_served(a) -> int:
               f"reruns {summ['reruns']}, peak arena {summ['peak_arena_bytes'] / 2**20:.0f} MiB, "
               f"{summ['gbs']:.1f} GB/s; host {summ['host_seconds']}", file=sys.stderr, flush=True)
     net.close()
-    same = out["single"]["rows"] == out["two-pass"]["rows"] == out["sized"]["rows"] == out["plain"]["rows"]
-    s1, s2, s0 = out["sized"], out["two-pass"], out["single"]
+    rows_seen = [o["rows"] for o in out.values()]
+    same = all(r == rows_seen[0] for r in rows_seen)
+
+    def brief(m):
+        return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in out[m].items() if k != "rows"}
+
+    def ratio(num, den, key, inv=False):
+        if num not in out or den not in out:
+            return None
+        x, y = out[num][key], out[den][key]
+        return round((y / x) if inv else (x / max(y, 1e-12)), 3)
+
+    head = "sized" if "sized" in out else next(iter(out))
+    s1 = out[head]
+    names = {"sized": "sized_two_pass", "estimate": "sized_by_estimate", "single": "sized_single_pass",
+             "two-pass": "two_pass_every_row", "plain": "plain_unbatched"}
+    cfg = {"workload": f"served: Poisson arrivals at {a.rate:.0f} q/s for {a.duration:.1f} s of random "
+                       "op-programs (4-40 ops) over 10 resident tables of log-uniform 1e5-5e7 rows",
+           "table_rows": sizes, "requests": n_req, "arena_gib": a.arena_gib,
+           "server": {"runners": a.runners, "max_batch": a.max_batch, "split": a.split, "grid_div": a.grid_div},
+           "predictor": {"examples": int(len(x)), "train_s": round(t2 - t1, 3), "loss": round(loss, 6),
+                         "heldout_mae": round(mae, 4), "independence_estimate_mae": round(mae_est, 4),
+                         "mean_selectivity_mae": round(mae_const, 4), "margin": margin,
+                         "estimate_margin": margin_est}}
+    for m in out:
+        cfg[names.get(m, m)] = brief(m)
+    cfg["rerun_rate"] = round(s1["reruns"] / n_req, 4)
+    cfg["batched_vs_plain"] = {"throughput": ratio("two-pass", "plain", "queries_per_s"),
+                               "p99": ratio("two-pass", "plain", "p99_ms", inv=True),
+                               "p50": ratio("two-pass", "plain", "p50_ms", inv=True)}
+    cfg["sized_vs_plain"] = {"throughput": ratio("sized", "plain", "queries_per_s"),
+                             "p99": ratio("sized", "plain", "p99_ms", inv=True),
+                             "p50": ratio("sized", "plain", "p50_ms", inv=True)}
+    cfg["gain_vs_two_pass"] = {"throughput": ratio("sized", "two-pass", "queries_per_s"),
+                               "p99": ratio("sized", "two-pass", "p99_ms", inv=True),
+                               "reserved_bytes": ratio("sized", "two-pass", "reserved_bytes", inv=True)}
+    cfg["gain_vs_estimate"] = {"reserved_bytes": ratio("sized", "estimate", "reserved_bytes", inv=True),
+                               "reruns": [out[m]["reruns"] for m in ("sized", "estimate") if m in out]}
+    cfg["identical_row_counts"] = same
     line = {
         "metric": "GB/s scanned by served filter SELECT queries (learned sizing vs buffers for every row)",
         "value": round(s1["gbs"], 2), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": round(s1["seconds"] * 1e3, 2), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+i32", "data": "synthetic (seeded Philox generator, seeded programs)",
-        "config": {"workload": f"served: Poisson arrivals at {a.rate:.0f} q/s for {a.duration:.1f} s of random "
-                               "op-programs (4-40 ops) over 10 resident tables of log-uniform 1e5-5e7 rows",
-                   "table_rows": sizes, "requests": n_req, "arena_gib": a.arena_gib,
-                   "predictor": {"examples": int(len(x)), "train_s": round(t2 - t1, 3), "loss": round(loss, 6),
-                                 "heldout_mae": round(mae, 4), "independence_estimate_mae": round(mae_est, 4),
-                                 "mean_selectivity_mae": round(mae_const, 4), "margin": margin},
-                   "sized_two_pass": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s1.items()
-                                      if k != "rows"},
-                   "sized_single_pass": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s0.items()
-                                         if k != "rows"},
-                   "two_pass_every_row": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s2.items()
-                                          if k != "rows"},
-                   "plain_unbatched": {k: (round(v, 4) if isinstance(v, float) else v)
-                                       for k, v in out["plain"].items() if k != "rows"},
-                   "rerun_rate": round(s1["reruns"] / n_req, 4),
-                   "batched_vs_plain": {"throughput": round(s2["queries_per_s"] / out["plain"]["queries_per_s"], 3),
-                                        "p99": round(out["plain"]["p99_ms"] / s2["p99_ms"], 3)},
-                   "gain_vs_two_pass": {"throughput": round(s1["queries_per_s"] / s2["queries_per_s"], 3),
-                                        "p99": round(s2["p99_ms"] / s1["p99_ms"], 3),
-                                        "peak_arena": round(s2["peak_arena_bytes"] / max(s1["peak_arena_bytes"], 1), 3),
-                                        "reserved_bytes": round(s2["reserved_bytes"] / max(s1["reserved_bytes"], 1), 3)},
-                   "identical_row_counts": same},
+        "config": cfg,
         "roofline": {"bound": "hbm", "achieved": round(s1["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(s1["gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": None,
@@ -750,6 +778,11 @@ def main() -> int:
     ap.add_argument("--rerun-target", type=float, default=0.005,
                     help="served: rerun rate the predictor's margin is calibrated for")
     ap.add_argument("--train-programs", type=int, default=600, help="served: predictor training runs")
+    ap.add_argument("--runners", type=int, default=3, help="served: runs in flight at once (streams)")
+    ap.add_argument("--max-batch", type=int, default=64, help="served: requests per run")
+    ap.add_argument("--split", type=int, default=0, help="served: 1 = one launch group per batch")
+    ap.add_argument("--grid-div", type=int, default=0, help="served: batches launch 1/grid-div of the grid")
+    ap.add_argument("--modes", default="sized,estimate,single,two-pass,plain", help="served: servers to run")
     ap.add_argument("--queries", type=int, default=256, help="batched: queries per batch")
     ap.add_argument("--programs", type=int, default=0,
                     help="batched: distinct program structures (0: every query its own)")

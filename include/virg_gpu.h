@@ -357,6 +357,96 @@ int vg_select_noalloc(const virg_op *ops, unsigned num_ops, const void *const *c
 	const int *col_types, unsigned ncols, const void *key, int key_type, unsigned long long rows,
 	void *const *out, unsigned long long capacity, void *result, void *stream, char *err, size_t errlen);
 
+/* Serving loop (no counterpart in the reference, whose queries run one at
+ * a time: src/vm/execute.c:28-102).  Requests are generated op-programs
+ * over tables already loaded; they arrive over time and are served with
+ * continuous batching by one host thread:
+ *   - a request joins the waiting line once the server's clock passes its
+ *     arrival time;
+ *   - while a runner (a stream) is free, waiting requests are admitted in
+ *     arrival order (reruns first) as long as their result buffers fit in
+ *     the arena (device memory taken once at creation): admission by HBM
+ *     footprint;
+ *   - one admitted request runs alone (its count and write launches; the
+ *     write pass stores the outcome into pinned host memory, which the loop
+ *     polls), several run as one persistent batch (vg_pbatch) whose kernels
+ *     store every outcome into pinned memory; runs on different runners
+ *     overlap on the device;
+ *   - a finished run completes its requests, except a result that did not
+ *     fit its buffers: that request is admitted again, before new ones, with
+ *     buffers of the exact size the run counted.
+ * Result buffers are sized (`sizing`) for every row of the query's table,
+ * or per tablet as rows * min(1, max(p, 0) + margin), f32, summed, plus
+ * 64, where p is the independence estimate of the filter over the table's
+ * histograms (feature 3 of vg_program_features) or the result-size
+ * network's output.  Results never depend on the sizing.  With runners = 1
+ * and max_batch = 1 this is the plain way: each request alone, in arrival
+ * order, waited for, then the next.  Latency of a request = when the loop
+ * saw its result complete in HBM (`ready`) minus its arrival.  A server is
+ * used by one host thread at a time. */
+#define VG_SIZE_ALL      0   /* buffers for every row of the table */
+#define VG_SIZE_ESTIMATE 1   /* the independence estimate over the histograms */
+#define VG_SIZE_NET      2   /* the result-size network (vg_mlp) */
+typedef struct {
+	unsigned long long arena_bytes;   /* result arena (0: 8 GiB) */
+	unsigned runners;                 /* runs in flight at once, one stream each (0: 3) */
+	unsigned max_batch;               /* requests per run (0: 64) */
+	unsigned tiles_per_item;          /* batch work items (0: 8) */
+	unsigned sizing;                  /* VG_SIZE_* */
+	float margin;                     /* added to each tablet's predicted fraction */
+	unsigned single;                  /* 1: single-pass batches (every run a batch) */
+	unsigned split;                   /* vg_pbatch_config.split */
+	unsigned grid_div;                /* vg_pbatch_config.grid_div */
+	unsigned keep;                    /* 1: each result is copied to host memory the server
+	                                   * holds (until vg_server_release) before the request
+	                                   * counts as done */
+	const vg_mlp *net;                /* VG_SIZE_NET: the trained network */
+} vg_server_config;
+typedef struct {
+	unsigned long long done, failed, reruns;
+	unsigned long long lone;          /* runs of one request */
+	unsigned long long batches;       /* runs of several */
+	unsigned long long reserved_bytes;/* result bytes reserved, over all admissions */
+	unsigned long long arena_peak;    /* most arena bytes in use at once */
+	unsigned long long in_flight_peak;
+	double admit_s, launch_s, collect_s;  /* host seconds per step of the loop */
+} vg_server_stats;
+/* state: 0 not arrived, 1 waiting, 2 running, 3 done, 4 failed */
+typedef struct {
+	unsigned state;
+	unsigned reruns;
+	unsigned long long rows;
+	unsigned long long capacity;      /* result rows reserved for its last run */
+	double arrival, ready;            /* seconds on the server's clock */
+} vg_server_req;
+typedef struct vg_server vg_server;
+vg_server *vg_server_create(virginian *v, const vg_server_config *cfg, char *err, size_t errlen);
+void vg_server_free(vg_server *s);
+/* Statistics of a table's tablets for sizing by estimate or network
+ * (copied). */
+int vg_server_table(vg_server *s, unsigned table_id, const vg_table_stats *st);
+/* Queue a request arriving at `arrival` seconds on the server's clock
+ * (negative: now); *id numbers requests from 0 in submission order.  The
+ * ops are copied (ResultColumn names are not kept). */
+int vg_server_submit(vg_server *s, const virg_op *ops, unsigned num_ops, double arrival,
+	unsigned long long *id);
+/* One pass of the loop (never waits); the number of requests completed,
+ * or -1 when a launch failed. */
+int vg_server_step(vg_server *s);
+/* Restart the clock at 0 and serve until every submitted request is done
+ * or failed (timeout_s > 0: give up after that long). */
+int vg_server_run(vg_server *s, double timeout_s);
+double vg_server_now(vg_server *s);
+int vg_server_request(vg_server *s, unsigned long long id, vg_server_req *out);
+/* keep = 1: a done request's result rows into host arrays, one per result
+ * column, in the reference's single-core row order (as vg_query_fetch);
+ * release drops the server's copy. */
+int vg_server_fetch(vg_server *s, unsigned long long id, void **cols, unsigned long long max_rows);
+int vg_server_release(vg_server *s, unsigned long long id);
+int vg_server_get_stats(vg_server *s, vg_server_stats *out);
+/* The last error's message ("" when none). */
+const char *vg_server_error(vg_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,7 +32,7 @@ LIBNAME = "libvirginian_amd.so"
 
 HOST_C = ["db.c", "dbfile.c", "tablet.c", "table.c", "vm.c", "reader.c", "sql.c", "gen.c", "features.c"]
 GPU_C = ["decode.c"]
-GPU_HIP = ["vm_kernel.hip", "device.hip", "mlp.hip", "gen.hip", "jit.hip"]
+GPU_HIP = ["vm_kernel.hip", "device.hip", "mlp.hip", "gen.hip", "jit.hip", "serve.hip"]
 
 CFLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", "-Wextra",
           "-Wno-unused-parameter", "-pthread", f"-I{INCLUDE}", f"-I{CSRC / 'host'}",

@@ -396,7 +396,7 @@ static int build_plan(vgd_device *d, const std::vector<const virg_tablet_meta *>
 }
 
 static int launch(const vg_prog &P, const plan &p, launch_res &lr, void *const *out,
-	uint64_t capacity, hipStream_t stream, hipEvent_t mid = nullptr)
+	uint64_t capacity, hipStream_t stream, hipEvent_t mid = nullptr, vg_bresult *hres = nullptr)
 {
 	if (p.num_tiles == 0) {
 		if (mid)
@@ -415,6 +415,7 @@ static int launch(const vg_prog &P, const plan &p, launch_res &lr, void *const *
 	L.vbits = lr.vbits;
 	L.tcount = lr.tcount;
 	L.wgcount = lr.wgcount;
+	L.hres = hres;
 	for (unsigned j = 0; j < P.nout; j++)
 		L.out[j] = out[j];
 	int rc = vg_kernel_launch(&P, &L, stream, mid);
@@ -1318,6 +1319,26 @@ static int query_launch(vg_query *q, hipStream_t stream, hipEvent_t mid = nullpt
 		return VIRG_SUCCESS;
 	}
 	return launch(q->P, *q->p, q->lr, q->out, q->capacity, stream, mid);
+}
+
+/* For the serving loop: run q alone on `stream` (its count and write
+ * passes), the write pass storing the outcome into res (pinned host
+ * memory, cleared here) once every result row is in place.  Interpreter
+ * queries only. */
+extern "C" int vgd_query_launch_lone(vg_query *q, hipStream_t stream, vg_bresult *res)
+{
+	if (q->jk || q->batch_refs)
+		return VIRG_FAIL;
+	vg_bresult *d_res = nullptr;
+	HCHK(hipHostGetDevicePointer((void **)&d_res, res, 0), "outcome");
+	memset(res, 0, sizeof *res);
+	if (q->pooled_ctrl)
+		take_pooled_lr(q->d, q->lr, q->p->num_tiles, vg_kernel_grid(&q->P, q->p->num_tiles ? q->p->num_tiles : 1));
+	if (q->p->num_tiles == 0) {
+		res->done = 1;
+		return VIRG_SUCCESS;
+	}
+	return launch(q->P, *q->p, q->lr, q->out, q->capacity, stream, nullptr, d_res);
 }
 
 extern "C" void vg_query_free(vg_query *q)
@@ -2468,6 +2489,12 @@ extern "C" int vg_pbatch_reload(vg_pbatch *b, vg_query *const *qs, unsigned n, c
 		return VIRG_SUCCESS;
 	}
 	return pbatch_build(b, err, errlen);
+}
+
+/* The stream a batch runs on (for the serving loop). */
+extern "C" hipStream_t vgd_pbatch_stream(vg_pbatch *b)
+{
+	return b->stream;
 }
 
 /* 1 when the batch's last run has finished (does not wait). */

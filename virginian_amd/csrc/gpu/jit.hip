@@ -1075,7 +1075,7 @@ extern "C" __global__ __launch_bounds__(256) void vgxb_reset(const vgxb_args B)
 		c->overflow = 0;
 		B.done[i] = 0;
 		vg_bresult r;
-		r.total = 0; r.t_done = 0; r.t_start = 0; r.flags = 0; r.pad = 0;
+		r.total = 0; r.t_done = 0; r.t_start = 0; r.flags = 0; r.done = 0;
 		B.hres[i] = r;
 	}
 	if (i == 0)
@@ -1139,8 +1139,11 @@ extern "C" __global__ __launch_bounds__(256) void vgxb_reset(const vgxb_args B)
 				r.flags = __hip_atomic_load(&A.ctrl->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 				r.t_done = wall_clock64();
 				r.t_start = B.state[2];
-				r.pad = 0;
+				r.done = 0;
 				B.hres[q] = r;
+				/* the fields, then the flag a polling host waits for */
+				__threadfence_system();
+				__hip_atomic_store(&B.hres[q].done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 			}
 			s_item = s_next;
 		}

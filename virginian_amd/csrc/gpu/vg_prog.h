@@ -131,6 +131,19 @@ typedef struct {
 	uint32_t pad[5];
 } vg_ctrl;
 
+/* A query's outcome of a run, stored by the kernel that finishes the
+ * query into pinned host memory (a batch's reset, or the host before a
+ * lone run, clears it): the host reads it where it lies, with no copy that
+ * would queue behind kernels still holding the compute units.  `done` is
+ * stored last (system scope, release). */
+typedef struct {
+	uint64_t total;          /* rows selected */
+	uint64_t t_done;         /* device clock when its last item finished */
+	uint64_t t_start;        /* device clock when the run started */
+	uint32_t flags;          /* overflow bits */
+	uint32_t done;           /* 1 once the fields above are final */
+} vg_bresult;
+
 /* kernel arguments besides the program.  Count-pass workgroup g owns
  * tiles [g * tiles_per_wg, (g + 1) * tiles_per_wg), write-pass workgroup g
  * tiles [g * wtiles_per_wg, (g + 1) * wtiles_per_wg); each pass has the
@@ -147,6 +160,8 @@ typedef struct {
 	uint8_t *vbits;          /* one validity byte per write-pass lane per tile */
 	uint32_t *tcount;        /* rows selected per tile and wave: 4 per tile */
 	uint32_t *wgcount;       /* rows selected per count-pass workgroup */
+	vg_bresult *hres;        /* NULL, or where a launch after the write pass
+	                          * stores the run's outcome (pinned host memory) */
 	void *out[VG_MAX_OUT];
 } vg_launch;
 
@@ -189,18 +204,6 @@ typedef struct {
 	uint64_t t_done;         /* device clock when its last item finished */
 	uint64_t total;          /* rows selected, when finished */
 } vg_bquery;
-
-/* A query's outcome of a run, stored by the kernel that finishes the
- * query into pinned host memory (the reset clears it): once the run's
- * stream is idle the host reads it where it lies, with no copy that would
- * queue behind kernels still holding the compute units. */
-typedef struct {
-	uint64_t total;          /* rows selected */
-	uint64_t t_done;         /* device clock when its last item finished */
-	uint64_t t_start;        /* device clock when the run started */
-	uint32_t flags;          /* overflow bits */
-	uint32_t pad;
-} vg_bresult;
 
 typedef struct {
 	uint64_t t_start;        /* device clock when the run started */

@@ -938,6 +938,25 @@ __device__ __forceinline__ uint64_t write_tiles(const vg_prog &P, const vg_launc
 	return chunk_base;
 }
 
+/* A run launched with an outcome pointer (a query run alone while the
+ * host polls): one thread after the write pass stores the row count and
+ * overflow bits into pinned host memory, `done` last.  The launch boundary
+ * orders it after every result store of the write pass. */
+__global__ __launch_bounds__(64) void vg_publish_kernel(const vg_ctrl *c, vg_bresult *h)
+{
+	if (threadIdx.x != 0)
+		return;
+	vg_bresult r;
+	r.total = c->total;
+	r.flags = c->overflow;
+	r.t_done = wall_clock64();
+	r.t_start = 0;
+	r.done = 0;
+	*h = r;
+	__threadfence_system();
+	__hip_atomic_store(&h->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ __launch_bounds__(VG_WRITE_THREADS) void vg_write_kernel(const vg_prog P, const vg_launch L)
 {
 	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
@@ -1033,11 +1052,13 @@ __device__ __forceinline__ void finish_query(const vg_batch_args &A, uint32_t q)
 	r.flags = __hip_atomic_load(&c->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	r.t_done = wall_clock64();
 	r.t_start = ldc(&A.head)->t_start;
-	r.pad = 0;
+	r.done = 0;
 	A.q[q].total = r.total;
 	A.q[q].flags = r.flags;
 	A.q[q].t_done = r.t_done;
 	A.hres[q] = r;
+	__threadfence_system();
+	__hip_atomic_store(&A.hres[q].done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 /* The single pass's next item: each query's tiles are taken in order
@@ -1162,10 +1183,12 @@ __global__ __launch_bounds__(64) void vg_batch_finish_kernel(const vg_batch_args
 		r.flags = ldc(&A.launch[q].ctrl)->overflow;
 		r.t_done = t;
 		r.t_start = ldc(&A.head)->t_start;
-		r.pad = 0;
+		r.done = 0;
 		A.q[q].flags = r.flags;
 		A.q[q].t_done = t;
 		A.hres[q] = r;
+		__threadfence_system();
+		__hip_atomic_store(&A.hres[q].done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 	}
 }
 
@@ -1460,6 +1483,8 @@ extern "C" int vg_kernel_launch(const vg_prog *P, vg_launch *L, hipStream_t stre
 	if (mid && hipEventRecord(mid, stream) != hipSuccess)
 		return -2;
 	hipLaunchKernelGGL(vg_write_kernel, dim3(wgrid), dim3(VG_WRITE_THREADS), write_lds_bytes(P), stream, *P, *L);
+	if (L->hres)
+		hipLaunchKernelGGL(vg_publish_kernel, dim3(1), dim3(64), 0, stream, (const vg_ctrl *)L->ctrl, L->hres);
 	return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
