@@ -504,7 +504,8 @@ def bench_served(a) -> int:
         srv = S.Server(db, tids, a.arena_gib << 30, mode=mode,
                        predictor=net if mode in ("sized", "single") else None,
                        margin=margin_est if mode == "estimate" else margin,
-                       runners=1 if mode == "single" else a.runners, max_batch=a.max_batch, keep_results=False,
+                       runners=a.single_runners if mode == "single" else a.runners, max_batch=a.max_batch,
+                       keep_results=False,
                        split=a.split, grid_div=a.grid_div)
         reqs = [S.Request(arrival=ar, ops=ops, rid=i) for i, (ar, ops) in enumerate(stream)]
         try:
@@ -780,6 +781,7 @@ def main() -> int:
     ap.add_argument("--train-programs", type=int, default=600, help="served: predictor training runs")
     ap.add_argument("--runners", type=int, default=3, help="served: runs in flight at once (streams)")
     ap.add_argument("--max-batch", type=int, default=64, help="served: requests per run")
+    ap.add_argument("--single-runners", type=int, default=1, help="served: runners of the single-pass server")
     ap.add_argument("--split", type=int, default=0, help="served: 1 = one launch group per batch")
     ap.add_argument("--grid-div", type=int, default=0, help="served: batches launch 1/grid-div of the grid")
     ap.add_argument("--modes", default="sized,estimate,single,two-pass,plain", help="served: servers to run")

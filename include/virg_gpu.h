@@ -130,9 +130,12 @@ int vg_batch_time(vg_batch *b, int warmup, int iters, double *ms);
  * items g, g + grid, ...), a scan launch turning each query's item counts
  * into output offsets, a write launch dealing out the write items the same
  * way, and a finish launch recording each query's outcome.  The single
- * pass (config.single) instead takes each query's tiles in order through
- * that query's own counter and finds each tile's output offset by
- * look-back.  The batch holds its queries (vg_query_free and
+ * pass (config.single) is one launch instead: items are chunks of a query's
+ * tiles, taken in order through that query's own counter; a workgroup
+ * counts its chunk, publishes the count, finds the chunk's output offset
+ * by look-back over the query's earlier chunks and writes the chunk's rows
+ * there, and the workgroup finishing a query's last chunk records its
+ * outcome.  The batch holds its queries (vg_query_free and
  * vg_query_set_outputs refuse them) until it is freed or reloaded.  Queries are grouped by the occupancy their
  * register files allow, each group with its own pair of launches on its
  * own branch; the reset launch and every group's pair are captured as one
@@ -150,10 +153,11 @@ typedef struct {
 	                            * (for batches whose queries change: vg_pbatch_reload) */
 	virginian *db;             /* direct batches created empty: the database */
 	unsigned single;           /* 1: one pass into each query's capacity as given (an
-	                            * estimate): every tile counts, takes its output offset
-	                            * by decoupled look-back over the query's earlier tiles
-	                            * and writes its rows there; a query whose result does
-	                            * not fit reports it (vg_pbatch_results) */
+	                            * estimate): every item (tiles_per_item tiles) counts,
+	                            * takes its output offset by decoupled look-back over
+	                            * the query's earlier items and writes its rows there;
+	                            * a query whose result does not fit reports it
+	                            * (vg_pbatch_results) */
 	unsigned jit;              /* 1: one persistent launch of specialised code: the
 	                            * batch's programs compiled into one module at build
 	                            * time (hiprtc), workgroups taking (query, chunk) items

@@ -1908,6 +1908,14 @@ struct vg_pbatch {
 	std::vector<hipStream_t> branch;        /* one per group (graph capture) */
 	std::vector<hipEvent_t> join;
 	hipEvent_t fork = nullptr;
+	/* what every run reads, uploaded with one copy per build: programs,
+	 * launch arguments, per-query info, group query lists and item lists,
+	 * each at its offset in d_blob (staged in h_blob) */
+	char *d_blob = nullptr;
+	size_t blob_cap = 0;
+	pinned_buf h_blob;
+	std::vector<vg_prog> progs_h;
+	std::vector<vg_launch> launch_h;
 	vg_prog *d_progs = nullptr;
 	vg_launch *d_launch = nullptr;
 	vg_bqinfo *d_info = nullptr;
@@ -1919,8 +1927,6 @@ struct vg_pbatch {
 	uint64_t *d_ibase = nullptr;
 	uint64_t *d_stamp = nullptr;            /* finish time of each write item */
 	uint32_t *d_gq = nullptr;               /* query ids, group by group */
-	size_t gq_cap = 0;
-	pinned_buf h_gq;
 	size_t items_cap = 0;
 	uint64_t *d_status = nullptr;           /* single pass: a word per tile */
 	size_t status_cap = 0;
@@ -1934,7 +1940,6 @@ struct vg_pbatch {
 	/* pinned staging: uploads and the state readback are asynchronous on
 	 * the batch's stream (a synchronous copy would wait behind whatever
 	 * holds the compute units) */
-	pinned_buf h_progs, h_launch, h_info, h_items, h_witems;
 	/* every query's outcome, stored there by the kernels (coherent, so it
 	 * can be read once the stream is idle) */
 	pinned_buf h_res;
@@ -1984,8 +1989,7 @@ extern "C" void vg_pbatch_free(vg_pbatch *b)
 	refs_drop(b->qs);
 	pbatch_drop_graph(b);
 	pbatch_free_jit(b);
-	void *bufs[] = { b->d_progs, b->d_launch, b->d_info, b->d_q, b->d_head, b->d_items, b->d_witems,
-		b->d_icount, b->d_ibase, b->d_status, b->d_stamp, b->d_gq };
+	void *bufs[] = { b->d_blob, b->d_q, b->d_head, b->d_icount, b->d_ibase, b->d_status, b->d_stamp };
 	for (void *p : bufs)
 		if (p)
 			(void)hipFree(p);
@@ -1993,8 +1997,7 @@ extern "C" void vg_pbatch_free(vg_pbatch *b)
 		(void)hipEventDestroy(e);
 	for (hipStream_t s : b->branch)
 		(void)hipStreamDestroy(s);
-	pinned_buf *pins[] = { &b->h_progs, &b->h_launch, &b->h_info, &b->h_items, &b->h_witems, &b->h_res,
-		&b->h_gq };
+	pinned_buf *pins[] = { &b->h_blob, &b->h_res };
 	for (pinned_buf *pb : pins)
 		pb->release();
 	if (b->fork)
@@ -2175,9 +2178,7 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		return pbatch_build_jit(b, err, errlen);
 	pbatch_free_jit(b);
 	const unsigned n = (unsigned)b->qs.size();
-	/* a single-pass item is one tile: a tile waits for the counts of the
-	 * tiles before it, which must not sit behind other tiles of one item */
-	const uint32_t tpi = b->cfg.single ? 1u : b->cfg.tiles_per_item ? b->cfg.tiles_per_item : 8;
+	const uint32_t tpi = b->cfg.tiles_per_item ? b->cfg.tiles_per_item : 8;
 	/* groups, keyed by the write launch's workgroups per device */
 	std::vector<uint32_t> key(n, 0);
 	if (b->cfg.split == 0)
@@ -2198,10 +2199,10 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 		const uint32_t nt = b->qs[i]->p->num_tiles;
 		info[i].nitem = (nt + tpi - 1) / tpi;
 		info[i].ioff = ioff;
-		info[i].soff = soff;
+		info[i].soff = soff;           /* single pass: a status word per item */
 		info[i].wstart = 0;
 		ioff += info[i].nitem;
-		soff += nt;
+		soff += info[i].nitem;
 	}
 	const bool single = b->cfg.single != 0;
 	if (single && soff > b->status_cap) {
@@ -2251,19 +2252,14 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 	const size_t ni = items.size();
 	if (ni > b->items_cap) {
 		const size_t want = ni > 2 * b->items_cap ? ni : 2 * b->items_cap;
-		if (b->d_items) (void)hipFree(b->d_items);
-		if (b->d_witems) (void)hipFree(b->d_witems);
 		if (b->d_icount) (void)hipFree(b->d_icount);
 		if (b->d_ibase) (void)hipFree(b->d_ibase);
 		if (b->d_stamp) (void)hipFree(b->d_stamp);
-		b->d_items = b->d_witems = nullptr;
 		b->d_icount = nullptr;
 		b->d_ibase = nullptr;
 		b->d_stamp = nullptr;
 		b->items_cap = 0;
-		if (herr(hipMalloc((void **)&b->d_items, want * sizeof(vg_item)), "batch items") ||
-			herr(hipMalloc((void **)&b->d_witems, want * sizeof(vg_item)), "batch items") ||
-			herr(hipMalloc((void **)&b->d_icount, want * sizeof(uint32_t)), "batch counts") ||
+		if (herr(hipMalloc((void **)&b->d_icount, want * sizeof(uint32_t)), "batch counts") ||
 			herr(hipMalloc((void **)&b->d_ibase, want * sizeof(uint64_t)), "batch offsets") ||
 			herr(hipMalloc((void **)&b->d_stamp, want * sizeof(uint64_t)), "batch stamps"))
 			return qerr(err, errlen, "cannot allocate the batch's item lists");
@@ -2288,41 +2284,47 @@ static int pbatch_build(vg_pbatch *b, char *err, size_t errlen)
 			return qerr(err, errlen, "cannot create the batch's events");
 		b->join.push_back(e);
 	}
-	if (n > b->gq_cap) {
-		const size_t want = n > 2 * b->gq_cap ? n : 2 * b->gq_cap;
-		if (b->d_gq) (void)hipFree(b->d_gq);
-		b->d_gq = nullptr;
-		b->gq_cap = 0;
-		if (herr(hipMalloc((void **)&b->d_gq, want * sizeof(uint32_t)), "batch groups"))
-			return qerr(err, errlen, "cannot allocate the batch's groups");
-		b->gq_cap = want;
-	}
 	{
-		uint32_t *hg = (uint32_t *)b->h_gq.get(n * sizeof(uint32_t));
-		if (!hg)
+		/* one upload: programs, launch arguments, info, group lists, items */
+		auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+		const size_t o_launch = al(n * sizeof(vg_prog));
+		const size_t o_info = o_launch + al(n * sizeof(vg_launch));
+		const size_t o_gq = o_info + al(n * sizeof(vg_bqinfo));
+		const size_t o_items = o_gq + al(n * sizeof(uint32_t));
+		const size_t o_witems = o_items + (single ? 0 : al(ni * sizeof(vg_item)));
+		const size_t total = o_witems + al(ni * sizeof(vg_item));
+		if (total > b->blob_cap) {
+			const size_t want = total > 2 * b->blob_cap ? total : 2 * b->blob_cap;
+			if (b->d_blob) (void)hipFree(b->d_blob);
+			b->d_blob = nullptr;
+			b->blob_cap = 0;
+			if (herr(hipMalloc((void **)&b->d_blob, want), "batch"))
+				return qerr(err, errlen, "cannot allocate the batch");
+			b->blob_cap = want;
+		}
+		char *h = (char *)b->h_blob.get(total);
+		if (!h)
 			return qerr(err, errlen, "cannot allocate pinned staging");
+		memcpy(h, b->progs_h.data(), n * sizeof(vg_prog));
+		memcpy(h + o_launch, b->launch_h.data(), n * sizeof(vg_launch));
+		memcpy(h + o_info, info.data(), n * sizeof(vg_bqinfo));
+		uint32_t *hg = (uint32_t *)(h + o_gq);
 		size_t k = 0;
 		for (pgroup &g : b->groups)
 			for (unsigned q : g.qids)
 				hg[k++] = q;
-		if (herr(hipMemcpyAsync(b->d_gq, hg, n * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream), "groups"))
-			return qerr(err, errlen, "cannot upload the batch's groups");
-	}
-	{
-		void *hi = b->h_items.get(ni * sizeof(vg_item)), *hw = b->h_witems.get(ni * sizeof(vg_item));
-		void *hf = b->h_info.get(n * sizeof(vg_bqinfo));
-		if (!hi || !hw || !hf)
-			return qerr(err, errlen, "cannot allocate pinned staging");
-		memcpy(hi, items.data(), ni * sizeof(vg_item));
-		memcpy(hw, witems.data(), ni * sizeof(vg_item));
-		memcpy(hf, info.data(), n * sizeof(vg_bqinfo));
-		if ((ni && (herr(hipMemcpyAsync(b->d_items, hi, ni * sizeof(vg_item), hipMemcpyHostToDevice, b->stream),
-				"batch items") ||
-			herr(hipMemcpyAsync(b->d_witems, hw, ni * sizeof(vg_item), hipMemcpyHostToDevice, b->stream),
-				"batch items"))) ||
-			herr(hipMemcpyAsync(b->d_info, hf, n * sizeof(vg_bqinfo), hipMemcpyHostToDevice, b->stream),
-				"batch info"))
-			return qerr(err, errlen, "cannot upload the batch's item lists");
+		if (!single && ni)
+			memcpy(h + o_items, items.data(), ni * sizeof(vg_item));
+		if (ni)
+			memcpy(h + o_witems, witems.data(), ni * sizeof(vg_item));
+		b->d_progs = (vg_prog *)b->d_blob;
+		b->d_launch = (vg_launch *)(b->d_blob + o_launch);
+		b->d_info = (vg_bqinfo *)(b->d_blob + o_info);
+		b->d_gq = (uint32_t *)(b->d_blob + o_gq);
+		b->d_items = (vg_item *)(b->d_blob + (single ? o_witems : o_items));
+		b->d_witems = (vg_item *)(b->d_blob + o_witems);
+		if (herr(hipMemcpyAsync(b->d_blob, h, total, hipMemcpyHostToDevice, b->stream), "batch"))
+			return qerr(err, errlen, "cannot upload the batch");
 		/* graph replays read the lists later: they must have landed */
 		if (!b->cfg.direct && herr(hipStreamSynchronize(b->stream), "batch items"))
 			return qerr(err, errlen, "cannot upload the batch's item lists");
@@ -2393,14 +2395,11 @@ static int pbatch_set_queries(vg_pbatch *b, vg_query *const *qs, unsigned n, cha
 	for (unsigned i = 0; i < n; i++) {
 		vg_query *q = qs[i];
 		const uint32_t nt = q->p->num_tiles;
-		if (!b->cfg.single && q->pooled_ctrl)
+		/* both forms keep validity bytes and strip counts per tile */
+		if (q->pooled_ctrl)
 			take_pooled_lr(q->d, q->lr, nt, vg_kernel_grid(&q->P, nt ? nt : 1));
-		if (b->cfg.single) {
-			if (!q->lr.ctrl && ensure_lr(q->lr, nt, vg_kernel_grid(&q->P, nt ? nt : 1), q->stream) == VIRG_FAIL)
-				return qerr(err, errlen, "cannot set up query %u", i);
-		} else if (ensure_lr(q->lr, nt, vg_kernel_grid(&q->P, nt ? nt : 1), q->stream) == VIRG_FAIL) {
+		if (ensure_lr(q->lr, nt, vg_kernel_grid(&q->P, nt ? nt : 1), q->stream) == VIRG_FAIL)
 			return qerr(err, errlen, "cannot set up query %u", i);
-		}
 		progs[i] = q->P;
 		fill_launch(q, launches[i]);
 	}
@@ -2408,34 +2407,20 @@ static int pbatch_set_queries(vg_pbatch *b, vg_query *const *qs, unsigned n, cha
 		if (!qs[i]->pooled_ctrl && herr(hipStreamSynchronize(qs[i]->stream), "query setup"))
 			return qerr(err, errlen, "query setup failed");
 	if (n > b->qcap) {
-		void *bufs[] = { b->d_progs, b->d_launch, b->d_info, b->d_q };
-		for (void *p : bufs)
-			if (p)
-				(void)hipFree(p);
-		b->d_progs = nullptr;
-		b->d_launch = nullptr;
-		b->d_info = nullptr;
+		const size_t cap = n < 64 ? 64 : n > 2 * b->qcap ? n : 2 * b->qcap;
+		if (b->d_q)
+			(void)hipFree(b->d_q);
 		b->d_q = nullptr;
 		b->qcap = 0;
-		const size_t cap = n < 64 ? 64 : n > 2 * b->qcap ? n : 2 * b->qcap;
-		if (herr(hipMalloc((void **)&b->d_progs, cap * sizeof(vg_prog)), "batch") ||
-			herr(hipMalloc((void **)&b->d_launch, cap * sizeof(vg_launch)), "batch") ||
-			herr(hipMalloc((void **)&b->d_info, cap * sizeof(vg_bqinfo)), "batch") ||
-			herr(hipMalloc((void **)&b->d_q, cap * sizeof(vg_bquery)), "batch"))
+		if (herr(hipMalloc((void **)&b->d_q, cap * sizeof(vg_bquery)), "batch"))
 			return qerr(err, errlen, "cannot allocate the batch");
 		b->qcap = cap;
 	}
-	if (n) {
-		void *hp = b->h_progs.get(n * sizeof(vg_prog)), *hl = b->h_launch.get(n * sizeof(vg_launch));
-		if (!hp || !hl || !b->h_res.get(n * sizeof(vg_bresult), hipHostMallocCoherent))
-			return qerr(err, errlen, "cannot allocate pinned staging");
-		memcpy(hp, progs.data(), n * sizeof(vg_prog));
-		memcpy(hl, launches.data(), n * sizeof(vg_launch));
-		if (herr(hipMemcpyAsync(b->d_progs, hp, n * sizeof(vg_prog), hipMemcpyHostToDevice, b->stream), "batch") ||
-			herr(hipMemcpyAsync(b->d_launch, hl, n * sizeof(vg_launch), hipMemcpyHostToDevice, b->stream),
-				"batch"))
-			return qerr(err, errlen, "cannot upload the batch");
-	}
+	if (n && !b->h_res.get(n * sizeof(vg_bresult), hipHostMallocCoherent))
+		return qerr(err, errlen, "cannot allocate pinned staging");
+	/* uploaded with the item lists when the batch is built */
+	b->progs_h.swap(progs);
+	b->launch_h.swap(launches);
 	return VIRG_SUCCESS;
 }
 
@@ -2491,10 +2476,16 @@ extern "C" int vg_pbatch_reload(vg_pbatch *b, vg_query *const *qs, unsigned n, c
 	return pbatch_build(b, err, errlen);
 }
 
-/* The stream a batch runs on (for the serving loop). */
+/* The stream a batch runs on, and where its kernels store each query's
+ * outcome (pinned host memory, `done` last), for the serving loop. */
 extern "C" hipStream_t vgd_pbatch_stream(vg_pbatch *b)
 {
 	return b->stream;
+}
+
+extern "C" const vg_bresult *vgd_pbatch_outcomes(vg_pbatch *b)
+{
+	return (const vg_bresult *)b->h_res.p;
 }
 
 /* 1 when the batch's last run has finished (does not wait). */

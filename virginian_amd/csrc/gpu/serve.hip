@@ -45,6 +45,7 @@
 
 extern "C" int vgd_query_launch_lone(vg_query *q, hipStream_t stream, vg_bresult *res);
 extern "C" hipStream_t vgd_pbatch_stream(vg_pbatch *b);
+extern "C" const vg_bresult *vgd_pbatch_outcomes(vg_pbatch *b);
 
 namespace {
 
@@ -139,6 +140,7 @@ struct request {
 	vg_query *q = nullptr;
 	std::vector<unsigned> widths;
 	uint64_t table_rows = 0;
+	double seen = -1;           /* when its outcome was seen complete in a running batch */
 	std::vector<std::vector<char>> host;   /* kept results, one array per column */
 };
 
@@ -433,6 +435,7 @@ static void finish(vg_server *s, request &r, uint64_t total, unsigned flags)
 		if (r.exact == 0)
 			r.exact = 1;
 		r.cap = 0;
+		r.seen = -1;
 		r.reruns++;
 		s->st.reruns++;
 		release_buffers(s, r);
@@ -457,7 +460,9 @@ static void finish(vg_server *s, request &r, uint64_t total, unsigned flags)
 			(void)hipGetLastError();
 	}
 	r.state = R_DONE;
-	r.ready = vg_server_now(s);
+	/* complete in HBM when the batch stored its outcome (kept results:
+	 * once copied out) */
+	r.ready = !s->cfg.keep && r.seen >= 0 ? r.seen : vg_server_now(s);
 	s->done++;
 	vg_query_free(r.q);
 	r.q = nullptr;
@@ -480,8 +485,21 @@ static int collect(vg_server *s)
 			rn.reqs.clear();
 			s->st.collect_s += mono_now() - ta;
 		} else if (rn.busy == 2) {
-			if (!vg_pbatch_done(rn.pb))
+			if (!vg_pbatch_done(rn.pb)) {
+				/* the single pass stores each query's outcome when its
+				 * last chunk is written: note complete results early */
+				const vg_bresult *o = vgd_pbatch_outcomes(rn.pb);
+				double now = -1;
+				for (size_t i = 0; o && i < rn.reqs.size(); i++) {
+					request &r = s->reqs[rn.reqs[i]];
+					if (r.seen >= 0 || __atomic_load_n(&o[i].done, __ATOMIC_ACQUIRE) == 0 || o[i].flags)
+						continue;
+					if (now < 0)
+						now = vg_server_now(s);
+					r.seen = now;
+				}
 				continue;
+			}
 			const double ta = mono_now();
 			const size_t n = rn.reqs.size();
 			std::vector<unsigned long long> rows(n);
@@ -543,8 +561,13 @@ static bool admit(vg_server *s, std::deque<size_t> &line, runner &rn)
 		for (size_t i : take)
 			qs.push_back(s->reqs[i].q);
 		char e[256];
-		ok = vg_pbatch_reload(rn.pb, qs.data(), (unsigned)qs.size(), e, sizeof e) == VIRG_SUCCESS &&
-			vg_pbatch_launch(rn.pb) == VIRG_SUCCESS;
+		ok = vg_pbatch_reload(rn.pb, qs.data(), (unsigned)qs.size(), e, sizeof e) == VIRG_SUCCESS;
+		/* the last run's outcomes stay until this run's reset clears them:
+		 * clear them first, the runner being idle */
+		vg_bresult *o = (vg_bresult *)vgd_pbatch_outcomes(rn.pb);
+		if (ok && o)
+			memset(o, 0, qs.size() * sizeof(vg_bresult));
+		ok = ok && vg_pbatch_launch(rn.pb) == VIRG_SUCCESS;
 		if (!ok)
 			snprintf(s->msg, sizeof s->msg, "%s", e);
 		rn.busy = 2;

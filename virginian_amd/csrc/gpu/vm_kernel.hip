@@ -1244,21 +1244,23 @@ __device__ __forceinline__ uint64_t lookback(uint64_t *st, uint32_t tile, uint32
 	return sum;
 }
 
-/* Single pass over tile items (each query's tiles listed in order): the 4
- * waves load every column the program uses for their strips of the tile,
- * run the whole program once, count, take the tile's output offset by
- * look-back, and write the selected rows there.  The output capacity is
- * the caller's estimate; past it rows are dropped and VG_OVF_CAPACITY set,
- * and the caller runs the exact two passes instead. */
+/* Single pass over chunk items (each query's chunks of tiles listed in
+ * order, taken in order through the query's own counter): the workgroup
+ * counts the chunk as the count pass does (validity bytes and per-strip
+ * counts into the query's two-pass arrays), publishes the chunk's count at
+ * once, takes the chunk's output offset by look-back over the query's
+ * earlier chunks (one status word per chunk), and writes the chunk's
+ * selected rows there as the write pass does, reading the filter columns
+ * again from L2.  The output capacity is the caller's estimate; past it
+ * rows are dropped and VG_OVF_CAPACITY set (the exact count is still
+ * produced).  The workgroup finishing a query's last chunk stores the
+ * query's outcome at once. */
 __global__ __launch_bounds__(256) void vg_batch_single_kernel(const vg_batch_args A)
 {
-	constexpr uint32_t T = VG_T;
 	extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
 	__shared__ vg_item s_item;
-	__shared__ uint32_t s_wc[4];
 	__shared__ uint64_t s_pre;
 	const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
-	const strips<1> S{wave * 256u, 256u, lane};
 	uint32_t cur = A.ngq ? blockIdx.x % A.ngq : 0;
 	for (;;) {
 		uint32_t q, t0, t1, k;
@@ -1268,80 +1270,26 @@ __global__ __launch_bounds__(256) void vg_batch_single_kernel(const vg_batch_arg
 		const vg_prog &P = A.progs[q];
 		const vg_launch &L = A.launch[q];
 		uint64_t *st = A.status + ldc(&A.info[q].soff);
-		uint32_t *regs = lds;
-		uint16_t *srcrow = (uint16_t *)(lds + ldc(&P.nslot) * T) + wave * 256;
-		cursor C;
-		cursor_init(C, P, lane);
-		for (uint32_t tile = t0; tile < t1; ++tile) {
-			cursor_seek(C, P, L, tile, lane);
-			const uint32_t row0 = (tile - C.first) * T;
-			const uint32_t left = C.rows - row0;
-			const uint32_t nrows = left < T ? left : T;
-			uint32_t live = 0xFu;
-			if (nrows < T) {
-				live = 0;
-#pragma unroll
-				for (int j = 0; j < 4; ++j)
-					live |= S.row(0) + j < nrows ? 1u << j : 0u;
-			}
-			/* the filter first: the tile's count goes out before any
-			 * projection work, so the tiles after it wait less */
-			rowstate RS;
-			rows_init(RS, live);
-			load_cols<1, T>(P, C, row0, S, nrows, VG_LOAD_FILTER, regs);
-			const uint32_t nf = ldc(&P.nfilter), nu = ldc(&P.nuop);
-			run_uops<1, T>(P, 0, nf, ~0ull, regs, S, RS);
-			const uint32_t mybits = RS.valid & 0xFu;
-			const uint32_t wc = wave_sum((uint32_t)__builtin_popcount(mybits));
+		const uint32_t nitem = ldc(&A.info[q].nitem);
+		const uint32_t n = count_tiles<1>(P, L, t0, t1, lds);
+		if (wave == 0) {
+			const uint64_t pre = lookback(st, k, n, lane, A.spin_limit);
 			if (lane == 0)
-				s_wc[wave] = wc;
-			__syncthreads();
-			const uint32_t n = s_wc[0] + s_wc[1] + s_wc[2] + s_wc[3];
-			if (wave == 0) {
-				const uint64_t pre = lookback(st, tile, n, lane, A.spin_limit);
-				if (lane == 0)
-					s_pre = pre;
-			}
-			__syncthreads();
-			const uint64_t pre = s_pre;
-			if (pre == ~0ull) {
-				if (tid == 0)
-					atomicOr(&ldc(&L.ctrl)->overflow, VG_OVF_WAIT);
-			} else if (wc != 0) {
-				/* the rest of the program, only where rows were selected:
-				 * validity no longer changes after micro-op nf */
-				if (nu > nf) {
-					load_cols<1, T>(P, C, row0, S, nrows, VG_LOAD_PROJECT, regs);
-					run_uops<1, T>(P, nf, nu, ~0ull, regs, S, RS);
-				}
-				uint32_t wave_off = 0;
-				for (uint32_t w = 0; w < wave; ++w)
-					wave_off += s_wc[w];
-				uint32_t pos = 0;
-#pragma unroll
-				for (int r = 0; r < 4; ++r)
-					pos += rank_below(__ballot((mybits >> r) & 1u));
-#pragma unroll
-				for (int r = 0; r < 4; ++r)
-					if ((mybits >> r) & 1u)
-						srcrow[pos++] = (uint16_t)(S.row(0) + r);
-				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-				__builtin_amdgcn_wave_barrier();
-				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-				const uint64_t at = pre + uni(wave_off);
-				const uint64_t cap = ldc(&L.capacity);
-				const uint64_t room = cap > at ? cap - at : 0;
-				const uint32_t m = (uint64_t)wc < room ? wc : (uint32_t)room;
-				if (m < wc && lane == 0)
-					atomicOr(&ldc(&L.ctrl)->overflow, VG_OVF_CAPACITY);
-				write_rows(P, L, C, row0, regs, srcrow, at, m, lane);
-			}
-			if (pre != ~0ull && tile + 1 == ldc(&L.num_tiles) && tid == 0)
-				ldc(&L.ctrl)->total = pre + n;
-			__syncthreads();        /* registers, srcrow, s_wc, s_pre are reused */
+				s_pre = pre;
 		}
-		if (tid == 0 && __hip_atomic_fetch_add(&A.q[q].wdone, t1 - t0, __ATOMIC_ACQ_REL,
-				__HIP_MEMORY_SCOPE_AGENT) + (t1 - t0) == ldc(&L.num_tiles))
+		__syncthreads();
+		const uint64_t pre = s_pre;
+		if (pre == ~0ull) {
+			if (tid == 0)
+				atomicOr(&ldc(&L.ctrl)->overflow, VG_OVF_WAIT);
+		} else if (n != 0) {
+			write_tiles(P, L, t0, t1, pre, lds);
+		}
+		if (pre != ~0ull && k + 1 == nitem && tid == 0)
+			ldc(&L.ctrl)->total = pre + n;
+		__syncthreads();            /* the write phase's stores before the count below */
+		if (tid == 0 && __hip_atomic_fetch_add(&A.q[q].wdone, 1u, __ATOMIC_ACQ_REL,
+				__HIP_MEMORY_SCOPE_AGENT) + 1 == nitem)
 			finish_query(A, q);
 		__syncthreads();            /* see the write kernel: keeps the fetch in the loop */
 	}
