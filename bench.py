@@ -418,11 +418,14 @@ def bench_served(a) -> int:
     log-uniform 1e5-5e7 rows; continuous batching with admission by the
     HBM arena; results complete in HBM.  The predictor (3-layer MLP, 32
     hidden, f32) is trained on the GPU on per-tablet counts from two-pass
-    runs of other programs, its margin calibrated on held-out programs for
-    at most --rerun-target reruns (0.5 %: the p99 latency then excludes
-    them).  value = the predictor-sized two-pass server;
-    reported beside the predictor-sized single pass and the same stream
-    served with buffers for every table row (the method without it)."""
+    runs of other programs, for the --quantile quantile of each tablet's
+    selectivity (pinball loss), its margin calibrated on held-out programs
+    for at most --rerun-target reruns; a second network trained on the
+    mean absolute error gives the accuracy figure.  value = the
+    predictor-sized server; reported beside the same stream sized by the
+    independence estimate alone (its own margin, same target), the
+    predictor-sized single pass, buffers for every table row and plain
+    launches."""
     import numpy as np
     import virginian_amd as va
     from virginian_amd import predict as PR
@@ -459,11 +462,13 @@ def bench_served(a) -> int:
     x, t = PR.collect(db, programs(a.train_programs - nval), stats)
     xv, tv = PR.collect(db, programs(nval), stats)
     t1 = time.perf_counter()
-    net = PR.MLP(seed=a.seed)
-    fit = net.fit(x, t, xv, tv)
-    loss = fit["validation_mae"]
+    net = PR.MLP(seed=a.seed, loss="quantile", tau=a.quantile)
+    fitq = net.fit(x, t, xv, tv)
     t2 = time.perf_counter()
-    held = programs(200)
+    mnet = PR.MLP(seed=a.seed)
+    fit = mnet.fit(x, t, xv, tv)
+    loss = fit["validation_mae"]
+    held = programs(a.calib_programs)
     hx, ht = PR.collect(db, held, stats)
     pred = net.forward(hx)
     preds, true_rows, rowsl, k = [], [], [], 0
@@ -483,13 +488,15 @@ def bench_served(a) -> int:
         est.append(hx[k:k + n, 3])
         k += n
     margin_est = PR.calibrate(est, true_rows, rowsl, a.rerun_target)
-    mae = float(np.mean(np.abs(pred - ht)))
+    mae = float(np.mean(np.abs(mnet.forward(hx) - ht)))
+    mnet.close()
     mae_est = float(np.mean(np.abs(hx[:, 3] - ht)))
     mae_const = float(np.mean(np.abs(float(np.mean(t)) - ht)))
-    print(f"# predictor: {len(x)} + {len(xv)} examples collected in {t1 - t0:.1f}s, trained in {t2 - t1:.2f}s "
-          f"({fit['best_epoch']} of {fit['epochs']} epochs, validation MAE {loss:.4f} from "
-          f"{fit['validation_mae_start']:.4f}); held-out MAE {mae:.4f} (independence estimate {mae_est:.4f}, "
-          f"mean selectivity {mae_const:.4f}); margin {margin:.3f} (estimate alone {margin_est:.3f})",
+    print(f"# predictor: {len(x)} + {len(xv)} examples collected in {t1 - t0:.1f}s; sizing network "
+          f"(quantile {a.quantile}) trained in {t2 - t1:.2f}s ({fitq['best_epoch']} of {fitq['epochs']} epochs); "
+          f"point network validation MAE {loss:.4f} from {fit['validation_mae_start']:.4f}, held-out MAE "
+          f"{mae:.4f} (independence estimate {mae_est:.4f}, mean selectivity {mae_const:.4f}); margin "
+          f"{margin:.3f} (estimate alone {margin_est:.3f}) on {len(held)} held-out programs",
           file=sys.stderr, flush=True)
 
     n_req = int(a.rate * a.duration)
@@ -542,7 +549,8 @@ def bench_served(a) -> int:
                        "op-programs (4-40 ops) over 10 resident tables of log-uniform 1e5-5e7 rows",
            "table_rows": sizes, "requests": n_req, "arena_gib": a.arena_gib,
            "server": {"runners": a.runners, "max_batch": a.max_batch, "split": a.split, "grid_div": a.grid_div},
-           "predictor": {"examples": int(len(x)), "train_s": round(t2 - t1, 3), "loss": round(loss, 6),
+           "predictor": {"examples": int(len(x)), "train_s": round(t2 - t1, 3), "quantile": a.quantile,
+                         "point_network_validation_mae": round(loss, 6),
                          "heldout_mae": round(mae, 4), "independence_estimate_mae": round(mae_est, 4),
                          "mean_selectivity_mae": round(mae_const, 4), "margin": margin,
                          "estimate_margin": margin_est}}
@@ -778,7 +786,9 @@ def main() -> int:
     ap.add_argument("--arena-gib", type=int, default=8, help="served: HBM for result buffers")
     ap.add_argument("--rerun-target", type=float, default=0.005,
                     help="served: rerun rate the predictor's margin is calibrated for")
-    ap.add_argument("--train-programs", type=int, default=600, help="served: predictor training runs")
+    ap.add_argument("--train-programs", type=int, default=1600, help="served: predictor training runs")
+    ap.add_argument("--calib-programs", type=int, default=600, help="served: held-out runs for the margins")
+    ap.add_argument("--quantile", type=float, default=0.9, help="served: the sizing network's quantile")
     ap.add_argument("--runners", type=int, default=3, help="served: runs in flight at once (streams)")
     ap.add_argument("--max-batch", type=int, default=64, help="served: requests per run")
     ap.add_argument("--single-runners", type=int, default=1, help="served: runners of the single-pass server")

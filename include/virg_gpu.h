@@ -313,6 +313,10 @@ int vg_mlp_gradient(vg_mlp *m, const float *x, const float *t, unsigned n, float
 /* The loss vg_mlp_gradient and vg_mlp_train use: 0 mean squared error
  * (the default), 1 mean absolute error. */
 int vg_mlp_set_loss(vg_mlp *m, int kind);
+/* The pinball loss of quantile tau, mean over examples of
+ * max(tau (t - y), (1 - tau) (y - t)) (0 < tau < 1): the network learns
+ * the tau-quantile of the target, an upper bound where tau is high. */
+int vg_mlp_set_quantile(vg_mlp *m, float tau);
 int vg_mlp_train(vg_mlp *m, const float *x, const float *t, unsigned n, unsigned epochs, unsigned batch,
 	float lr, unsigned long long seed, float *loss);
 

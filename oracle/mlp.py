@@ -5,7 +5,8 @@ network with its own HIP kernels, virginian_amd/csrc/gpu/mlp.hip).
 The network has no counterpart in the reference, so there is no reference
 file to follow.  Its definition is the textbook one:
     h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), y = sigmoid(h2 W3 + b3)
-    loss = mean((y - t)^2), or mean(|y - t|)
+    loss = mean((y - t)^2), mean(|y - t|), or the pinball loss of quantile
+    tau, mean(max(tau (t - y), (1 - tau) (y - t))) (Koenker & Bassett 1978)
 with parameters packed flat as W1[nin][nh] b1[nh] W2[nh][nh] b2[nh] W3[nh] b3,
 and optionally `skip` times the first input added to the output's
 pre-activation (the network then learns a correction to an estimate given
@@ -48,21 +49,32 @@ def forward(p: np.ndarray, x: np.ndarray, nin: int, nh: int, skip: float = 0.0):
     return y, (h1, h2)
 
 
-def loss(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int, skip: float = 0.0,
-         absolute: bool = False) -> float:
-    y, _ = forward(p, x, nin, nh, skip)
+def _loss_of(y: np.ndarray, t: np.ndarray, absolute: bool, tau: float | None) -> float:
+    if tau is not None:
+        d = y - t
+        return float(np.mean(np.where(d > 0, (1.0 - tau) * d, -tau * d)))
     return float(np.mean(np.abs(y - t))) if absolute else float(np.mean((y - t) ** 2))
 
 
+def loss(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int, skip: float = 0.0,
+         absolute: bool = False, tau: float | None = None) -> float:
+    y, _ = forward(p, x, nin, nh, skip)
+    return _loss_of(y, t, absolute, tau)
+
+
 def gradient(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int, skip: float = 0.0,
-             absolute: bool = False):
+             absolute: bool = False, tau: float | None = None):
     """(loss, gradient packed like p) by backpropagation; the loss is the
-    mean squared error, or with absolute=True the mean absolute error
-    (derivative sign(y - t), 0 where y = t)."""
+    mean squared error, with absolute=True the mean absolute error
+    (derivative sign(y - t), 0 where y = t), with tau the pinball loss
+    (derivative 1 - tau above the target, -tau below it, 0 on it)."""
     w = unpack(p, nin, nh)
     y, (h1, h2) = forward(p, x, nin, nh, skip)
     B = x.shape[0]
-    dl = np.sign(y - t) if absolute else 2.0 * (y - t)
+    if tau is not None:
+        dl = np.where(y > t, 1.0 - tau, np.where(y < t, -tau, 0.0))
+    else:
+        dl = np.sign(y - t) if absolute else 2.0 * (y - t)
     d3 = (dl / B * y * (1.0 - y))[:, None]                 # dL/dz3
     gW3 = h2.T @ d3
     gb3 = d3.sum(axis=0)
@@ -73,4 +85,4 @@ def gradient(p: np.ndarray, x: np.ndarray, t: np.ndarray, nin: int, nh: int, ski
     gW1 = x.T @ d1
     gb1 = d1.sum(axis=0)
     g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
-    return (float(np.mean(np.abs(y - t))) if absolute else float(np.mean((y - t) ** 2))), g
+    return _loss_of(y, t, absolute, tau), g

@@ -17,12 +17,13 @@ from virginian_amd import workload as W
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("loss", ["squared", "absolute"])
+@pytest.mark.parametrize("loss", ["squared", "absolute", "quantile"])
 def test_forward_and_gradient_match_oracle(gpu_db, loss):
     absolute = loss == "absolute"
+    tau = 0.85 if loss == "quantile" else None
     for nin, nh, B, seed, skip in [(7, 5, 9, 1, 0.0), (PR.NFEATURES, 32, 300, 2, PR.SKIP), (40, 32, 1, 3, 0.0),
                                    (12, 32, 64, 4, 2.5)]:
-        net = PR.MLP(nin, nh, seed=seed, skip=skip, loss=loss)
+        net = PR.MLP(nin, nh, seed=seed, skip=skip, loss=loss, tau=tau or 0.9)
         rng = np.random.default_rng(seed)
         # biases away from zero so that ReLUs are on and off
         p = net.params().astype(np.float64)
@@ -39,7 +40,8 @@ def test_forward_and_gradient_match_oracle(gpu_db, loss):
         assert np.allclose(yh, y_ref, rtol=1e-4, atol=1e-6), (nin, np.abs(yh - y_ref).max())
         assert np.allclose(yh, y, rtol=1e-5, atol=1e-6), (nin, np.abs(yh - y).max())
         lv, g = net.gradient(x, t)
-        loss_ref, g_ref = M.gradient(p, x.astype(np.float64), t.astype(np.float64), nin, nh, skip, absolute)
+        loss_ref, g_ref = M.gradient(p, x.astype(np.float64), t.astype(np.float64), nin, nh, skip, absolute,
+                                     tau=tau)
         assert np.isclose(lv, loss_ref, rtol=1e-4)
         scale = np.abs(g_ref).max()
         assert np.allclose(g, g_ref, rtol=1e-4, atol=1e-5 * scale), (nin, np.abs(g - g_ref).max(), scale)
@@ -86,8 +88,11 @@ def learned(gpu_db):
     net = PR.MLP(seed=7)
     fit = net.fit(x, t, xv, tv)
     print("fit", fit, flush=True)
-    yield gpu_db, net, stats, train, held, (x, t)
+    qnet = PR.MLP(seed=7, loss="quantile", tau=0.9)
+    print("quantile fit", qnet.fit(x, t, xv, tv), flush=True)
+    yield gpu_db, net, stats, train, held, (x, t), qnet
     net.close()
+    qnet.close()
 
 
 def test_predictor_gains_over_sizing_without_it(learned):
@@ -95,7 +100,7 @@ def test_predictor_gains_over_sizing_without_it(learned):
     the network reserve less than buffers sized from the mean selectivity
     of the training runs (the method without it), and far less than the
     table (what a single pass needs with no estimate at all)."""
-    db, net, stats, train, held, (x, t) = learned
+    db, net, stats, train, held, (x, t), qnet = learned
     hx, ht = PR.collect(db, held, stats)
     mae_net = float(np.mean(np.abs(net.forward(hx) - ht)))
     mae_const = float(np.mean(np.abs(np.mean(t) - ht)))
@@ -127,10 +132,41 @@ def test_predictor_gains_over_sizing_without_it(learned):
     assert res_net < 0.7 * res_const
 
 
+def test_quantile_sizing_reserves_less_than_the_estimate(learned):
+    """What the network gains over the method it corrects: trained for the
+    0.9-quantile of each tablet's selectivity, it sizes buffers by how
+    uncertain each program is, so at the same overflow rate on held-out
+    programs (margins calibrated for each method alike) it reserves fewer
+    rows than the independence estimate of the histograms."""
+    db, net, stats, train, held, (x, t), qnet = learned
+    hx, ht = PR.collect(db, held, stats)
+    pq, pe, true_rows, rows, k = [], [], [], [], 0
+    yq = qnet.forward(hx)
+    for ops in held:
+        st = stats[PR.table_of(ops)]
+        n = len(st.rows)
+        pq.append(yq[k:k + n])
+        pe.append(hx[k:k + n, 3])
+        true_rows.append(float((ht[k:k + n] * st.rows).sum()))
+        rows.append(st.rows)
+        k += n
+    for target in (0.02, 0.05):
+        m_q = PR.calibrate(pq, true_rows, rows, target)
+        m_e = PR.calibrate(pe, true_rows, rows, target)
+        res_q = sum(PR.capacity(p, r, m_q) for p, r in zip(pq, rows))
+        res_e = sum(PR.capacity(p, r, m_e) for p, r in zip(pe, rows))
+        over_q = sum(PR.capacity(p, r, m_q) < n for p, n, r in zip(pq, true_rows, rows))
+        over_e = sum(PR.capacity(p, r, m_e) < n for p, n, r in zip(pe, true_rows, rows))
+        print(f"<= {target:.0%} overflow: quantile network {res_q} rows (margin {m_q:.3f}, {over_q} over), "
+              f"estimate {res_e} rows (margin {m_e:.3f}, {over_e} over), selected {sum(true_rows):.0f}")
+        assert over_q <= target * len(held) and over_e <= target * len(held)
+        assert res_q < res_e, (target, res_q, res_e)
+
+
 def test_single_pass_with_predicted_sizes(learned):
     """Single pass into network-sized buffers; a query that did not fit is
     rerun in two passes; every result equals the oracle's."""
-    db, net, stats, train, held, _ = learned
+    db, net, stats, train, held, _, _ = learned
     qs, want = [], []
     for ops in held[:40]:
         st = stats[PR.table_of(ops)]

@@ -59,6 +59,36 @@ def test_oracle_absolute_error_gradient_matches_finite_differences():
         assert not np.allclose(g, g2, rtol=1e-3)
 
 
+def test_oracle_pinball_gradient_matches_finite_differences():
+    """Pinball loss of quantile tau: away from y = t the gradient is
+    backpropagated 1 - tau (above) / -tau (below); tau = 1/2 is half the
+    absolute error; and at zero weights (y = 1/2) only b3 moves, by the
+    closed form mean(dl) / 4."""
+    for seed, tau in zip(range(4), (0.1, 0.5, 0.9, 0.97)):
+        p, x, t, nin, nh = _case(seed + 40)
+        y, _ = M.forward(p, x, nin, nh)
+        assert np.abs(y - t).min() > 1e-4
+        _, g = M.gradient(p, x, t, nin, nh, 0.0, tau=tau)
+        fd = np.zeros_like(p)
+        for i in range(len(p)):
+            a, b = p.copy(), p.copy()
+            a[i] += 1e-6
+            b[i] -= 1e-6
+            fd[i] = (M.loss(a, x, t, nin, nh, tau=tau) - M.loss(b, x, t, nin, nh, tau=tau)) / 2e-6
+        assert np.allclose(g, fd, rtol=1e-5, atol=1e-8), (seed, tau, np.abs(g - fd).max())
+    p, x, t, nin, nh = _case(45)
+    assert np.isclose(M.loss(p, x, t, nin, nh, tau=0.5), 0.5 * M.loss(p, x, t, nin, nh, absolute=True))
+    nin, nh = 3, 2
+    p = np.zeros(M.layout(nin, nh)["n"])
+    x = np.ones((4, nin))
+    t = np.array([0.0, 1.0, 0.25, 0.75])
+    loss, g = M.gradient(p, x, t, nin, nh, tau=0.8)
+    assert np.isclose(loss, np.mean([0.2 * 0.5, 0.8 * 0.5, 0.2 * 0.25, 0.8 * 0.25]))
+    L = M.layout(nin, nh)
+    assert np.isclose(g[L["b3"][0]], np.mean([0.2, -0.8, 0.2, -0.8]) * 0.25)
+    assert np.count_nonzero(g) == 1
+
+
 def test_finite_difference_check_catches_a_dropped_relu_mask():
     p, x, t, nin, nh = _case(11)
     w = M.unpack(p, nin, nh)

@@ -37,7 +37,8 @@ struct mlp_work {
 struct vg_mlp {
 	unsigned nin = 0, nh = 0, np = 0;
 	float skip = 0.0f;           /* output pre-activation += skip * x[0] */
-	int loss = 0;                /* 0: mean squared error, 1: mean absolute error */
+	int loss = 0;                /* 0: mean squared error, 1: mean absolute error, 2: pinball */
+	float tau = 0.5f;            /* the pinball loss's quantile */
 	float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
 	unsigned long long step = 0;
 	/* inference: a stream that does not wait for other streams' work,
@@ -95,15 +96,24 @@ __global__ void mlp_layer_fwd(const float *in, unsigned nin, const float *W, con
 	out[t] = act == 0 ? (z > 0.0f ? z : 0.0f) : 1.0f / (1.0f + expf(-z));
 }
 
-/* output delta: d(loss)/dz3 = 2 (y - t) / B * y (1 - y) for the squared
- * error, sign(y - t) / B * y (1 - y) for the absolute error */
-__global__ void mlp_out_delta(const float *y, const float *target, unsigned B, float *dz, int absolute)
+/* output delta: d(loss)/dz3 = dl / B * y (1 - y), dl = d(loss of one
+ * example)/dy: 2 (y - t) for the squared error, sign(y - t) for the
+ * absolute error, and for the pinball loss of quantile tau,
+ * max(tau (t - y), (1 - tau) (y - t)): 1 - tau above the target, -tau
+ * below it (0 on it) */
+__global__ void mlp_out_delta(const float *y, const float *target, unsigned B, float *dz, int kind, float tau)
 {
 	const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
 	if (b >= B)
 		return;
 	const float yy = y[b], r = yy - target[b];
-	const float dl = absolute ? (r > 0.0f ? 1.0f : r < 0.0f ? -1.0f : 0.0f) : 2.0f * r;
+	float dl;
+	if (kind == 1)
+		dl = r > 0.0f ? 1.0f : r < 0.0f ? -1.0f : 0.0f;
+	else if (kind == 2)
+		dl = r > 0.0f ? 1.0f - tau : r < 0.0f ? -tau : 0.0f;
+	else
+		dl = 2.0f * r;
 	dz[b] = dl / (float)B * yy * (1.0f - yy);
 }
 
@@ -141,14 +151,14 @@ __global__ void mlp_wgrad(const float *in, unsigned nin, const float *dz, unsign
 	}
 }
 
-/* mean squared (or absolute) error into *loss (one workgroup of 256) */
-__global__ void mlp_loss(const float *y, const float *target, unsigned B, float *loss, int absolute)
+/* mean loss into *loss (one workgroup of 256) */
+__global__ void mlp_loss(const float *y, const float *target, unsigned B, float *loss, int kind, float tau)
 {
 	__shared__ float part[256];
 	float s = 0.0f;
 	for (unsigned b = threadIdx.x; b < B; b += 256) {
 		const float d = y[b] - target[b];
-		s += absolute ? fabsf(d) : d * d;
+		s += kind == 1 ? fabsf(d) : kind == 2 ? (d > 0.0f ? (1.0f - tau) * d : -tau * d) : d * d;
 	}
 	part[threadIdx.x] = s;
 	__syncthreads();
@@ -227,7 +237,7 @@ static void backward(vg_mlp *m, mlp_work &w, unsigned B, const float *x, const f
 	const float *P = m->d_p;
 	float *G = m->d_g;
 	const unsigned h = m->nh;
-	hipLaunchKernelGGL(mlp_out_delta, dim3(blocks(B)), dim3(256), 0, 0, w.y, t, B, w.d3, m->loss);
+	hipLaunchKernelGGL(mlp_out_delta, dim3(blocks(B)), dim3(256), 0, 0, w.y, t, B, w.d3, m->loss, m->tau);
 	hipLaunchKernelGGL(mlp_wgrad, dim3(blocks((size_t)(h + 1) * 1)), dim3(256), 0, 0,
 		w.h2, h, w.d3, 1u, B, G + L.w3, G + L.b3);
 	hipLaunchKernelGGL(mlp_hidden_delta, dim3(blocks((size_t)B * h)), dim3(256), 0, 0,
@@ -287,6 +297,17 @@ extern "C" int vg_mlp_set_loss(vg_mlp *m, int kind)
 	if (kind != 0 && kind != 1)
 		return VIRG_FAIL;
 	m->loss = kind;
+	return VIRG_SUCCESS;
+}
+
+/* The pinball loss of quantile tau (0 < tau < 1): the network then
+ * predicts the tau-quantile of the target given the features. */
+extern "C" int vg_mlp_set_quantile(vg_mlp *m, float tau)
+{
+	if (!(tau > 0.0f && tau < 1.0f))
+		return VIRG_FAIL;
+	m->loss = 2;
+	m->tau = tau;
 	return VIRG_SUCCESS;
 }
 
@@ -447,7 +468,7 @@ extern "C" int vg_mlp_gradient(vg_mlp *m, const float *x, const float *t, unsign
 	if (ok) {
 		forward(m, w, n, w.x);
 		backward(m, w, n, w.x, w.t);
-		hipLaunchKernelGGL(mlp_loss, dim3(1), dim3(256), 0, 0, w.y, w.t, n, w.loss, m->loss);
+		hipLaunchKernelGGL(mlp_loss, dim3(1), dim3(256), 0, 0, w.y, w.t, n, w.loss, m->loss, m->tau);
 		ok = hipGetLastError() == hipSuccess &&
 			hipMemcpy(loss, w.loss, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess &&
 			hipMemcpy(grad, m->d_g, m->np * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
@@ -500,7 +521,7 @@ extern "C" int vg_mlp_train(vg_mlp *m, const float *x, const float *t, unsigned 
 		}
 	if (ok) {
 		forward(m, w, n, w.x);
-		hipLaunchKernelGGL(mlp_loss, dim3(1), dim3(256), 0, 0, w.y, w.t, n, w.loss, m->loss);
+		hipLaunchKernelGGL(mlp_loss, dim3(1), dim3(256), 0, 0, w.y, w.t, n, w.loss, m->loss, m->tau);
 		ok = hipGetLastError() == hipSuccess &&
 			hipMemcpy(loss, w.loss, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
 	}

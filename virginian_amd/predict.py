@@ -128,19 +128,33 @@ class MLP:
     """The predictor network on the GPU (vg_mlp_* in include/virg_gpu.h)."""
 
     def __init__(self, nin: int = NFEATURES, nh: int = 32, seed: int = 1, skip: float = SKIP,
-                 loss: str = "absolute"):
-        """loss: "absolute" (mean absolute error, what sizing is judged by;
-        the squared error pulls predictions toward the mean of skewed
-        selectivities) or "squared"."""
+                 loss: str = "absolute", tau: float = 0.9):
+        """loss: "absolute" (mean absolute error, what an estimate is
+        judged by; the squared error pulls predictions toward the mean of
+        skewed selectivities), "squared", or "quantile" (the pinball loss
+        of quantile tau: the network learns an upper bound whose width
+        follows how uncertain each program is, which is what sizing
+        buffers for a target rerun rate needs)."""
         _bind()
         self.nin, self.nh, self.skip = nin, nh, skip
         self.absolute = loss == "absolute"
+        self.tau = tau if loss == "quantile" else None
         self.h = lib.vg_mlp_create(nin, nh, skip, seed)
         if not self.h:
             raise VirginianError("vg_mlp_create failed (no device?)")
-        if lib.vg_mlp_set_loss(self.h, 1 if self.absolute else 0) != VIRG_SUCCESS:
-            raise VirginianError("vg_mlp_set_loss failed")
+        ok = (lib.vg_mlp_set_quantile(self.h, tau) if self.tau is not None
+              else lib.vg_mlp_set_loss(self.h, 1 if self.absolute else 0))
+        if ok != VIRG_SUCCESS:
+            raise VirginianError("setting the network's loss failed")
         self.nparams = lib.vg_mlp_num_params(self.h)
+
+    def criterion(self, y: np.ndarray, t: np.ndarray) -> float:
+        """The validation criterion: mean absolute error, or for a quantile
+        network its pinball loss."""
+        if self.tau is None:
+            return float(np.mean(np.abs(y - t)))
+        d = y.astype(np.float64) - t
+        return float(np.mean(np.where(d > 0, (1.0 - self.tau) * d, -self.tau * d)))
 
     def params(self) -> np.ndarray:
         p = np.empty(self.nparams, dtype=np.float32)
@@ -200,18 +214,19 @@ class MLP:
     def fit(self, x: np.ndarray, t: np.ndarray, xv: np.ndarray, tv: np.ndarray, max_epochs: int = 300,
             patience: int = 60, batch: int = 256, lr: float = 2e-3, seed: int = 1) -> dict:
         """Train epoch by epoch (vg_mlp_train, Adam state carried over) and
-        keep the parameters with the least mean absolute error on the
-        validation examples (xv, tv: tablets of programs kept out of x);
+        keep the parameters with the least criterion (mean absolute error,
+        or the pinball loss of a quantile network) on the validation
+        examples (xv, tv: tablets of programs kept out of x);
         stop after `patience` epochs without improvement (the first tens
         of epochs, while the correction grows from zero, may not improve
         it).  A few hundred programs are easy to memorise: without this
         the held-out error rises above the estimate the network corrects."""
-        best = float(np.mean(np.abs(self.forward_host(xv) - tv)))
+        best = self.criterion(self.forward_host(xv), tv)
         start = best
         best_p, best_epoch, epoch = self.params(), 0, 0
         for epoch in range(1, max_epochs + 1):
             self.train(x, t, epochs=1, batch=batch, lr=lr, seed=seed + epoch)
-            mae = float(np.mean(np.abs(self.forward_host(xv) - tv)))
+            mae = self.criterion(self.forward_host(xv), tv)
             if mae < best:
                 best, best_p, best_epoch = mae, self.params(), epoch
             elif epoch - best_epoch >= patience:
@@ -251,6 +266,7 @@ def _bind() -> None:
         "vg_mlp_forward_host": (C.c_int, [P, F, C.c_uint, F]),
         "vg_mlp_gradient": (C.c_int, [P, F, F, C.c_uint, C.POINTER(C.c_float), F]),
         "vg_mlp_set_loss": (C.c_int, [P, C.c_int]),
+        "vg_mlp_set_quantile": (C.c_int, [P, C.c_float]),
         "vg_mlp_train": (C.c_int, [P, F, F, C.c_uint, C.c_uint, C.c_uint, C.c_float, C.c_ulonglong,
                                    C.POINTER(C.c_float)]),
         "vg_query_tablet_rows": (C.c_uint, [P, C.POINTER(C.c_ulonglong), C.c_uint]),
