@@ -513,7 +513,7 @@ def bench_served(a) -> int:
                        margin=margin_est if mode == "estimate" else margin,
                        runners=a.single_runners if mode == "single" else a.runners, max_batch=a.max_batch,
                        keep_results=False,
-                       split=a.split, grid_div=a.grid_div)
+                       split=a.split, grid_div=a.grid_div, batch_line=a.batch_line, costly=int(a.costly))
         reqs = [S.Request(arrival=ar, ops=ops, rid=i) for i, (ar, ops) in enumerate(stream)]
         try:
             summ = srv.run(reqs)
@@ -525,7 +525,8 @@ def bench_served(a) -> int:
         summ["bytes"] = nbytes
         out[mode] = summ
         print(f"# {mode}: {summ['requests']} requests in {summ['seconds']:.3f}s, "
-              f"{summ['queries_per_s']:.0f} q/s, p50 {summ['p50_ms']:.2f} ms, p99 {summ['p99_ms']:.2f} ms, "
+              f"{summ['queries_per_s']:.0f} q/s, p50 {summ['p50_ms']:.2f} ms, p90 {summ['p90_ms']:.2f} ms, "
+              f"p99 {summ['p99_ms']:.2f} ms, "
               f"reruns {summ['reruns']}, peak arena {summ['peak_arena_bytes'] / 2**20:.0f} MiB, "
               f"{summ['gbs']:.1f} GB/s; host {summ['host_seconds']}", file=sys.stderr, flush=True)
     net.close()
@@ -548,7 +549,8 @@ def bench_served(a) -> int:
     cfg = {"workload": f"served: Poisson arrivals at {a.rate:.0f} q/s for {a.duration:.1f} s of random "
                        "op-programs (4-40 ops) over 10 resident tables of log-uniform 1e5-5e7 rows",
            "table_rows": sizes, "requests": n_req, "arena_gib": a.arena_gib,
-           "server": {"runners": a.runners, "max_batch": a.max_batch, "split": a.split, "grid_div": a.grid_div},
+           "server": {"runners": a.runners, "max_batch": a.max_batch, "split": a.split, "grid_div": a.grid_div,
+                      "batch_line": a.batch_line, "costly": a.costly},
            "predictor": {"examples": int(len(x)), "train_s": round(t2 - t1, 3), "quantile": a.quantile,
                          "point_network_validation_mae": round(loss, 6),
                          "heldout_mae": round(mae, 4), "independence_estimate_mae": round(mae_est, 4),
@@ -557,12 +559,10 @@ def bench_served(a) -> int:
     for m in out:
         cfg[names.get(m, m)] = brief(m)
     cfg["rerun_rate"] = round(s1["reruns"] / n_req, 4)
-    cfg["batched_vs_plain"] = {"throughput": ratio("two-pass", "plain", "queries_per_s"),
-                               "p99": ratio("two-pass", "plain", "p99_ms", inv=True),
-                               "p50": ratio("two-pass", "plain", "p50_ms", inv=True)}
-    cfg["sized_vs_plain"] = {"throughput": ratio("sized", "plain", "queries_per_s"),
-                             "p99": ratio("sized", "plain", "p99_ms", inv=True),
-                             "p50": ratio("sized", "plain", "p50_ms", inv=True)}
+    for m in ("two-pass", "sized", "estimate", "single"):
+        cfg[f"{m.replace('-', '_')}_vs_plain"] = {
+            "throughput": ratio(m, "plain", "queries_per_s"), "p50": ratio(m, "plain", "p50_ms", inv=True),
+            "p90": ratio(m, "plain", "p90_ms", inv=True), "p99": ratio(m, "plain", "p99_ms", inv=True)}
     cfg["gain_vs_two_pass"] = {"throughput": ratio("sized", "two-pass", "queries_per_s"),
                                "p99": ratio("sized", "two-pass", "p99_ms", inv=True),
                                "reserved_bytes": ratio("sized", "two-pass", "reserved_bytes", inv=True)}
@@ -794,6 +794,8 @@ def main() -> int:
     ap.add_argument("--single-runners", type=int, default=1, help="served: runners of the single-pass server")
     ap.add_argument("--split", type=int, default=0, help="served: 1 = one launch group per batch")
     ap.add_argument("--grid-div", type=int, default=0, help="served: batches launch 1/grid-div of the grid")
+    ap.add_argument("--batch-line", type=int, default=16, help="served: waiting requests from which runs batch")
+    ap.add_argument("--costly", type=float, default=1e8, help="served: rows x ops of a costly request")
     ap.add_argument("--modes", default="sized,estimate,single,two-pass,plain", help="served: servers to run")
     ap.add_argument("--queries", type=int, default=256, help="batched: queries per batch")
     ap.add_argument("--programs", type=int, default=0,

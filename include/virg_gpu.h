@@ -375,9 +375,13 @@ int vg_select_noalloc(const virg_op *ops, unsigned num_ops, const void *const *c
  *     arrival order (reruns first) as long as their result buffers fit in
  *     the arena (device memory taken once at creation): admission by HBM
  *     footprint;
- *   - one admitted request runs alone (its count and write launches; the
- *     write pass stores the outcome into pinned host memory, which the loop
- *     polls), several run as one persistent batch (vg_pbatch) whose kernels
+ *   - while the line is short (below batch_line) each run takes one
+ *     request, run alone as its count and write launches (a last launch
+ *     stores the outcome into pinned host memory, which the loop polls),
+ *     and at most one costly request (rows x ops, see `costly`) runs at a
+ *     time: costly runs side by side only slow each other, while cheap ones
+ *     beside a costly one barely do; from batch_line on, runs take up to
+ *     max_batch requests as one persistent batch (vg_pbatch) whose kernels
  *     store every outcome into pinned memory; runs on different runners
  *     overlap on the device;
  *   - a finished run completes its requests, except a result that did not
@@ -409,6 +413,12 @@ typedef struct {
 	                                   * holds (until vg_server_release) before the request
 	                                   * counts as done */
 	const vg_mlp *net;                /* VG_SIZE_NET: the trained network */
+	unsigned batch_line;              /* waiting requests from which runs are batches; below
+	                                   * it each run takes one request (0: 16) */
+	unsigned long long costly;        /* rows x ops from which a request is costly: while the
+	                                   * line is short at most one costly run is in flight,
+	                                   * cheaper requests may pass a waiting costly one
+	                                   * (0: 1e8; ~0: no limit) */
 } vg_server_config;
 typedef struct {
 	unsigned long long done, failed, reruns;

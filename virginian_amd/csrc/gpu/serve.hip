@@ -16,10 +16,15 @@
  *      host memory first) and their buffers go back to the arena;
  *   3. while a runner (a stream) is free, waiting queries are admitted in
  *      arrival order (reruns first) as long as their result buffers fit in
- *      the arena: admission by HBM footprint.  One admitted query runs
- *      alone as its own count and write launches; several run as one
- *      persistent batch (vg_pbatch), while earlier work still runs on the
- *      other runners' streams, so their work overlaps on the device.
+ *      the arena: admission by HBM footprint.  While the line is short a
+ *      run takes one query, alone as its own count and write launches, and
+ *      at most one costly query (rows x ops) runs at a time -- the
+ *      interpreter is bound by its scalar dispatch, so two costly runs side
+ *      by side take as long as one after the other while each is slowed,
+ *      whereas cheap queries beside a costly one pass it without slowing it
+ *      much; once the line is long, runs take many queries as one
+ *      persistent batch (vg_pbatch).  Runs on different runners' streams
+ *      overlap on the device.
  *
  * Buffers are sized for every row of the query's table, by the
  * independence estimate of the filter over the table's histograms, or by
@@ -140,6 +145,7 @@ struct request {
 	vg_query *q = nullptr;
 	std::vector<unsigned> widths;
 	uint64_t table_rows = 0;
+	uint64_t cost = 0;          /* rows x ops: how long the interpreter runs it */
 	double seen = -1;           /* when its outcome was seen complete in a running batch */
 	std::vector<std::vector<char>> host;   /* kept results, one array per column */
 };
@@ -149,6 +155,7 @@ struct runner {
 	hipStream_t stream = nullptr;
 	vg_bresult *res = nullptr;  /* lone run's outcome (pinned, coherent) */
 	int busy = 0;               /* 0 free, 1 lone run, 2 batch */
+	bool costly = false;        /* a lone run of a costly request */
 	std::vector<size_t> reqs;
 };
 
@@ -224,6 +231,10 @@ extern "C" vg_server *vg_server_create(virginian *v, const vg_server_config *cfg
 		c.max_batch = 64;
 	if (!c.tiles_per_item)
 		c.tiles_per_item = 8;
+	if (!c.batch_line)
+		c.batch_line = 16;
+	if (!c.costly)
+		c.costly = 100000000ull;
 	memset(&s->st, 0, sizeof s->st);
 	if (c.sizing > VG_SIZE_NET || (c.sizing == VG_SIZE_NET && !c.net)) {
 		serr(s, err, errlen, "vg_server_create: sizing by network needs the network");
@@ -371,6 +382,7 @@ static bool prepare(vg_server *s, request &r)
 		unsigned tiles, rb, wb, nout;
 		vg_query_info(r.q, &rows, &tiles, &rb, &wb, &nout);
 		r.table_rows = rows;
+		r.cost = rows * (uint64_t)r.ops.size();
 	}
 	if (r.cap == 0) {
 		if (size_request(s, r) != VIRG_SUCCESS)
@@ -522,29 +534,39 @@ static int collect(vg_server *s)
 	return completed;
 }
 
-/* Admit from the front of `line` onto runner rn; false when nothing was
- * admitted. */
-static bool admit(vg_server *s, std::deque<size_t> &line, runner &rn)
+/* Admit from `line` onto runner rn, in arrival order, at most max_take
+ * requests; a costly request only when allow_costly (those left wait in
+ * place, cheaper ones behind them may go first).  *blocked = the arena had
+ * no room for the next one.  False when nothing was admitted. */
+static bool admit(vg_server *s, std::deque<size_t> &line, runner &rn, size_t max_take, bool allow_costly,
+	bool *blocked)
 {
 	const double ta = mono_now();
 	std::vector<size_t> take;
-	while (!line.empty() && take.size() < s->cfg.max_batch) {
-		request &r = s->reqs[line.front()];
+	*blocked = false;
+	for (size_t i = 0; i < line.size() && take.size() < max_take;) {
+		request &r = s->reqs[line[i]];
 		if (!prepare(s, r)) {
 			fail_request(s, r);
-			line.pop_front();
+			line.erase(line.begin() + (long)i);
 			continue;
 		}
 		if (r.need > s->ar.size) {
 			snprintf(s->msg, sizeof s->msg, "a request's result buffers (%zu bytes) exceed the arena", r.need);
 			fail_request(s, r);
-			line.pop_front();
+			line.erase(line.begin() + (long)i);
 			continue;
 		}
-		if (!place(s, r))
+		if (!allow_costly && r.cost >= s->cfg.costly) {
+			++i;
+			continue;
+		}
+		if (!place(s, r)) {
+			*blocked = true;
 			break;                  /* admission is in arrival order */
-		take.push_back(line.front());
-		line.pop_front();
+		}
+		take.push_back(line[i]);
+		line.erase(line.begin() + (long)i);
 	}
 	const double tb = mono_now();
 	s->st.admit_s += tb - ta;
@@ -555,6 +577,7 @@ static bool admit(vg_server *s, std::deque<size_t> &line, runner &rn)
 		request &r = s->reqs[take[0]];
 		ok = vgd_query_launch_lone(r.q, rn.stream, rn.res) == VIRG_SUCCESS;
 		rn.busy = 1;
+		rn.costly = r.cost >= s->cfg.costly;
 		s->st.lone++;
 	} else {
 		std::vector<vg_query *> qs;
@@ -611,12 +634,21 @@ extern "C" int vg_server_step(vg_server *s)
 		std::deque<size_t> &line = !s->reruns.empty() ? s->reruns : s->waiting;
 		if (line.empty())
 			break;
-		bool busy_any = false;
-		for (runner &o : s->runners)
+		bool busy_any = false, costly_running = false;
+		for (runner &o : s->runners) {
 			busy_any = busy_any || o.busy;
-		if (!admit(s, line, rn)) {
+			costly_running = costly_running || (o.busy == 1 && o.costly);
+		}
+		/* a short line: runs of one request each, at most one costly run
+		 * at a time (costly runs side by side only slow each other); a
+		 * line of batch_line or more: batches */
+		const bool light = line.size() < s->cfg.batch_line;
+		bool blocked = false;
+		if (!admit(s, line, rn, light ? 1 : s->cfg.max_batch, !light || !costly_running, &blocked)) {
 			if (rn.busy)
 				return -1;      /* a launch failed */
+			if (!blocked)
+				break;          /* only costly requests wait, behind a costly run */
 			if (!busy_any && !line.empty() && s->reqs[line.front()].state == R_WAITING) {
 				/* nothing runs and the head does not fit: it never will */
 				request &r = s->reqs[line.front()];

@@ -36,7 +36,7 @@ class ServerConfig(C.Structure):
     _fields_ = [("arena_bytes", C.c_ulonglong), ("runners", C.c_uint), ("max_batch", C.c_uint),
                 ("tiles_per_item", C.c_uint), ("sizing", C.c_uint), ("margin", C.c_float),
                 ("single", C.c_uint), ("split", C.c_uint), ("grid_div", C.c_uint), ("keep", C.c_uint),
-                ("net", C.c_void_p)]
+                ("net", C.c_void_p), ("batch_line", C.c_uint), ("costly", C.c_ulonglong)]
 
 
 class ServerStats(C.Structure):
@@ -109,7 +109,8 @@ class Server:
 
     def __init__(self, db, table_ids: list[int], arena_bytes: int, mode: str = "single", predictor=None,
                  margin: float = 0.05, runners: int = 3, max_batch: int = 64, tiles_per_item: int = 8,
-                 keep_results: bool = True, split: int = 0, grid_div: int = 0):
+                 keep_results: bool = True, split: int = 0, grid_div: int = 0, batch_line: int = 0,
+                 costly: int = 0):
         _bind()
         if mode not in ("sized", "estimate", "single", "two-pass", "plain"):
             raise ValueError(f"unknown serving mode {mode}")
@@ -123,7 +124,7 @@ class Server:
             runners, max_batch = 1, 1
         cfg = ServerConfig(arena_bytes, runners, max_batch, tiles_per_item, sizing, margin,
                            1 if mode == "single" else 0, split, grid_div, 1 if keep_results else 0,
-                           predictor.h if sizing == SIZE_NET else None)
+                           predictor.h if sizing == SIZE_NET else None, batch_line, costly)
         err = C.create_string_buffer(256)
         self.h = lib.vg_server_create(C.cast(db.ptr, C.c_void_p), C.byref(cfg), err, 256)
         if not self.h:
@@ -186,6 +187,7 @@ class Server:
         lat = np.array([r.ready - r.arrival for r in reqs]) * 1e3
         return {"requests": len(reqs), "seconds": end, "queries_per_s": len(reqs) / end if end else 0.0,
                 "p50_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
+                "p90_ms": float(np.percentile(lat, 90)) if len(lat) else 0.0,
                 "p99_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0,
                 "reruns": sum(r.rerun for r in reqs), "peak_arena_bytes": st["arena_peak"],
                 "reserved_bytes": st["reserved_bytes"],
