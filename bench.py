@@ -512,8 +512,9 @@ def bench_served(a) -> int:
                        predictor=net if mode in ("sized", "single") else None,
                        margin=margin_est if mode == "estimate" else margin,
                        runners=a.single_runners if mode == "single" else a.runners, max_batch=a.max_batch,
-                       keep_results=False,
-                       split=a.split, grid_div=a.grid_div, batch_line=a.batch_line, costly=int(a.costly))
+                       keep_results=a.keep,
+                       split=a.split, grid_div=a.grid_div, batch_line=a.batch_line, costly=int(a.costly),
+                       host_arena=a.host_arena)
         reqs = [S.Request(arrival=ar, ops=ops, rid=i) for i, (ar, ops) in enumerate(stream)]
         try:
             summ = srv.run(reqs)
@@ -550,7 +551,7 @@ def bench_served(a) -> int:
                        "op-programs (4-40 ops) over 10 resident tables of log-uniform 1e5-5e7 rows",
            "table_rows": sizes, "requests": n_req, "arena_gib": a.arena_gib,
            "server": {"runners": a.runners, "max_batch": a.max_batch, "split": a.split, "grid_div": a.grid_div,
-                      "batch_line": a.batch_line, "costly": a.costly},
+                      "batch_line": a.batch_line, "costly": a.costly, "keep": a.keep, "host_arena": a.host_arena},
            "predictor": {"examples": int(len(x)), "train_s": round(t2 - t1, 3), "quantile": a.quantile,
                          "point_network_validation_mae": round(loss, 6),
                          "heldout_mae": round(mae, 4), "independence_estimate_mae": round(mae_est, 4),
@@ -795,6 +796,8 @@ def main() -> int:
     ap.add_argument("--split", type=int, default=0, help="served: 1 = one launch group per batch")
     ap.add_argument("--grid-div", type=int, default=0, help="served: batches launch 1/grid-div of the grid")
     ap.add_argument("--batch-line", type=int, default=16, help="served: waiting requests from which runs batch")
+    ap.add_argument("--keep", action="store_true", help="served: each result to host memory before it counts done")
+    ap.add_argument("--host-arena", action="store_true", help="served: kernels write results into pinned host memory")
     ap.add_argument("--costly", type=float, default=1e8, help="served: rows x ops of a costly request")
     ap.add_argument("--modes", default="sized,estimate,single,two-pass,plain", help="served: servers to run")
     ap.add_argument("--queries", type=int, default=256, help="batched: queries per batch")

@@ -419,6 +419,9 @@ typedef struct {
 	                                   * line is short at most one costly run is in flight,
 	                                   * cheaper requests may pass a waiting costly one
 	                                   * (0: 1e8; ~0: no limit) */
+	unsigned host_arena;              /* 1: the arena is pinned host memory the kernels write
+	                                   * results into directly (no copy to the host; the
+	                                   * writes cross the link) */
 } vg_server_config;
 typedef struct {
 	unsigned long long done, failed, reruns;

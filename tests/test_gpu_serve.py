@@ -61,6 +61,22 @@ def test_served_results_equal_alone(setup, mode, margin):
         assert summ["reruns"] > 0                 # some estimates were short: reruns exercised
 
 
+@pytest.mark.parametrize("mode", ["two-pass", "sized", "single"])
+def test_served_results_into_host_memory(setup, mode):
+    """With the arena in pinned host memory the kernels write each result
+    straight to the host; the results are the same as alone."""
+    db, tids, reqs, want = setup
+    srv = S.Server(db, tids, 256 << 20, mode=mode, predictor=None, margin=0.05, runners=3, max_batch=16,
+                   host_arena=True)
+    try:
+        done = _fresh(reqs)
+        summ = srv.run(done)
+    finally:
+        srv.close()
+    print(mode, "host arena", summ, flush=True)
+    _check(done, want, f"{mode} host arena")
+
+
 def test_alone_equals_in_mix(setup):
     """One at a time (one runner, batches of one) gives the same results
     as the mix."""

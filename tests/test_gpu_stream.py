@@ -9,7 +9,7 @@ import pytest
 
 import virginian_amd as va
 from oracle import oracle
-from tests.helpers import CONFIG1_SQL, assert_same, config2_sql
+from tests.helpers import CONFIG1_SQL, QUERY_SUITE, assert_same, config2_sql
 
 pytestmark = pytest.mark.gpu
 
@@ -51,6 +51,22 @@ def test_use_stream_boundary(gpu_db):
         want, _ = oracle.select(gpu_db, sql, threads=8)
         assert_same(got, want, sql)
     gpu_db.use_stream = False
+
+
+def test_use_mmap_boundary(gpu_db):
+    """virg_vm_gpu with use_mmap (the reference's mapped mode, exercised by
+    its ConditionsGPUMapped test, src/test/sql.cc:316-330): the data stays
+    in host memory; every query of the published suite and the benchmark
+    query give the oracle's rows, in order, in the host result tablets."""
+    gpu_db.gen_table("test", (2 << 20) + 77, 1 << 19, seed=29)
+    gpu_db.use_mmap = True
+    try:
+        for sql in [CONFIG1_SQL, CONFIG3_SQL] + QUERY_SUITE:
+            got = gpu_db.query(sql)
+            want, _ = oracle.select(gpu_db, sql, threads=8)
+            assert_same(got, want, sql)
+    finally:
+        gpu_db.use_mmap = False
 
 
 def test_stream_capacity_error(gpu_db):

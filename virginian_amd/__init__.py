@@ -355,6 +355,16 @@ class Virginian:
         self._st.use_stream = 1 if on else 0
 
     @property
+    def use_mmap(self) -> bool:
+        """virginian.use_mmap: the reference's mapped mode -- the data stays
+        in host memory (here it streams through HBM as with use_stream)."""
+        return bool(self._st.use_mmap)
+
+    @use_mmap.setter
+    def use_mmap(self, on: bool) -> None:
+        self._st.use_mmap = 1 if on else 0
+
+    @property
     def state(self) -> State:
         return self._st
 

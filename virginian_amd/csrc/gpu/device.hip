@@ -959,12 +959,20 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 	 * meta, and the streaming path, which reads host memory later, keeps a
 	 * lock on every tablet until it is done. */
 	const bool paged = v->dbfd >= 0;
+	/* The reference's mapped mode (use_mmap, src/vm/vm_gpu.cu:445-560) runs
+	 * each tablet where it lies in host memory and writes the results into
+	 * the host result tablets.  Here both host-memory modes take the
+	 * streaming path: the data stays in host memory and crosses the link
+	 * once, by DMA copies of only the referenced columns, which reach the
+	 * link's rate where compute-unit loads of mapped memory would not;
+	 * results land in the host result tablets as in the reference. */
+	const bool host_data = v->use_stream || v->use_mmap;
 	std::vector<const virg_tablet_meta *> tabs;
 	std::vector<virg_tablet_meta> metas;
 	held_locks held(v);
 	unsigned proced = 0;
 	while (1) {
-		if (paged && !v->use_stream) {
+		if (paged && !host_data) {
 			VIRG_CHECK(!resident_tablet(d, tab[0], d->stream) || hipStreamSynchronize(d->stream) != hipSuccess,
 				"could not make a tablet resident");
 			metas.push_back(*tab[0]);
@@ -983,9 +991,9 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 	for (const virg_tablet_meta &m : metas)
 		tabs.push_back(&m);
 
-	if (v->use_stream) {
-		/* the reference's streaming mode: the tablets stay in host memory
-		 * and stream through HBM batch by batch */
+	if (host_data) {
+		/* the reference's streaming and mapped modes: the tablets stay in
+		 * host memory and stream through HBM batch by batch */
 		tablet_sink sink(v, vm, res);
 		vg_stream_stats st;
 		memset(&st, 0, sizeof st);
@@ -1026,14 +1034,17 @@ extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 	}
 	size_t need = (size_t)p.rows * total_w + 256 * (P.nout + 1);
 	if (need > d->outbuf_bytes) {
+		/* grown at least twofold, so a run of ever larger results
+		 * reallocates (and waits for the device to free) only a few times */
+		const size_t want = need > 2 * d->outbuf_bytes ? need : 2 * d->outbuf_bytes;
 		if (d->outbuf) {
 			HCHK(hipStreamSynchronize(d->stream), "sync");
 			HCHK(hipFree(d->outbuf), "free");
 		}
 		d->outbuf = nullptr;
 		d->outbuf_bytes = 0;
-		HCHK(hipMalloc(&d->outbuf, need), "result buffer");
-		d->outbuf_bytes = need;
+		HCHK(hipMalloc(&d->outbuf, want), "result buffer");
+		d->outbuf_bytes = want;
 	}
 	void *out[VG_MAX_OUT];
 	size_t off = 0;

@@ -174,6 +174,7 @@ struct vg_server {
 	bool order_dirty = false;
 	std::vector<runner> runners;
 	hipStream_t copy = nullptr;  /* kept results come back on it */
+	char *hbase = nullptr;       /* host_arena: the host's view of the arena (ar.base is the device's) */
 	double t0 = 0;
 	size_t done = 0, failed = 0;
 	vg_server_stats st;
@@ -210,7 +211,9 @@ extern "C" void vg_server_free(vg_server *s)
 			vg_query_free(q.q);
 	if (s->copy)
 		(void)hipStreamDestroy(s->copy);
-	if (s->ar.base)
+	if (s->hbase)
+		(void)hipHostFree(s->hbase);
+	else if (s->ar.base)
 		vg_device_free(s->v, s->ar.base);
 	delete s;
 }
@@ -241,7 +244,20 @@ extern "C" vg_server *vg_server_create(virginian *v, const vg_server_config *cfg
 		delete s;
 		return nullptr;
 	}
-	s->ar.base = (char *)vg_device_alloc(v, c.arena_bytes);
+	if (c.host_arena) {
+		/* results written by the kernels straight into pinned host memory */
+		if (hipHostMalloc((void **)&s->hbase, c.arena_bytes, hipHostMallocMapped | hipHostMallocCoherent) !=
+				hipSuccess ||
+			hipHostGetDevicePointer((void **)&s->ar.base, s->hbase, 0) != hipSuccess) {
+			(void)hipGetLastError();
+			if (s->hbase)
+				(void)hipHostFree(s->hbase);
+			s->hbase = nullptr;
+			s->ar.base = nullptr;
+		}
+	} else {
+		s->ar.base = (char *)vg_device_alloc(v, c.arena_bytes);
+	}
 	if (!s->ar.base) {
 		serr(s, err, errlen, "vg_server_create: cannot allocate the result arena");
 		delete s;
@@ -249,7 +265,7 @@ extern "C" vg_server *vg_server_create(virginian *v, const vg_server_config *cfg
 	}
 	s->ar.size = c.arena_bytes;
 	s->ar.free_[0] = c.arena_bytes;
-	if (c.keep && hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess) {
+	if (c.keep && !c.host_arena && hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess) {
 		(void)hipGetLastError();
 		vg_server_free(s);
 		serr(nullptr, err, errlen, "vg_server_create: cannot create a stream");
@@ -463,12 +479,14 @@ static void finish(vg_server *s, request &r, uint64_t total, unsigned flags)
 		const uint64_t c = r.cap ? r.cap : 1;
 		for (size_t j = 0; j < r.widths.size(); j++) {
 			r.host[j].resize(total * r.widths[j]);
-			if (total && hipMemcpyAsync(r.host[j].data(), s->ar.base + o, total * r.widths[j],
+			if (s->hbase)               /* already in host memory: no copy engine */
+				memcpy(r.host[j].data(), s->hbase + o, total * r.widths[j]);
+			else if (total && hipMemcpyAsync(r.host[j].data(), s->ar.base + o, total * r.widths[j],
 					hipMemcpyDeviceToHost, s->copy) != hipSuccess)
 				(void)hipGetLastError();
 			o += round256(c * r.widths[j]);
 		}
-		if (hipStreamSynchronize(s->copy) != hipSuccess)
+		if (!s->hbase && hipStreamSynchronize(s->copy) != hipSuccess)
 			(void)hipGetLastError();
 	}
 	r.state = R_DONE;

@@ -36,7 +36,8 @@ class ServerConfig(C.Structure):
     _fields_ = [("arena_bytes", C.c_ulonglong), ("runners", C.c_uint), ("max_batch", C.c_uint),
                 ("tiles_per_item", C.c_uint), ("sizing", C.c_uint), ("margin", C.c_float),
                 ("single", C.c_uint), ("split", C.c_uint), ("grid_div", C.c_uint), ("keep", C.c_uint),
-                ("net", C.c_void_p), ("batch_line", C.c_uint), ("costly", C.c_ulonglong)]
+                ("net", C.c_void_p), ("batch_line", C.c_uint), ("costly", C.c_ulonglong),
+                ("host_arena", C.c_uint)]
 
 
 class ServerStats(C.Structure):
@@ -110,7 +111,7 @@ class Server:
     def __init__(self, db, table_ids: list[int], arena_bytes: int, mode: str = "single", predictor=None,
                  margin: float = 0.05, runners: int = 3, max_batch: int = 64, tiles_per_item: int = 8,
                  keep_results: bool = True, split: int = 0, grid_div: int = 0, batch_line: int = 0,
-                 costly: int = 0):
+                 costly: int = 0, host_arena: bool = False):
         _bind()
         if mode not in ("sized", "estimate", "single", "two-pass", "plain"):
             raise ValueError(f"unknown serving mode {mode}")
@@ -124,7 +125,8 @@ class Server:
             runners, max_batch = 1, 1
         cfg = ServerConfig(arena_bytes, runners, max_batch, tiles_per_item, sizing, margin,
                            1 if mode == "single" else 0, split, grid_div, 1 if keep_results else 0,
-                           predictor.h if sizing == SIZE_NET else None, batch_line, costly)
+                           predictor.h if sizing == SIZE_NET else None, batch_line, costly,
+                           1 if host_arena else 0)
         err = C.create_string_buffer(256)
         self.h = lib.vg_server_create(C.cast(db.ptr, C.c_void_p), C.byref(cfg), err, 256)
         if not self.h:
