@@ -5,8 +5,11 @@ the GPU (vg_jit_program: decode, shape, immediates, source), then its
 per-chunk function is compiled for the host with the stand-ins of
 tests/kernel_cpu_shim.h and run over the host tablets, one workgroup of
 256 lanes (coroutines of one thread) taking the chunks in order (so every
-look-back finds its predecessors published).  The kernel entry itself, the chunk counter and
-the epoch handling are GPU-only and not exercised here.
+look-back finds its predecessors published).  A pipelined shape runs in the
+pipelined kernel's order: each chunk's filter phase into the other staging
+buffer before the previous chunk's result phase.  The kernel entry itself,
+the chunk counter and the epoch handling are GPU-only and not exercised
+here.
 
     cols, rows, flags, shape = select(db, prog)
 
@@ -37,7 +40,7 @@ MAX_COLS, MAX_UOPS, MAX_OUT = 17, 48, 16
 class Shape(C.Structure):
     _fields_ = [("R", C.c_int), ("S", C.c_uint32), ("TR", C.c_uint32), ("fbytes_row", C.c_uint32),
                 ("ftile_bytes", C.c_uint32), ("stage_off", C.c_uint32), ("stage_wave_bytes", C.c_uint32),
-                ("lds_bytes", C.c_uint32), ("aligned", C.c_int), ("staged", C.c_int)]
+                ("lds_bytes", C.c_uint32), ("aligned", C.c_int), ("staged", C.c_int), ("pipe", C.c_int)]
 
 
 class TabDesc(C.Structure):
@@ -98,23 +101,55 @@ void kc_barrier::wait()
 static const vgx_launch_args *kc_A;
 static u8 *kc_lds;
 static u32 kc_ovf;
-static u32 s_cnt[16][4];
+static u32 s_cnt[2][16][4];
 static u64 s_pre;
+static unsigned kc_stage_off;
+static int kc_pipe;
 
 static void kc_lane_main()
 {
-	for (u32 c = 0; c < kc_A->nchunks; c++) {
-		const u32 o = p0::chunk(*kc_A, c, 0ull, s_cnt, &s_pre, kc_lds, [] {});
-		kc_ovf |= o;
-		__syncthreads();
+	const u32 nc = kc_A->nchunks;
+	if (!kc_pipe) {
+		for (u32 c = 0; c < nc; c++) {
+			const u32 o = p0::chunk(*kc_A, c, 0ull, s_cnt[0], &s_pre, kc_lds, [] {});
+			kc_ovf |= o;
+			__syncthreads();
+		}
+	} else {
+		/* as the pipelined kernel: chunk c+1's filter phase into the other
+		 * buffer, then chunk c's result phase */
+		const unsigned buf = kc_stage_off / 2;
+		u32 b = 0, n0 = 0;
+		u64 vb0 = 0;
+		if (nc) {
+			p0::stage(*kc_A, 0, kc_lds);
+			n0 = p0::phase1(*kc_A, 0, 0ull, s_cnt[0], kc_lds, vb0);
+		}
+		for (u32 c = 0; c < nc; c++) {
+			__syncthreads();
+			u32 n1 = 0;
+			u64 vb1 = 0;
+			u8 *const buf1 = kc_lds + (b ^ 1u) * buf;
+			if (c + 1 < nc) {
+				p0::stage(*kc_A, c + 1, buf1);
+				n1 = p0::phase1(*kc_A, c + 1, 0ull, s_cnt[b ^ 1u], buf1, vb1);
+			}
+			kc_ovf |= p0::phase2(*kc_A, c, 0ull, s_cnt[b], &s_pre, kc_lds + b * buf, kc_lds + kc_stage_off,
+				vb0, n0, [] {});
+			n0 = n1;
+			vb0 = vb1;
+			b ^= 1u;
+		}
 	}
 	kc_state[kc_cur] = 2;
 }
 
 /* every chunk in order, one workgroup of 256 lanes; VG_OVF_* bits into
  * the control block.  Returns 0, or -1 when lanes wait for ever. */
-extern "C" int kc_run(const vgx_launch_args *A, unsigned lds_bytes)
+extern "C" int kc_run(const vgx_launch_args *A, unsigned lds_bytes, unsigned stage_off, int pipe)
 {
+	kc_stage_off = stage_off;
+	kc_pipe = pipe;
 	kc_block blk;
 	blk.all.n = 256;
 	for (int w = 0; w < 4; w++)
@@ -190,7 +225,7 @@ def _compile(text: str) -> C.CDLL:
             raise RuntimeError(f"host compile of {cpp} failed:\n{r.stderr[-4000:]}")
         os.replace(tmp, so)
     lib = C.CDLL(str(so))
-    lib.kc_run.argtypes = [C.POINTER(LaunchArgs), C.c_uint]
+    lib.kc_run.argtypes = [C.POINTER(LaunchArgs), C.c_uint, C.c_uint, C.c_int]
     lib.kc_run.restype = C.c_int
     return lib
 
@@ -265,7 +300,7 @@ def select(db, prog, capacity: int = 0):
     A.spin_limit = 1 << 20
     for j, o in enumerate(outs):
         A.out[j] = o.ctypes.data
-    assert lib.kc_run(C.byref(A), sh.lds_bytes) == 0
+    assert lib.kc_run(C.byref(A), sh.lds_bytes, sh.stage_off, sh.pipe) == 0
     total = ctrl.total
     got = min(total, cap)
     cols = {}

@@ -402,11 +402,12 @@ struct outsrc {
 
 } /* namespace */
 
-extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S_override, vgx_shape *sh, char *err,
-	size_t errlen)
+extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S_override, int pipe, vgx_shape *sh,
+	char *err, size_t errlen)
 {
 	memset(sh, 0, sizeof *sh);
 	sh->aligned = aligned ? 1 : 0;
+	sh->pipe = pipe ? 1 : 0;
 	if (P->nuop > VG_MAX_UOPS || P->nout > VG_MAX_OUT) {
 		snprintf(err, errlen, "program too large");
 		return -1;
@@ -435,7 +436,11 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 		if (P->col[c].phase == VG_PHASE_FILTER)
 			fbytes += P->col[c].width == 1 ? 1u : P->col[c].width;
 	const uint32_t two_per_cu = 79u * 1024u - 4u * stage_wave;
-	const uint32_t budget = two_per_cu < 64u * 1024u ? two_per_cu : 64u * 1024u;
+	uint32_t budget = two_per_cu < 64u * 1024u ? two_per_cu : 64u * 1024u;
+	/* a pipelined kernel holds two chunks' filter columns at once */
+	if (pipe)
+		budget /= 2;
+	const uint32_t min_rows = pipe ? 8192u : 16384u;
 	/* a chunk is 16K rows (one grab and one look-back each); staging its
 	 * filter columns in LDS saves reading them twice, when they fit at
 	 * that size; otherwise they are loaded into registers tile by tile */
@@ -449,7 +454,7 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 		while (S > 1 && ftile * S > budget)
 			S--;
 	const char *mode = getenv("VG_JIT_STAGE");
-	int staged = S * TR >= 16384u && ftile <= budget;
+	int staged = S * TR >= min_rows && ftile <= budget;
 	if (mode && (mode[0] == '0' || mode[0] == '1'))
 		staged = mode[0] == '1' && ftile <= 96u * 1024u;
 	if (!staged) {
@@ -470,7 +475,7 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 	sh->fbytes_row = fbytes;
 	sh->ftile_bytes = ftile;
 	sh->staged = staged;
-	sh->stage_off = ftile * S;
+	sh->stage_off = ftile * S * (pipe ? 2u : 1u);
 	sh->stage_wave_bytes = stage_wave;
 	sh->lds_bytes = sh->stage_off + 4u * sh->stage_wave_bytes;
 	return 0;
@@ -545,19 +550,21 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 			decl("\t\t\tu32 e%u[R], e%uh[R];\n#pragma unroll\n\t\t\tfor (int r = 0; r < R; ++r) { e%u[r] = 0u; e%uh[r] = 0u; }\n",
 				c, c, c, c);
 
-	if (outlined)
-		o("template <typename AT, typename G>\n__device__ __attribute__((noinline)) u32 chunk(AT &A, const u32 c, "
-		  "const u64 epoch, las u32 (*s_cnt3)[4], las u64 *s_pre3, las u8 *lds3, G grab_next)\n{\n"
-		  "\tu32 (*const s_cnt)[4] = (u32 (*)[4])s_cnt3;\n\tu64 *const s_pre = (u64 *)s_pre3;\n"
-		  "\tu8 *const lds = (u8 *)lds3;\n");
-	else
-		o("template <typename AT, typename G>\n__device__ __attribute__((always_inline)) inline u32 chunk(AT &A, "
-		  "const u32 c, const u64 epoch, u32 (*s_cnt)[4], u64 *s_pre, u8 *lds, G grab_next)\n{\n");
-	o("\tu32 ovf = 0;\n");
-	o("\tconst u32 tid = threadIdx.x, lane = tid & 63u, wave = uni(tid >> 6);\n");
-	o("\tu8 *const ostage = lds + %u + wave * %u;\n", sh->stage_off, sh->stage_wave_bytes);
-	o("\t{\n");
-	o("\t\tconst u32 t0 = c * S_;\n\t\tconst u32 nt = uni(min((u32)S_, A.num_tiles - t0));\n");
+	/* A chunk is worked in three steps, each a function: stage() issues
+	 * the loads of the chunk's filter columns into the LDS staging buffer
+	 * `lds` (staged shapes; nothing waits), phase1() evaluates the filter,
+	 * counts and publishes the chunk's count, phase2() finds the chunk's
+	 * output offset by look-back and writes its rows.  chunk() runs the
+	 * three in a row; a pipelined kernel runs phase1() of its next chunk
+	 * before phase2() of this one. */
+	const char *pre_fn =
+		"\tconst u32 tid = threadIdx.x, lane = tid & 63u, wave = uni(tid >> 6);\n"
+		"\t(void)tid; (void)lane; (void)wave;\n"
+		"\t{\n"
+		"\t\tconst u32 t0 = c * S_;\n\t\tconst u32 nt = uni(min((u32)S_, A.num_tiles - t0));\n"
+		"\t\t(void)t0; (void)nt;\n";
+	o("template <typename AT>\n__device__ __attribute__((always_inline)) inline void stage(AT &A, const u32 c, u8 *lds)\n{\n");
+	o("%s", pre_fn);
 
 	/* tile lookup */
 	const char *tile_info =
@@ -601,8 +608,9 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 		}
 		o("\t\t\t\t}\n\t\t\t}\n");
 	}
-	o("\t\t}\n\t\twait_vm();\n");
+	o("\t\t}\n");
 	}
+	o("\t}\n}\n\n");
 
 	/* read filter column c of tile s from LDS into its slots (this lane's rows) */
 	auto lds_read = [&](out_buf &b, uint32_t c, const char *ind) {
@@ -673,6 +681,11 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 	};
 
 	/* phase 1 evaluation */
+	o("template <typename AT>\n__device__ __attribute__((always_inline)) inline u32 phase1(AT &A, const u32 c, "
+	  "const u64 epoch, u32 (*s_cnt)[4], u8 *lds, u64 &vb_out)\n{\n");
+	o("%s", pre_fn);
+	if (sh->staged)
+		o("\t\twait_vm();          /* the staged filter columns */\n");
 	o("\t\tu64 vb = 0;\n");
 	o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
 	o("\t\t\t(void)fast;\n");
@@ -703,8 +716,14 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 	o("\t\tn = uni(n);\n");
 	o("\t\tif (tid == 0)\n\t\t\t__hip_atomic_store(&A.status[c], (c == 0 ? ST_PRE : ST_AGG) | (epoch << ST_VBITS) | n,\n"
 	  "\t\t\t\t__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n");
+	o("\t\tvb_out = vb;\n\t\treturn n;\n\t}\n}\n\n");
 
 	/* phase 2 */
+	o("template <typename AT, typename G>\n__device__ __attribute__((always_inline)) inline u32 phase2(AT &A, const u32 c, "
+	  "const u64 epoch, u32 (*s_cnt)[4], u64 *s_pre, u8 *lds, u8 *ost, const u64 vb, const u32 n, G grab_next)\n{\n");
+	o("\tu32 ovf = 0;\n");
+	o("%s", pre_fn);
+	o("\t\tu8 *const ostage = ost + wave * %uu;\n", sh->stage_wave_bytes);
 	o("\t\t/* phase 2: result columns, tile by tile */\n");
 	o("\t\tu64 pre = 0;\n\t\tu32 tdone = 0;\n");
 	o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
@@ -797,7 +816,20 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 	}
 	o("\t\t\t\t}\n\t\t\t\tat += qn;\n\t\t\t}\n");
 	o("\t\t\ttdone += s_cnt[s][0] + s_cnt[s][1] + s_cnt[s][2] + s_cnt[s][3];\n\t\t}\n");
-	o("\t}\n\treturn ovf;\n}\n} /* namespace %s */\n", ns);
+	o("\t}\n\treturn ovf;\n}\n\n");
+
+	/* the three steps in a row, with the output staging after one buffer */
+	if (outlined)
+		o("template <typename AT, typename G>\n__device__ __attribute__((noinline)) u32 chunk(AT &A, const u32 c, "
+		  "const u64 epoch, las u32 (*s_cnt3)[4], las u64 *s_pre3, las u8 *lds3, G grab_next)\n{\n"
+		  "\tu32 (*const s_cnt)[4] = (u32 (*)[4])s_cnt3;\n\tu64 *const s_pre = (u64 *)s_pre3;\n"
+		  "\tu8 *const lds = (u8 *)lds3;\n");
+	else
+		o("template <typename AT, typename G>\n__device__ __attribute__((always_inline)) inline u32 chunk(AT &A, "
+		  "const u32 c, const u64 epoch, u32 (*s_cnt)[4], u64 *s_pre, u8 *lds, G grab_next)\n{\n");
+	o("\tstage(A, c, lds);\n\tu64 vb = 0;\n\tconst u32 n = phase1(A, c, epoch, s_cnt, lds, vb);\n"
+	  "\treturn phase2(A, c, epoch, s_cnt, s_pre, lds, lds + %uu, vb, n, grab_next);\n}\n", sh->stage_off);
+	o("} /* namespace %s */\n", ns);
 }
 
 static void gen_prelude(out_buf &o)
@@ -816,18 +848,48 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 	gen_program(o, P, sh, "p0");
 	o("\nextern \"C\" __global__ __launch_bounds__(256) void vgx_main(const vgx_args A)\n{\n");
 	o("\textern __shared__ __attribute__((aligned(16))) u8 lds[];\n");
-	o("\t__shared__ u32 s_cnt[%u][4];\n\t__shared__ u64 s_pre;\n\t__shared__ u32 s_chunk;\n\t__shared__ u32 s_epoch;\n",
-	  sh->S);
+	o("\t__shared__ u32 s_cnt[%u][%u][4];\n\t__shared__ u64 s_pre;\n\t__shared__ u32 s_chunk;\n\t__shared__ u32 s_epoch;\n",
+	  sh->pipe ? 2u : 1u, sh->S);
 	o("\tconst u32 tid = threadIdx.x;\n");
 	o("\tif (tid == 0) {\n"
 	  "\t\tconst u64 e = __hip_atomic_load(&A.state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
 	  "\t\twait_vm();   /* the epoch is in hand before the first grab */\n"
 	  "\t\ts_epoch = (u32)e;\n\t\ts_chunk = grab(A, e);\n\t}\n\t__syncthreads();\n");
 	o("\tconst u64 epoch = s_epoch;\n\tu32 ovf = 0;\n");
-	o("\tfor (;;) {\n\t\tconst u32 c = uni(s_chunk);\n\t\tif (c >= A.nchunks)\n\t\t\tbreak;\n");
-	o("\t\tu32 nxt = 0;\n");
-	o("\t\tovf |= p0::chunk(A, c, epoch, s_cnt, &s_pre, lds, [&]() { nxt = grab(A, epoch); });\n");
-	o("\t\t__syncthreads();\n\t\tif (tid == 0)\n\t\t\ts_chunk = nxt;\n\t\t__syncthreads();\n\t}\n");
+	if (sh->pipe) {
+		/* Pipelined: a chunk's filter phase (which waits for nothing but
+		 * its own loads) runs right after the chunk is taken, and the
+		 * previous chunk's look-back and writes come after it, so the
+		 * chunks before that one have had a whole filter phase to publish
+		 * their counts.  Every chunk taken has its count published
+		 * without waiting on another, so no look-back can wait for ever. */
+		/* The chunk after next is taken as this chunk's last tile starts
+		 * (thread 0's atomic flies under that tile's work): a workgroup
+		 * makes exactly one grab past the last chunk. */
+		const uint32_t buf = sh->stage_off / 2u;
+		o("\tu32 c0 = uni(s_chunk), c1 = 0xFFFFFFFFu, b = 0, n0 = 0;\n\tu64 vb0 = 0;\n");
+		o("\tif (c0 < A.nchunks) {\n\t\tp0::stage(A, c0, lds);\n"
+		  "\t\tn0 = p0::phase1(A, c0, epoch, s_cnt[0], lds, vb0);\n"
+		  "\t\t__syncthreads();\n\t\tif (tid == 0)\n\t\t\ts_chunk = grab(A, epoch);\n\t\t__syncthreads();\n"
+		  "\t\tc1 = uni(s_chunk);\n\t}\n");
+		o("\twhile (c0 < A.nchunks) {\n"
+		  "\t\tu32 n1 = 0;\n\t\tu64 vb1 = 0;\n"
+		  "\t\tu8 *const buf1 = lds + (b ^ 1u) * %uu;\n"
+		  "\t\tif (c1 < A.nchunks) {\n\t\t\tp0::stage(A, c1, buf1);\n"
+		  "\t\t\tn1 = p0::phase1(A, c1, epoch, s_cnt[b ^ 1u], buf1, vb1);\n\t\t}\n"
+		  "\t\tu32 nxt = 0xFFFFFFFFu;\n"
+		  "\t\tconst bool more = c1 < A.nchunks;\n"
+		  "\t\tovf |= p0::phase2(A, c0, epoch, s_cnt[b], &s_pre, lds + b * %uu, lds + %uu, vb0, n0,\n"
+		  "\t\t\t[&]() { if (more) nxt = grab(A, epoch); });\n", buf, buf, sh->stage_off);
+		o("\t\t__syncthreads();   /* s_chunk, and the buffer the next chunk stages into */\n"
+		  "\t\tif (tid == 0)\n\t\t\ts_chunk = nxt;\n\t\t__syncthreads();\n"
+		  "\t\tc0 = c1;\n\t\tc1 = uni(s_chunk);\n\t\tn0 = n1;\n\t\tvb0 = vb1;\n\t\tb ^= 1u;\n\t}\n");
+	} else {
+		o("\tfor (;;) {\n\t\tconst u32 c = uni(s_chunk);\n\t\tif (c >= A.nchunks)\n\t\t\tbreak;\n");
+		o("\t\tu32 nxt = 0;\n");
+		o("\t\tovf |= p0::chunk(A, c, epoch, s_cnt[0], &s_pre, lds, [&]() { nxt = grab(A, epoch); });\n");
+		o("\t\t__syncthreads();\n\t\tif (tid == 0)\n\t\t\ts_chunk = nxt;\n\t\t__syncthreads();\n\t}\n");
+	}
 	o("\tif (ovf)\n\t\tatomicOr(&A.ctrl->overflow, ovf);\n}\n");
 
 	char *m = (char *)malloc(o.s.size() + 1);
@@ -961,7 +1023,8 @@ static int env_int(const char *name, int dflt)
 extern "C" vgx_kernel *vgx_get(const vg_prog *P, int aligned, char *err, size_t errlen)
 {
 	vgx_shape sh;
-	if (vgx_shape_of(P, aligned, env_int("VG_JIT_R", 0), env_int("VG_JIT_S", 0), &sh, err, errlen) != 0)
+	if (vgx_shape_of(P, aligned, env_int("VG_JIT_R", 0), env_int("VG_JIT_S", 0), env_int("VG_JIT_PIPE", 1), &sh,
+			err, errlen) != 0)
 		return nullptr;
 	char *src = nullptr;
 	if (vgx_source(P, &sh, &src) != 0) {
@@ -1021,7 +1084,7 @@ static std::string vgxb_text(const vg_prog *const *P, const int *aligned, unsign
 	uint32_t nk = 0, maxlds = 0, maxs = 1;
 	for (unsigned i = 0; i < n; i++) {
 		vgx_shape sh;
-		if (vgx_shape_of(P[i], aligned[i], 0, 0, &sh, err, errlen) != 0)
+		if (vgx_shape_of(P[i], aligned[i], 0, 0, 0, &sh, err, errlen) != 0)
 			return std::string();
 		out_buf body;
 		gen_program(body, P[i], &sh, "pK", true);
@@ -1207,7 +1270,8 @@ extern "C" long vg_jit_source(const virg_op *ops, unsigned num_ops, unsigned pc,
 	if (vg_decode(ops, num_ops, pc, tab, nullptr, &P, err, errlen) != VIRG_SUCCESS)
 		return -1;
 	vgx_shape sh;
-	if (vgx_shape_of(&P, 1, env_int("VG_JIT_R", 0), env_int("VG_JIT_S", 0), &sh, err, errlen) != 0)
+	if (vgx_shape_of(&P, 1, env_int("VG_JIT_R", 0), env_int("VG_JIT_S", 0), env_int("VG_JIT_PIPE", 1), &sh, err,
+			errlen) != 0)
 		return -1;
 	char *src = nullptr;
 	if (vgx_source(&P, &sh, &src) != 0)
@@ -1245,7 +1309,8 @@ extern "C" long vg_jit_program(const virg_op *ops, unsigned num_ops, unsigned pc
 	vg_prog P;
 	if (vg_decode(ops, num_ops, pc, tab, nullptr, &P, err, errlen) != VIRG_SUCCESS)
 		return -1;
-	if (vgx_shape_of(&P, aligned, env_int("VG_JIT_R", 0), env_int("VG_JIT_S", 0), sh, err, errlen) != 0)
+	if (vgx_shape_of(&P, aligned, env_int("VG_JIT_R", 0), env_int("VG_JIT_S", 0), env_int("VG_JIT_PIPE", 1), sh,
+			err, errlen) != 0)
 		return -1;
 	memset(a, 0, sizeof *a);
 	vgx_immediates(&P, a);
@@ -1312,7 +1377,7 @@ extern "C" int vg_jit_compile_check(const virg_op *ops, unsigned num_ops, unsign
 	if (vg_decode(ops, num_ops, pc, tab, nullptr, &P, err, errlen) != VIRG_SUCCESS)
 		return 0;
 	vgx_shape sh;
-	if (vgx_shape_of(&P, 1, 0, 0, &sh, err, errlen) != 0)
+	if (vgx_shape_of(&P, 1, 0, 0, env_int("VG_JIT_PIPE", 1), &sh, err, errlen) != 0)
 		return 0;
 	char *src = nullptr;
 	if (vgx_source(&P, &sh, &src) != 0)

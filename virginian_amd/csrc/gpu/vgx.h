@@ -26,13 +26,17 @@ typedef struct {
 	                            * 0: loaded into registers tile by tile, and those
 	                            * the result phase needs loaded again like the
 	                            * result columns (for the rows selected) */
+	int pipe;                  /* 1: the kernel counts its next chunk before it looks
+	                            * back for the current one (two staging buffers) */
 } vgx_shape;
 
 typedef struct vgx_kernel vgx_kernel;
 
-/* shape for a program (R_override, S_override: 0 = chosen from the program) */
-int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S_override, vgx_shape *sh, char *err,
-	size_t errlen);
+/* shape for a program (R_override, S_override: 0 = chosen from the program;
+ * pipe: for the one-query kernel that overlaps a chunk's look-back with
+ * the next chunk's filter phase) */
+int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S_override, int pipe, vgx_shape *sh,
+	char *err, size_t errlen);
 /* the immediates of P as its kernel reads them (a->imm, a->oimm) */
 void vgx_immediates(const vg_prog *P, vgx_launch_args *a);
 /* kernel source (malloc'd) */
