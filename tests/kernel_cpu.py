@@ -40,7 +40,7 @@ MAX_COLS, MAX_UOPS, MAX_OUT = 17, 48, 16
 class Shape(C.Structure):
     _fields_ = [("R", C.c_int), ("S", C.c_uint32), ("TR", C.c_uint32), ("fbytes_row", C.c_uint32),
                 ("ftile_bytes", C.c_uint32), ("stage_off", C.c_uint32), ("stage_wave_bytes", C.c_uint32),
-                ("lds_bytes", C.c_uint32), ("aligned", C.c_int), ("staged", C.c_int), ("pipe", C.c_int)]
+                ("lds_bytes", C.c_uint32), ("aligned", C.c_int), ("staged", C.c_int), ("pipe", C.c_int), ("held", C.c_int)]
 
 
 class TabDesc(C.Structure):
@@ -119,11 +119,12 @@ static void kc_lane_main()
 		/* as the pipelined kernel: chunk c+1's filter phase into the other
 		 * buffer, then chunk c's result phase */
 		const unsigned buf = kc_stage_off / 2;
+		p0::held H[2];           /* a held shape's registers for the two chunks (this lane's) */
 		u32 b = 0, n0 = 0;
 		u64 vb0 = 0;
 		if (nc) {
-			p0::stage(*kc_A, 0, kc_lds);
-			n0 = p0::phase1(*kc_A, 0, 0ull, s_cnt[0], kc_lds, vb0);
+			p0::stage(*kc_A, 0, kc_lds, H[0]);
+			n0 = p0::phase1(*kc_A, 0, 0ull, s_cnt[0], kc_lds, vb0, H[0]);
 		}
 		for (u32 c = 0; c < nc; c++) {
 			__syncthreads();
@@ -131,11 +132,11 @@ static void kc_lane_main()
 			u64 vb1 = 0;
 			u8 *const buf1 = kc_lds + (b ^ 1u) * buf;
 			if (c + 1 < nc) {
-				p0::stage(*kc_A, c + 1, buf1);
-				n1 = p0::phase1(*kc_A, c + 1, 0ull, s_cnt[b ^ 1u], buf1, vb1);
+				p0::stage(*kc_A, c + 1, buf1, H[b ^ 1u]);
+				n1 = p0::phase1(*kc_A, c + 1, 0ull, s_cnt[b ^ 1u], buf1, vb1, H[b ^ 1u]);
 			}
 			kc_ovf |= p0::phase2(*kc_A, c, 0ull, s_cnt[b], &s_pre, kc_lds + b * buf, kc_lds + kc_stage_off,
-				vb0, n0, [] {});
+				vb0, n0, H[b], [] {});
 			n0 = n1;
 			vb0 = vb1;
 			b ^= 1u;

@@ -1,7 +1,9 @@
 """The specialised kernels' generated code, run on the CPU (tests/kernel_cpu.py)
-against the oracle, bit for bit and in row order: both ways of feeding the
-filter (columns staged in LDS, or loaded into registers tile by tile and
-the ones the result phase needs loaded again for the selected rows), over
+against the oracle, bit for bit and in row order: the three ways of feeding
+the filter (columns staged in LDS; loaded into registers tile by tile and
+the ones the result phase needs loaded again for the selected rows; a
+one-column filter held in registers for both phases, the default shape's
+choice for such filters), over
 the benchmark queries, the reference's KATs and suite, typed and cast
 programs, ragged layouts, random programs and a capacity that does not
 fit.  The same source is what hiprtc compiles for the GPU; the kernel
@@ -21,7 +23,7 @@ from tests.helpers import (CONFIG1_SQL, KAT_CONDITIONS, QUERY_SUITE, assert_same
                            simpledb, typed_program, typed_table)
 from virginian_amd import workload as W
 
-MODES = ["1", "0"]      # VG_JIT_STAGE: staged in LDS, registers
+MODES = ["1", "0", "held"]   # VG_JIT_STAGE: staged in LDS, registers; the default shape
 
 # A random program (seed 5, number 187 of the register-mode sweep) whose
 # filter drops the rows failing two OR-ed compares: the compiler once folded
@@ -36,9 +38,12 @@ P187 = [(0, 0, 0, 0, None), (1, 2, 0, 0, 'r0'), (2, 0, 0, 27, None), (4, 0, 2, 0
         (3, 0, 0, 0, None)]
 
 
-@pytest.fixture(params=MODES, ids=["staged", "registers"])
+@pytest.fixture(params=MODES, ids=["staged", "registers", "default"])
 def mode(request, monkeypatch):
-    monkeypatch.setenv("VG_JIT_STAGE", request.param)
+    if request.param == "held":
+        monkeypatch.delenv("VG_JIT_STAGE", raising=False)
+    else:
+        monkeypatch.setenv("VG_JIT_STAGE", request.param)
     return request.param
 
 
@@ -49,8 +54,9 @@ def _check(db, prog, what="", mode=None):
     assert flags == 0, (what, flags)
     assert total == len(next(iter(want.values()))), (what, total)
     assert_same(got, want, f"{what} {shape}")
-    if mode is not None:
+    if mode is not None and mode != "held":
         assert shape["staged"] == (1 if mode == "1" else 0) or shape["fbytes_row"] == 0, shape
+        assert shape["held"] == 0, shape
     return shape
 
 
@@ -67,7 +73,10 @@ def test_layouts(db, mode, rows, tablet_rows):
 def test_config2_thresholds(db, mode):
     db.gen_table("test", 1 << 17, 1 << 16, seed=2)
     for t in (0.1, 10.0, 90.0):
-        _check(db, config2_sql(t), f"T={t}", mode)
+        shape = _check(db, config2_sql(t), f"T={t}", mode)
+        # the benchmark query filters on one 4-byte column: the default
+        # shape holds it in registers
+        assert shape["held"] == (1 if mode == "held" else 0), shape
 
 
 def test_kats_and_suite(db, mode):

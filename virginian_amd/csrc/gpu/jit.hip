@@ -431,10 +431,12 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 	/* filter columns staged in LDS: bytes per row; with the staging and
 	 * the kernel's own LDS (under 1 KB) a workgroup takes at most 79 KB,
 	 * so two fit a CU's 160 KB */
-	uint32_t fbytes = 0;
+	uint32_t fbytes = 0, fwords = 0;
 	for (uint32_t c = 0; c < P->ncol; c++)
-		if (P->col[c].phase == VG_PHASE_FILTER)
+		if (P->col[c].phase == VG_PHASE_FILTER) {
 			fbytes += P->col[c].width == 1 ? 1u : P->col[c].width;
+			fwords += P->col[c].width == 8 ? 2u : 1u;
+		}
 	const uint32_t two_per_cu = 79u * 1024u - 4u * stage_wave;
 	uint32_t budget = two_per_cu < 64u * 1024u ? two_per_cu : 64u * 1024u;
 	/* a pipelined kernel holds two chunks' filter columns at once */
@@ -463,6 +465,20 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 		ftile = 0;
 		S = 16384u / TR;
 	}
+	/* a pipelined filter of one 4-byte (or 1-byte) column: held in
+	 * registers instead, two chunks of 8K rows at a time (2 x 32 values a
+	 * lane at 16 rows per lane), which leaves the LDS free for more
+	 * workgroups per CU and reads the column once */
+	int held = 0;
+	const char *hm = getenv("VG_JIT_HELD");
+	if (pipe && fwords == 1 && !(hm && hm[0] == '0') && !(mode && (mode[0] == '0' || mode[0] == '1'))) {
+		held = 1;
+		staged = 0;
+		R = R0;
+		TR = 256u * (uint32_t)R;
+		ftile = 0;
+		S = 8192u / TR;
+	}
 	if (S_override >= 1 && S_override <= 16)
 		S = (uint32_t)S_override;
 	if (S * (uint32_t)R > 64u) {
@@ -475,6 +491,7 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 	sh->fbytes_row = fbytes;
 	sh->ftile_bytes = ftile;
 	sh->staged = staged;
+	sh->held = held;
 	sh->stage_off = ftile * S * (pipe ? 2u : 1u);
 	sh->stage_wave_bytes = stage_wave;
 	sh->lds_bytes = sh->stage_off + 4u * sh->stage_wave_bytes;
@@ -563,8 +580,24 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 		"\t{\n"
 		"\t\tconst u32 t0 = c * S_;\n\t\tconst u32 nt = uni(min((u32)S_, A.num_tiles - t0));\n"
 		"\t\t(void)t0; (void)nt;\n";
-	o("template <typename AT>\n__device__ __attribute__((always_inline)) inline void stage(AT &A, const u32 c, u8 *lds)\n{\n");
+	/* a held shape's registers for one chunk's filter column(s) */
+	o("struct held {");
+	if (sh->held) {
+		for (uint32_t c = 0; c < P->ncol; c++)
+			if (is_filter[c]) {
+				o(" u32 f%u[S_][R];", c);
+				if (P->col[c].width == 8)
+					o(" u32 f%uh[S_][R];", c);
+			}
+	} else {
+		o(" u32 unused;");
+	}
+	o(" };\n");
+	if (!sh->held) {
+	o("template <typename AT>\n__device__ __attribute__((always_inline)) inline void stage(AT &A, const u32 c, u8 *lds, held &H)\n{\n");
+	o("\t(void)H;\n");
 	o("%s", pre_fn);
+	}
 
 	/* tile lookup */
 	const char *tile_info =
@@ -610,7 +643,8 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 	}
 	o("\t\t}\n");
 	}
-	o("\t}\n}\n\n");
+	if (!sh->held)
+		o("\t}\n}\n\n");
 
 	/* read filter column c of tile s from LDS into its slots (this lane's rows) */
 	auto lds_read = [&](out_buf &b, uint32_t c, const char *ind) {
@@ -638,11 +672,14 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 	 * groups (they read nothing and return 0), so no lane branches and all
 	 * loads in flight together; tiles past a tablet's last row are cut off
 	 * by the resource's size */
-	auto col_load = [&](uint32_t c) {
+	auto col_load = [&](uint32_t c, const char *tlo = nullptr, const char *thi = nullptr) {
 		const vg_col &cl = P->col[c];
 		const bool emit = cl.phase == VG_PHASE_EMIT;
-		char lo[32], hi[32];
-		if (emit) {
+		char lo[64], hi[64];
+		if (tlo) {
+			snprintf(lo, sizeof lo, "%s", tlo);
+			snprintf(hi, sizeof hi, "%s", thi ? thi : tlo);
+		} else if (emit) {
 			snprintf(lo, sizeof lo, "e%u", c);
 			snprintf(hi, sizeof hi, "e%uh", c);
 		} else {
@@ -680,20 +717,55 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 		o("\t\t\t\t}\n\t\t\t}\n");
 	};
 
+	/* a held shape's stage(): every tile's filter column(s) into the
+	 * chunk's registers, all loads in flight together (rows past a
+	 * tablet's end read 0 through the resource's size) */
+	if (sh->held) {
+		o("template <typename AT>\n__device__ __attribute__((always_inline)) inline void stage(AT &A, const u32 c, u8 *lds, held &H)\n{\n");
+		o("\t(void)lds;\n");
+		o("%s", pre_fn);
+		o("#pragma unroll\n\t\tfor (u32 s = 0; s < S_; ++s) {\n\t\t\tif (s >= nt)\n\t\t\t\tbreak;\n%s", tile_info);
+		o("\t\t\t(void)fast;\n\t\t\tconst u32 sel = RMASK;\n");
+		for (uint32_t c = 0; c < P->ncol; c++)
+			if (is_filter[c]) {
+				char tl[48], th[48];
+				snprintf(tl, sizeof tl, "H.f%u[s]", c);
+				snprintf(th, sizeof th, "H.f%uh[s]", c);
+				col_load(c, tl, th);
+			}
+		o("\t\t}\n\t}\n}\n\n");
+	}
+	/* a held shape: column c's values of tile s from the chunk's registers */
+	auto held_read = [&](uint32_t c, const char *ind) {
+		const vg_col &cl = P->col[c];
+		o("#pragma unroll\n%sfor (int r = 0; r < R; ++r) s%u[r] = H.f%u[s][r];\n", ind, cl.slot, c);
+		if (cl.width == 8)
+			o("#pragma unroll\n%sfor (int r = 0; r < R; ++r) s%u[r] = H.f%uh[s][r];\n", ind, cl.slot_hi, c);
+	};
+
 	/* phase 1 evaluation */
 	o("template <typename AT>\n__device__ __attribute__((always_inline)) inline u32 phase1(AT &A, const u32 c, "
-	  "const u64 epoch, u32 (*s_cnt)[4], u8 *lds, u64 &vb_out)\n{\n");
+	  "const u64 epoch, u32 (*s_cnt)[4], u8 *lds, u64 &vb_out, held &H)\n{\n");
+	o("\t(void)H;\n");
 	o("%s", pre_fn);
 	if (sh->staged)
 		o("\t\twait_vm();          /* the staged filter columns */\n");
 	o("\t\tu64 vb = 0;\n");
-	o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
+	/* a held shape indexes its registers by tile: the loop is unrolled */
+	if (sh->held)
+		o("#pragma unroll\n\t\tfor (u32 s = 0; s < S_; ++s) {\n\t\t\tif (s >= nt)\n\t\t\t\tbreak;\n%s", tile_info);
+	else
+		o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
 	o("\t\t\t(void)fast;\n");
 	o("\t\t\tu32 live = RMASK;\n\t\t\tif (nrows < TR) {\n\t\t\t\tlive = 0;\n"
 	  "#pragma unroll\n\t\t\t\tfor (int q = 0; q < NQ; ++q)\n#pragma unroll\n\t\t\t\t\tfor (int j = 0; j < 4; ++j)\n"
 	  "\t\t\t\t\t\tlive |= (wave * (NQ * 256u) + q * 256u + lane * 4u + j < nrows) ? 1u << (4 * q + j) : 0u;\n\t\t\t}\n");
 	o("%s", decl.s.c_str());
-	if (sh->staged) {
+	if (sh->held) {
+		for (uint32_t c = 0; c < P->ncol; c++)
+			if (is_filter[c])
+				held_read(c, "\t\t\t");
+	} else if (sh->staged) {
 		for (uint32_t c = 0; c < P->ncol; c++)
 			if (is_filter[c])
 				lds_read(o, c, "\t\t\t");
@@ -720,13 +792,18 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 
 	/* phase 2 */
 	o("template <typename AT, typename G>\n__device__ __attribute__((always_inline)) inline u32 phase2(AT &A, const u32 c, "
-	  "const u64 epoch, u32 (*s_cnt)[4], u64 *s_pre, u8 *lds, u8 *ost, const u64 vb, const u32 n, G grab_next)\n{\n");
-	o("\tu32 ovf = 0;\n");
+	  "const u64 epoch, u32 (*s_cnt)[4], u64 *s_pre, u8 *lds, u8 *ost, const u64 vb, const u32 n, held &H, "
+	  "G grab_next)\n{\n");
+	o("\tu32 ovf = 0;\n\t(void)H;\n");
 	o("%s", pre_fn);
 	o("\t\tu8 *const ostage = ost + wave * %uu;\n", sh->stage_wave_bytes);
 	o("\t\t/* phase 2: result columns, tile by tile */\n");
 	o("\t\tu64 pre = 0;\n\t\tu32 tdone = 0;\n");
-	o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
+	/* a held shape indexes its registers by tile: the loop is unrolled */
+	if (sh->held)
+		o("#pragma unroll\n\t\tfor (u32 s = 0; s < S_; ++s) {\n\t\t\tif (s >= nt)\n\t\t\t\tbreak;\n%s", tile_info);
+	else
+		o("\t\tfor (u32 s = 0; s < nt; ++s) {\n%s", tile_info);
 	/* the next chunk is taken only as this one's last tile starts: a chunk
 	 * taken is a chunk worked on, so later chunks never wait for one that
 	 * sits behind a whole chunk of its workgroup's work */
@@ -736,12 +813,15 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 	/* result-phase columns: project / emit columns, and filter columns
 	 * that were not staged */
 	for (uint32_t c = 0; c < P->ncol; c++)
-		if (need2[c] && !(is_filter[c] && sh->staged))
+		if (need2[c] && !(is_filter[c] && (sh->staged || sh->held)))
 			col_load(c);
-	/* filter columns again, from LDS */
+	/* filter columns again, from LDS or the registers that hold them */
 	for (uint32_t c = 0; c < P->ncol; c++)
 		if (is_filter[c] && need2[c] && sh->staged)
 			lds_read(o, c, "\t\t\t");
+	for (uint32_t c = 0; c < P->ncol; c++)
+		if (is_filter[c] && need2[c] && sh->held)
+			held_read(c, "\t\t\t");
 	/* the chunk's output offset: wave 0 looks back while the loads fly */
 	o("\t\t\tif (s == 0) {\n\t\t\t\tif (wave == 0) {\n\t\t\t\t\tbool gave = false;\n"
 	  "\t\t\t\t\tconst u64 p = lookback(A.status, c, n, lane, epoch, A.spin_limit, &gave);\n"
@@ -827,8 +907,8 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 	else
 		o("template <typename AT, typename G>\n__device__ __attribute__((always_inline)) inline u32 chunk(AT &A, "
 		  "const u32 c, const u64 epoch, u32 (*s_cnt)[4], u64 *s_pre, u8 *lds, G grab_next)\n{\n");
-	o("\tstage(A, c, lds);\n\tu64 vb = 0;\n\tconst u32 n = phase1(A, c, epoch, s_cnt, lds, vb);\n"
-	  "\treturn phase2(A, c, epoch, s_cnt, s_pre, lds, lds + %uu, vb, n, grab_next);\n}\n", sh->stage_off);
+	o("\theld H;\n\tstage(A, c, lds, H);\n\tu64 vb = 0;\n\tconst u32 n = phase1(A, c, epoch, s_cnt, lds, vb, H);\n"
+	  "\treturn phase2(A, c, epoch, s_cnt, s_pre, lds, lds + %uu, vb, n, H, grab_next);\n}\n", sh->stage_off);
 	o("} /* namespace %s */\n", ns);
 }
 
@@ -867,23 +947,53 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 		 * (thread 0's atomic flies under that tile's work): a workgroup
 		 * makes exactly one grab past the last chunk. */
 		const uint32_t buf = sh->stage_off / 2u;
+		if (!sh->held) {
+		o("\tp0::held Hd;\n");
 		o("\tu32 c0 = uni(s_chunk), c1 = 0xFFFFFFFFu, b = 0, n0 = 0;\n\tu64 vb0 = 0;\n");
-		o("\tif (c0 < A.nchunks) {\n\t\tp0::stage(A, c0, lds);\n"
-		  "\t\tn0 = p0::phase1(A, c0, epoch, s_cnt[0], lds, vb0);\n"
+		o("\tif (c0 < A.nchunks) {\n\t\tp0::stage(A, c0, lds, Hd);\n"
+		  "\t\tn0 = p0::phase1(A, c0, epoch, s_cnt[0], lds, vb0, Hd);\n"
 		  "\t\t__syncthreads();\n\t\tif (tid == 0)\n\t\t\ts_chunk = grab(A, epoch);\n\t\t__syncthreads();\n"
 		  "\t\tc1 = uni(s_chunk);\n\t}\n");
 		o("\twhile (c0 < A.nchunks) {\n"
 		  "\t\tu32 n1 = 0;\n\t\tu64 vb1 = 0;\n"
 		  "\t\tu8 *const buf1 = lds + (b ^ 1u) * %uu;\n"
-		  "\t\tif (c1 < A.nchunks) {\n\t\t\tp0::stage(A, c1, buf1);\n"
-		  "\t\t\tn1 = p0::phase1(A, c1, epoch, s_cnt[b ^ 1u], buf1, vb1);\n\t\t}\n"
+		  "\t\tif (c1 < A.nchunks) {\n\t\t\tp0::stage(A, c1, buf1, Hd);\n"
+		  "\t\t\tn1 = p0::phase1(A, c1, epoch, s_cnt[b ^ 1u], buf1, vb1, Hd);\n\t\t}\n"
 		  "\t\tu32 nxt = 0xFFFFFFFFu;\n"
 		  "\t\tconst bool more = c1 < A.nchunks;\n"
-		  "\t\tovf |= p0::phase2(A, c0, epoch, s_cnt[b], &s_pre, lds + b * %uu, lds + %uu, vb0, n0,\n"
+		  "\t\tovf |= p0::phase2(A, c0, epoch, s_cnt[b], &s_pre, lds + b * %uu, lds + %uu, vb0, n0, Hd,\n"
 		  "\t\t\t[&]() { if (more) nxt = grab(A, epoch); });\n", buf, buf, sh->stage_off);
 		o("\t\t__syncthreads();   /* s_chunk, and the buffer the next chunk stages into */\n"
 		  "\t\tif (tid == 0)\n\t\t\ts_chunk = nxt;\n\t\t__syncthreads();\n"
 		  "\t\tc0 = c1;\n\t\tc1 = uni(s_chunk);\n\t\tn0 = n1;\n\t\tvb0 = vb1;\n\t\tb ^= 1u;\n\t}\n");
+		} else {
+		/* held: the two chunks' registers take turns, the loop unrolled by
+		 * two so that each half names its own */
+		o("\tp0::held H0, H1;\n");
+		o("\tu32 c0 = uni(s_chunk), c1 = 0xFFFFFFFFu, n0 = 0;\n\tu64 vb0 = 0;\n");
+		o("\tif (c0 < A.nchunks) {\n\t\tp0::stage(A, c0, lds, H0);\n"
+		  "\t\tn0 = p0::phase1(A, c0, epoch, s_cnt[0], lds, vb0, H0);\n"
+		  "\t\t__syncthreads();\n\t\tif (tid == 0)\n\t\t\ts_chunk = grab(A, epoch);\n\t\t__syncthreads();\n"
+		  "\t\tc1 = uni(s_chunk);\n\t}\n");
+		o("\tfor (;;) {\n");
+		for (int half = 0; half < 2; half++) {
+			const char *cur = half ? "H1" : "H0", *nx = half ? "H0" : "H1";
+			const unsigned bc = half ? 1u : 0u, bn = half ? 0u : 1u;
+			o("\t\tif (c0 >= A.nchunks)\n\t\t\tbreak;\n\t\t{\n"
+			  "\t\t\tu32 n1 = 0;\n\t\t\tu64 vb1 = 0;\n"
+			  "\t\t\tif (c1 < A.nchunks) {\n\t\t\t\tp0::stage(A, c1, lds, %s);\n"
+			  "\t\t\t\tn1 = p0::phase1(A, c1, epoch, s_cnt[%u], lds, vb1, %s);\n\t\t\t}\n"
+			  "\t\t\tu32 nxt = 0xFFFFFFFFu;\n"
+			  "\t\t\tconst bool more = c1 < A.nchunks;\n"
+			  "\t\t\tovf |= p0::phase2(A, c0, epoch, s_cnt[%u], &s_pre, lds, lds + %uu, vb0, n0, %s,\n"
+			  "\t\t\t\t[&]() { if (more) nxt = grab(A, epoch); });\n"
+			  "\t\t\t__syncthreads();\n"
+			  "\t\t\tif (tid == 0)\n\t\t\t\ts_chunk = nxt;\n\t\t\t__syncthreads();\n"
+			  "\t\t\tc0 = c1;\n\t\t\tc1 = uni(s_chunk);\n\t\t\tn0 = n1;\n\t\t\tvb0 = vb1;\n\t\t}\n",
+			  nx, bn, nx, bc, sh->stage_off, cur);
+		}
+		o("\t}\n");
+		}
 	} else {
 		o("\tfor (;;) {\n\t\tconst u32 c = uni(s_chunk);\n\t\tif (c >= A.nchunks)\n\t\t\tbreak;\n");
 		o("\t\tu32 nxt = 0;\n");

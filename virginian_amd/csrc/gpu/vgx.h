@@ -28,6 +28,9 @@ typedef struct {
 	                            * result columns (for the rows selected) */
 	int pipe;                  /* 1: the kernel counts its next chunk before it looks
 	                            * back for the current one (two staging buffers) */
+	int held;                  /* 1: the filter's one column is loaded into registers
+	                            * once per chunk and held there for both phases (no
+	                            * LDS staging, no second read) */
 } vgx_shape;
 
 typedef struct vgx_kernel vgx_kernel;
