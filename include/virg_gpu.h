@@ -77,7 +77,11 @@ int vg_jit_policy(int policy);
 /* What runs q: *fused = 1 for its specialised single-pass kernel (rows per
  * lane, tiles per chunk, workgroups, compile milliseconds; *mode bit 0: the
  * variant for 16-byte aligned columns, bit 1: filter columns staged in
- * LDS), 0 for the interpreter's count and write passes. */
+ * LDS, bit 2: a one-column filter held in registers for both phases, bit
+ * 3: pipelined (a workgroup counts its next chunk before it looks back for
+ * the current one), bits 8-15: the waves per SIMD the kernel was compiled
+ * for, 0 for the compiler's choice), 0 for the interpreter's count and
+ * write passes. */
 int vg_query_kernel(vg_query *q, int *fused, int *rows_per_lane, unsigned *tiles_per_chunk,
 	unsigned *grid, double *compile_ms, int *mode);
 

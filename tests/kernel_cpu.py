@@ -40,7 +40,8 @@ MAX_COLS, MAX_UOPS, MAX_OUT = 17, 48, 16
 class Shape(C.Structure):
     _fields_ = [("R", C.c_int), ("S", C.c_uint32), ("TR", C.c_uint32), ("fbytes_row", C.c_uint32),
                 ("ftile_bytes", C.c_uint32), ("stage_off", C.c_uint32), ("stage_wave_bytes", C.c_uint32),
-                ("lds_bytes", C.c_uint32), ("aligned", C.c_int), ("staged", C.c_int), ("pipe", C.c_int), ("held", C.c_int)]
+                ("lds_bytes", C.c_uint32), ("aligned", C.c_int), ("staged", C.c_int), ("pipe", C.c_int), ("held", C.c_int),
+                ("waves", C.c_int)]
 
 
 class TabDesc(C.Structure):

@@ -49,6 +49,24 @@ def test_config2(gpu_db, jit, threshold):
     assert_same(got, want, sql)
 
 
+def test_held_launch_bound(gpu_db, jit):
+    """A one-column filter is held in registers.  The benchmark query's
+    kernel is compiled for three waves per SIMD (a few dwords spilled).
+    Selecting all six columns spills well past the limit at that bound, so
+    its kernel is rebuilt at the compiler's own occupancy.  Both are bit
+    exact against the oracle."""
+    gpu_db.gen_table("test", 4 << 20, 1 << 20, seed=4)
+    cases = [(config2_sql(10.0), 3),
+             ("select id, uniformf, normalf5, uniformi, normali20, normalf20 from test where uniformf < 50.0", 0)]
+    for sql, waves in cases:
+        got, k = _run(gpu_db, sql)
+        print(f"{sql}: {len(next(iter(got.values())))} rows, kernel {k}", flush=True)
+        assert k["held"] and k["pipelined"] and k["waves"] == waves, k
+        want, ill = oracle.select(gpu_db, sql, threads=8)
+        assert ill == 0
+        assert_same(got, want, sql)
+
+
 @pytest.mark.parametrize("tablet_rows", [0, 1 << 20])
 def test_config1(gpu_db, jit, tablet_rows):
     gpu_db.gen_table("test", 1 << 20, tablet_rows, seed=1)

@@ -683,7 +683,8 @@ class DeviceQuery:
         f, r, sc, g, ms, m = C.c_int(), C.c_int(), C.c_uint(), C.c_uint(), C.c_double(), C.c_int()
         lib.vg_query_kernel(self.h, C.byref(f), C.byref(r), C.byref(sc), C.byref(g), C.byref(ms), C.byref(m))
         return {"fused": bool(f.value), "rows_per_lane": r.value, "tiles_per_chunk": sc.value,
-                "grid": g.value, "compile_ms": ms.value, "aligned": bool(m.value & 1), "staged": bool(m.value & 2)}
+                "grid": g.value, "compile_ms": ms.value, "aligned": bool(m.value & 1), "staged": bool(m.value & 2),
+                "held": bool(m.value & 4), "pipelined": bool(m.value & 8), "waves": (m.value >> 8) & 0xFF}
 
     def time_passes(self, warmup: int, iters: int) -> tuple[float, float, int]:
         """(count-pass ms, write-pass ms, rows) from HIP events on the query's stream."""

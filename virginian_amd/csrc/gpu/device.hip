@@ -1677,8 +1677,15 @@ extern "C" int vg_query_kernel(vg_query *q, int *fused, int *rows_per_lane, unsi
 	unsigned *grid, double *compile_ms, int *mode)
 {
 	*fused = q->jk != nullptr;
-	/* bit 0: the aligned variant; bit 1: filter columns staged in LDS */
-	*mode = q->jk ? (vgx_kernel_shape(q->jk)->aligned ? 1 : 0) | (vgx_kernel_shape(q->jk)->staged ? 2 : 0) : 0;
+	/* bit 0: the aligned variant; bit 1: filter columns staged in LDS;
+	 * bit 2: the filter column held in registers; bit 3: pipelined; bits
+	 * 8-15: the waves per SIMD it was compiled for (0: the compiler's) */
+	*mode = 0;
+	if (q->jk) {
+		const vgx_shape *sh = vgx_kernel_shape(q->jk);
+		*mode = (sh->aligned ? 1 : 0) | (sh->staged ? 2 : 0) | (sh->held ? 4 : 0) | (sh->pipe ? 8 : 0) |
+			((sh->waves & 0xff) << 8);
+	}
 	*rows_per_lane = q->jk ? vgx_kernel_shape(q->jk)->R : 0;
 	*tiles_per_chunk = q->jk ? vgx_kernel_shape(q->jk)->S : 0;
 	*grid = q->jk ? q->jgrid : 0;

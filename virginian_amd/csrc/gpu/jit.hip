@@ -492,6 +492,15 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 	sh->ftile_bytes = ftile;
 	sh->staged = staged;
 	sh->held = held;
+	/* a held shape keeps two chunks' filter values and a tile's result
+	 * loads in VGPRs (about 180 at 16 rows per lane, two waves per SIMD);
+	 * compiled for three waves per SIMD it fits 168 with a few dwords
+	 * spilled and runs three workgroups per CU, which hides more of each
+	 * chunk's load and look-back latency */
+	sh->waves = held ? 3 : 0;
+	const char *wv = getenv("VG_JIT_WAVES");
+	if (wv && *wv)
+		sh->waves = atoi(wv);
 	sh->stage_off = ftile * S * (pipe ? 2u : 1u);
 	sh->stage_wave_bytes = stage_wave;
 	sh->lds_bytes = sh->stage_off + 4u * sh->stage_wave_bytes;
@@ -926,7 +935,10 @@ extern "C" int vgx_source(const vg_prog *P, const vgx_shape *sh, char **out_src)
 	out_buf o;
 	gen_prelude(o);
 	gen_program(o, P, sh, "p0");
-	o("\nextern \"C\" __global__ __launch_bounds__(256) void vgx_main(const vgx_args A)\n{\n");
+	if (sh->waves > 0)
+		o("\nextern \"C\" __global__ __launch_bounds__(256, %d) void vgx_main(const vgx_args A)\n{\n", sh->waves);
+	else
+		o("\nextern \"C\" __global__ __launch_bounds__(256) void vgx_main(const vgx_args A)\n{\n");
 	o("\textern __shared__ __attribute__((aligned(16))) u8 lds[];\n");
 	o("\t__shared__ u32 s_cnt[%u][%u][4];\n\t__shared__ u64 s_pre;\n\t__shared__ u32 s_chunk;\n\t__shared__ u32 s_epoch;\n",
 	  sh->pipe ? 2u : 1u, sh->S);
@@ -1020,7 +1032,13 @@ struct vgx_kernel {
 	double compile_ms = 0;
 };
 
+/* scratch a lane may use for spills under a shape's launch bound (the
+ * benchmark's held filter spills 24 bytes at three waves per SIMD) */
+#define VGX_MAX_SPILL_BYTES 64
+
 static std::mutex g_mu;
+/* compiled kernels by source, kept for the life of the process (a kernel
+ * rebuilt without its launch bound is found under both sources) */
 static std::unordered_map<std::string, vgx_kernel *> g_cache;
 static std::string g_arch;
 
@@ -1148,6 +1166,31 @@ extern "C" vgx_kernel *vgx_get(const vg_prog *P, int aligned, char *err, size_t 
 	if (it != g_cache.end())
 		return it->second;
 	vgx_kernel *k = compile(key, sh, err, errlen);
+	if (k && sh.waves > 0) {
+		/* the launch bound may make the compiler spill; a few dwords a
+		 * lane cost less than the occupancy they buy, more do not: then
+		 * the kernel is built again at the compiler's own occupancy */
+		int scratch = 0;
+		if (hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, k->fn) != hipSuccess) {
+			(void)hipGetLastError();
+			scratch = 0;
+		}
+		if (scratch > VGX_MAX_SPILL_BYTES) {
+			(void)hipModuleUnload(k->mod);
+			delete k;
+			sh.waves = 0;
+			if (vgx_source(P, &sh, &src) != 0) {
+				snprintf(err, errlen, "out of memory");
+				return nullptr;
+			}
+			std::string key2(src);
+			free(src);
+			auto it2 = g_cache.find(key2);
+			k = it2 != g_cache.end() ? it2->second : compile(key2, sh, err, errlen);
+			if (k)
+				g_cache[key2] = k;
+		}
+	}
 	if (k)
 		g_cache[key] = k;
 	return k;
@@ -1487,7 +1530,8 @@ extern "C" int vg_jit_compile_check(const virg_op *ops, unsigned num_ops, unsign
 	if (vg_decode(ops, num_ops, pc, tab, nullptr, &P, err, errlen) != VIRG_SUCCESS)
 		return 0;
 	vgx_shape sh;
-	if (vgx_shape_of(&P, 1, 0, 0, env_int("VG_JIT_PIPE", 1), &sh, err, errlen) != 0)
+	if (vgx_shape_of(&P, 1, env_int("VG_JIT_R", 0), env_int("VG_JIT_S", 0), env_int("VG_JIT_PIPE", 1), &sh, err,
+			errlen) != 0)
 		return 0;
 	char *src = nullptr;
 	if (vgx_source(&P, &sh, &src) != 0)

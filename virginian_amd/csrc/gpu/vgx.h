@@ -6,6 +6,7 @@
 #define VGX_H
 
 #include <stdint.h>
+#include <hip/hip_runtime.h>
 #include "vg_prog.h"
 #include "vgx_abi.h"
 
@@ -31,6 +32,8 @@ typedef struct {
 	int held;                  /* 1: the filter's one column is loaded into registers
 	                            * once per chunk and held there for both phases (no
 	                            * LDS staging, no second read) */
+	int waves;                 /* minimum waves per SIMD the kernel is compiled for
+	                            * (its launch bound), 0 = the compiler's choice */
 } vgx_shape;
 
 typedef struct vgx_kernel vgx_kernel;
@@ -57,9 +60,7 @@ vgx_kernel *vgxb_get(const vg_prog *const *P, const int *aligned, unsigned n, ui
 	size_t errlen);
 unsigned vgxb_grid(const vgx_kernel *k, unsigned nitems, unsigned cus);
 
-#ifdef __HIP_PLATFORM_AMD__
 int vgx_launch(const vgx_kernel *k, const vgx_launch_args *a, unsigned grid, hipStream_t stream);
 int vgxb_launch(const vgx_kernel *k, const vgxb_args *a, unsigned grid, hipStream_t stream);
-#endif
 
 #endif
