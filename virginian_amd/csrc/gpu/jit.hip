@@ -76,8 +76,13 @@ typedef const __attribute__((address_space(4))) vgx_args cvgx_args;
 #define ST_EMASK ((1ull << 24) - 1)
 
 __device__ __forceinline__ u32 uni(u32 x) { return __builtin_amdgcn_readfirstlane(x); }
+/* a uniform read of read-only metadata through the constant address space
+ * (a scalar load).  Scalars only: a struct copy is rewritten into loads
+ * from the global address space, which become vector loads (a wait for
+ * every load in flight before their values are used) */
 template <typename T> __device__ __forceinline__ T ldc(const T *p)
 {
+	static_assert(__is_scalar(T), "ldc reads scalars; read a struct field by field");
 	return *(const __attribute__((address_space(4))) T *)p;
 }
 template <typename T> __device__ __forceinline__ const __attribute__((address_space(1))) T *gp(const void *p)
@@ -610,7 +615,7 @@ static void gen_program(out_buf &o, const vg_prog *P, const vgx_shape *sh, const
 
 	/* tile lookup */
 	const char *tile_info =
-		"\t\t\tconst vgx_tile ti = ldc(&A.tiles[t0 + s]);\n"
+		"\t\t\tconst vgx_tile ti = {ldc(&A.tiles[t0 + s].tab), ldc(&A.tiles[t0 + s].row0)};\n"
 		"\t\t\tconst vgx_tab *tb = A.tabs + ti.tab;\n"
 		"\t\t\tconst u32 trows = ldc(&tb->rows);\n"
 		"\t\t\tconst u32 nrows = uni(min((u32)TR, trows - ti.row0));\n"
@@ -1323,7 +1328,7 @@ extern "C" __global__ __launch_bounds__(256) void vgxb_reset(const vgxb_args B)
 		const u32 i = uni(s_item);
 		if (i >= B.nitems)
 			break;
-		const vgxb_item it = ldc(B.items + i);
+		const vgxb_item it = {ldc(&B.items[i].q), ldc(&B.items[i].c)};
 		const u32 q = uni(it.q), c = uni(it.c);
 		cvgx_args &A = *(cvgx_args *)(B.qa + q);
 		const next_item next = {B.state, B.nitems, epoch, &s_next};
