@@ -477,17 +477,27 @@ static void finish(vg_server *s, request &r, uint64_t total, unsigned flags)
 		r.host.assign(r.widths.size(), std::vector<char>());
 		size_t o = r.off;
 		const uint64_t c = r.cap ? r.cap : 1;
+		bool copied = true;
 		for (size_t j = 0; j < r.widths.size(); j++) {
 			r.host[j].resize(total * r.widths[j]);
 			if (s->hbase)               /* already in host memory: no copy engine */
 				memcpy(r.host[j].data(), s->hbase + o, total * r.widths[j]);
 			else if (total && hipMemcpyAsync(r.host[j].data(), s->ar.base + o, total * r.widths[j],
 					hipMemcpyDeviceToHost, s->copy) != hipSuccess)
-				(void)hipGetLastError();
+				copied = false;
 			o += round256(c * r.widths[j]);
 		}
 		if (!s->hbase && hipStreamSynchronize(s->copy) != hipSuccess)
+			copied = false;
+		if (!copied) {
+			/* no result to hand over: the request fails, it is not done */
 			(void)hipGetLastError();
+			std::vector<std::vector<char>>().swap(r.host);
+			snprintf(s->msg, sizeof s->msg, "a kept result could not be copied to host memory");
+			release_buffers(s, r);
+			fail_request(s, r);
+			return;
+		}
 	}
 	r.state = R_DONE;
 	/* complete in HBM when the batch stored its outcome (kept results:
