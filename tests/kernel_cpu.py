@@ -218,7 +218,8 @@ def _compile(text: str) -> C.CDLL:
     h = hashlib.sha256(text.encode()).hexdigest()[:24]
     so = CACHE / f"k_{h}.so"
     if not so.exists():
-        cpp = CACHE / f"k_{h}.cpp"
+        # per process: parallel test workers may compile the same source at once
+        cpp = CACHE / f"k_{h}.{os.getpid()}.cpp"
         cpp.write_text(text)
         tmp = CACHE / f"k_{h}.{os.getpid()}.so"
         r = subprocess.run([CXX, "-std=c++17", "-O1", "-ffp-contract=off", "-fPIC", "-shared",
