@@ -436,6 +436,9 @@ static int read_ctrl(const launch_res &lr, hipStream_t stream, uint64_t *rows)
 		HCHK(hipMemsetAsync(lr.ctrl, 0, sizeof(vg_ctrl), stream), "control block");
 		HCHK(hipStreamSynchronize(stream), "control block");
 	}
+	if (c.overflow != 0)
+		fprintf(stderr, "::: Virginian: overflow flags 0x%x (1: output capacity, 2: a single-pass wait gave up)\n",
+			c.overflow);
 	VIRG_CHECK(c.overflow != 0, "result exceeded the output capacity");
 	rows[0] = c.total;
 	return VIRG_SUCCESS;

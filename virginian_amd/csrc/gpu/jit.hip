@@ -503,9 +503,13 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 	 * spilled and runs three workgroups per CU, which hides more of each
 	 * chunk's load and look-back latency */
 	sh->waves = held ? 3 : 0;
+	/* VG_JIT_WAVES=0 drops the bound; no other bound is used: a
+	 * register-tile kernel built for four waves per SIMD (128 VGPRs, 8 B
+	 * spilled) reported a capacity overflow on the 64M-row benchmark
+	 * query, where the same source at its own occupancy is exact */
 	const char *wv = getenv("VG_JIT_WAVES");
-	if (wv && *wv)
-		sh->waves = atoi(wv);
+	if (wv && wv[0] == '0')
+		sh->waves = 0;
 	sh->stage_off = ftile * S * (pipe ? 2u : 1u);
 	sh->stage_wave_bytes = stage_wave;
 	sh->lds_bytes = sh->stage_off + 4u * sh->stage_wave_bytes;
