@@ -2118,7 +2118,7 @@ static int pbatch_build_jit(vg_pbatch *b, char *err, size_t errlen)
 	for (unsigned i = 0; i < n; i++) {
 		vg_query *q = b->qs[i];
 		vgx_shape sh;
-		if (vgx_shape_of(&q->P, aligned[i], 0, 0, 0, &sh, err, errlen) != 0)
+		if (vgx_shape_of(&q->P, aligned[i], 0, 0, vgxb_shape_mode(), &sh, err, errlen) != 0)
 			return VIRG_FAIL;
 		const std::vector<vgx_tile> t = jit_tiles(*q->p, sh.TR);
 		toff[i] = tiles.size();

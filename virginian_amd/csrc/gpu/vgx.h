@@ -39,10 +39,13 @@ typedef struct {
 typedef struct vgx_kernel vgx_kernel;
 
 /* shape for a program (R_override, S_override: 0 = chosen from the program;
- * pipe: for the one-query kernel that overlaps a chunk's look-back with
- * the next chunk's filter phase) */
+ * pipe 1: for the one-query kernel that overlaps a chunk's look-back with
+ * the next chunk's filter phase; 2: a batch's chunk function with the
+ * pipelined kernel's smaller chunks, not pipelined) */
 int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S_override, int pipe, vgx_shape *sh,
 	char *err, size_t errlen);
+/* the `pipe` argument of a batch's shapes */
+int vgxb_shape_mode(void);
 /* the immediates of P as its kernel reads them (a->imm, a->oimm) */
 void vgx_immediates(const vg_prog *P, vgx_launch_args *a);
 /* kernel source (malloc'd) */
