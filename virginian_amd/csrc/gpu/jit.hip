@@ -414,15 +414,21 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 	sh->aligned = aligned ? 1 : 0;
 	/* pipe 2: a batch shape with the pipelined shape's staging budget and
 	 * chunk size (half the staging), neither pipelined nor held */
-	const bool small = pipe != 0;
+	const bool small = pipe != 0, batch = pipe == 2;
 	pipe = pipe == 1;
 	sh->pipe = pipe ? 1 : 0;
 	if (P->nuop > VG_MAX_UOPS || P->nout > VG_MAX_OUT) {
 		snprintf(err, errlen, "program too large");
 		return -1;
 	}
-	/* rows per lane from the register file size (values are held in VGPRs) */
-	int R = P->nslot <= 10 ? 16 : P->nslot <= 20 ? 8 : 4;
+	/* rows per lane from the register file size (values are held in VGPRs);
+	 * a batch's chunk functions run at three workgroups per CU (168
+	 * VGPRs), so they take fewer rows per lane from 8 slots on (1.316
+	 * against 1.336 ms per run of the 256-program batch; from 6 slots:
+	 * 1.396).  VG_JIT_BATCH_RSLOTS: the most slots at 16 rows per lane */
+	const char *rse = getenv("VG_JIT_BATCH_RSLOTS");
+	const uint32_t rs = !batch ? 10u : rse && *rse ? (uint32_t)atoi(rse) : 7u;
+	int R = P->nslot <= rs ? 16 : P->nslot <= 2 * rs ? 8 : 4;
 	if (R_override == 4 || R_override == 8 || R_override == 16)
 		R = R_override;
 	/* output staging: per wave, 256 rows x the widest group of outputs
