@@ -634,22 +634,34 @@ def dropin_times(db, per: list, a) -> dict:
 PMC_RUNS = 3
 
 
-def pmc_run(va, queries) -> int:
+def pmc_run(va, db, queries) -> int:
     """Counter runs (rocprofv3 --pmc): each query's kernel PMC_RUNS times
     in threshold order, one at a time, so the dispatches can be told apart
-    by their order."""
+    by their order; then the interpreter's count and write passes on the
+    same queries, PMC_RUNS times each (the kernels before specialisation)."""
     for q in queries:
         for _ in range(PMC_RUNS):
             q.run()
-    print(json.dumps({"pmc_run": [q.last_rows for q in queries], "runs_per_query": PMC_RUNS}))
+    interp = []
+    for t in THRESHOLDS:
+        qi = db.prepare(sql_for(t), flags=va.VG_QUERY_INTERP)
+        for _ in range(PMC_RUNS):
+            qi.run()
+        interp.append(qi.last_rows)
+        qi.close()
+    print(json.dumps({"pmc_run": [q.last_rows for q in queries], "interpreter": interp,
+                      "runs_per_query": PMC_RUNS}))
     return 0
 
 
-def summarize_counters(outdir: str, dest: str) -> int:
+def summarize_counters(outdir: str, dest: str, workload: str = "batched") -> int:
     """Per-kernel means of every counter in the rocprofv3 --pmc runs under
-    outdir (one directory per pass), for the batched workload's counter
-    runs (--workload batched --pmc-run): dispatches, and per dispatch and
-    per run the counter values (FETCH_SIZE and WRITE_SIZE in KiB)."""
+    outdir (one directory per pass), for a workload's counter runs
+    (--workload W --pmc-run: the batched workload, or the resident one):
+    dispatches, and per dispatch and per run the counter values
+    (FETCH_SIZE and WRITE_SIZE in KiB)."""
+    wl = "--workload batched " if workload == "batched" else ""
+    tag = "batched" if workload == "batched" else "config2"
     import csv
     from collections import defaultdict
     from pathlib import Path as P
@@ -667,13 +679,13 @@ def summarize_counters(outdir: str, dest: str) -> int:
            "commands": [
                "cd /tmp && export TMPDIR=/tmp",
                "rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_SMEM SQ_INSTS_LDS SQ_WAVES SQ_WAVE_CYCLES "
-               "SQ_BUSY_CYCLES SQ_WAIT_ANY --output-format csv -d <out>/sq -o batched -- python3 -u <repo>/bench.py "
-               "--workload batched --pmc-run",
-               "rocprofv3 --pmc FETCH_SIZE --output-format csv -d <out>/fetch -o batched -- python3 -u "
-               "<repo>/bench.py --workload batched --pmc-run",
-               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d <out>/write -o batched -- python3 -u "
-               "<repo>/bench.py --workload batched --pmc-run",
-               f"python3 <repo>/bench.py --summarize-counters <out> {dest}"],
+               f"SQ_BUSY_CYCLES SQ_WAIT_ANY --output-format csv -d <out>/sq -o {tag} -- python3 -u <repo>/bench.py "
+               f"{wl}--pmc-run",
+               f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d <out>/fetch -o {tag} -- python3 -u "
+               f"<repo>/bench.py {wl}--pmc-run",
+               f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d <out>/write -o {tag} -- python3 -u "
+               f"<repo>/bench.py {wl}--pmc-run",
+               f"python3 <repo>/bench.py {wl}--summarize-counters <out> {dest}"],
            "kernels": kernels,
            "hardware": "MI355X (gfx950, 256 CUs)"}
     P(dest).write_text(json.dumps(doc, indent=1) + "\n")
@@ -815,7 +827,7 @@ def main() -> int:
     if a.summarize_pmc:
         return summarize_pmc(*a.summarize_pmc)
     if a.summarize_counters:
-        return summarize_counters(*a.summarize_counters)
+        return summarize_counters(*a.summarize_counters, workload=a.workload)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the executor on one GPU; figures for several GPUs are "
               "not produced (tablets shard across GPUs by design, but no multi-GPU path is "
@@ -852,7 +864,7 @@ def main() -> int:
         q.run()
     kinfo = queries[0].kernel()
     if a.pmc_run:
-        return pmc_run(va, queries)
+        return pmc_run(va, db, queries)
 
     # per query: kernel time back to back (HIP events on the query's
     # stream), beside the interpreter's count + write passes for the same
