@@ -169,6 +169,104 @@ def bench_config1(a) -> int:
     return 0
 
 
+# the reference's published mean query time for its 10-query suite
+# (virginian_amd.workload.SUITE_QUERIES) over the 8M-row table of
+# db/generate.c, on a GTX 570 / i7-960 (paper section 7, Table 1), in ms
+SUITE_PUBLISHED_MS = {"cached_gpu": 14.0, "mapped_gpu": 29.0, "multi_core_cpu": 113.0, "single_core_cpu": 505.0}
+
+
+def bench_suite(a) -> int:
+    """The reference's own published benchmark on this GPU: the 10-query
+    suite over an 8M-row table of its generator's distributions.  Per
+    query: the specialised kernel with the table resident in HBM (HIP
+    events; the reference's "cached GPU" time is its kernel time,
+    src/test/timing.cc:74) and the interpreter's passes, the drop-in
+    virg_vm_execute with the tablets in the device cache and with
+    use_mmap (the host tablets streamed over the link: the reference's
+    mapped mode), and the CPU VM (the oracle) on one core and on every
+    core, each beside the paper's Table 1 figure.  value = the mean
+    resident query time."""
+    import statistics
+    import virginian_amd as va
+    from oracle import oracle
+    from virginian_amd.workload import SUITE_QUERIES, suite_table
+    rows = a.rows or 8_000_000
+    db = va.Virginian()
+    name, cus = db.device()
+    if cus == 0:
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+    t0 = time.perf_counter()
+    suite_table(db, rows, a.tablet_rows, a.seed)
+    print(f"# the reference's suite table: {rows} rows in {a.tablet_rows}-row tablets, built in "
+          f"{time.perf_counter() - t0:.1f}s on {name} ({cus} CUs)", file=sys.stderr, flush=True)
+    per = []
+    for i, sql in enumerate(SUITE_QUERIES):
+        q = db.prepare(sql)
+        q.run()
+        ms, n = q.time(a.warmup, a.steps)
+        fused = q.kernel()["fused"]
+        read_b, write_b = q.rows * q.read_bytes_per_row, n * q.write_bytes_per_row
+        q.close()
+        qi = db.prepare(sql, flags=va.VG_QUERY_INTERP)
+        qi.run()
+        ms_i, _ = qi.time(a.warmup, a.steps)
+        qi.close()
+        _execute_ms(va, db, sql)
+        warm = statistics.median(_execute_ms(va, db, sql)[0] for _ in range(3))
+        db.use_mmap = True
+        try:
+            mapped = statistics.median(_execute_ms(va, db, sql)[0] for _ in range(3))
+        finally:
+            db.use_mmap = False
+        per.append({"query": i, "selected": n, "selectivity": round(n / q.rows, 5), "kernel_ms": round(ms, 4),
+                    "specialised": fused, "read_bytes": read_b, "write_bytes": write_b,
+                    "interpreter_ms": round(ms_i, 4), "dropin_cached_ms": round(warm, 3),
+                    "dropin_mapped_ms": round(mapped, 3)})
+        print(f"# Q{i}: {n} rows ({n / q.rows:.2%}) {ms * 1e3:.1f} us resident ({'specialised' if fused else 'interpreter'})"
+              f", interpreter {ms_i * 1e3:.1f} us; virg_vm_execute cached {warm:.2f} ms, mapped {mapped:.2f} ms",
+              file=sys.stderr, flush=True)
+    # the CPU VM on the same table: one pass of the suite on every core, one on one core
+    threads = cpu_threads()
+    t0 = time.perf_counter()
+    for sql in SUITE_QUERIES:
+        oracle.select(db, sql, threads=threads, ordered=False)
+    cpu_multi = (time.perf_counter() - t0) * 1e3 / len(SUITE_QUERIES)
+    t0 = time.perf_counter()
+    for sql in SUITE_QUERIES:
+        oracle.select(db, sql)
+    cpu_single = (time.perf_counter() - t0) * 1e3 / len(SUITE_QUERIES)
+    print(f"# CPU VM mean per query: {cpu_single:.1f} ms on one core, {cpu_multi:.1f} ms on {threads}",
+          file=sys.stderr, flush=True)
+    mean = statistics.mean(p["kernel_ms"] for p in per)
+    alg = statistics.mean(p["read_bytes"] + p["write_bytes"] for p in per)
+    means = {"resident_specialised": round(mean, 4),
+             "resident_interpreter": round(statistics.mean(p["interpreter_ms"] for p in per), 4),
+             "dropin_cached": round(statistics.mean(p["dropin_cached_ms"] for p in per), 3),
+             "dropin_mapped": round(statistics.mean(p["dropin_mapped_ms"] for p in per), 3),
+             "cpu_multi_core": round(cpu_multi, 2), "cpu_single_core": round(cpu_single, 2)}
+    line = {
+        "metric": "mean query time of the reference's 10-query filter SELECT suite (src/test/queries.h), 8M rows",
+        "value": round(mean, 4), "unit": "ms", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(mean * len(SUITE_QUERIES), 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": round(mean / SUITE_PUBLISHED_MS["cached_gpu"], 6), "dtype": "i32+f32",
+        "data": "synthetic (the reference generator's distributions, seeded)",
+        "config": {"workload": "the reference's 10-query suite over its 8M-row test table resident in HBM",
+                   "rows": rows, "tablet_rows": a.tablet_rows, "means_ms": means,
+                   "published_ms": SUITE_PUBLISHED_MS,
+                   "published_hardware": "GTX 570 (152 GB/s) and i7-960 (4 cores, 8 threads), paper section 7",
+                   "cpu_threads": threads, "cpu_model": cpu_model(), "per_query": per},
+        "roofline": {"bound": "hbm", "achieved": round(alg / mean / 1e6, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / mean / 1e6 / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": {"value": round(cpu_multi, 2), "unit": "ms", "cores": threads, "kind": "port",
+                         "sample": f"one pass of the suite on the same table, {threads} threads; one core: "
+                                   f"{cpu_single:.1f} ms per query"},
+    }
+    print(json.dumps(line))
+    db.close()
+    return 0
+
+
 CONFIG3_SQL = ("select id, uniformi, normalf20 from test "
                "where uniformf < 9.24 and normalf5 > 2.0 or normali20 > 10")
 
@@ -792,7 +890,7 @@ def main() -> int:
     ap.add_argument("--pin-limit-gib", type=int, default=52,
                     help="out-of-core: pin at most this much tablet memory")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
-    ap.add_argument("--workload", choices=["resident", "outofcore", "batched", "served", "config1"],
+    ap.add_argument("--workload", choices=["resident", "outofcore", "batched", "served", "config1", "suite"],
                     default="resident")
     ap.add_argument("--rate", type=float, default=2000.0, help="served: arrivals per second")
     ap.add_argument("--duration", type=float, default=2.0, help="served: seconds of arrivals")
@@ -840,6 +938,8 @@ def main() -> int:
         return bench_outofcore(a)
     if a.workload == "config1":
         return bench_config1(a)
+    if a.workload == "suite":
+        return bench_suite(a)
     if a.workload == "batched":
         return bench_batched(a)
     if a.workload == "served":

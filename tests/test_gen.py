@@ -107,3 +107,35 @@ def test_device_builder_needs_a_device():
             db.gen_table("t", 100, 100, seed=1, device=True)
     finally:
         db.close()
+
+
+def test_reference_suite_table():
+    """The reference's test table (db/generate.c:41-48): int key 0..n-1,
+    C-truncated ints of U(-100, 100) and N(0, sigma), f32 floats of the
+    same.  The suite's first query selects trunc(U) > 60 and trunc(N(0, 5))
+    < 0, with probability 39/200 x Phi(-1/5) (trunc(x) < 0 iff x <= -1 for
+    the integer draws, trunc(u) > 60 iff u >= 61), checked through the
+    oracle against that closed form."""
+    from oracle import oracle
+    from virginian_amd.workload import SUITE_QUERIES, suite_table
+    rows = 400_000
+    db = va.Virginian()
+    try:
+        tid = suite_table(db, rows, 1 << 16, seed=3)
+        assert db.numrows(tid) == rows
+        got, ill = oracle.select(db, "select id, uniformi, normali5, uniformf from test where id >= 0", threads=8)
+        assert ill == 0
+        assert np.array_equal(got["id"], np.arange(rows, dtype=np.int32))
+        u, g5, uf = got["uniformi"], got["normali5"], got["uniformf"]
+        assert u.dtype == np.int32 and uf.dtype == np.float32
+        assert u.min() == -99 and u.max() == 99               # truncation toward zero, never +-100
+        assert abs(np.mean(u == 0) - 2 / 200) < 0.002         # 0 takes (-1, 1): twice a bin's share
+        assert abs(np.mean(g5 == 0) - (math.erf(1 / (5 * math.sqrt(2))))) < 0.004
+        assert -100 <= uf.min() and uf.max() < 100
+        want, ill = oracle.select(db, SUITE_QUERIES[0], threads=8)
+        assert ill == 0
+        p = 39 / 200 * 0.5 * (1 + math.erf(-0.2 / math.sqrt(2)))
+        sel = len(want["id"]) / rows
+        assert abs(sel - p) < 4 * math.sqrt(p * (1 - p) / rows), (sel, p)
+    finally:
+        db.close()

@@ -87,6 +87,20 @@ def test_kats_and_suite(gpu_db, jit):
         assert_same(got, want, q)
 
 
+def test_reference_suite_table(gpu_db, jit):
+    """The reference's published suite over a table of its own generator's
+    distributions (int key, truncated ints, src/test/queries.h over
+    db/generate.c's table) in 300K-row tablets (about the reference's 8 MB
+    tablets, the last one ragged)."""
+    from virginian_amd.workload import suite_table
+    suite_table(gpu_db, 1 << 20, 300_000, seed=5)
+    for q in QUERY_SUITE:
+        got, _ = _run(gpu_db, q)
+        want, ill = oracle.select(gpu_db, q, threads=8)
+        assert ill == 0, q
+        assert_same(got, want, q)
+
+
 def test_query_suite(gpu_db, jit):
     gpu_db.gen_table("test", 1 << 20, 1 << 18, seed=3)
     for q in QUERY_SUITE:

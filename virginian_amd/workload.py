@@ -209,3 +209,44 @@ def _negate(op: int) -> int:
 
 def log_uniform_sizes(rng: np.random.Generator, n: int, lo: float, hi: float) -> list[int]:
     return [int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(n)]
+
+
+# The reference's published benchmark queries (src/test/queries.h:1-13),
+# over the table suite_table() builds.
+SUITE_QUERIES = [
+    "select id, uniformi, normali5 from test where uniformi > 60 and normali5 < 0",
+    "select id, uniformf, normalf5 from test where uniformf > 60.0 and normalf5 < 0.0",
+    "select id, uniformi, normali5 from test where uniformi > -60 and normali5 < 5",
+    "select id, uniformf, normalf5 from test where uniformf > -60.0 and normalf5 < 5.0",
+    "select id, uniformi, normali20 from test where (normali20 + 40) > (uniformi - 10)",
+    "select id, uniformf, normalf20 from test where (normalf20 + 40.0) > (uniformf - 10.0)",
+    "select id, normali5, normali20 from test where normali5 * normali20 >= -5 and normali5 * normali20 <= 5",
+    "select id, normalf5, normalf20 from test where normalf5 * normalf20 >= -5.0 and normalf5 * normalf20 <= 5.0",
+    "select id, uniformi, normali5, normali20 from test where uniformi >= -1 and uniformi <= 1 or "
+    "normali5 >= -1 and normali5 <= 1 or normali20 >= -1 and normali20 <= 1",
+    "select id, uniformf, normalf5, normalf20 from test where uniformf >= -1.0 and uniformf <= 1.0 or "
+    "normalf5 >= -1.0 and normalf5 <= 1.0 or normalf20 >= -1.0 and normalf20 <= 1.0",
+]
+
+
+def suite_table(db, rows: int, tablet_rows: int, seed: int, name: str = "test") -> int:
+    """The reference's test table (db/generate.c:41-48, 8M rows in its
+    benchmark, src/Makefile:85): int key 0..rows-1; uniformi, normali5,
+    normali20 = C int casts (toward zero) of U(-100, 100), N(0, 5),
+    N(0, 20); uniformf, normalf5, normalf20 = the same distributions as
+    f32.  Seeded numpy draws stand in for the reference's gsl ranlxd2
+    seeded by the clock.  Returns the table id."""
+    rng = np.random.default_rng(seed)
+    tid = db.create_table(name, VIRG_INT, SCHEMA)
+    for r0 in range(0, rows, tablet_rows):
+        n = min(tablet_rows, rows - r0)
+        key = np.arange(r0, r0 + n, dtype=np.int32)
+        cols = [np.trunc(rng.uniform(-100, 100, n)).astype(np.int32),
+                np.trunc(rng.normal(0, 5, n)).astype(np.int32),
+                np.trunc(rng.normal(0, 20, n)).astype(np.int32),
+                rng.uniform(-100, 100, n).astype(np.float32),
+                rng.normal(0, 5, n).astype(np.float32),
+                rng.normal(0, 20, n).astype(np.float32)]
+        db.append_tablet(tid, key, cols)
+    return tid
+
