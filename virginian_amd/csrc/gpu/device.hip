@@ -93,7 +93,9 @@ struct vgd_device {
 	std::vector<vg_ctrl *> ctrl_free;                   /* pooled control blocks */
 	std::vector<vg_ctrl *> ctrl_chunks;
 	std::vector<launch_res> lr_free;                    /* pooled two-pass arrays */
+	struct stream_cache *scache = nullptr;              /* streaming buffers, kept between calls */
 };
+static void free_stream_cache(vgd_device *d);
 
 static void free_plan(plan &p);
 static int build_plan(vgd_device *d, const std::vector<const virg_tablet_meta *> &tabs,
@@ -209,6 +211,7 @@ extern "C" void vgd_close(virginian *v)
 		(void)hipFree(c);
 	for (launch_res &lr : d->lr_free)
 		free_lr(lr);
+	free_stream_cache(d);
 	(void)hipStreamDestroy(d->stream);
 	delete d;
 	s->dev = nullptr;
@@ -462,6 +465,7 @@ struct sbuf {
 	char *stage = nullptr;
 	size_t stage_cap = 0;
 	char *h_stage = nullptr;        /* pinned host copy of `stage` for pageable tablets */
+	size_t h_stage_cap = 0;
 	char *out = nullptr;
 	size_t out_cap = 0;
 	vg_tabdesc *d_desc = nullptr, *h_desc = nullptr;
@@ -489,6 +493,29 @@ static void free_sbuf(sbuf &b)
 	if (b.freed) (void)hipEventDestroy(b.freed);
 	if (b.stream) (void)hipStreamDestroy(b.stream);
 	b = sbuf();
+}
+
+/* A device's streaming buffers, kept from one streaming call to the next
+ * (each grows to the largest batch seen and is held until the instance
+ * closes): allocating and pinning them again on every call cost several
+ * milliseconds a query. */
+struct stream_cache {
+	sbuf B[2];
+	hipStream_t down = nullptr;     /* result copies */
+};
+
+static void free_stream_cache(vgd_device *d)
+{
+	if (!d->scache)
+		return;
+	for (int k = 0; k < 2; k++)
+		free_sbuf(d->scache->B[k]);
+	if (d->scache->down) {
+		(void)hipStreamSynchronize(d->scache->down);
+		(void)hipStreamDestroy(d->scache->down);
+	}
+	delete d->scache;
+	d->scache = nullptr;
 }
 
 /* receives each batch's results, in order: n rows of every result column
@@ -613,33 +640,78 @@ static int stream_select(vgd_device *d, const vg_prog &P, const std::vector<cons
 		pageable[i] = !host_pinned(tabs[i]);
 		any_pageable = any_pageable || pageable[i];
 	}
-	sbuf B[2];
-	hipStream_t down = nullptr;
 	int rc = VIRG_SUCCESS;
 	uint64_t rows_out = 0, h2d = 0, d2h = 0, scanned = 0;
 	auto t0 = std::chrono::steady_clock::now();
+	if (!d->scache)
+		d->scache = new stream_cache();
+	sbuf *const B = d->scache->B;
+	hipStream_t &down = d->scache->down;
 #define SCHK(call, what) do { if (herr((call), (what))) { rc = VIRG_FAIL; goto done; } } while (0)
-	SCHK(hipStreamCreateWithFlags(&down, hipStreamNonBlocking), "stream");
+	if (!down)
+		SCHK(hipStreamCreateWithFlags(&down, hipStreamNonBlocking), "stream");
 	for (int k = 0; k < 2; k++) {
 		sbuf &b = B[k];
 		/* staging: every loaded column of every tablet, 256-byte aligned */
-		b.stage_cap = (size_t)maxrows * row_in + (size_t)maxtabs * P.ncol * 256 + 256;
-		b.out_cap = (size_t)maxrows * row_out + 256 * (P.nout + 1);
-		b.desc_cap = maxtabs;
-		b.tile_cap = maxtiles;
-		SCHK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking), "stream");
-		SCHK(hipEventCreateWithFlags(&b.done, hipEventDisableTiming), "event");
-		SCHK(hipEventCreateWithFlags(&b.freed, hipEventDisableTiming), "event");
-		SCHK(hipMalloc((void **)&b.stage, b.stage_cap), "stream staging");
-		if (any_pageable)
-			SCHK(hipHostMalloc((void **)&b.h_stage, b.stage_cap, hipHostMallocDefault),
+		const size_t stage = (size_t)maxrows * row_in + (size_t)maxtabs * P.ncol * 256 + 256;
+		const size_t outb = (size_t)maxrows * row_out + 256 * (P.nout + 1);
+		b.busy = false;
+		if (!b.stream) {
+			SCHK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking), "stream");
+			SCHK(hipEventCreateWithFlags(&b.done, hipEventDisableTiming), "event");
+			SCHK(hipEventCreateWithFlags(&b.freed, hipEventDisableTiming), "event");
+			SCHK(hipHostMalloc((void **)&b.h_ctrl, sizeof(vg_ctrl), hipHostMallocDefault), "control block");
+		}
+		/* the buffers of the last call, grown where this one needs more */
+		if (b.stage_cap < stage) {
+			if (b.stage)
+				(void)hipFree(b.stage);
+			b.stage = nullptr;
+			b.stage_cap = 0;
+			SCHK(hipMalloc((void **)&b.stage, stage), "stream staging");
+			b.stage_cap = stage;
+		}
+		if (any_pageable && b.h_stage_cap < stage) {
+			if (b.h_stage)
+				(void)hipHostFree(b.h_stage);
+			b.h_stage = nullptr;
+			b.h_stage_cap = 0;
+			SCHK(hipHostMalloc((void **)&b.h_stage, stage, hipHostMallocDefault),
 				"pinned staging for pageable tablets");
-		SCHK(hipMalloc((void **)&b.out, b.out_cap), "stream output");
-		SCHK(hipMalloc((void **)&b.d_desc, b.desc_cap * sizeof(vg_tabdesc)), "descriptors");
-		SCHK(hipMalloc((void **)&b.d_tile, b.tile_cap * sizeof(uint32_t)), "tile table");
-		SCHK(hipHostMalloc((void **)&b.h_desc, b.desc_cap * sizeof(vg_tabdesc), hipHostMallocDefault), "descriptors");
-		SCHK(hipHostMalloc((void **)&b.h_tile, b.tile_cap * sizeof(uint32_t), hipHostMallocDefault), "tile table");
-		SCHK(hipHostMalloc((void **)&b.h_ctrl, sizeof(vg_ctrl), hipHostMallocDefault), "control block");
+			b.h_stage_cap = stage;
+		}
+		if (b.out_cap < outb) {
+			if (b.out)
+				(void)hipFree(b.out);
+			b.out = nullptr;
+			b.out_cap = 0;
+			SCHK(hipMalloc((void **)&b.out, outb), "stream output");
+			b.out_cap = outb;
+		}
+		if (b.desc_cap < maxtabs) {
+			if (b.d_desc)
+				(void)hipFree(b.d_desc);
+			if (b.h_desc)
+				(void)hipHostFree(b.h_desc);
+			b.d_desc = b.h_desc = nullptr;
+			b.desc_cap = 0;
+			SCHK(hipMalloc((void **)&b.d_desc, maxtabs * sizeof(vg_tabdesc)), "descriptors");
+			SCHK(hipHostMalloc((void **)&b.h_desc, maxtabs * sizeof(vg_tabdesc), hipHostMallocDefault),
+				"descriptors");
+			b.desc_cap = maxtabs;
+		}
+		if (b.tile_cap < maxtiles) {
+			if (b.d_tile)
+				(void)hipFree(b.d_tile);
+			if (b.h_tile)
+				(void)hipHostFree(b.h_tile);
+			b.d_tile = b.h_tile = nullptr;
+			b.tile_cap = 0;
+			SCHK(hipMalloc((void **)&b.d_tile, maxtiles * sizeof(uint32_t)), "tile table");
+			SCHK(hipHostMalloc((void **)&b.h_tile, maxtiles * sizeof(uint32_t), hipHostMallocDefault),
+				"tile table");
+			b.tile_cap = maxtiles;
+		}
 		if (ensure_lr(b.lr, (uint32_t)maxtiles, vg_kernel_grid(&P, (uint32_t)maxtiles), b.stream) == VIRG_FAIL) {
 			rc = VIRG_FAIL;
 			goto done;
@@ -746,12 +818,10 @@ static int stream_select(vgd_device *d, const vg_prog &P, const std::vector<cons
 	}
 done:
 #undef SCHK
-	for (int k = 0; k < 2; k++)
-		free_sbuf(B[k]);
-	if (down) {
-		(void)hipStreamSynchronize(down);
-		(void)hipStreamDestroy(down);
-	}
+	/* the buffers stay for the next call; after a failure they go, so that
+	 * the next call starts from nothing */
+	if (rc == VIRG_FAIL)
+		free_stream_cache(d);
 	return rc;
 }
 
@@ -771,6 +841,9 @@ struct array_sink : stream_sink {
 		return VIRG_SUCCESS;
 	}
 };
+
+/* largest result tablet the device path allocates (rows) */
+#define VG_RESULT_TABLET_ROWS (1u << 26)
 
 /* results into the host result tablet chain of a virg_vm (as the reference
  * fills result tablets, allocating the next one when one is full) */
@@ -797,7 +870,14 @@ struct tablet_sink : stream_sink {
 				break;
 			VIRG_CHECK(r->possible_rows == 0, "result tablet has no capacity");
 			virg_tablet_meta *old = r;
-			VIRG_CHECK(virg_vm_allocresult(v, vm, res, old) == VIRG_FAIL, "result allocation failed");
+			/* the rest of the batch in one tablet (few large copies instead
+			 * of one per 8 MB), unless a database file must be able to hold
+			 * every tablet */
+			const uint64_t left = n - done;
+			const unsigned want = (unsigned)(left < VG_RESULT_TABLET_ROWS ? left : VG_RESULT_TABLET_ROWS);
+			VIRG_CHECK((v->dbfd < 0 && want > old->possible_rows
+					? virg_vm_allocresult_rows(v, vm, res, old, want)
+					: virg_vm_allocresult(v, vm, res, old)) == VIRG_FAIL, "result allocation failed");
 			virg_tablet_unlock(v, old->id);
 		}
 		return VIRG_SUCCESS;
@@ -935,8 +1015,6 @@ extern "C" const size_t *virg_gpu_getsizes(void)
 }
 
 /* reference: src/vm/vm_gpu.cu:107 */
-/* largest result tablet the device path allocates (rows) */
-#define VG_RESULT_TABLET_ROWS (1u << 26)
 
 extern "C" int virg_vm_gpu(virginian *v, virg_vm *vm, virg_tablet_meta **tab,
 	virg_tablet_meta **res, unsigned num_tablets)
