@@ -5,22 +5,26 @@ tablets, resident in HBM;
     select (uniformf * 2.0) + normalf5, uniformi - normali20,
            uniformf / (normalf20 + 1000.0) from test where uniformf < T
 for T in 0.1, 10, 50, 90 (selectivity 0.1 %, 10 %, 50 %, 90 %).  One step
-runs the four queries, each one count-pass + write-pass launch pair over
-all 64 tablets on its own stream.  value = scanned bytes (referenced
-columns at full length) of all queries / wall time of the K timed steps.
+runs the four queries, each one launch of its specialised single-pass
+kernel over all 64 tablets on its own stream.  value = scanned bytes
+(referenced columns at full length) of all queries / wall time of the K
+timed steps.  The 64M-row results are checked against the oracle's
+digests outside the timed region (verified).
 
 Also reported:
-  roofline      the query's kernel pair: algorithmic bytes per launch
+  roofline      the query's kernel: algorithmic bytes per launch
                 (referenced columns x rows + selected rows x result bytes)
-                / mean launch-pair time from HIP events on the query's
-                stream, against 8 TB/s; per pass in config.per_query;
-  traffic       HBM bytes per launch pair measured with rocprofv3
-                FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a
-                device copy of known size (profiles/traffic_*.json), or null;
+                / mean launch time from HIP events on the query's stream,
+                against 8 TB/s; per query in config.per_query, beside the
+                interpreter's count + write launch pair;
+  traffic       HBM bytes per launch measured with rocprofv3 FETCH_SIZE /
+                WRITE_SIZE, scaled by the same counters on a device copy
+                of known size (profiles/traffic_*.json), or null;
   cpu_baseline  the oracle's multi-threaded CPU VM on this host's cores
                 over a bounded sample of the same workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --workload config1|outofcore|batched|served|suite
     python bench.py --copy-probe 256     # known-size device copies (counter calibration)
 """
 from __future__ import annotations
@@ -117,7 +121,7 @@ CONFIG1_SQL = "select id, uniformi, normalf5 from test where uniformi < 50 and n
 def bench_config1(a) -> int:
     """The plain filter over one 1M-row tablet (select id, uniformi, normalf5
     where uniformi < 50 and normali5 > 0; 23 % selected): latency, best of
-    5 and median of 20, of the resident launch pair (HIP events) and of the
+    5 and median of 20, of the resident query's launch (HIP events) and of the
     drop-in virg_vm_execute (wall clock, results into host result tablets),
     beside the oracle's single-core CPU VM on the same table.  At 21 MB
     read the GPU side is launch-bound; the figures are latencies."""
@@ -145,18 +149,19 @@ def bench_config1(a) -> int:
     def stats(v):
         return {"best_ms": round(min(sorted(v)[:5]), 4), "median_ms": round(statistics.median(v), 4)}
 
-    print(f"# config 1: {q.last_rows} of {q.rows} rows; launch pair median {statistics.median(pair):.3f} ms, "
+    print(f"# config 1: {q.last_rows} of {q.rows} rows; resident launch median {statistics.median(pair):.3f} ms "
+          f"({'specialised single pass' if q.kernel()['fused'] else 'interpreter launch pair'}), "
           f"virg_vm_execute median {statistics.median(dropin):.3f} ms, single-core CPU VM median "
           f"{statistics.median(cpu):.1f} ms", file=sys.stderr, flush=True)
     line = {
-        "metric": "latency of the plain filter SELECT over one 1M-row tablet (launch pair)",
+        "metric": "latency of the plain filter SELECT over one 1M-row tablet (resident query launch)",
         "value": round(statistics.median(pair), 4), "unit": "ms", "n_gpus": 1, "steps": 20, "warmup": 1,
         "ms_per_step": round(statistics.median(pair), 4), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "i32+f32", "data": "synthetic (seeded Philox generator, 7-column schema)",
         "verified": check["verified"],
         "config": {"workload": "plain filter SELECT (23 % selected) over one 1M-row tablet resident in HBM",
                    "rows": q.rows, "selected": q.last_rows, "read_bytes": read_b,
-                   "launch_pair": stats(pair), "dropin_virg_vm_execute": stats(dropin)},
+                   "kernel": q.kernel(), "launch": stats(pair), "dropin_virg_vm_execute": stats(dropin)},
         "roofline": {"bound": "hbm", "achieved": round(read_b / statistics.median(pair) / 1e6, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(read_b / statistics.median(pair) / 1e6 / HBM_PEAK_GBS, 4), "traffic": None},

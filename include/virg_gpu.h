@@ -72,7 +72,9 @@ void vg_query_free(vg_query *q);
 /* Set when prepared queries get a specialised kernel (policy >= 0) and
  * return the previous policy: 0 never, 1 (the default, or the value of the
  * environment variable VG_JIT) queries that are neither light nor flagged
- * VG_QUERY_INTERP, 2 light queries too. */
+ * VG_QUERY_INTERP and whose tables hold at least 8192 rows per compute unit
+ * (smaller ones cannot fill the device with the single pass's chunks), 2
+ * every query not flagged VG_QUERY_INTERP. */
 int vg_jit_policy(int policy);
 /* What runs q: *fused = 1 for its specialised single-pass kernel (rows per
  * lane, tiles per chunk, workgroups, compile milliseconds; *mode bit 0: the

@@ -18,7 +18,9 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def jit():
-    old = va.jit_policy(1)
+    # every query, the small tables too (the default leaves tables of fewer
+    # than 8192 rows per CU on the interpreter)
+    old = va.jit_policy(2)
     yield
     va.jit_policy(old)
 

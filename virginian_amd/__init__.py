@@ -273,8 +273,9 @@ lib = _Lib()
 
 def jit_policy(policy: int = -1) -> int:
     """vg_jit_policy: when prepared queries get a specialised single-pass
-    kernel (0 never, 1 non-light queries, 2 all); returns the previous
-    policy, and only reads it when policy < 0."""
+    kernel (0 never; 1, the default, queries that are not light, over
+    tables of at least 8192 rows per compute unit; 2 all); returns the
+    previous policy, and only reads it when policy < 0."""
     return lib.vg_jit_policy(policy)
 
 
