@@ -2,7 +2,8 @@
 referenced columns stream through HBM (vg_stream_select, and virg_vm_gpu
 with use_stream set as in the reference).  Bit-exact against the oracle in
 row order, over batch sizes that split the table at tablet boundaries,
-with a ragged last tablet."""
+with a ragged last tablet, and over calls in a row that share the
+instance's streaming buffers."""
 from __future__ import annotations
 
 import pytest
@@ -88,3 +89,29 @@ def test_stream_pageable_tablets(gpu_db):
         got, st = gpu_db.stream_select(CONFIG3_SQL, batch_rows=batch_rows)
         assert st.h2d_bytes == ((6 << 20) + 5) * 28
         assert_same(got, want, f"pageable batch_rows={batch_rows}")
+
+
+def test_stream_buffers_kept_between_calls(gpu_db):
+    """The streaming buffers outlive a call: calls on one instance whose
+    batches grow (1M rows, the whole table, 1M again), under programs of
+    different column and result widths, a call that fails (its capacity is
+    too small, which frees them) and the calls after it, through both
+    vg_stream_select and the drop-in with use_mmap, all give the oracle's
+    rows."""
+    gpu_db.gen_table("test", (4 << 20) + 321, 1 << 20, seed=26)
+    sqls = [CONFIG3_SQL, "select id from test where uniformi < 30", config2_sql(70.0)]
+    wants = {sql: oracle.select(gpu_db, sql, threads=8)[0] for sql in sqls}
+    for batch_rows in (1 << 20, 1 << 30, 1 << 20):
+        for sql in sqls:
+            got, _ = gpu_db.stream_select(sql, batch_rows=batch_rows)
+            assert_same(got, wants[sql], f"{sql} batch_rows={batch_rows}")
+    with pytest.raises(va.VirginianError):
+        gpu_db.stream_select(sqls[1], capacity=10)
+    got, _ = gpu_db.stream_select(sqls[2], batch_rows=1 << 30)
+    assert_same(got, wants[sqls[2]], "after a failed call")
+    gpu_db.use_mmap = True
+    try:
+        for sql in sqls:
+            assert_same(gpu_db.query(sql), wants[sql], f"use_mmap {sql}")
+    finally:
+        gpu_db.use_mmap = False
