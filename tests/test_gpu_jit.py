@@ -232,3 +232,30 @@ def test_mask_logic_regression(gpu_db, jit, monkeypatch, stage):
         got, k = _run(gpu_db, prog)
         assert k["staged"] == (stage == "1")
         assert_same(got, want, f"{prog}")
+
+
+def test_small_tables_stay_on_the_interpreter(gpu_db):
+    """Under the default policy a table of fewer than 8192 rows per CU runs
+    on the interpreter (its chunks could not fill the device); policy 2
+    takes the specialised kernel for it too; a larger table gets it by
+    default.  The results are the oracle's either way."""
+    name, cus = gpu_db.device()
+    gpu_db.gen_table("small", 1 << 20, 1 << 20, seed=7)
+    gpu_db.gen_table("test", 4 << 20, 1 << 20, seed=7)
+    assert (1 << 20) < cus * 8192 <= (4 << 20), cus
+    small_sql = CONFIG1_SQL.replace("from test", "from small")
+    old = va.jit_policy(1)
+    try:
+        for policy, sql, fused in ((1, small_sql, False), (2, small_sql, True), (1, CONFIG1_SQL, True)):
+            va.jit_policy(policy)
+            q = gpu_db.prepare(sql)
+            try:
+                assert q.kernel()["fused"] == fused, (policy, sql)
+                q.run()
+                want, ill = oracle.select(gpu_db, sql)
+                assert ill == 0
+                assert_same(q.fetch(), want, f"policy {policy}: {sql}")
+            finally:
+                q.close()
+    finally:
+        va.jit_policy(old)
