@@ -51,19 +51,26 @@ def test_config2(gpu_db, jit, threshold):
     assert_same(got, want, sql)
 
 
-def test_held_launch_bound(gpu_db, jit):
-    """A one-column filter is held in registers.  The benchmark query's
-    kernel is compiled for three waves per SIMD (a few dwords spilled).
-    Selecting all six columns spills well past the limit at that bound, so
-    its kernel is rebuilt at the compiler's own occupancy.  Both are bit
-    exact against the oracle."""
+def test_held_launch_bound(gpu_db, jit, monkeypatch):
+    """A one-column filter is held in registers, by default in 16K-row
+    chunks at the compiler's own occupancy.  With 8K-row chunks and the
+    three-wave launch bound (VG_JIT_S=2, VG_JIT_WAVES=3) the benchmark
+    query's kernel keeps the bound (a few dwords spilled), while selecting
+    all six columns spills past the limit and is rebuilt without it.  All
+    are bit exact against the oracle."""
     gpu_db.gen_table("test", 4 << 20, 1 << 20, seed=4)
-    cases = [(config2_sql(10.0), 3),
-             ("select id, uniformf, normalf5, uniformi, normali20, normalf20 from test where uniformf < 50.0", 0)]
-    for sql, waves in cases:
+    six = "select id, uniformf, normalf5, uniformi, normali20, normalf20 from test where uniformf < 50.0"
+    cases = [({}, config2_sql(10.0), 0, 4),
+             ({"VG_JIT_S": "2", "VG_JIT_WAVES": "3"}, config2_sql(10.0), 3, 2),
+             ({"VG_JIT_S": "2", "VG_JIT_WAVES": "3"}, six, 0, 2)]
+    for env, sql, waves, tiles in cases:
+        for k in ("VG_JIT_S", "VG_JIT_WAVES"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
         got, k = _run(gpu_db, sql)
-        print(f"{sql}: {len(next(iter(got.values())))} rows, kernel {k}", flush=True)
-        assert k["held"] and k["pipelined"] and k["waves"] == waves, k
+        print(f"{env} {sql}: {len(next(iter(got.values())))} rows, kernel {k}", flush=True)
+        assert k["held"] and k["pipelined"] and k["waves"] == waves and k["tiles_per_chunk"] == tiles, k
         want, ill = oracle.select(gpu_db, sql, threads=8)
         assert ill == 0
         assert_same(got, want, sql)

@@ -481,9 +481,12 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 		S = 16384u / TR;
 	}
 	/* a pipelined filter of one 4-byte (or 1-byte) column: held in
-	 * registers instead, two chunks of 8K rows at a time (2 x 32 values a
-	 * lane at 16 rows per lane), which leaves the LDS free for more
-	 * workgroups per CU and reads the column once */
+	 * registers instead, two chunks of 16K rows at a time (2 x 64 values a
+	 * lane, S x R = 64 validity bits a lane), which reads the column once
+	 * and needs no LDS for it.  Chunks of 16K rows at two workgroups per CU
+	 * beat 8K rows at three (0.699 against 0.675 of HBM on the 64M-row
+	 * sweep, the 0.1 % query 114 against 131 us): each chunk's chain of
+	 * waits is paid half as often */
 	int held = 0;
 	const char *hm = getenv("VG_JIT_HELD");
 	if (pipe && fwords == 1 && !(hm && hm[0] == '0') && !(mode && (mode[0] == '0' || mode[0] == '1'))) {
@@ -492,7 +495,7 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 		R = R0;
 		TR = 256u * (uint32_t)R;
 		ftile = 0;
-		S = 8192u / TR;
+		S = 16384u / TR;
 	}
 	if (S_override >= 1 && S_override <= 16)
 		S = (uint32_t)S_override;
@@ -507,19 +510,16 @@ extern "C" int vgx_shape_of(const vg_prog *P, int aligned, int R_override, int S
 	sh->ftile_bytes = ftile;
 	sh->staged = staged;
 	sh->held = held;
-	/* a held shape keeps two chunks' filter values and a tile's result
-	 * loads in VGPRs (about 180 at 16 rows per lane, two waves per SIMD);
-	 * compiled for three waves per SIMD it fits 168 with a few dwords
-	 * spilled and runs three workgroups per CU, which hides more of each
-	 * chunk's load and look-back latency */
-	sh->waves = held ? 3 : 0;
-	/* VG_JIT_WAVES=0 drops the bound; no other bound is used: a
-	 * register-tile kernel built for four waves per SIMD (128 VGPRs, 8 B
+	/* A launch bound of three waves per SIMD for a held shape
+	 * (VG_JIT_WAVES=3): with 8K-row chunks (VG_JIT_S) it fits 168 VGPRs
+	 * with 24 B spilled and runs three workgroups per CU (0.675, against
+	 * 0.658 unbounded); with the default 16K-row chunks it would spill
+	 * 288 B a lane and is built again unbounded.  No other bound is used:
+	 * a register-tile kernel built for four waves per SIMD (128 VGPRs, 8 B
 	 * spilled) reported a capacity overflow on the 64M-row benchmark
-	 * query, where the same source at its own occupancy is exact */
+	 * query, where the same source at its own occupancy is exact. */
 	const char *wv = getenv("VG_JIT_WAVES");
-	if (wv && wv[0] == '0')
-		sh->waves = 0;
+	sh->waves = held && wv && wv[0] == '3' ? 3 : 0;
 	sh->stage_off = ftile * S * (pipe ? 2u : 1u);
 	sh->stage_wave_bytes = stage_wave;
 	sh->lds_bytes = sh->stage_off + 4u * sh->stage_wave_bytes;
